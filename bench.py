@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark: PCG solve to 1e-8 relative residual on the 128^3 heterogeneous
+7-point operator (log10 k ~ N(0, 1.5^2), harmonic transmissibilities,
+Dirichlet faces), hypre-default hierarchy (aggressive HMIS + HMIS, ext+i
+trunc 0.3) with l1-Jacobi V(1,1), FP64 outer PCG.
+
+One step = one PCG solve from x = 0 with the hierarchy and b resident in HBM.
+The headline value is seconds per solve with the FP32 cycle; the FP64-cycle
+solve time, the speedup, iteration counts, V-cycle time and bandwidth, the
+roofline of the dominant kernel and a CPU baseline (the oracle's OpenMP
+restatement of the same solve on a bounded sample) ride along in the line.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "draugr.jl_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "s/PCG solve to 1e-8 rel resid, 128³ 7-pt; FP32-cycle vs FP64 speedup; GB/s"
+HBM_PEAK_GBS = 8000.0
+SEED = 20261202
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--sigma", type=float, default=1.5)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
+    ap.add_argument("--cpu-sample-iters", type=int, default=3,
+                    help="PCG iterations of the CPU baseline sample (its time is scaled to a full solve)")
+    ap.add_argument("--skip-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_solves(dh, torch, h, cfg, b, steps, warmup, tol):
+    x = torch.zeros_like(b)
+    its = []
+    for _ in range(warmup):
+        x.zero_()
+        dh.pcg_solve_(x, b, h, cfg, tol=tol, maxiter=500)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        x.zero_()
+        _, it, rel = dh.pcg_solve_(x, b, h, cfg, tol=tol, maxiter=500)
+        its.append(it)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return dt, its, rel, x
+
+
+def event_time(torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the design for sharding is in "
+              "the documentation); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import draugr_hip as dh
+
+    P = dh.problems
+    m = args.size
+    t0 = time.perf_counter()
+    logk = P.log_permeability(m, args.sigma, SEED + 1)
+    A = P.heterogeneous3d(m, logk)
+    b_host = P.splitmix_normal(SEED + 2, A.n)
+    log("problem %d^3: n=%d nnz=%d (%.1fs)" % (m, A.n, A.nnz, time.perf_counter() - t0))
+    cfg64 = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())
+    cfg32 = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), precision=args.precision)
+    t0 = time.perf_counter()
+    h = dh.amg_setup(A, cfg64)
+    t_setup = time.perf_counter() - t0
+    log(h.summary())
+    log("setup + upload %.2fs, resident %.1f MB" % (t_setup, h.resident_bytes() / 1e6))
+
+    b = torch.tensor(b_host, device="cuda")
+    t64, its64, rel64, x64 = time_solves(dh, torch, h, cfg64, b, args.steps, args.warmup, args.tol)
+    log("FP64 cycle: %.4f s/solve, iterations %s, relres %.2e" % (t64, its64, rel64))
+    t32, its32, rel32, x32 = time_solves(dh, torch, h, cfg32, b, args.steps, args.warmup, args.tol)
+    log("%s cycle: %.4f s/solve, iterations %s, relres %.2e" % (args.precision, t32, its32, rel32))
+    xr = x64.cpu().numpy()
+    x3 = x32.cpu().numpy()
+    true32 = float(np.linalg.norm(b_host - A.matvec(x3)) / np.linalg.norm(b_host))
+    err32 = float(np.linalg.norm(x3 - xr) / np.linalg.norm(xr))
+
+    # V-cycle time and bandwidth (zero-guess preconditioner application)
+    z = torch.zeros_like(b)
+    t_cyc32 = event_time(torch, lambda: dh.amg_cycle_(z, b, h, cfg32, zero_guess=True), 20)
+    t_cyc64 = event_time(torch, lambda: dh.amg_cycle_(z, b, h, cfg64, zero_guess=True), 20)
+    bytes32, bytes64 = h.cycle_bytes(cfg32), h.cycle_bytes(cfg64)
+
+    # dominant kernel: level-0 Jacobi step in the cycle's precision, timed live
+    tdt = torch.float32
+    n0, nnz0 = A.n, A.nnz
+    xa = torch.randn(n0, dtype=tdt, device="cuda")
+    ba = torch.randn(n0, dtype=tdt, device="cuda")
+    oa = torch.empty(n0, dtype=tdt, device="cuda")
+    prec_code = args.precision
+    t_k = event_time(torch, lambda: h.level_op(0, 2, prec_code, xa, ba, oa), 50)
+    sv = 4 if args.precision == "fp32" else 2
+    kbytes = nnz0 * (sv + 4) + (n0 + 1) * 4 + 4 * n0 * 4
+    ach = kbytes / t_k / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": t32,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 cycle / f64 PCG" if args.precision == "fp32" else "bf16-value cycle / f64 PCG",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D 7-point heterogeneous diffusion %d^3 (n=%d, nnz=%d), log10 k ~ N(0,%.1f^2), "
+                        "hypre_default_config + l1-Jacobi V(1,1), PCG to %.0e" % (m, A.n, A.nnz, args.sigma,
+                                                                                args.tol),
+            "n": A.n, "nnz": A.nnz, "levels": h.nlevels, "seed": SEED,
+        },
+        "fp64_cycle_s_per_solve": t64,
+        "speedup_fp32_cycle_vs_fp64": t64 / t32,
+        "iters_fp64": its64[-1],
+        "iters_mixed": its32[-1],
+        "mixed_true_relres": true32,
+        "mixed_rel_err_vs_fp64_solution": err32,
+        "vcycle_ms": {"mixed": t_cyc32 * 1e3, "fp64": t_cyc64 * 1e3},
+        "vcycle_GBps_algorithmic": {"mixed": bytes32 / t_cyc32 / 1e9, "fp64": bytes64 / t_cyc64 / 1e9},
+        "vcycle_bytes": {"mixed": bytes32, "fp64": bytes64},
+        "setup_s": t_setup,
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "level-0 %s Jacobi step (k_csr_rows EpiJacobi)" % args.precision,
+                     "bytes_per_launch": kbytes, "avg_launch_us": t_k * 1e6},
+    }
+
+    if not args.skip_cpu:
+        from oracle import oracle as orc
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        levels = [h.level(l, with_rmap=False) for l in range(h.nlevels)]
+        c = h.coarsest()
+        oh = orc.OracleHierarchy(levels, c["rp"], c["cv"], c["nz"], smoother="l1_jacobi")
+        k = args.cpu_sample_iters
+        t0 = time.perf_counter()
+        _, itc, _ = oh.pcg(np.zeros(A.n), b_host, 0.0, k)  # tol 0: exactly k iterations
+        tc = time.perf_counter() - t0
+        per_solve = tc / k * its64[-1]
+        out["cpu_baseline"] = {"value": per_solve, "unit": "s", "cores": threads, "kind": "port",
+                               "sample": "%d FP64 PCG iterations (oracle OpenMP restatement, same hierarchy), "
+                                         "scaled to %d iterations" % (k, its64[-1])}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

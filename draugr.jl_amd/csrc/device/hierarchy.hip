@@ -1,0 +1,1002 @@
+// Device-resident hierarchy and the stream-ordered drivers of the hot path:
+// V/W cycle (src/cycle.jl:12-99), stationary solve (src/cycle.jl:110-160),
+// FP64 PCG around a FP64 / FP32 / BF16-storage cycle, and the partial numeric
+// resetup (src/resetup.jl:20-45).  No host synchronisation happens inside a
+// cycle; the solvers read one scalar per iteration back to the host.
+#include "hierarchy.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+
+namespace draugr {
+
+using namespace dev;
+
+namespace {
+
+constexpr int kRedBlocks = 1024;
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_map(int64_t n, F f) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) f(i);
+}
+
+template <class F>
+void launch_map(int64_t n, F f, hipStream_t st) {
+    if (n <= 0) return;
+    int64_t nb = (n + kBlock - 1) / kBlock;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_map<F>, dim3((unsigned)nb), dim3(kBlock), 0, st, n, f);
+}
+
+__device__ __forceinline__ void store_agent(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_agent(const double* p) {
+    return __longlong_as_double((long long)__hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Block sum in LDS (fixed tree order) -> partial[block]; the last block to
+// arrive sums the partials in index order and calls fin(total) on one lane.
+// Deterministic for a fixed grid size.
+template <class FIN>
+__device__ __forceinline__ void block_reduce_finish(double v, double* sh, double* partial, uint32_t* counter,
+                                                    int nblocks, int block, FIN fin) {
+    __shared__ bool last;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        store_agent(&partial[block], sh[0]);
+        __threadfence();
+        const uint32_t t = atomicAdd(counter, 1u);
+        last = (t == (uint32_t)nblocks - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += kBlock) acc += load_agent(&partial[i]);
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+#pragma unroll
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        fin(sh[0]);
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <class F, class FIN>
+__global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, F f, double* partial, uint32_t* counter, FIN fin) {
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) acc += f(i);
+    block_reduce_finish(acc, sh, partial, counter, gridDim.x, blockIdx.x, fin);
+}
+
+template <class F, class FIN>
+void launch_reduce(int64_t n, F f, FIN fin, double* partial, uint32_t* counter, hipStream_t st) {
+    int64_t nb = (n + kBlock - 1) / kBlock;
+    nb = std::max<int64_t>(1, std::min<int64_t>(nb, kRedBlocks));
+    hipLaunchKernelGGL((k_reduce<F, FIN>), dim3((unsigned)nb), dim3(kBlock), 0, st, n, f, partial, counter, fin);
+}
+
+// CSR row-group pass whose epilogue returns a contribution to a global sum
+// (e.g. q = A p fused with p.q).  Rows are split in contiguous chunks per block.
+template <int G, typename TV, typename TX, class XF, class EPI, class FIN>
+__global__ __launch_bounds__(kBlock) void k_rows_reduce(int nrows, int chunk, const int* __restrict__ rp,
+                                                        const int* __restrict__ cv, const TV* __restrict__ av,
+                                                        XF xf, EPI epi, double* partial, uint32_t* counter,
+                                                        FIN fin) {
+    __shared__ double sh[kBlock];
+    const int b = xcd_block();
+    const int lane = threadIdx.x % G, slot = threadIdx.x / G;
+    constexpr int kRows = kBlock / G;
+    const int r0 = b * chunk, r1 = min(nrows, r0 + chunk);
+    double dot = 0.0;
+    for (int base = r0; base < r1; base += kRows) {
+        const int row = base + slot;
+        if (row < r1) {
+            const int s = rp[row], e = rp[row + 1];
+            TX acc = 0;
+            for (int k = s + lane; k < e; k += G) acc += ld<TX, TV>(av, k) * xf(ldi(cv, k));
+            acc = group_sum<G>(acc);
+            if (lane == 0) dot += epi(row, acc);
+        }
+    }
+    block_reduce_finish(dot, sh, partial, counter, gridDim.x, blockIdx.x, fin);
+}
+
+template <int G, typename TV, typename TX, class XF, class EPI, class FIN>
+void launch_rows_reduce_g(int nrows, const int* rp, const int* cv, const TV* av, XF xf, EPI epi, FIN fin,
+                          double* partial, uint32_t* counter, hipStream_t st) {
+    constexpr int kRows = kBlock / G;
+    int64_t nb = ((int64_t)nrows + kRows - 1) / kRows;
+    nb = std::max<int64_t>(1, std::min<int64_t>(nb, kRedBlocks));
+    int chunk = (int)(((int64_t)nrows + nb - 1) / nb);
+    chunk = ((chunk + kRows - 1) / kRows) * kRows;
+    nb = ((int64_t)nrows + chunk - 1) / chunk;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL((k_rows_reduce<G, TV, TX, XF, EPI, FIN>), dim3((unsigned)nb), dim3(kBlock), 0, st, nrows,
+                       chunk, rp, cv, av, xf, epi, partial, counter, fin);
+}
+
+template <typename TV, typename TX, class XF, class EPI, class FIN>
+void launch_rows_reduce(int G, int nrows, const int* rp, const int* cv, const TV* av, XF xf, EPI epi, FIN fin,
+                        double* partial, uint32_t* counter, hipStream_t st) {
+    switch (G) {
+        case 1: launch_rows_reduce_g<1, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
+        case 2: launch_rows_reduce_g<2, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
+        case 4: launch_rows_reduce_g<4, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
+        case 8: launch_rows_reduce_g<8, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
+        case 16: launch_rows_reduce_g<16, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
+        case 32: launch_rows_reduce_g<32, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
+        default: launch_rows_reduce_g<64, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
+    }
+}
+
+// Inverse diagonal for the smoothers: plain (src/smoothers.jl:47-63) or l1
+// (src/smoothers.jl:816-827).  One row group per row.
+template <int G, bool L1>
+__global__ __launch_bounds__(kBlock) void k_invdiag(int n, const int* __restrict__ rp, const int* __restrict__ cv,
+                                                    const double* __restrict__ av, double* __restrict__ d) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int row = t / G, lane = t % G;
+    if (row >= n) return;
+    double nrm = 0.0;
+    double dg = 0.0;
+    int dgpos = -1;
+    for (int k = rp[row] + lane; k < rp[row + 1]; k += G) {
+        nrm += fabs(av[k]);
+        if (cv[k] == row) { dg = av[k]; dgpos = k; }
+    }
+    nrm = group_sum<G>(nrm);
+    // the last diagonal match in row order wins
+#pragma unroll
+    for (int off = G >> 1; off > 0; off >>= 1) {
+        const int op = __shfl_xor(dgpos, off, G);
+        const double ov = __shfl_xor(dg, off, G);
+        if (op > dgpos) { dgpos = op; dg = ov; }
+    }
+    if (lane != 0) return;
+    const double eps = 2.220446049250313e-16;
+    if (L1) {
+        d[row] = nrm > eps ? 1.0 / nrm : 0.0;
+    } else {
+        const double thr = eps * fmax(1.0, nrm);
+        d[row] = fabs(dg) > thr ? 1.0 / dg : 0.0;
+    }
+}
+
+// Galerkin numeric product into the fixed coarse pattern:
+// Ac[k] = sum_t P[pi_t] * A[ai_t] * P[pj_t] over the triples of nonzero k
+// (src/galerkin.jl:194-222).  One row group per coarse nonzero.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nnzc, const int64_t* __restrict__ off,
+                                                     const int* __restrict__ pi, const int* __restrict__ ai,
+                                                     const int* __restrict__ pj, const double* __restrict__ P,
+                                                     const double* __restrict__ A, double* __restrict__ Ac) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t k = t / G;
+    const int lane = (int)(t % G);
+    if (k >= nnzc) return;
+    double acc = 0.0;
+    for (int64_t q = off[k] + lane; q < off[k + 1]; q += G) acc += (P[pi[q]] * A[ai[q]]) * P[pj[q]];
+    acc = group_sum<G>(acc);
+    if (lane == 0) Ac[k] = acc;
+}
+
+// Dense LU with partial pivoting of the coarsest operator and its inverse,
+// one workgroup (n_c is small: <= max_coarse_size normally).  Pivot: largest
+// |a| in the column, first index on ties (LAPACK getrf).  flag = 1 if singular.
+__global__ __launch_bounds__(kBlock) void k_dense_lu_inverse(int n, double* __restrict__ a,
+                                                             double* __restrict__ inv, int* __restrict__ piv,
+                                                             int* __restrict__ flag) {
+    __shared__ double sv[kBlock];
+    __shared__ int si[kBlock];
+    auto at = [&](int r, int c) -> double& { return a[(size_t)c * n + r]; };
+    for (int k = 0; k < n; ++k) {
+        double best = -1.0;
+        int bi = n;
+        for (int r = k + threadIdx.x; r < n; r += kBlock) {
+            const double v = fabs(at(r, k));
+            if (v > best) { best = v; bi = r; }
+        }
+        sv[threadIdx.x] = best;
+        si[threadIdx.x] = bi;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                const double ov = sv[threadIdx.x + s];
+                const int oi = si[threadIdx.x + s];
+                if (ov > sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+                    sv[threadIdx.x] = ov;
+                    si[threadIdx.x] = oi;
+                }
+            }
+            __syncthreads();
+        }
+        const int p = si[0];
+        __syncthreads();
+        if (threadIdx.x == 0) piv[k] = p;
+        if (at(p, k) == 0.0) {
+            if (threadIdx.x == 0) *flag = 1;
+            return;
+        }
+        if (p != k)
+            for (int c = threadIdx.x; c < n; c += kBlock) {
+                const double tmp = at(k, c);
+                at(k, c) = at(p, c);
+                at(p, c) = tmp;
+            }
+        __syncthreads();
+        const double dkk = at(k, k);
+        for (int r = k + 1 + threadIdx.x; r < n; r += kBlock) at(r, k) /= dkk;
+        __syncthreads();
+        const int m = n - k - 1;
+        for (int64_t e = threadIdx.x; e < (int64_t)m * m; e += kBlock) {
+            const int r = k + 1 + (int)(e % m), c = k + 1 + (int)(e / m);
+            at(r, c) -= at(r, k) * at(k, c);
+        }
+        __syncthreads();
+    }
+    // inverse columns: solve L U x = P e_c, one column per thread
+    for (int c = threadIdx.x; c < n; c += kBlock) {
+        double* x = inv + (size_t)c * n;
+        for (int r = 0; r < n; ++r) x[r] = (r == c) ? 1.0 : 0.0;
+        for (int k = 0; k < n; ++k) {
+            const int p = piv[k];
+            const double tmp = x[k];
+            x[k] = x[p];
+            x[p] = tmp;
+        }
+        for (int r = 0; r < n; ++r) {
+            double s = x[r];
+            for (int k = 0; k < r; ++k) s -= at(r, k) * x[k];
+            x[r] = s;
+        }
+        for (int r = n - 1; r >= 0; --r) {
+            double s = x[r];
+            for (int k = r + 1; k < n; ++k) s -= at(r, k) * x[k];
+            x[r] = s / at(r, r);
+        }
+    }
+}
+
+// x = Ainv b for the coarsest level (column-major inverse: lanes read
+// consecutive rows of one column, b[j] is a broadcast load).
+template <typename TX>
+__global__ __launch_bounds__(kBlock) void k_coarse_apply(int n, const TX* __restrict__ inv, const TX* __restrict__ b,
+                                                         TX* __restrict__ x) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    TX acc = 0;
+    for (int j = 0; j < n; ++j) acc += inv[(size_t)j * n + i] * b[j];
+    x[i] = acc;
+}
+
+template <typename TV> const TV* vals(const DevCSR& m);
+template <> const double* vals<double>(const DevCSR& m) { return m.v64.p; }
+template <> const float* vals<float>(const DevCSR& m) { return m.v32.p; }
+template <> const bf16_t* vals<bf16_t>(const DevCSR& m) { return m.v16.p; }
+
+// Transfer operators use FP32 values when the cycle stores A in BF16.
+template <typename TV> struct TransferT { typedef TV type; };
+template <> struct TransferT<bf16_t> { typedef float type; };
+
+template <typename TX> TX* r_of(DevLevel& l);
+template <> double* r_of<double>(DevLevel& l) { return l.r64.p; }
+template <> float* r_of<float>(DevLevel& l) { return l.r32.p; }
+template <typename TX> TX* t_of(DevLevel& l);
+template <> double* t_of<double>(DevLevel& l) { return l.t64.p; }
+template <> float* t_of<float>(DevLevel& l) { return l.t32.p; }
+template <typename TX> TX* xc_of(DevLevel& l);
+template <> double* xc_of<double>(DevLevel& l) { return l.xc64.p; }
+template <> float* xc_of<float>(DevLevel& l) { return l.xc32.p; }
+template <typename TX> TX* bc_of(DevLevel& l);
+template <> double* bc_of<double>(DevLevel& l) { return l.bc64.p; }
+template <> float* bc_of<float>(DevLevel& l) { return l.bc32.p; }
+template <typename TX> const TX* d_of(const DevLevel& l);
+template <> const double* d_of<double>(const DevLevel& l) { return l.d64.p; }
+template <> const float* d_of<float>(const DevLevel& l) { return l.d32.p; }
+
+void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
+    d.nrow = h.nrow;
+    d.ncol = h.ncol;
+    d.nnz = h.nnz();
+    d.rp.upload(h.rp, st);
+    d.cv.upload(h.cv, st);
+    d.v64.upload(h.nz, st);
+}
+
+bool is_l1(Smoother s) { return s == Smoother::L1Jacobi || s == Smoother::L1ColoredGS; }
+bool is_gs(Smoother s) { return s == Smoother::ColoredGS || s == Smoother::L1ColoredGS; }
+
+}  // namespace
+
+// ─────────────────────────────────────────────────────────────────────────────
+DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
+    : smoother_(cfg.smoother), omega_(cfg.jacobi_omega) {
+    if (!(smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || is_gs(smoother_)))
+        throw Error("smoother not available in the HIP backend (supported: jacobi, l1_jacobi, colored_gs, "
+                    "l1_colored_gs)");
+    DRAUGR_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    hipStream_t st = stream_;
+    n0_ = h.levels.empty() ? h.A_coarsest.nrow : h.levels[0].A.nrow;
+    L_.resize(h.levels.size());
+    for (size_t l = 0; l < h.levels.size(); ++l) {
+        const HostLevel& H = h.levels[l];
+        DevLevel& D = L_[l];
+        upload_csr(D.A, H.A, st);
+        upload_csr(D.P, H.P, st);
+        D.Pt.nrow = H.P.ncol;
+        D.Pt.ncol = H.P.nrow;
+        D.Pt.nnz = H.P.nnz();
+        D.Pt.rp.upload(H.Pt.offsets, st);
+        D.Pt.cv.upload(H.Pt.fine_rows, st);
+        D.pt_pnz.upload(H.Pt.p_nz_idx, st);
+        std::vector<double> ptv(H.Pt.p_nz_idx.size());
+        for (size_t k = 0; k < ptv.size(); ++k) ptv[k] = H.P.nz[H.Pt.p_nz_idx[k]];
+        D.Pt.v64.upload(ptv, st);
+        DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // ptv is pageable and local
+        const int gs = cfg.group_size;
+        D.A.group = gs > 0 ? gs : pick_group(D.A.nnz, D.A.nrow);
+        D.P.group = gs > 0 ? gs : pick_group(D.P.nnz, D.P.nrow);
+        D.Pt.group = gs > 0 ? gs : pick_group(D.Pt.nnz, D.Pt.nrow);
+        D.has_rmap = H.has_rmap;
+        if (H.has_rmap) {
+            D.ntriples = H.R.ntriples();
+            D.r_off.upload(H.R.nz_offsets, st);
+            D.r_pi.upload(H.R.pi, st);
+            D.r_ai.upload(H.R.ai, st);
+            D.r_pj.upload(H.R.pj, st);
+            const int64_t nnzc = (int64_t)H.R.nz_offsets.size() - 1;
+            D.rap_group = pick_group(D.ntriples, nnzc);
+        }
+        if (is_gs(smoother_)) {
+            D.ncolors = H.coloring.num_colors;
+            D.color_off = H.coloring.color_offsets;
+            D.color_order.upload(H.coloring.color_order, st);
+        }
+        const int32_t n = H.A.nrow, nc = H.P.ncol;
+        D.d64.alloc(n);
+        D.r64.alloc(n);
+        D.t64.alloc(n);
+        D.xc64.alloc(nc);
+        D.bc64.alloc(nc);
+    }
+    upload_csr(Ac_, h.A_coarsest, st);
+    Ac_.group = pick_group(Ac_.nnz, Ac_.nrow);
+    nc_ = h.n_coarse;
+    cdense_.alloc((size_t)nc_ * nc_);
+    cinv64_.alloc((size_t)nc_ * nc_);
+    flag_.alloc(1 + nc_);
+    const int64_t n0 = n0_;
+    w_r_.alloc(n0);
+    w_p0_.alloc(n0);
+    w_p1_.alloc(n0);
+    w_q_.alloc(n0);
+    w_z_.alloc(n0);
+    w_x_.alloc(n0);
+    scal_.alloc(S_COUNT_);
+    partial_.alloc(kRedBlocks);
+    counter_.alloc(1);
+    DRAUGR_HIP_CHECK(hipMemsetAsync(counter_.p, 0, sizeof(uint32_t), st));
+    DRAUGR_HIP_CHECK(hipMemsetAsync(scal_.p, 0, S_COUNT_ * sizeof(double), st));
+    DRAUGR_HIP_CHECK(hipHostMalloc((void**)&pinned_, 16 * sizeof(double), hipHostMallocDefault));
+    for (int l = 0; l < nlevels(); ++l) compute_dinv(l, st);
+    coarse_factor(st);
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+    int fl = 0;
+    DRAUGR_HIP_CHECK(hipMemcpy(&fl, flag_.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (fl) throw Error("coarsest-level matrix is singular (dense LU found a zero pivot)");
+}
+
+DeviceHierarchy::~DeviceHierarchy() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (pinned_) (void)hipHostFree(pinned_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int64_t DeviceHierarchy::bytes_resident() const {
+    int64_t b = 0;
+    auto csr = [&](const DevCSR& m) { b += m.rp.bytes() + m.cv.bytes() + m.v64.bytes() + m.v32.bytes() + m.v16.bytes(); };
+    for (auto& l : L_) {
+        csr(l.A); csr(l.P); csr(l.Pt);
+        b += l.pt_pnz.bytes() + l.r_off.bytes() + l.r_pi.bytes() + l.r_ai.bytes() + l.r_pj.bytes();
+        b += l.d64.bytes() + l.d32.bytes() + l.color_order.bytes();
+        b += l.r64.bytes() + l.t64.bytes() + l.xc64.bytes() + l.bc64.bytes();
+        b += l.r32.bytes() + l.t32.bytes() + l.xc32.bytes() + l.bc32.bytes();
+    }
+    csr(Ac_);
+    b += cdense_.bytes() + cinv64_.bytes() + cinv32_.bytes();
+    b += w_r_.bytes() + w_p0_.bytes() + w_p1_.bytes() + w_q_.bytes() + w_z_.bytes() + w_x_.bytes();
+    b += w_r32_.bytes() + w_z32_.bytes();
+    return b;
+}
+
+void DeviceHierarchy::compute_dinv(int l, hipStream_t st) {
+    DevLevel& D = L_[l];
+    const int n = D.A.nrow;
+    const int G = D.A.group;
+    const int64_t nb = ((int64_t)n * G + kBlock - 1) / kBlock;
+    if (n == 0) return;
+#define DRAUGR_INVDIAG(GG)                                                                                  \
+    case GG:                                                                                                \
+        if (is_l1(smoother_))                                                                               \
+            hipLaunchKernelGGL((k_invdiag<GG, true>), dim3((unsigned)nb), dim3(kBlock), 0, st, n, D.A.rp.p, \
+                               D.A.cv.p, D.A.v64.p, D.d64.p);                                                \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_invdiag<GG, false>), dim3((unsigned)nb), dim3(kBlock), 0, st, n, D.A.rp.p, \
+                               D.A.cv.p, D.A.v64.p, D.d64.p);                                                \
+        break;
+    switch (G) {
+        DRAUGR_INVDIAG(1) DRAUGR_INVDIAG(2) DRAUGR_INVDIAG(4) DRAUGR_INVDIAG(8) DRAUGR_INVDIAG(16)
+        DRAUGR_INVDIAG(32) default : DRAUGR_INVDIAG(64)
+    }
+#undef DRAUGR_INVDIAG
+    if (D.d32.p) {
+        const double* s = D.d64.p;
+        float* d = D.d32.p;
+        launch_map(n, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
+    }
+}
+
+void DeviceHierarchy::galerkin(int l, hipStream_t st) {
+    DevLevel& D = L_[l];
+    if (!D.has_rmap) throw Error("partial resetup needs the restriction maps (setup with allow_partial_resetup=1)");
+    DevCSR& out = (l + 1 < nlevels()) ? L_[l + 1].A : Ac_;
+    const int64_t nnzc = out.nnz;
+    if (nnzc == 0) return;
+    const int G = D.rap_group;
+    const int64_t nb = (nnzc * G + kBlock - 1) / kBlock;
+#define DRAUGR_GAL(GG)                                                                                      \
+    case GG:                                                                                                \
+        hipLaunchKernelGGL((k_galerkin<GG>), dim3((unsigned)nb), dim3(kBlock), 0, st, nnzc, D.r_off.p,        \
+                           D.r_pi.p, D.r_ai.p, D.r_pj.p, D.P.v64.p, D.A.v64.p, out.v64.p);                  \
+        break;
+    switch (G) {
+        DRAUGR_GAL(1) DRAUGR_GAL(2) DRAUGR_GAL(4) DRAUGR_GAL(8) DRAUGR_GAL(16) DRAUGR_GAL(32)
+        default : DRAUGR_GAL(64)
+    }
+#undef DRAUGR_GAL
+}
+
+void DeviceHierarchy::coarse_factor(hipStream_t st) {
+    const int n = nc_;
+    if (n == 0) return;
+    double* dn = cdense_.p;
+    launch_map((int64_t)n * n, [=] __device__(int64_t i) { dn[i] = 0.0; }, st);
+    const int* rp = Ac_.rp.p;
+    const int* cv = Ac_.cv.p;
+    const double* av = Ac_.v64.p;
+    launch_map(n, [=] __device__(int64_t i) {
+        for (int k = rp[i]; k < rp[i + 1]; ++k) dn[(size_t)cv[k] * n + i] = av[k];
+    }, st);
+    int* fl = flag_.p;
+    launch_map(1, [=] __device__(int64_t) { fl[0] = 0; }, st);
+    hipLaunchKernelGGL(k_dense_lu_inverse, dim3(1), dim3(kBlock), 0, st, n, cdense_.p, cinv64_.p, flag_.p + 1,
+                       flag_.p);
+    if (cinv32_.p) {
+        const double* s = cinv64_.p;
+        float* d = cinv32_.p;
+        launch_map((int64_t)n * n, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
+    }
+}
+
+void DeviceHierarchy::refresh_level_mirrors(int l, hipStream_t st) {
+    DevLevel& D = L_[l];
+    const double* s = D.A.v64.p;
+    if (D.A.v32.p) {
+        float* d = D.A.v32.p;
+        launch_map(D.A.nnz, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
+    }
+    if (D.A.v16.p) {
+        bf16_t* d = D.A.v16.p;
+        launch_map(D.A.nnz, [=] __device__(int64_t i) { d[i] = float_to_bf16((float)s[i]); }, st);
+    }
+}
+
+void DeviceHierarchy::ensure_precision(Precision p) {
+    if (p == Precision::FP64) return;
+    hipStream_t st = stream_;
+    if (!have32_) {
+        for (auto& D : L_) {
+            D.A.v32.alloc(D.A.nnz);
+            D.P.v32.alloc(D.P.nnz);
+            D.Pt.v32.alloc(D.Pt.nnz);
+            D.d32.alloc(D.A.nrow);
+            D.r32.alloc(D.A.nrow);
+            D.t32.alloc(D.A.nrow);
+            D.xc32.alloc(D.P.ncol);
+            D.bc32.alloc(D.P.ncol);
+            for (DevCSR* m : {&D.A, &D.P, &D.Pt}) {
+                const double* s = m->v64.p;
+                float* d = m->v32.p;
+                launch_map(m->nnz, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
+            }
+            const double* s = D.d64.p;
+            float* d = D.d32.p;
+            launch_map(D.A.nrow, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
+        }
+        cinv32_.alloc((size_t)nc_ * nc_);
+        const double* s = cinv64_.p;
+        float* d = cinv32_.p;
+        launch_map((int64_t)nc_ * nc_, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
+        w_r32_.alloc(n0_);
+        w_z32_.alloc(n0_);
+        have32_ = true;
+    }
+    if (p == Precision::BF16 && !have16_) {
+        for (auto& D : L_) {
+            D.A.v16.alloc(D.A.nnz);
+            const double* s = D.A.v64.p;
+            bf16_t* d = D.A.v16.p;
+            launch_map(D.A.nnz, [=] __device__(int64_t i) { d[i] = float_to_bf16((float)s[i]); }, st);
+        }
+        have16_ = true;
+    }
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// ── smoothing (src/smoothers.jl:78-124, 203-241, 296-313, 835-854) ──────────
+template <typename TV, typename TX>
+void DeviceHierarchy::smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bool& zero, bool fuse_resid,
+                             bool& resid_done, hipStream_t st) {
+    DevLevel& D = L_[l];
+    const int n = D.A.nrow;
+    const TX* d = d_of<TX>(D);
+    const TX w = (TX)omega_;
+    if (is_gs(smoother_)) {
+        if (zero) {
+            TX* xx = x;
+            launch_map(n, [=] __device__(int64_t i) { xx[i] = 0; }, st);
+            zero = false;
+        }
+        for (int s = 0; s < steps; ++s)
+            for (int c = 0; c < D.ncolors; ++c) {
+                const int off = D.color_off[c], cnt = D.color_off[c + 1] - off;
+                if (cnt == 0) continue;
+                launch_rows<TV, TX>(D.A.group, cnt, D.A.rp.p, D.A.cv.p, vals<TV>(D.A),
+                                    RowList{D.color_order.p + off}, XVec<TX>{x}, EpiGS<TX>{x, b, d}, st);
+            }
+        return;
+    }
+    for (int s = 0; s < steps; ++s) {
+        if (zero) {
+            if (fuse_resid && steps == 1) {
+                launch_rows<TV, TX>(D.A.group, n, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{},
+                                    XZeroJacobi<TX>{d, b, w}, EpiZeroJacobiResid<TX>{b, d, alt, r_of<TX>(D), w}, st);
+                resid_done = true;
+            } else {
+                TX* o = alt;
+                launch_map(n, [=] __device__(int64_t i) { o[i] = (w * d[i]) * b[i]; }, st);
+            }
+            zero = false;
+        } else {
+            launch_rows<TV, TX>(D.A.group, n, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{}, XVec<TX>{x},
+                                EpiJacobi<TX>{x, b, d, alt, w}, st);
+        }
+        std::swap(x, alt);
+    }
+    if (zero) {
+        TX* xx = x;
+        launch_map(n, [=] __device__(int64_t i) { xx[i] = 0; }, st);
+        zero = false;
+    }
+}
+
+template <typename TX>
+void DeviceHierarchy::coarse_solve(TX* x, const TX* b, hipStream_t st) {
+    const int n = nc_;
+    if (n == 0) return;
+    const TX* inv;
+    if constexpr (sizeof(TX) == 8) inv = (const TX*)cinv64_.p;
+    else inv = (const TX*)cinv32_.p;
+    const int nb = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_coarse_apply<TX>, dim3(nb), dim3(kBlock), 0, st, n, inv, b, x);
+}
+
+// One visit of level l (src/cycle.jl:63-99).
+template <typename TV, typename TX>
+void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const CycleOpts& o, hipStream_t st) {
+    typedef typename TransferT<TV>::type TP;
+    DevLevel& D = L_[l];
+    TX* xcur = x;
+    TX* xalt = t_of<TX>(D);
+    TX* r = r_of<TX>(D);
+    TX* xc = xc_of<TX>(D);
+    TX* bc = bc_of<TX>(D);
+    bool resid_done = false;
+    smooth<TV, TX>(l, xcur, xalt, b, o.pre, zero, true, resid_done, st);
+    if (!resid_done)
+        launch_rows<TV, TX>(D.A.group, D.A.nrow, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{}, XVec<TX>{xcur},
+                            EpiResid<TX>{b, r}, st);
+    launch_rows<TP, TX>(D.Pt.group, D.Pt.nrow, D.Pt.rp.p, D.Pt.cv.p, vals<TP>(D.Pt), RowId{}, XVec<TX>{r},
+                        EpiStore<TX>{bc}, st);
+    if (l + 1 < nlevels()) {
+        const int reps = o.w_cycle ? 2 : 1;
+        for (int rep = 0; rep < reps; ++rep) cycle_level<TV, TX>(l + 1, xc, bc, rep == 0, o, st);
+    } else {
+        coarse_solve<TX>(xc, bc, st);
+    }
+    launch_rows<TP, TX>(D.P.group, D.P.nrow, D.P.rp.p, D.P.cv.p, vals<TP>(D.P), RowId{}, XVec<TX>{xc},
+                        EpiAdd<TX>{xcur}, st);
+    bool z = false, dummy = false;
+    smooth<TV, TX>(l, xcur, xalt, b, o.post, z, false, dummy, st);
+    if (xcur != x) {
+        const TX* s = xcur;
+        launch_map(D.A.nrow, [=] __device__(int64_t i) { x[i] = s[i]; }, st);
+    }
+}
+
+template <typename TV, typename TX>
+void DeviceHierarchy::cycle_top(TX* x, const TX* b, bool zero, const CycleOpts& o, hipStream_t st) {
+    if (nlevels() == 0) {
+        coarse_solve<TX>(x, b, st);
+        return;
+    }
+    cycle_level<TV, TX>(0, x, b, zero, o, st);
+}
+
+void DeviceHierarchy::cycle(double* x, const double* b, bool zero, const CycleOpts& o, hipStream_t st) {
+    if (o.prec == Precision::FP64 || nlevels() == 0) {
+        // no AMG levels: the cycle is the direct solve (src/cycle.jl:18-24), kept in FP64
+        cycle_top<double, double>(x, b, zero, o, st);
+        return;
+    }
+    ensure_precision(o.prec);
+    const int n = n0_;
+    float* r32 = w_r32_.p;
+    float* z32 = w_z32_.p;
+    if (zero) {
+        launch_map(n, [=] __device__(int64_t i) { r32[i] = (float)b[i]; }, st);
+    } else {
+        // correction form: x += V(0; b - A x), the same affine map in exact arithmetic
+        DevLevel& D = L_[0];
+        double* r = w_r_.p;
+        launch_rows<double, double>(D.A.group, n, D.A.rp.p, D.A.cv.p, D.A.v64.p, RowId{}, XVec<double>{x},
+                                    EpiResid<double>{b, r}, st);
+        launch_map(n, [=] __device__(int64_t i) { r32[i] = (float)r[i]; }, st);
+    }
+    if (o.prec == Precision::FP32) cycle_top<float, float>(z32, r32, true, o, st);
+    else cycle_top<bf16_t, float>(z32, r32, true, o, st);
+    if (zero) launch_map(n, [=] __device__(int64_t i) { x[i] = (double)z32[i]; }, st);
+    else launch_map(n, [=] __device__(int64_t i) { x[i] += (double)z32[i]; }, st);
+}
+
+double DeviceHierarchy::reduce_to_host(hipStream_t st, int slot) {
+    DRAUGR_HIP_CHECK(hipMemcpyAsync(pinned_ + slot, scal_.p + slot, sizeof(double), hipMemcpyDeviceToHost, st));
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+    return pinned_[slot];
+}
+
+// Stationary AMG iteration with the true-residual stopping test
+// (src/cycle.jl:110-160).  In FP32/BF16 cycle mode each iteration is the
+// correction form x += V(0; b - A x) with the residual in FP64.
+int DeviceHierarchy::solve(double* x, const double* b, double tol, int maxiter, const CycleOpts& o,
+                           hipStream_t st, double* relres, int verbose) {
+    const int n = n0_;
+    double* scal = scal_.p;
+    launch_reduce(n, [=] __device__(int64_t i) { return b[i] * b[i]; },
+                  [=] __device__(double s) { scal[S_BB] = s; }, partial_.p, counter_.p, st);
+    const double bnorm = std::sqrt(reduce_to_host(st, S_BB));
+    if (relres) *relres = 0.0;
+    if (bnorm == 0.0) {
+        launch_map(n, [=] __device__(int64_t i) { x[i] = 0.0; }, st);
+        DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+        return 0;
+    }
+    if (nlevels() == 0) {
+        cycle(x, b, true, o, st);
+        DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+        return 1;
+    }
+    DevLevel& D = L_[0];
+    double* r = w_r_.p;
+    double rel = 1.0;
+    for (int it = 1; it <= maxiter; ++it) {
+        cycle(x, b, false, o, st);
+        launch_rows<double, double>(D.A.group, n, D.A.rp.p, D.A.cv.p, D.A.v64.p, RowId{}, XVec<double>{x},
+                                    EpiResid<double>{b, r}, st);
+        launch_reduce(n, [=] __device__(int64_t i) { return r[i] * r[i]; },
+                      [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, counter_.p, st);
+        rel = std::sqrt(reduce_to_host(st, S_RR)) / bnorm;
+        if (relres) *relres = rel;
+        if (verbose >= 2) std::printf("  AMG iter %4d: relative residual = %.6e\n", it, rel);
+        if (rel < tol) return it;
+        if (!std::isfinite(rel)) throw Error("AMG iteration diverged (non-finite residual)");
+    }
+    return maxiter;
+}
+
+// FP64 preconditioned CG (Hestenes-Stiefel) with z = V(0; r).  Stopping test
+// on the recurrence residual ||r||/||b|| < tol.  Kernel fusion: p = z + beta p
+// is formed inside the q = A p pass (ping-pong p), the pass also reduces p.q
+// and sets alpha on the device; the x/r update reduces r.r.  The cycle for the
+// next z is enqueued before the host reads r.r, so the GPU never idles on the
+// per-iteration convergence check (the last cycle is speculative).
+int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, const CycleOpts& o,
+                         hipStream_t st, double* relres, int verbose) {
+    const int n = n0_;
+    double* scal = scal_.p;
+    double* r = w_r_.p;
+    double* q = w_q_.p;
+    double* z = w_z_.p;
+    double* pa = w_p0_.p;
+    double* pb = w_p1_.p;
+    const bool mixed = o.prec != Precision::FP64;
+    if (mixed) ensure_precision(o.prec);
+    float* r32 = w_r32_.p;
+    float* z32 = w_z32_.p;
+    launch_reduce(n, [=] __device__(int64_t i) { return b[i] * b[i]; },
+                  [=] __device__(double s) { scal[S_BB] = s; }, partial_.p, counter_.p, st);
+    const double bb = reduce_to_host(st, S_BB);
+    if (relres) *relres = 0.0;
+    if (bb == 0.0) {
+        launch_map(n, [=] __device__(int64_t i) { x[i] = 0.0; }, st);
+        DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+        return 0;
+    }
+    // r = b - A x
+    if (nlevels() > 0) {
+        DevLevel& D = L_[0];
+        launch_rows<double, double>(D.A.group, n, D.A.rp.p, D.A.cv.p, D.A.v64.p, RowId{}, XVec<double>{x},
+                                    EpiResid<double>{b, r}, st);
+    } else {
+        launch_rows<double, double>(Ac_.group, n, Ac_.rp.p, Ac_.cv.p, Ac_.v64.p, RowId{}, XVec<double>{x},
+                                    EpiResid<double>{b, r}, st);
+    }
+    launch_reduce(n, [=] __device__(int64_t i) { return r[i] * r[i]; },
+                  [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, counter_.p, st);
+    double rel = std::sqrt(reduce_to_host(st, S_RR) / bb);
+    if (relres) *relres = rel;
+    if (rel < tol) return 0;
+    const DevCSR& A = nlevels() > 0 ? L_[0].A : Ac_;
+    launch_map(n, [=] __device__(int64_t i) { pa[i] = 0.0; }, st);
+    auto precondition = [&]() {
+        if (mixed) {
+            if (o.prec == Precision::FP32) cycle_top<float, float>(z32, r32, true, o, st);
+            else cycle_top<bf16_t, float>(z32, r32, true, o, st);
+        } else {
+            cycle_top<double, double>(z, r, true, o, st);
+        }
+    };
+    if (mixed) launch_map(n, [=] __device__(int64_t i) { r32[i] = (float)r[i]; }, st);
+    precondition();
+    // rz = r.z ; beta = 0 for the first direction
+    if (mixed)
+        launch_reduce(n, [=] __device__(int64_t i) { return r[i] * (double)z32[i]; },
+                      [=] __device__(double s) { scal[S_RZ] = s; scal[S_BETA] = 0.0; }, partial_.p, counter_.p, st);
+    else
+        launch_reduce(n, [=] __device__(int64_t i) { return r[i] * z[i]; },
+                      [=] __device__(double s) { scal[S_RZ] = s; scal[S_BETA] = 0.0; }, partial_.p, counter_.p, st);
+    hipEvent_t ev;
+    DRAUGR_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    int it = 0;
+    int result = maxiter;
+    try {
+        for (it = 1; it <= maxiter; ++it) {
+            // p_new = z + beta p_old; q = A p_new; alpha = rz / (p_new . q)
+            const double* betap = scal + S_BETA;
+            auto fin_pq = [=] __device__(double pq) { scal[S_PQ] = pq; scal[S_ALPHA] = scal[S_RZ] / pq; };
+            if (mixed) {
+                auto xf = [=] __device__(int j) -> double { return (double)z32[j] + *betap * pa[j]; };
+                auto epi = [=] __device__(int i, double s) -> double {
+                    const double pn = (double)z32[i] + *betap * pa[i];
+                    pb[i] = pn;
+                    q[i] = s;
+                    return pn * s;
+                };
+                launch_rows_reduce<double, double>(A.group, n, A.rp.p, A.cv.p, A.v64.p, xf, epi, fin_pq, partial_.p,
+                                                   counter_.p, st);
+            } else {
+                auto xf = [=] __device__(int j) -> double { return z[j] + *betap * pa[j]; };
+                auto epi = [=] __device__(int i, double s) -> double {
+                    const double pn = z[i] + *betap * pa[i];
+                    pb[i] = pn;
+                    q[i] = s;
+                    return pn * s;
+                };
+                launch_rows_reduce<double, double>(A.group, n, A.rp.p, A.cv.p, A.v64.p, xf, epi, fin_pq, partial_.p,
+                                                   counter_.p, st);
+            }
+            // x += alpha p; r -= alpha q; r.r  (and the FP32 copy of r)
+            const double* alphap = scal + S_ALPHA;
+            if (mixed)
+                launch_reduce(n,
+                              [=] __device__(int64_t i) {
+                                  const double a = *alphap;
+                                  x[i] += a * pb[i];
+                                  const double ri = r[i] - a * q[i];
+                                  r[i] = ri;
+                                  r32[i] = (float)ri;
+                                  return ri * ri;
+                              },
+                              [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, counter_.p, st);
+            else
+                launch_reduce(n,
+                              [=] __device__(int64_t i) {
+                                  const double a = *alphap;
+                                  x[i] += a * pb[i];
+                                  const double ri = r[i] - a * q[i];
+                                  r[i] = ri;
+                                  return ri * ri;
+                              },
+                              [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, counter_.p, st);
+            DRAUGR_HIP_CHECK(hipMemcpyAsync(pinned_ + S_RR, scal + S_RR, sizeof(double), hipMemcpyDeviceToHost, st));
+            DRAUGR_HIP_CHECK(hipEventRecord(ev, st));
+            // speculative: next preconditioned residual and beta
+            precondition();
+            auto fin_rz = [=] __device__(double s) {
+                scal[S_BETA] = s / scal[S_RZ];
+                scal[S_RZ] = s;
+            };
+            if (mixed)
+                launch_reduce(n, [=] __device__(int64_t i) { return r[i] * (double)z32[i]; }, fin_rz, partial_.p,
+                              counter_.p, st);
+            else
+                launch_reduce(n, [=] __device__(int64_t i) { return r[i] * z[i]; }, fin_rz, partial_.p, counter_.p,
+                              st);
+            std::swap(pa, pb);
+            DRAUGR_HIP_CHECK(hipEventSynchronize(ev));
+            rel = std::sqrt(pinned_[S_RR] / bb);
+            if (relres) *relres = rel;
+            if (verbose >= 2) std::printf("  PCG iter %4d: relative residual = %.6e\n", it, rel);
+            if (rel < tol) { result = it; break; }
+            if (!std::isfinite(rel)) throw Error("PCG diverged (non-finite residual)");
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        (void)hipEventDestroy(ev);
+        throw;
+    }
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+    DRAUGR_HIP_CHECK(hipEventDestroy(ev));
+    return result;
+}
+
+// Partial resetup (src/resetup.jl:27-45): new fine values -> smoother ->
+// Galerkin level by level -> coarsest dense operator and its factorisation.
+void DeviceHierarchy::resetup(const double* nzval, hipStream_t st) {
+    if (nlevels() == 0) {
+        DRAUGR_HIP_CHECK(hipMemcpyAsync(Ac_.v64.p, nzval, Ac_.nnz * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+        DRAUGR_HIP_CHECK(
+            hipMemcpyAsync(L_[0].A.v64.p, nzval, L_[0].A.nnz * sizeof(double), hipMemcpyDeviceToDevice, st));
+        compute_dinv(0, st);
+        for (int l = 0; l < nlevels(); ++l) {
+            galerkin(l, st);
+            if (l + 1 < nlevels()) compute_dinv(l + 1, st);
+        }
+        for (int l = 0; l < nlevels(); ++l) refresh_level_mirrors(l, st);
+    }
+    coarse_factor(st);
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+    int fl = 0;
+    DRAUGR_HIP_CHECK(hipMemcpy(&fl, flag_.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (fl) throw Error("coarsest-level matrix is singular after resetup (dense LU found a zero pivot)");
+}
+
+// Algorithmic bytes of one cycle from a zero guess, counting every array a
+// kernel must touch once (value size sv, vector size sx, int32 indices).
+double DeviceHierarchy::cycle_bytes(const CycleOpts& o) const {
+    const double sv = o.prec == Precision::FP64 ? 8 : (o.prec == Precision::FP32 ? 4 : 2);
+    const double sp = o.prec == Precision::FP64 ? 8 : 4;
+    const double sx = o.prec == Precision::FP64 ? 8 : 4;
+    double total = 0.0, visits = 1.0;
+    for (int l = 0; l < nlevels(); ++l) {
+        const DevLevel& D = L_[l];
+        const double n = D.A.nrow, nc = D.P.ncol;
+        const double BA = D.A.nnz * (sv + 4) + (n + 1) * 4;
+        double lv = 0.0;
+        if (is_gs(smoother_)) {
+            lv += n * sx;  // zero fill
+            lv += (o.pre + o.post) * (BA + 4 * n * sx);
+            lv += BA + 3 * n * sx;  // residual
+        } else {
+            // pre-smoothing from zero (fused with the residual when pre == 1)
+            if (o.pre == 1) lv += BA + 4 * n * sx;
+            else if (o.pre == 0) lv += n * sx + BA + 3 * n * sx;
+            else lv += 3 * n * sx + (o.pre - 1) * (BA + 4 * n * sx) + BA + 3 * n * sx;
+            lv += o.post * (BA + 4 * n * sx);
+            if ((o.pre + o.post) % 2) lv += 2 * n * sx;
+        }
+        lv += D.Pt.nnz * (sp + 4) + (nc + 1) * 4 + n * sx + nc * sx;      // restrict
+        lv += D.P.nnz * (sp + 4) + (n + 1) * 4 + nc * sx + 2 * n * sx;    // prolong
+        total += visits * lv;
+        if (o.w_cycle && l > 0) visits *= 2;
+        else if (o.w_cycle && l == 0) visits *= 2;
+    }
+    total += visits * ((double)nc_ * nc_ * sx + 2.0 * nc_ * sx);
+    return total;
+}
+
+// ── per-kernel hooks for parity tests and measurement ───────────────────────
+void DeviceHierarchy::level_op(int lvl, int op, int prec, const void* in1, const void* in2, void* out, void* out2,
+                               hipStream_t st) {
+    if (op == 7) {
+        if (prec == 0) coarse_solve<double>((double*)out, (const double*)in1, st);
+        else { ensure_precision(Precision::FP32); coarse_solve<float>((float*)out, (const float*)in1, st); }
+        return;
+    }
+    if (lvl < 0 || lvl >= nlevels()) throw Error("level index out of range");
+    DevLevel& D = L_[lvl];
+    if (op == 8) { galerkin(lvl, st); return; }
+    if (op == 9) {
+        compute_dinv(lvl, st);
+        DRAUGR_HIP_CHECK(hipMemcpyAsync(out, D.d64.p, D.A.nrow * sizeof(double), hipMemcpyDeviceToDevice, st));
+        return;
+    }
+    auto run = [&](auto tv_tag, auto tx_tag) {
+        typedef decltype(tv_tag) TV;
+        typedef decltype(tx_tag) TX;
+        typedef typename TransferT<TV>::type TP;
+        const int n = D.A.nrow;
+        const TX* d = d_of<TX>(D);
+        const TX w = (TX)omega_;
+        switch (op) {
+            case 0: launch_rows<TV, TX>(D.A.group, n, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{},
+                                        XVec<TX>{(const TX*)in1}, EpiStore<TX>{(TX*)out}, st); break;
+            case 1: launch_rows<TV, TX>(D.A.group, n, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{},
+                                        XVec<TX>{(const TX*)in1}, EpiResid<TX>{(const TX*)in2, (TX*)out}, st); break;
+            case 2: launch_rows<TV, TX>(D.A.group, n, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{},
+                                        XVec<TX>{(const TX*)in1},
+                                        EpiJacobi<TX>{(const TX*)in1, (const TX*)in2, d, (TX*)out, w}, st); break;
+            case 3: launch_rows<TV, TX>(D.A.group, n, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{},
+                                        XZeroJacobi<TX>{d, (const TX*)in2, w},
+                                        EpiZeroJacobiResid<TX>{(const TX*)in2, d, (TX*)out, (TX*)out2, w}, st); break;
+            case 4: launch_rows<TP, TX>(D.Pt.group, D.Pt.nrow, D.Pt.rp.p, D.Pt.cv.p, vals<TP>(D.Pt), RowId{},
+                                        XVec<TX>{(const TX*)in1}, EpiStore<TX>{(TX*)out}, st); break;
+            case 5: launch_rows<TP, TX>(D.P.group, D.P.nrow, D.P.rp.p, D.P.cv.p, vals<TP>(D.P), RowId{},
+                                        XVec<TX>{(const TX*)in1}, EpiAdd<TX>{(TX*)out}, st); break;
+            case 6:
+                for (int c = 0; c < D.ncolors; ++c) {
+                    const int off = D.color_off[c], cnt = D.color_off[c + 1] - off;
+                    if (cnt == 0) continue;
+                    launch_rows<TV, TX>(D.A.group, cnt, D.A.rp.p, D.A.cv.p, vals<TV>(D.A),
+                                        RowList{D.color_order.p + off}, XVec<TX>{(TX*)out},
+                                        EpiGS<TX>{(TX*)out, (const TX*)in2, d}, st);
+                }
+                break;
+            default: throw Error("unknown level op");
+        }
+    };
+    if (prec == 0) run(double(), double());
+    else if (prec == 1) { ensure_precision(Precision::FP32); run(float(), float()); }
+    else { ensure_precision(Precision::BF16); run(bf16_t(), float()); }
+}
+
+void DeviceHierarchy::export_level_values(int lvl, int which, double* dst) {
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(stream_));
+    if (which == 2) {
+        DRAUGR_HIP_CHECK(hipMemcpy(dst, cinv64_.p, cinv64_.bytes(), hipMemcpyDeviceToHost));
+        return;
+    }
+    if (which == 3) {
+        DRAUGR_HIP_CHECK(hipMemcpy(dst, Ac_.v64.p, Ac_.v64.bytes(), hipMemcpyDeviceToHost));
+        return;
+    }
+    if (lvl < 0 || lvl >= nlevels()) throw Error("level index out of range");
+    const DevLevel& D = L_[lvl];
+    if (which == 0) DRAUGR_HIP_CHECK(hipMemcpy(dst, D.A.v64.p, D.A.v64.bytes(), hipMemcpyDeviceToHost));
+    else if (which == 1) DRAUGR_HIP_CHECK(hipMemcpy(dst, D.d64.p, D.d64.bytes(), hipMemcpyDeviceToHost));
+    else throw Error("unknown export selector");
+}
+
+}  // namespace draugr

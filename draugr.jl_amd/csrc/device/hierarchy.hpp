@@ -1,0 +1,124 @@
+// Device-resident AMG hierarchy: the level table uploaded once to HBM, and the
+// stream-ordered cycle / solve / PCG / resetup drivers that run on it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "../host/types.hpp"
+#include "dbuf.hpp"
+#include "kernels.hpp"
+
+namespace draugr {
+
+struct DevCSR {
+    int32_t nrow = 0, ncol = 0;
+    int64_t nnz = 0;
+    DBuf<int32_t> rp, cv;
+    DBuf<double> v64;
+    DBuf<float> v32;
+    DBuf<dev::bf16_t> v16;
+    int group = 1;  // lanes per row
+};
+
+struct DevLevel {
+    DevCSR A;                 // level operator
+    DevCSR P;                 // prolongation n_l x n_{l+1}
+    DevCSR Pt;                // P^T as CSR (values gathered from P)
+    DBuf<int32_t> pt_pnz;     // Pt entry -> P entry
+    bool has_rmap = false;
+    int64_t ntriples = 0;
+    int rap_group = 1;
+    DBuf<int64_t> r_off;      // Galerkin triple map of A_{l+1}
+    DBuf<int32_t> r_pi, r_ai, r_pj;
+    DBuf<double> d64;         // smoother inverse diagonal (plain or l1)
+    DBuf<float> d32;
+    int ncolors = 0;
+    std::vector<int32_t> color_off;  // host copy (loop control)
+    DBuf<int32_t> color_order;
+    // workspaces: residual, ping-pong smoother buffer, coarse x and b
+    DBuf<double> r64, t64, xc64, bc64;
+    DBuf<float> r32, t32, xc32, bc32;
+};
+
+// Device scalars used by the fused reductions of PCG / stationary solve.
+enum Scalar : int {
+    S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_COUNT_ = 16
+};
+
+struct CycleOpts {
+    bool w_cycle = false;
+    int pre = 1, post = 1;
+    Precision prec = Precision::FP64;
+};
+
+class DeviceHierarchy {
+public:
+    DeviceHierarchy(const HostHierarchy& h, const Config& cfg);
+    ~DeviceHierarchy();
+
+    int32_t n_fine() const { return n0_; }
+    int nlevels() const { return (int)L_.size(); }
+    hipStream_t stream() const { return stream_; }
+
+    // One V/W cycle on device vectors x (in/out) and b; zero_guess asserts x == 0
+    // on entry (its content is ignored then).
+    void cycle(double* x, const double* b, bool zero_guess, const CycleOpts& o, hipStream_t st);
+    // Stationary AMG iteration (src/cycle.jl:110-160); returns the iteration count.
+    int solve(double* x, const double* b, double tol, int maxiter, const CycleOpts& o, hipStream_t st,
+              double* relres, int verbose);
+    // FP64 PCG preconditioned by one cycle; returns the iteration count.
+    int pcg(double* x, const double* b, double tol, int maxiter, const CycleOpts& o, hipStream_t st,
+            double* relres, int verbose);
+    // Partial numeric resetup from new fine-level values (device pointer, FP64).
+    void resetup(const double* nzval_dev, hipStream_t st);
+
+    // ── test / measurement hooks ─────────────────────────────────────────────
+    // op: 0 y=A x, 1 r=b-Ax, 2 jacobi step xo=x+w d(b-Ax), 3 zero-guess jacobi
+    // + residual (x1, r), 4 restrict bc=P^T r, 5 prolong x+=P xc, 6 GS sweep
+    // (in place, all colours), 7 coarse solve, 8 galerkin (A_{l+1} values),
+    // 9 inverse diagonal (returned in out)
+    void level_op(int lvl, int op, int prec, const void* in1, const void* in2, void* out, void* out2,
+                  hipStream_t st);
+    void export_level_values(int lvl, int which, double* host_dst);  // 0 A vals, 1 dinv, 2 coarse inverse
+    int64_t bytes_resident() const;
+    double cycle_bytes(const CycleOpts& o) const;  // algorithmic bytes of one V/W cycle (zero guess)
+    void ensure_precision(Precision p);
+
+private:
+    template <typename TV, typename TX>
+    void cycle_level(int l, TX* x, const TX* b, bool zero_guess, const CycleOpts& o, hipStream_t st);
+    template <typename TV, typename TX>
+    void smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bool& zero, bool fuse_resid, bool& resid_done,
+                hipStream_t st);
+    template <typename TX> void coarse_solve(TX* x, const TX* b, hipStream_t st);
+    template <typename TV, typename TX> void cycle_top(TX* x, const TX* b, bool zero_guess, const CycleOpts& o,
+                                                       hipStream_t st);
+    void refresh_level_mirrors(int l, hipStream_t st);
+    void compute_dinv(int l, hipStream_t st);
+    void galerkin(int l, hipStream_t st);
+    void coarse_factor(hipStream_t st);
+    double reduce_to_host(hipStream_t st, int slot);
+
+    std::vector<DevLevel> L_;
+    int32_t n0_ = 0;
+    int32_t nc_ = 0;
+    DevCSR Ac_;                        // coarsest operator (sparse; values by Galerkin)
+    DBuf<double> cdense_, cinv64_;
+    DBuf<float> cinv32_;
+    DBuf<int32_t> flag_;
+    Smoother smoother_;
+    double omega_;
+    bool have32_ = false, have16_ = false;
+    // solver workspaces at the finest size
+    DBuf<double> w_r_, w_p0_, w_p1_, w_q_, w_z_, w_x_;
+    DBuf<float> w_r32_, w_z32_;
+    DBuf<double> scal_, partial_;
+    DBuf<uint32_t> counter_;
+    double* pinned_ = nullptr;
+    hipStream_t stream_ = nullptr;
+};
+
+}  // namespace draugr

@@ -1,0 +1,585 @@
+// Host-side coarsening (setup phase, runs once; its output is uploaded to HBM).
+//
+// Behaviour follows the reference algorithms; each function cites the file:line
+// of the Draugr.jl routine it restates.  All arrays are 0-based here; the
+// C/F state uses the reference's codes: 1 = C, -1 = F, 0 = undecided,
+// -2 = tentative fine (Z_PT, HMIS first pass only).
+#include "types.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <numeric>
+
+namespace draugr {
+
+namespace {
+
+constexpr double kEps = std::numeric_limits<double>::epsilon();
+
+inline bool strong_offdiag(const std::vector<uint8_t>& s, int64_t nz, int32_t j, int32_t i) {
+    return j != i && s[nz];
+}
+
+int32_t uf_find(std::vector<int32_t>& parent, int32_t i) {
+    while (parent[i] != i) {
+        parent[i] = parent[parent[i]];
+        i = parent[i];
+    }
+    return i;
+}
+
+void uf_union(std::vector<int32_t>& parent, int32_t a, int32_t b) {
+    int32_t ra = uf_find(parent, a), rb = uf_find(parent, b);
+    if (ra != rb) parent[rb] = ra;
+}
+
+// Number of rows that strongly depend on each node (column count of S).
+// src/coarsening.jl:179-192
+std::vector<int32_t> strong_transpose_count(const CSR& A, const std::vector<uint8_t>& s) {
+    std::vector<int32_t> cnt(A.nrow, 0);
+    for (int32_t i = 0; i < A.nrow; ++i)
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (strong_offdiag(s, k, A.cv[k], i)) cnt[A.cv[k]]++;
+    return cnt;
+}
+
+// Transpose adjacency of S: for node j, the rows i (ascending) with i -> j strong.
+// src/coarsening.jl:827-860
+void strong_transpose_adj(const CSR& A, const std::vector<uint8_t>& s,
+                          std::vector<int64_t>& off, std::vector<int32_t>& src) {
+    const int32_t n = A.nrow;
+    off.assign(n + 1, 0);
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (strong_offdiag(s, k, A.cv[k], i)) off[A.cv[k] + 1]++;
+    for (int32_t j = 0; j < n; ++j) off[j + 1] += off[j];
+    src.resize(off[n]);
+    std::vector<int64_t> pos(off.begin(), off.end() - 1);
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (strong_offdiag(s, k, A.cv[k], i)) src[pos[A.cv[k]]++] = i;
+}
+
+bool has_strong_offdiag(const CSR& A, const std::vector<uint8_t>& s, int32_t i) {
+    for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+        if (strong_offdiag(s, k, A.cv[k], i)) return true;
+    return false;
+}
+
+// Doubly linked buckets keyed by the measure lambda, head insertion.
+// Same selection semantics as the reference's bucket lists
+// (src/coarsening.jl:450-507, 706-776).
+struct Buckets {
+    std::vector<int32_t> head, next, prev;  // node ids, -1 = none
+    std::vector<int32_t>& lam;
+    Buckets(int32_t n, int32_t nb, std::vector<int32_t>& l)
+        : head(std::max(nb, 1), -1), next(n, -1), prev(n, -1), lam(l) {}
+    void push(int32_t i) {
+        int32_t k = lam[i];
+        if (k >= (int32_t)head.size()) head.resize(k + 1, -1);
+        int32_t h = head[k];
+        head[k] = i;
+        next[i] = h;
+        prev[i] = -1;
+        if (h >= 0) prev[h] = i;
+    }
+    void remove(int32_t i) {
+        int32_t k = lam[i], p = prev[i], nx = next[i];
+        if (p >= 0) next[p] = nx; else head[k] = nx;
+        if (nx >= 0) prev[nx] = p;
+        next[i] = prev[i] = -1;
+    }
+    void update(int32_t i, int32_t v) {
+        remove(i);
+        lam[i] = v;
+        push(i);
+    }
+};
+
+// Ruge-Stueben greedy first pass.  With zpt = true (HMIS), nodes found to have
+// zero measure become Z_PT (-2) instead of F; src/coarsening.jl:406-567.
+// With classical = true the reference's coarsen_rs variant is run instead
+// (no zero-measure preprocessing, plain decrement): src/coarsening.jl:671-805.
+std::vector<int8_t> rs_first_pass(const CSR& A, const std::vector<uint8_t>& s, bool zpt, bool classical) {
+    const int32_t n = A.nrow;
+    const int8_t fpnt = zpt ? -2 : -1;
+    std::vector<int32_t> lam = strong_transpose_count(A, s);
+    std::vector<int64_t> st_off;
+    std::vector<int32_t> st_src;
+    strong_transpose_adj(A, s, st_off, st_src);
+    std::vector<int8_t> cf(n, 0);
+    for (int32_t i = 0; i < n; ++i)
+        if (!has_strong_offdiag(A, s, i) && lam[i] == 0) cf[i] = 1;
+    if (!classical) {
+        for (int32_t i = 0; i < n; ++i) {
+            if (cf[i] != 0 || lam[i] != 0) continue;
+            cf[i] = fpnt;
+            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+                int32_t j = A.cv[k];
+                if (strong_offdiag(s, k, j, i) && cf[j] == 0) lam[j]++;
+            }
+        }
+    }
+    int32_t maxlam = 0;
+    for (int32_t i = 0; i < n; ++i)
+        if (cf[i] == 0) maxlam = std::max(maxlam, lam[i]);
+    Buckets B(n, classical ? std::max(maxlam + 1, n) : maxlam + 1, lam);
+    for (int32_t i = 0; i < n; ++i)
+        if (cf[i] == 0) B.push(i);
+    int32_t top = maxlam;
+    for (;;) {
+        while (top >= 0 && B.head[top] < 0) --top;
+        if (top < 0) break;
+        const int32_t best = B.head[top];
+        B.remove(best);
+        cf[best] = 1;
+        for (int64_t t = st_off[best]; t < st_off[best + 1]; ++t) {
+            const int32_t j = st_src[t];
+            if (cf[j] != 0) continue;
+            B.remove(j);
+            cf[j] = -1;
+            for (int32_t k2 = A.rp[j]; k2 < A.rp[j + 1]; ++k2) {
+                const int32_t k = A.cv[k2];
+                if (strong_offdiag(s, k2, k, j) && cf[k] == 0) {
+                    const int32_t v = lam[k] + 1;
+                    B.update(k, v);
+                    if (v > top) top = v;
+                }
+            }
+        }
+        for (int32_t kk = A.rp[best]; kk < A.rp[best + 1]; ++kk) {
+            const int32_t j = A.cv[kk];
+            if (!(strong_offdiag(s, kk, j, best) && cf[j] == 0)) continue;
+            const int32_t v = std::max(0, lam[j] - 1);
+            B.update(j, v);
+            if (!classical && v == 0) {
+                B.remove(j);
+                cf[j] = fpnt;
+                for (int32_t k2 = A.rp[j]; k2 < A.rp[j + 1]; ++k2) {
+                    const int32_t k = A.cv[k2];
+                    if (strong_offdiag(s, k2, k, j) && cf[k] == 0) {
+                        const int32_t v2 = lam[k] + 1;
+                        B.update(k, v2);
+                        if (v2 > top) top = v2;
+                    }
+                }
+            }
+        }
+    }
+    return cf;
+}
+
+// PMIS sweeps on the undecided nodes (cf == 0); src/coarsening.jl:575-624.
+void pmis_on_undecided(std::vector<int8_t>& cf, const CSR& A, const std::vector<uint8_t>& s,
+                       const std::vector<double>& measure) {
+    const int32_t n = A.nrow;
+    for (int64_t iter = 0; iter <= (int64_t)n; ++iter) {
+        bool all_decided = true;
+        for (int32_t i = 0; i < n; ++i) {
+            if (cf[i] != 0) continue;
+            all_decided = false;
+            if (measure[i] < 1.0) { cf[i] = -1; continue; }
+            bool is_max = true;
+            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+                const int32_t j = A.cv[k];
+                if (strong_offdiag(s, k, j, i) && cf[j] != -1 && measure[j] > measure[i]) {
+                    is_max = false;
+                    break;
+                }
+            }
+            if (is_max) cf[i] = 1;
+        }
+        for (int32_t i = 0; i < n; ++i) {
+            if (cf[i] != 0) continue;
+            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+                if (s[k] && cf[A.cv[k]] == 1) { cf[i] = -1; break; }
+        }
+        if (all_decided) break;
+    }
+    for (auto& c : cf)
+        if (c == 0) c = -1;
+}
+
+// Promote F points without a strong C neighbour; src/coarsening.jl:291-313.
+void ensure_fine_have_coarse_neighbor(std::vector<int8_t>& cf, const CSR& A, const std::vector<uint8_t>& s) {
+    bool changed = true;
+    while (changed) {
+        changed = false;
+        for (int32_t i = 0; i < A.nrow; ++i) {
+            if (cf[i] != -1) continue;
+            bool has_c = false;
+            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+                if (strong_offdiag(s, k, A.cv[k], i) && cf[A.cv[k]] == 1) { has_c = true; break; }
+            if (!has_c) { cf[i] = 1; changed = true; }
+        }
+    }
+}
+
+int32_t count_coarse(const std::vector<int8_t>& cf) {
+    int32_t nc = 0;
+    for (auto c : cf) nc += (c == 1);
+    return nc;
+}
+
+}  // namespace
+
+// Absolute strength of connection; src/strength.jl:24-61.
+std::vector<uint8_t> strength_absolute(const CSR& A, double theta) {
+    std::vector<uint8_t> s(A.nnz(), 0);
+    for (int32_t i = 0; i < A.nrow; ++i) {
+        double mx = 0.0;
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (A.cv[k] != i) mx = std::max(mx, std::fabs(A.nz[k]));
+        const double thr = theta * mx;
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            s[k] = (A.cv[k] != i) && (std::fabs(A.nz[k]) >= thr);
+    }
+    return s;
+}
+
+// Max-row-sum dependency weakening, used for strength only; src/strength.jl:183-213.
+CSR apply_max_row_sum(const CSR& A, double threshold) {
+    CSR W = A;
+    for (int32_t i = 0; i < A.nrow; ++i) {
+        double diag = 0.0, rsum = 0.0;
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            rsum += A.nz[k];
+            if (A.cv[k] == i) diag = A.nz[k];
+        }
+        const double ad = std::fabs(diag);
+        if (ad < kEps) continue;
+        if (std::fabs(rsum) > ad * threshold && threshold < 1.0)
+            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+                if (A.cv[k] != i) W.nz[k] = 0.0;
+    }
+    return W;
+}
+
+// Greedy aggregation with distance-2 seeds, neighbour assignment and singleton
+// merging; src/coarsening.jl:14-168.  agg[i] is 0-based.
+void coarsen_aggregation(const CSR& A, double theta, std::vector<int32_t>& agg, int32_t& nc) {
+    const int32_t n = A.nrow;
+    if (n <= 1) {
+        agg.assign(n, 0);
+        nc = n;
+        return;
+    }
+    const std::vector<uint8_t> s = strength_absolute(A, theta);
+    agg.assign(n, -1);
+    nc = 0;
+    std::vector<int32_t> scount(n, 0);
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (strong_offdiag(s, k, A.cv[k], i)) scount[i]++;
+    // phase 1: seeds in decreasing strong-count order (stable)
+    std::vector<int8_t> state(n, 0);
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return scount[a] > scount[b]; });
+    for (int32_t idx = 0; idx < n; ++idx) {
+        const int32_t i = order[idx];
+        if (state[i] != 0) continue;
+        bool can_seed = true;
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const int32_t j = A.cv[k];
+            if (strong_offdiag(s, k, j, i) && (state[j] == 1 || state[j] == -1)) { can_seed = false; break; }
+        }
+        if (!can_seed) continue;
+        state[i] = 1;
+        agg[i] = nc++;
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const int32_t j = A.cv[k];
+            if (!(strong_offdiag(s, k, j, i) && state[j] == 0)) continue;
+            state[j] = -1;
+            agg[j] = agg[i];
+            for (int32_t k2 = A.rp[j]; k2 < A.rp[j + 1]; ++k2) {
+                const int32_t m = A.cv[k2];
+                if (strong_offdiag(s, k2, m, j) && state[m] == 0) state[m] = -2;
+            }
+        }
+    }
+    // phase 2: attach the rest to the strongest aggregated neighbour
+    for (int32_t i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        int32_t best = -1;
+        double bv = 0.0;
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const int32_t j = A.cv[k];
+            if (strong_offdiag(s, k, j, i) && agg[j] >= 0 && std::fabs(A.nz[k]) > bv) {
+                bv = std::fabs(A.nz[k]);
+                best = agg[j];
+            }
+        }
+        if (best < 0)
+            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+                const int32_t j = A.cv[k];
+                if (j != i && agg[j] >= 0 && std::fabs(A.nz[k]) > bv) {
+                    bv = std::fabs(A.nz[k]);
+                    best = agg[j];
+                }
+            }
+        agg[i] = best >= 0 ? best : nc++;
+    }
+    // phase 3: merge aggregates of size <= 1 into the strongest neighbour
+    std::vector<int32_t> size(nc, 0);
+    for (int32_t i = 0; i < n; ++i) size[agg[i]]++;
+    std::vector<int32_t> parent(nc);
+    std::iota(parent.begin(), parent.end(), 0);
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t mine = uf_find(parent, agg[i]);
+        if (size[mine] > 1) continue;
+        int32_t target = -1;
+        double bv = 0.0;
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const int32_t j = A.cv[k];
+            if (j == i) continue;
+            const int32_t rj = uf_find(parent, agg[j]);
+            if (rj != mine && std::fabs(A.nz[k]) > bv) {
+                bv = std::fabs(A.nz[k]);
+                target = rj;
+            }
+        }
+        if (target >= 0) {
+            const int32_t sm = size[mine], st = size[target];
+            uf_union(parent, target, mine);
+            size[uf_find(parent, target)] = sm + st;
+        }
+    }
+    std::vector<int32_t> new_id(nc, -1);
+    int32_t cnt = 0;
+    for (int32_t a = 0; a < nc; ++a) {
+        const int32_t r = uf_find(parent, a);
+        if (new_id[r] < 0) new_id[r] = cnt++;
+        new_id[a] = new_id[r];
+    }
+    for (int32_t i = 0; i < n; ++i) agg[i] = new_id[uf_find(parent, agg[i])];
+    nc = cnt;
+}
+
+// PMIS with the column-count measure plus a uniform random; src/coarsening.jl:202-283.
+void coarsen_pmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf, int32_t& nc) {
+    const int32_t n = A.nrow;
+    if (n <= 1) {
+        cf.assign(n, 1);
+        nc = n;
+        return;
+    }
+    const std::vector<uint8_t> s = strength_absolute(A, theta);
+    const std::vector<int32_t> st = strong_transpose_count(A, s);
+    std::vector<double> measure(n);
+    for (int32_t i = 0; i < n; ++i) measure[i] = (double)st[i] + rng.uniform();
+    cf.assign(n, 0);
+    for (int32_t i = 0; i < n; ++i)
+        if (!has_strong_offdiag(A, s, i) && st[i] == 0) cf[i] = 1;
+    for (int64_t iter = 0; iter <= (int64_t)n; ++iter) {
+        bool all_decided = true;
+        for (int32_t i = 0; i < n; ++i) {
+            if (cf[i] != 0) continue;
+            all_decided = false;
+            bool is_max = true;
+            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+                const int32_t j = A.cv[k];
+                if (strong_offdiag(s, k, j, i) && cf[j] != -1 && measure[j] > measure[i]) {
+                    is_max = false;
+                    break;
+                }
+            }
+            if (is_max) cf[i] = 1;
+        }
+        for (int32_t i = 0; i < n; ++i) {
+            if (cf[i] != 0) continue;
+            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+                if (s[k] && cf[A.cv[k]] == 1) { cf[i] = -1; break; }
+        }
+        if (all_decided) break;
+    }
+    for (auto& c : cf)
+        if (c == 0) c = 1;
+    nc = count_coarse(cf);
+}
+
+// HMIS = RS first pass (Z_PT variant) + PMIS on the rest; src/coarsening.jl:330-394.
+void coarsen_hmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf, int32_t& nc) {
+    const int32_t n = A.nrow;
+    if (n <= 1) {
+        cf.assign(n, 1);
+        nc = n;
+        return;
+    }
+    const std::vector<uint8_t> s = strength_absolute(A, theta);
+    cf = rs_first_pass(A, s, /*zpt=*/true, /*classical=*/false);
+    const std::vector<int32_t> st = strong_transpose_count(A, s);
+    std::vector<double> measure(n);
+    for (int32_t i = 0; i < n; ++i) measure[i] = (double)st[i] + rng.uniform();
+    for (int32_t i = 0; i < n; ++i) {
+        if (cf[i] != -2) continue;
+        cf[i] = (measure[i] >= 1.0 || has_strong_offdiag(A, s, i)) ? 0 : -1;
+    }
+    pmis_on_undecided(cf, A, s, measure);
+    nc = count_coarse(cf);
+}
+
+// Classical Ruge-Stueben (first + second pass); src/coarsening.jl:671-818.
+void coarsen_rs(const CSR& A, double theta, std::vector<int8_t>& cf, int32_t& nc) {
+    const int32_t n = A.nrow;
+    if (n <= 1) {
+        cf.assign(n, 1);
+        nc = n;
+        return;
+    }
+    const std::vector<uint8_t> s = strength_absolute(A, theta);
+    cf = rs_first_pass(A, s, /*zpt=*/false, /*classical=*/true);
+    ensure_fine_have_coarse_neighbor(cf, A, s);
+    nc = count_coarse(cf);
+}
+
+// Two-pass aggressive C/F splitting: base splitting, then PMIS on the
+// distance-2 C graph; src/coarsening.jl:990-1116.
+void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& rng,
+                           std::vector<int8_t>& cf, int32_t& nc) {
+    const int32_t n = A_in.nrow;
+    if (n <= 1) {
+        cf.assign(n, 1);
+        nc = n;
+        return;
+    }
+    std::vector<int8_t> cf1;
+    int32_t nc1 = 0;
+    if (hmis_base) coarsen_hmis(A_in, theta, rng, cf1, nc1);
+    else coarsen_pmis(A_in, theta, rng, cf1, nc1);
+    const CSR& A = A_in;
+    const std::vector<uint8_t> s = strength_absolute(A, theta);
+
+    std::vector<int32_t> c_idx, c_loc(n, -1);
+    for (int32_t i = 0; i < n; ++i)
+        if (cf1[i] == 1) { c_loc[i] = (int32_t)c_idx.size(); c_idx.push_back(i); }
+    const int32_t m = (int32_t)c_idx.size();
+
+    // Strong C neighbours of every row (CSR), and its transpose restricted to C.
+    std::vector<int64_t> sc_off(n + 1, 0);
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (strong_offdiag(s, k, A.cv[k], i) && cf1[A.cv[k]] == 1) sc_off[i + 1]++;
+    for (int32_t i = 0; i < n; ++i) sc_off[i + 1] += sc_off[i];
+    std::vector<int32_t> sc(sc_off[n]);
+    {
+        int64_t p = 0;
+        for (int32_t i = 0; i < n; ++i)
+            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+                if (strong_offdiag(s, k, A.cv[k], i) && cf1[A.cv[k]] == 1) sc[p++] = c_loc[A.cv[k]];
+    }
+    std::vector<int64_t> tr_off(m + 1, 0);
+    for (int64_t t = 0; t < sc_off[n]; ++t) tr_off[sc[t] + 1]++;
+    for (int32_t a = 0; a < m; ++a) tr_off[a + 1] += tr_off[a];
+    std::vector<int32_t> tr(tr_off[m]);
+    {
+        std::vector<int64_t> pos(tr_off.begin(), tr_off.end() - 1);
+        for (int32_t i = 0; i < n; ++i)
+            for (int64_t t = sc_off[i]; t < sc_off[i + 1]; ++t) tr[pos[sc[t]]++] = i;
+    }
+    // Neighbour set of C point a (local ids): pairs of strong C neighbours of a
+    // common row, plus direct strong C-C connections in both directions.
+    std::vector<int64_t> g_off(m + 1, 0);
+    std::vector<int32_t> g;
+    g.reserve((size_t)m * 16);
+    std::vector<int32_t> mark(m, -1);
+    for (int32_t a = 0; a < m; ++a) {
+        auto add = [&](int32_t b) {
+            if (b != a && mark[b] != a) { mark[b] = a; g.push_back(b); }
+        };
+        const int32_t ia = c_idx[a];
+        for (int64_t t = sc_off[ia]; t < sc_off[ia + 1]; ++t) add(sc[t]);
+        for (int64_t t = tr_off[a]; t < tr_off[a + 1]; ++t) {
+            const int32_t i = tr[t];
+            for (int64_t u = sc_off[i]; u < sc_off[i + 1]; ++u) add(sc[u]);
+            if (cf1[i] == 1) add(c_loc[i]);
+        }
+        g_off[a + 1] = (int64_t)g.size();
+    }
+    std::vector<double> measure(m);
+    for (int32_t a = 0; a < m; ++a) measure[a] = (double)(g_off[a + 1] - g_off[a]) + rng.uniform();
+    std::vector<int8_t> cf2(m, 0);
+    for (int64_t iter = 0; iter <= (int64_t)m; ++iter) {
+        bool all_decided = true;
+        for (int32_t a = 0; a < m; ++a) {
+            if (cf2[a] != 0) continue;
+            all_decided = false;
+            bool is_max = true;
+            for (int64_t t = g_off[a]; t < g_off[a + 1]; ++t) {
+                const int32_t b = g[t];
+                if (cf2[b] != -1 && measure[b] > measure[a]) { is_max = false; break; }
+            }
+            if (is_max) cf2[a] = 1;
+        }
+        for (int32_t a = 0; a < m; ++a) {
+            if (cf2[a] != 0) continue;
+            for (int64_t t = g_off[a]; t < g_off[a + 1]; ++t)
+                if (cf2[g[t]] == 1) { cf2[a] = -1; break; }
+        }
+        if (all_decided) break;
+    }
+    for (auto& c : cf2)
+        if (c == 0) c = 1;
+    cf = cf1;
+    for (int32_t a = 0; a < m; ++a)
+        if (cf2[a] == -1) cf[c_idx[a]] = -1;
+    ensure_fine_have_coarse_neighbor(cf, A, s);
+    nc = count_coarse(cf);
+}
+
+// Aggregation-style aggressive coarsening built on PMIS (AggressiveCoarsening
+// with a base other than :pmis/:hmis); src/coarsening.jl:869-957.
+void coarsen_aggressive_agg(const CSR& A, double theta, Rng& rng, std::vector<int32_t>& agg, int32_t& nc) {
+    const int32_t n = A.nrow;
+    std::vector<int8_t> cf;
+    int32_t nc1 = 0;
+    coarsen_pmis(A, theta, rng, cf, nc1);
+    const std::vector<uint8_t> s = strength_absolute(A, theta);
+    agg.assign(n, -1);
+    int32_t cnt = 0;
+    for (int32_t i = 0; i < n; ++i)
+        if (cf[i] == 1) agg[i] = cnt++;
+    for (int32_t i = 0; i < n; ++i) {
+        if (cf[i] == 1) continue;
+        int32_t best = -1;
+        double bv = 0.0;
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const int32_t j = A.cv[k];
+            if (j != i && cf[j] == 1 && std::fabs(A.nz[k]) > bv) { bv = std::fabs(A.nz[k]); best = agg[j]; }
+        }
+        if (best >= 0) agg[i] = best;
+    }
+    std::vector<int32_t> parent(cnt);
+    std::iota(parent.begin(), parent.end(), 0);
+    std::vector<int32_t> roots;
+    for (int32_t i = 0; i < n; ++i) {
+        if (cf[i] != -1) continue;
+        roots.clear();
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (s[k] && cf[A.cv[k]] == 1) roots.push_back(uf_find(parent, agg[A.cv[k]]));
+        for (size_t r = 1; r < roots.size(); ++r) uf_union(parent, roots[0], roots[r]);
+    }
+    std::vector<int32_t> new_id(cnt, -1);
+    nc = 0;
+    for (int32_t a = 0; a < cnt; ++a) {
+        const int32_t r = uf_find(parent, a);
+        if (new_id[r] < 0) new_id[r] = nc++;
+        new_id[a] = new_id[r];
+    }
+    for (int32_t i = 0; i < n; ++i)
+        if (agg[i] >= 0) agg[i] = new_id[uf_find(parent, agg[i])];
+    for (int32_t i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        int32_t best = -1;
+        double bv = 0.0;
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const int32_t j = A.cv[k];
+            if (j != i && agg[j] >= 0 && std::fabs(A.nz[k]) > bv) { bv = std::fabs(A.nz[k]); best = agg[j]; }
+        }
+        agg[i] = best >= 0 ? best : nc++;
+    }
+}
+
+}  // namespace draugr

@@ -1,0 +1,150 @@
+// Host-side hierarchy construction (setup phase, run once; the result is
+// uploaded to HBM and stays resident).
+#include "types.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <sstream>
+
+namespace draugr {
+
+namespace {
+
+bool uses_cf(Coarsen c) {
+    return c == Coarsen::PMIS || c == Coarsen::HMIS || c == Coarsen::RS ||
+           c == Coarsen::AggressivePMIS || c == Coarsen::AggressiveHMIS;
+}
+
+// Coarsen once and build P; src/setup.jl:256-327.
+CSR coarsen_and_build_p(const CSR& A, const CoarsenSpec& alg, const Config& cfg, Rng& rng, int32_t& nc) {
+    const bool weaken = cfg.max_row_sum < 1.0;
+    CSR Aw;
+    if (weaken) Aw = apply_max_row_sum(A, cfg.max_row_sum);
+    const CSR& As = weaken ? Aw : A;  // matrix the splitting / aggregation sees
+    if (uses_cf(alg.type)) {
+        std::vector<int8_t> cf;
+        switch (alg.type) {
+            case Coarsen::PMIS: coarsen_pmis(As, alg.theta, rng, cf, nc); break;
+            case Coarsen::HMIS: coarsen_hmis(As, alg.theta, rng, cf, nc); break;
+            case Coarsen::RS: coarsen_rs(As, alg.theta, cf, nc); break;
+            case Coarsen::AggressivePMIS: coarsen_aggressive_cf(As, alg.theta, false, rng, cf, nc); break;
+            case Coarsen::AggressiveHMIS: coarsen_aggressive_cf(As, alg.theta, true, rng, cf, nc); break;
+            default: break;
+        }
+        CSR P = interpolation_cf(A, cf, nc, alg.interp, alg.theta);
+        if (alg.interp.trunc_factor > 0) P = filter_prolongation(P, alg.interp.trunc_factor);
+        return P;
+    }
+    std::vector<int32_t> agg;
+    if (alg.type == Coarsen::AggressiveAggregation) {
+        coarsen_aggressive_agg(As, alg.theta, rng, agg, nc);
+        return prolongation_from_aggregates(agg, nc);
+    }
+    coarsen_aggregation(As, alg.theta, agg, nc);
+    CSR P = prolongation_from_aggregates(agg, nc);
+    if (alg.type == Coarsen::SmoothedAggregation) P = smooth_prolongation(A, P, alg.sa_omega);
+    if (alg.filtering) P = filter_prolongation(P, alg.filter_tol);
+    return P;
+}
+
+// Retry with theta halved (up to 3 times) when n_coarse >= 0.8 n;
+// src/setup.jl:219-237.
+CSR coarsen_with_fallback(const CSR& A, const CoarsenSpec& alg, const Config& cfg, Rng& rng, int32_t& nc) {
+    const int32_t n = A.nrow;
+    CSR P = coarsen_and_build_p(A, alg, cfg, rng, nc);
+    if (nc < 0.8 * n || n <= cfg.max_coarse_size) return P;
+    for (int attempt = 1; attempt <= 3; ++attempt) {
+        CoarsenSpec red = alg;
+        red.theta = alg.theta * std::pow(0.5, attempt);
+        int32_t nc2 = 0;
+        CSR P2 = coarsen_and_build_p(A, red, cfg, rng, nc2);
+        if (nc2 < nc) {
+            P = std::move(P2);
+            nc = nc2;
+        }
+        if (nc < 0.8 * n) break;
+    }
+    return P;
+}
+
+bool needs_coloring(Smoother s) { return s == Smoother::ColoredGS || s == Smoother::L1ColoredGS; }
+
+}  // namespace
+
+// Level loop of src/setup.jl:123-166 and the dense coarse operator of
+// src/setup.jl:177-209.
+HostHierarchy amg_setup_host(const CSR& A_in, const Config& cfg, bool allow_partial_resetup) {
+    auto t0 = std::chrono::steady_clock::now();
+    HostHierarchy h;
+    Rng rng(cfg.seed);
+    CSR A = A_in;
+    for (int lvl = 1; lvl <= cfg.max_levels - 1; ++lvl) {
+        const int32_t n = A.nrow;
+        if (n <= cfg.max_coarse_size) break;
+        const CoarsenSpec& alg = lvl <= cfg.initial_coarsening_levels ? cfg.initial_coarsening : cfg.coarsening;
+        int32_t nc = 0;
+        CSR P = coarsen_with_fallback(A, alg, cfg, rng, nc);
+        if (nc >= n || nc == 0) break;
+        HostLevel L;
+        CSR Ac;
+        coarse_sparsity(A, P, nc, Ac, allow_partial_resetup ? &L.R : nullptr);
+        L.has_rmap = allow_partial_resetup;
+        build_transpose_map(P, L.Pt);
+        if (needs_coloring(cfg.smoother)) greedy_coloring(A, L.coloring);
+        L.A = std::move(A);
+        L.P = std::move(P);
+        h.levels.push_back(std::move(L));
+        A = std::move(Ac);
+    }
+    h.n_coarse = A.nrow;
+    h.coarse_dense.assign((size_t)A.nrow * A.nrow, 0.0);
+    for (int32_t i = 0; i < A.nrow; ++i)
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            h.coarse_dense[(size_t)A.cv[k] * A.nrow + i] = A.nz[k];
+    h.A_coarsest = std::move(A);
+    h.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return h;
+}
+
+// Hierarchy table in the shape of src/setup.jl:405-473.
+std::string hierarchy_summary(const HostHierarchy& h, const Config& cfg, int64_t n_finest) {
+    std::ostringstream o;
+    char buf[256];
+    const size_t L = h.levels.size();
+    o << "+=============================================================================+\n";
+    o << "                         AMG Hierarchy Summary (HIP, gfx950)\n";
+    o << "+=============================================================================+\n";
+    std::snprintf(buf, sizeof buf, "  Setup time:      %.4f s\n", h.setup_seconds);
+    o << buf;
+    o << "  Levels:          " << L + 1 << " (" << L << " AMG + 1 coarse direct)\n";
+    o << "  Cycle type:      " << (cfg.w_cycle ? "W" : "V") << "-cycle\n";
+    o << "  Level     Rows        NNZ      Ratio\n";
+    int64_t tot_nnz = 0, tot_rows = 0;
+    for (size_t i = 0; i < L; ++i) {
+        const CSR& A = h.levels[i].A;
+        tot_nnz += A.nnz();
+        tot_rows += A.nrow;
+        if (i == 0)
+            std::snprintf(buf, sizeof buf, "  %5zu   %9d  %10lld        -\n", i + 1, A.nrow, (long long)A.nnz());
+        else
+            std::snprintf(buf, sizeof buf, "  %5zu   %9d  %10lld    %5.3f\n", i + 1, A.nrow, (long long)A.nnz(),
+                          (double)A.nrow / h.levels[i - 1].A.nrow);
+        o << buf;
+    }
+    int64_t cnnz = 0;
+    for (double v : h.coarse_dense) cnnz += (v != 0.0);
+    tot_nnz += cnnz;
+    tot_rows += h.n_coarse;
+    std::snprintf(buf, sizeof buf, "  %5zu   %9d  %10lld    %5.3f   Direct (LU)\n", L + 1, h.n_coarse,
+                  (long long)cnnz, L ? (double)h.n_coarse / h.levels[L - 1].A.nrow : 1.0);
+    o << buf;
+    if (L) {
+        std::snprintf(buf, sizeof buf, "  Operator complexity: %.3f\n  Grid complexity:     %.3f\n",
+                      (double)tot_nnz / h.levels[0].A.nnz(), (double)tot_rows / n_finest);
+        o << buf;
+    }
+    return o.str();
+}
+
+}  // namespace draugr

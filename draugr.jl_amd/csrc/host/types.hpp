@@ -1,0 +1,164 @@
+// Host-side data layout of the AMG hierarchy and its configuration.
+//
+// Everything here is 0-based, int32 indices, FP64 values.  The layout mirrors
+// the reference's level contract (src/types.jl:368-467): per level the operator
+// A (CSR, columns sorted within rows), the prolongation P (CSR, n_fine x n_coarse),
+// the transpose map of P (coarse row -> fine rows, P nz index), the Galerkin
+// triple map grouped by destination coarse nonzero, and the smoother colouring.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace draugr {
+
+struct Error : std::runtime_error {
+    explicit Error(const std::string& m) : std::runtime_error(m) {}
+};
+
+// Compressed sparse row matrix, 0-based.
+struct CSR {
+    int32_t nrow = 0, ncol = 0;
+    std::vector<int32_t> rp;   // nrow + 1
+    std::vector<int32_t> cv;   // nnz
+    std::vector<double> nz;    // nnz
+    int64_t nnz() const { return (int64_t)cv.size(); }
+};
+
+// ── configuration (field set and defaults follow src/types.jl:497-536) ──────
+enum class Interp : int { Direct = 0, Standard = 1, ExtendedI = 2 };
+enum class Coarsen : int {
+    Aggregation = 0, PMIS = 1, HMIS = 2, RS = 3,
+    AggressivePMIS = 4, AggressiveHMIS = 5, SmoothedAggregation = 6,
+    AggressiveAggregation = 7  // AggressiveCoarsening with a base other than :pmis/:hmis
+};
+enum class Smoother : int {
+    Jacobi = 0, ColoredGS = 1, SerialGS = 2, SPAI0 = 3, SPAI1 = 4, L1Jacobi = 5,
+    L1ColoredGS = 6, L1SerialGS = 7, Chebyshev = 8, ILU0 = 9, SerialILU0 = 10
+};
+enum class Precision : int { FP64 = 0, FP32 = 1, BF16 = 2 };
+
+struct InterpSpec {
+    Interp type = Interp::ExtendedI;
+    double trunc_factor = 0.0;
+    int max_elements = 0;
+    int norm_p = 1;
+    bool rescale = false;
+};
+
+struct CoarsenSpec {
+    Coarsen type = Coarsen::HMIS;
+    double theta = 0.5;
+    InterpSpec interp;
+    bool filtering = false;     // aggregation / smoothed aggregation
+    double filter_tol = 0.1;
+    double sa_omega = 2.0 / 3.0;  // smoothed aggregation damping
+};
+
+struct Config {
+    CoarsenSpec coarsening;
+    Smoother smoother = Smoother::L1ColoredGS;
+    int max_levels = 20;
+    int max_coarse_size = 50;
+    int pre_smoothing_steps = 1;
+    int post_smoothing_steps = 1;
+    double jacobi_omega = 2.0 / 3.0;
+    int verbose = 0;
+    CoarsenSpec initial_coarsening;
+    int initial_coarsening_levels = 0;
+    double max_row_sum = 1.0;
+    bool w_cycle = false;
+    bool signed_strength = false;  // stored only: coarsening always uses absolute strength
+    bool coarse_solve_on_cpu = false;
+    // ── extensions of this backend (no counterpart in the reference config) ──
+    Precision precision = Precision::FP64;  // arithmetic of the V/W cycle
+    uint64_t seed = 0x5EEDull;              // seed of the setup RNG (PMIS/HMIS measures)
+    int group_size = 0;                     // lanes per CSR row, 0 = autotune
+};
+
+// ── hierarchy (host copy; uploaded once to HBM) ──────────────────────────────
+struct TransposeMap {            // src/types.jl:394-398
+    std::vector<int32_t> offsets;    // n_coarse + 1
+    std::vector<int32_t> fine_rows;  // nnz(P)
+    std::vector<int32_t> p_nz_idx;   // nnz(P)
+};
+
+struct RestrictionMap {          // src/types.jl:413-418
+    std::vector<int64_t> nz_offsets; // nnz(A_coarse) + 1
+    std::vector<int32_t> pi, ai, pj; // ntriples each
+    int64_t ntriples() const { return (int64_t)pi.size(); }
+};
+
+struct Coloring {                // src/types.jl:206-228
+    int num_colors = 0;
+    std::vector<int32_t> colors;        // 1-based colour per row
+    std::vector<int32_t> color_offsets; // num_colors + 1 (0-based offsets into order)
+    std::vector<int32_t> color_order;   // rows sorted by colour
+};
+
+struct HostLevel {               // src/types.jl:431-440
+    CSR A;
+    CSR P;
+    TransposeMap Pt;
+    bool has_rmap = false;
+    RestrictionMap R;
+    Coloring coloring;           // only for the coloured Gauss-Seidel smoothers
+};
+
+struct HostHierarchy {           // src/types.jl:457-467
+    std::vector<HostLevel> levels;
+    CSR A_coarsest;                   // sparse coarsest operator (n_c x n_c)
+    int32_t n_coarse = 0;
+    std::vector<double> coarse_dense; // column-major n_c x n_c
+    double setup_seconds = 0.0;
+};
+
+// Deterministic generator for the setup's random measures: SplitMix64 with
+// 53-bit uniforms in [0, 1).
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+// ── setup entry points (host) ────────────────────────────────────────────────
+std::vector<uint8_t> strength_absolute(const CSR& A, double theta);
+CSR apply_max_row_sum(const CSR& A, double threshold);
+
+void coarsen_aggregation(const CSR& A, double theta, std::vector<int32_t>& agg, int32_t& nc);
+void coarsen_aggressive_agg(const CSR& A, double theta, Rng& rng, std::vector<int32_t>& agg, int32_t& nc);
+void coarsen_pmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf, int32_t& nc);
+void coarsen_hmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf, int32_t& nc);
+void coarsen_rs(const CSR& A, double theta, std::vector<int8_t>& cf, int32_t& nc);
+void coarsen_aggressive_cf(const CSR& A, double theta, bool hmis_base, Rng& rng,
+                           std::vector<int8_t>& cf, int32_t& nc);
+
+CSR prolongation_from_aggregates(const std::vector<int32_t>& agg, int32_t nc);
+CSR smooth_prolongation(const CSR& A, const CSR& Ptent, double omega);
+CSR filter_prolongation(const CSR& P, double tol);
+CSR interpolation_cf(const CSR& A, const std::vector<int8_t>& cf, int32_t nc,
+                     const InterpSpec& ip, double theta);
+
+void coarse_sparsity(const CSR& A, const CSR& P, int32_t nc, CSR& Ac, RestrictionMap* rmap);
+void build_transpose_map(const CSR& P, TransposeMap& t);
+void greedy_coloring(const CSR& A, Coloring& c);
+
+HostHierarchy amg_setup_host(const CSR& A, const Config& cfg, bool allow_partial_resetup);
+std::string hierarchy_summary(const HostHierarchy& h, const Config& cfg, int64_t n_finest);
+
+// Dense LU with partial pivoting (column-major), and its explicit inverse.
+// Returns false when the matrix is singular to working precision.
+bool dense_lu_inverse(int n, const double* A_colmajor, double* inv_colmajor);
+
+// JSON configuration (keys of c_api/draugr_amg.h:58-89 plus this backend's).
+Config config_from_json(const char* json);
+
+}  // namespace draugr

@@ -1,0 +1,142 @@
+/*
+ * draugr_hip.h — C-ABI of the MI355X (gfx950) AMG solve / resetup backend.
+ *
+ * Part 1 mirrors the reference C API one to one: same symbol names,
+ * argument meaning, return codes and error convention as
+ * c_api/draugr_amg.h (implemented there by src/cfunction_interface.jl).
+ * A host that links libdraugr.so can link libdraugr_hip.so instead; no Julia
+ * runtime is involved, so init_julia/shutdown_julia have no counterpart.
+ *
+ * Part 2 adds entry points the reference does not have: device-pointer
+ * (HBM-resident) vectors with an explicit HIP stream, the FP64 PCG driver,
+ * resetup from device values, and inspection / per-kernel hooks used by the
+ * parity tests.  Plain pointers and sizes only; a stream is passed as void*
+ * (a hipStream_t, or NULL for the hierarchy's own stream).
+ *
+ * Errors: every call returns -1 on failure and stores a message retrievable
+ * with draugr_amg_last_error().  The message is per calling thread and valid
+ * until that thread's next draugr_* call.
+ *
+ * Threading: the handle tables are protected by a lock; calls on one
+ * hierarchy handle are serialised by a per-hierarchy lock.
+ */
+#ifndef DRAUGR_HIP_H
+#define DRAUGR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ── Part 1: reference C API ─────────────────────────────────────────────── */
+
+/* Last error message ("" if none).  Replaces c_api/draugr_amg.h:46. */
+const char *draugr_amg_last_error(void);
+
+/* Config from JSON; all keys optional, unknown keys ignored, defaults of
+ * AMGConfig() (src/types.jl:514-536).  Keys of c_api/draugr_amg.h:58-89 plus
+ * "precision" ("fp64" | "fp32" | "bf16": arithmetic of the V/W cycle),
+ * "seed" (setup RNG seed), "group_size" (lanes per CSR row, 0 = auto),
+ * "filtering", "filter_tol".  Returns a handle > 0, or -1.
+ * Replaces c_api/draugr_amg.h:94. */
+int32_t draugr_amg_config_from_json(const char *json_str);
+
+/* Build the hierarchy on the host (setup phase) and upload it once to HBM.
+ * Inputs are copied.  index_base 0 or 1.  Returns a handle > 0, or -1.
+ * Replaces c_api/draugr_amg.h:115-120. */
+int32_t draugr_amg_setup(int32_t n, int32_t nnz, const int32_t *rowptr, const int32_t *colval,
+                         const double *nzval, int32_t config_handle, int32_t index_base,
+                         int32_t allow_partial_resetup);
+
+/* partial=1: new values with the same pattern -> Galerkin products, smoothers
+ * and coarse factorisation recomputed on the GPU.  partial=0: full rebuild.
+ * Returns 0 or -1.  Replaces c_api/draugr_amg.h:143-149. */
+int32_t draugr_amg_resetup(int32_t handle, int32_t n, int32_t nnz, const int32_t *rowptr,
+                           const int32_t *colval, const double *nzval, int32_t config_handle,
+                           int32_t index_base, int32_t partial, int32_t allow_partial_resetup);
+
+/* Stationary AMG iteration on host vectors (x in/out).  Returns the number
+ * of iterations (maxiter when not converged), or -1.
+ * Replaces c_api/draugr_amg.h:167-170. */
+int32_t draugr_amg_solve(int32_t handle, int32_t n, double *x, const double *b, int32_t config_handle,
+                         double tol, int32_t maxiter);
+
+/* One V/W cycle on host vectors (x in/out).  Returns 0 or -1.
+ * Replaces c_api/draugr_amg.h:178-180. */
+int32_t draugr_amg_cycle(int32_t handle, int32_t n, double *x, const double *b, int32_t config_handle);
+
+/* Replace c_api/draugr_amg.h:188 and :194.  Return 0, or -1 for an unknown handle. */
+int32_t draugr_amg_free(int32_t handle);
+int32_t draugr_amg_config_free(int32_t handle);
+
+/* ── Part 2: backend extensions (no counterpart in the reference) ────────── */
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int32_t draugr_hip_device_count(void);
+
+/* Setup on the host only (no GPU needed): same arguments as draugr_amg_setup.
+ * The handle supports the inspection calls below; draugr_hip_upload() makes
+ * it usable for solves. */
+int32_t draugr_hip_setup_host(int32_t n, int32_t nnz, const int32_t *rowptr, const int32_t *colval,
+                              const double *nzval, int32_t config_handle, int32_t index_base,
+                              int32_t allow_partial_resetup);
+int32_t draugr_hip_upload(int32_t handle);
+
+/* out[0..]: n_levels, n_coarse, n_fine, setup microseconds, uploaded (0/1),
+ * total operator nnz, total rows.  Returns the number of values written. */
+int32_t draugr_hip_hierarchy_info(int32_t handle, int64_t *out, int32_t nout);
+
+/* out[0..]: n, nnz(A), n_coarse, nnz(P), ntriples, num_colors, has_rmap. */
+int32_t draugr_hip_level_info(int32_t handle, int32_t level, int64_t *out, int32_t nout);
+
+/* Copy a host array of level `level` (0-based) into dst, 0-based indices:
+ *  0 A.rowptr  1 A.colval  2 A.nzval(f64)  3 P.rowptr  4 P.colval  5 P.nzval(f64)
+ *  6 Pt.offsets 7 Pt.fine_rows 8 Pt.p_nz_idx 9 R.nz_offsets(i64) 10 R.pi 11 R.ai
+ * 12 R.pj 13 color_offsets 14 color_order
+ * level == -1: 0 coarsest rowptr 1 colval 2 nzval 15 dense coarse (col-major f64). */
+int32_t draugr_hip_level_export(int32_t handle, int32_t level, int32_t which, void *dst);
+
+/* Hierarchy table as text (src/setup.jl:405-473 layout); returns length. */
+int32_t draugr_hip_summary(int32_t handle, char *buf, int32_t buflen);
+
+/* Device-pointer variants: x, b are FP64 arrays in HBM of length n. */
+int32_t draugr_hip_cycle_device(int32_t handle, int32_t n, double *x, const double *b, int32_t config_handle,
+                                int32_t zero_guess, void *stream);
+int32_t draugr_hip_solve_device(int32_t handle, int32_t n, double *x, const double *b, int32_t config_handle,
+                                double tol, int32_t maxiter, void *stream, double *relres);
+/* FP64 PCG preconditioned by one cycle (cycle precision from the config). */
+int32_t draugr_hip_pcg_device(int32_t handle, int32_t n, double *x, const double *b, int32_t config_handle,
+                              double tol, int32_t maxiter, void *stream, double *relres);
+int32_t draugr_hip_pcg(int32_t handle, int32_t n, double *x, const double *b, int32_t config_handle,
+                       double tol, int32_t maxiter, double *relres);
+
+/* Partial resetup from new fine-level values already in HBM (same pattern). */
+int32_t draugr_hip_resetup_device(int32_t handle, int64_t nnz, const double *nzval, void *stream);
+
+/* One hot-path kernel on level `level`, device pointers (prec 0 f64, 1 f32,
+ * 2 bf16 values with f32 vectors):
+ *  op 0 out = A in1           op 1 out = in2 - A in1
+ *  op 2 out = in1 + w d (in2 - A in1)          (Jacobi / l1-Jacobi step)
+ *  op 3 out = w d in2, out2 = in2 - A out      (zero-guess step + residual)
+ *  op 4 out = P^T in1 (restriction)  op 5 out += P in1 (prolongation)
+ *  op 6 out: one coloured Gauss-Seidel sweep in place, rhs in2
+ *  op 7 out = A_coarse^-1 in1        op 8 Galerkin values of level+1
+ *  op 9 out = inverse diagonal of the smoother (f64) */
+int32_t draugr_hip_level_op(int32_t handle, int32_t level, int32_t op, int32_t prec, const void *in1,
+                            const void *in2, void *out, void *out2, void *stream);
+
+/* Device-side FP64 values to the host: which 0 A.nzval, 1 smoother inverse
+ * diagonal, 2 coarse inverse (col-major), 3 coarsest operator values. */
+int32_t draugr_hip_export_device_values(int32_t handle, int32_t level, int32_t which, double *dst);
+
+/* Algorithmic bytes of one cycle from a zero guess under the given config,
+ * and the HBM bytes the hierarchy holds. */
+int32_t draugr_hip_cycle_bytes(int32_t handle, int32_t config_handle, double *bytes);
+int32_t draugr_hip_resident_bytes(int32_t handle, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DRAUGR_HIP_H */

@@ -1,0 +1,286 @@
+"""GPU parity: every hot-path kernel, the V/W cycle, the stationary solve, the
+FP64 / mixed-precision PCG and the partial resetup against the FP64 CPU
+oracle, on the same hierarchy and the same seeded inputs.
+
+Tolerances (relative 2-norm of the difference):
+  FP64 single kernels 1e-13 (summation order differs: row groups vs. the
+  reference's sequential row sums), FP64 cycle 1e-12, FP64 solutions 1e-10,
+  FP32 kernels 1e-5, BF16-value kernels 1e-2 (8-bit mantissa storage).
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_of, rel_err
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261202
+
+
+def _case(dh, name):
+    P = dh.problems
+    if name == "agg2d":
+        A = P.poisson2d(48)
+        cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType())
+        sm = "jacobi"
+    elif name == "hypre3d":
+        A = P.heterogeneous3d(20, P.log_permeability(20, 1.5, SEED + 1))
+        cfg = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())
+        sm = "l1_jacobi"
+    elif name == "gs2d":
+        A = P.poisson2d(40)
+        cfg = dh.AMGConfig(coarsening=dh.PMISCoarsening(0.25), smoother=dh.ColoredGaussSeidelType())
+        sm = "colored_gs"
+    elif name == "l1gs3d":
+        A = P.heterogeneous3d(16, P.log_permeability(16, 1.0, SEED + 3))
+        cfg = dh.AMGConfig(coarsening=dh.HMISCoarsening(0.5, dh.ExtendedIInterpolation(0.3)),
+                           smoother=dh.L1ColoredGaussSeidelType())
+        sm = "l1_colored_gs"
+    else:
+        raise KeyError(name)
+    return A, cfg, sm
+
+
+CASES = ["agg2d", "hypre3d", "gs2d", "l1gs3d"]
+_cache = {}
+
+
+def _built(dh, orc, name):
+    if name not in _cache:
+        A, cfg, sm = _case(dh, name)
+        h = dh.amg_setup(A, cfg)
+        oh = oracle_of(h, orc, sm)
+        _cache[name] = (A, cfg, sm, h, oh)
+    return _cache[name]
+
+
+TOL = {"fp64": 1e-13, "fp32": 1e-5, "bf16": 1e-2}
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("name", CASES)
+def test_level_kernels(dh, orc, torch_cuda, name, prec):
+    torch = torch_cuda
+    A, cfg, sm, h, oh = _built(dh, orc, name)
+    tdt = torch.float64 if prec == "fp64" else torch.float32
+    tol = TOL[prec]
+    w = cfg.jacobi_omega
+    rng = np.random.default_rng(SEED)
+    for l in range(h.nlevels):
+        lv = oh.levels[l]
+        n, nc = len(lv["A_rp"]) - 1, len(lv["Pt_off"]) - 1
+        x, b, xc = rng.standard_normal(n), rng.standard_normal(n), rng.standard_normal(nc)
+        dev = lambda a: torch.tensor(a, dtype=tdt, device="cuda")  # noqa: E731
+        out, out2 = torch.zeros(n, dtype=tdt, device="cuda"), torch.zeros(n, dtype=tdt, device="cuda")
+        d = oh.dinv(l)
+        rp, cv, nz = lv["A_rp"], lv["A_cv"], lv["A_nz"]
+        print("case", name, prec, "level", l, "n", n, flush=True)
+        h.level_op(l, 0, prec, dev(x), None, out)
+        assert rel_err(out.cpu().numpy(), orc.spmv(rp, cv, nz, x)) < tol
+        h.level_op(l, 1, prec, dev(x), dev(b), out)
+        assert rel_err(out.cpu().numpy(), orc.residual(rp, cv, nz, x, b)) < tol
+        if sm in ("jacobi", "l1_jacobi"):
+            h.level_op(l, 2, prec, dev(x), dev(b), out)
+            assert rel_err(out.cpu().numpy(), orc.jacobi_step(rp, cv, nz, x, b, d, w)) < tol
+            h.level_op(l, 3, prec, None, dev(b), out, out2)
+            x1 = orc.jacobi_step(rp, cv, nz, np.zeros(n), b, d, w)
+            assert rel_err(out.cpu().numpy(), x1) < tol
+            assert rel_err(out2.cpu().numpy(), orc.residual(rp, cv, nz, x1, b)) < tol
+        else:
+            xg = dev(x)
+            h.level_op(l, 6, prec, None, dev(b), xg)
+            ref = orc.gs_sweep(rp, cv, nz, x, b, d, lv["c_off"], lv["c_order"])
+            assert rel_err(xg.cpu().numpy(), ref) < tol
+        bc = torch.zeros(nc, dtype=tdt, device="cuda")
+        h.level_op(l, 4, prec, dev(x), None, bc)
+        assert rel_err(bc.cpu().numpy(), orc.restrict(lv["Pt_off"], lv["Pt_rows"], lv["Pt_pnz"], lv["P_nz"], x)) < tol
+        xo = dev(x)
+        h.level_op(l, 5, prec, dev(xc), None, xo)
+        assert rel_err(xo.cpu().numpy(), orc.prolong(lv["P_rp"], lv["P_cv"], lv["P_nz"], xc, x)) < tol
+    if prec == "fp64":
+        for l in range(h.nlevels):
+            n = oh.levels[l]["A_rp"].shape[0] - 1
+            dd = torch.zeros(n, dtype=torch.float64, device="cuda")
+            h.level_op(l, 9, 0, None, None, dd)
+            assert rel_err(dd.cpu().numpy(), oh.dinv(l)) < 1e-15
+    # coarse direct solve
+    c = h.coarsest()
+    bc = rng.standard_normal(c["n"])
+    out = torch.zeros(c["n"], dtype=tdt, device="cuda")
+    h.level_op(0, 7, prec, torch.tensor(bc, dtype=tdt, device="cuda"), None, out)
+    ref = np.linalg.solve(c["dense"], bc)
+    assert rel_err(out.cpu().numpy(), ref) < (1e-11 if prec == "fp64" else 1e-3)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_galerkin_matches_oracle(dh, orc, torch_cuda, name):
+    A, cfg, sm, h, oh = _built(dh, orc, name)
+    for l in range(h.nlevels):
+        lv = oh.levels[l]
+        ref = orc.galerkin(lv["R_off"], lv["R_pi"], lv["R_ai"], lv["R_pj"], lv["P_nz"], lv["A_nz"])
+        nxt = h.level_info(l + 1)["nnz"] if l + 1 < h.nlevels else h.coarsest()["nnz"]
+        h.level_op(l, 8, 0)
+        got = h.device_values(l + 1, 0, nxt) if l + 1 < h.nlevels else h.device_values(0, 3, nxt)
+        assert rel_err(got, ref) < 1e-13
+        # and the setup-time values are the same sums in the same order
+        setup_vals = oh.level_values(l + 1)
+        assert rel_err(setup_vals, ref) < 1e-14
+
+
+@pytest.mark.parametrize("cycle", ["v", "w"])
+@pytest.mark.parametrize("name", CASES)
+def test_cycle_fp64(dh, orc, name, cycle):
+    A, cfg, sm, h, oh = _built(dh, orc, name)
+    oh.set_options(cycle=cycle, pre=1, post=1)
+    c2 = dh.AMGConfig(**{**cfg.__dict__, "cycle_type": cycle.upper()})
+    rng = np.random.default_rng(SEED + 7)
+    x0, b = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    x = x0.copy()
+    dh.amg_cycle_(x, b, h, c2)
+    ref = oh.cycle(x0, b)
+    assert rel_err(x, ref) < 1e-12
+    oh.set_options(cycle="v")
+
+
+@pytest.mark.parametrize("steps", [(2, 1), (0, 2), (3, 3)])
+def test_cycle_pre_post_steps(dh, orc, steps):
+    A, cfg, sm, h, oh = _built(dh, orc, "hypre3d")
+    pre, post = steps
+    oh.set_options(pre=pre, post=post)
+    c2 = dh.AMGConfig(**{**cfg.__dict__, "pre_smoothing_steps": pre, "post_smoothing_steps": post})
+    rng = np.random.default_rng(SEED + 11)
+    x0, b = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    x = x0.copy()
+    dh.amg_cycle_(x, b, h, c2)
+    assert rel_err(x, oh.cycle(x0, b)) < 1e-12
+    # preconditioner form (zero guess) on device vectors
+    import torch
+    xd = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+    dh.amg_cycle_(xd, torch.tensor(b, device="cuda"), h, c2, zero_guess=True)
+    assert rel_err(xd.cpu().numpy(), oh.cycle(np.zeros(A.n), b)) < 1e-12
+    oh.set_options(pre=1, post=1)
+
+
+@pytest.mark.parametrize("name", ["agg2d", "hypre3d", "l1gs3d"])
+def test_stationary_solve(dh, orc, name):
+    A, cfg, sm, h, oh = _built(dh, orc, name)
+    b = 2.0 * dh.problems.splitmix_uniform(SEED, A.n) - 1.0
+    x = np.zeros(A.n)
+    x, it = dh.amg_solve_(x, b, h, cfg, tol=1e-8, maxiter=300)
+    xr, itr, rel = oh.solve(np.zeros(A.n), b, 1e-8, 300)
+    assert abs(it - itr) <= 1
+    assert rel_err(x, xr) < 1e-9
+    assert np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b) < 1e-8 or it == 300
+
+
+@pytest.mark.parametrize("name", ["agg2d", "hypre3d", "l1gs3d"])
+def test_pcg_fp64(dh, orc, torch_cuda, name):
+    torch = torch_cuda
+    A, cfg, sm, h, oh = _built(dh, orc, name)
+    b = dh.problems.splitmix_normal(SEED + 5, A.n)
+    xd = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+    _, it, rel = dh.pcg_solve_(xd, torch.tensor(b, device="cuda"), h, cfg, tol=1e-8, maxiter=500)
+    torch.cuda.synchronize()
+    x = xd.cpu().numpy()
+    xr, itr, relr = oh.pcg(np.zeros(A.n), b, 1e-8, 500)
+    assert abs(it - itr) <= 1, (it, itr)
+    assert rel_err(x, xr) < 1e-10
+    assert np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b) < 1.01e-8
+    # host-vector entry point gives the same
+    xh = np.zeros(A.n)
+    _, it2, _ = dh.pcg_solve_(xh, b, h, cfg, tol=1e-8, maxiter=500)
+    assert it2 == it and rel_err(xh, x) < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("name", ["agg2d", "hypre3d"])
+def test_pcg_mixed(dh, orc, torch_cuda, name, prec):
+    """FP32 / BF16-storage cycle inside FP64 PCG reaches the FP64 tolerance."""
+    torch = torch_cuda
+    A, cfg, sm, h, oh = _built(dh, orc, name)
+    b = dh.problems.splitmix_normal(SEED + 6, A.n)
+    c2 = dh.AMGConfig(**{**cfg.__dict__, "precision": prec})
+    xd = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+    _, it, rel = dh.pcg_solve_(xd, torch.tensor(b, device="cuda"), h, c2, tol=1e-8, maxiter=500)
+    x = xd.cpu().numpy()
+    xr, itr, _ = oh.pcg(np.zeros(A.n), b, 1e-8, 500)
+    true_rel = np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b)
+    print(name, prec, "iters", it, "fp64 oracle iters", itr, "true relres", true_rel, "err", rel_err(x, xr))
+    assert true_rel < 1.01e-8
+    assert it <= itr + (2 if prec == "fp32" else 8)
+    assert rel_err(x, xr) < 1e-6
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_resetup_matches_oracle(dh, orc, torch_cuda, name):
+    torch = torch_cuda
+    A, cfg, sm, _, _ = _built(dh, orc, name)
+    h = dh.amg_setup(A, cfg)
+    oh = oracle_of(h, orc, sm)
+    rng = np.random.default_rng(SEED + 9)
+    # same pattern, new values: symmetric positive rescaling keeps an M-matrix
+    s = np.exp(0.3 * rng.standard_normal(A.n))
+    rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+    new = A.nzval * s[rows] * s[A.colval]
+    dh.amg_resetup_(h, dh.problems.CSR(A.rowptr, A.colval, new), cfg)
+    oh.resetup(new)
+    for l in range(1, h.nlevels):
+        got = h.device_values(l, 0, h.level_info(l)["nnz"])
+        assert rel_err(got, oh.level_values(l)) < 1e-13
+    got = h.device_values(0, 3, h.coarsest()["nnz"])
+    assert rel_err(got, oh.coarse["nz"]) < 1e-13
+    for l in range(h.nlevels):
+        got = h.device_values(l, 1, h.level_info(l)["n"])
+        assert rel_err(got, oh.dinv(l)) < 1e-13
+    b = rng.standard_normal(A.n)
+    x = np.zeros(A.n)
+    dh.amg_cycle_(x, b, h, cfg)
+    assert rel_err(x, oh.cycle(np.zeros(A.n), b)) < 1e-12
+    # resetup from device values
+    new2 = A.nzval * 2.0
+    dh.amg_resetup_(h, torch.tensor(new2, device="cuda"), cfg)
+    oh.resetup(new2)
+    x = np.zeros(A.n)
+    dh.amg_cycle_(x, b, h, cfg)
+    assert rel_err(x, oh.cycle(np.zeros(A.n), b)) < 1e-12
+
+
+def test_trivial_systems(dh):
+    """1x1 and 2x2 systems (test/runtests.jl:1798-1815): direct solve only."""
+    A1 = dh.problems.CSR(np.array([0, 1], np.int32), np.array([0], np.int32), np.array([5.0]))
+    cfg = dh.AMGConfig(max_coarse_size=10)
+    h = dh.amg_setup(A1, cfg)
+    x = np.zeros(1)
+    x, it = dh.amg_solve_(x, np.array([3.0]), h, cfg, tol=1e-10)
+    assert abs(x[0] - 0.6) < 1e-8 and it == 1
+    A2 = dh.problems.CSR(np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32),
+                         np.array([4.0, -1.0, -1.0, 4.0]))
+    h2 = dh.amg_setup(A2, cfg)
+    b = np.array([1.0, 1.0])
+    x = np.zeros(2)
+    x, it = dh.amg_solve_(x, b, h2, cfg, tol=1e-10)
+    assert np.linalg.norm(b - A2.matvec(x)) / np.linalg.norm(b) < 1e-10
+    x, it = dh.amg_solve_(x, np.zeros(2), h2, cfg)
+    assert it == 0 and not x.any()
+
+
+def test_gpu_error_paths(dh):
+    A = dh.problems.poisson2d(8)
+    cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(), smoother=dh.JacobiSmootherType())
+    h = dh.amg_setup(A, cfg)
+    import ctypes
+    L = dh.lib()
+    c = dh.core._Config(cfg)
+    x = np.zeros(A.n - 1)
+    assert L.draugr_amg_cycle(h.handle, A.n - 1, x.ctypes.data_as(ctypes.c_void_p),
+                              x.ctypes.data_as(ctypes.c_void_p), c.handle) == -1
+    assert "DimensionMismatch" in dh.last_error()
+    with pytest.raises(dh.DraugrError, match="DimensionMismatch"):
+        bad = dh.problems.poisson2d(9)
+        dh.amg_resetup_(h, bad, cfg)
+    h_np = dh.amg_setup(A, cfg, allow_partial_resetup=False)
+    with pytest.raises(dh.DraugrError, match="restriction maps"):
+        dh.amg_resetup_(h_np, A, cfg)
+    # full resetup rebuilds
+    dh.amg_resetup_(h_np, dh.problems.poisson2d(12), cfg, partial=False)
+    assert h_np.n == 144 and h_np.info()["n_fine"] == 144
