@@ -127,7 +127,8 @@ int32_t setup_impl(int32_t n, int32_t nnz, const int32_t* rp, const int32_t* cv,
     return h;
 }
 
-hipStream_t stream_of(Hier& H, void* s) { return s ? (hipStream_t)s : H.dev->stream(); }
+// The caller's stream as given: NULL is HIP's default (null) stream.
+hipStream_t stream_of(Hier&, void* s) { return (hipStream_t)s; }
 
 }  // namespace
 }  // namespace draugr
@@ -190,9 +191,11 @@ int32_t draugr_amg_resetup(int32_t handle, int32_t n, int32_t nnz, const int32_t
                 throw Error("partial resetup needs restriction maps: setup with allow_partial_resetup=1");
             require_device(*H);
             hipStream_t st = H->dev->stream();
+            H->dev->begin(st);
             H->hv.alloc(A.nnz());
             DRAUGR_HIP_CHECK(hipMemcpyAsync(H->hv.p, A.nz.data(), A.nnz() * sizeof(double), hipMemcpyHostToDevice, st));
             H->dev->resetup(H->hv.p, st);
+            H->dev->end(st);
             if (H->host.levels.empty()) H->host.A_coarsest.nz = A.nz;
             else H->host.levels[0].A.nz = A.nz;
         } else {
@@ -218,7 +221,9 @@ int32_t draugr_hip_resetup_device(int32_t handle, int64_t nnz, const double* nzv
         if (!H->allow_partial && !H->host.levels.empty())
             throw Error("partial resetup needs restriction maps: setup with allow_partial_resetup=1");
         hipStream_t st = stream_of(*H, stream);
+        H->dev->begin(st);
         H->dev->resetup(nzval, st);
+        H->dev->end(st);
         return 0;
     });
 }
@@ -234,6 +239,7 @@ static int32_t host_vector_call(int32_t handle, int32_t n, double* x, const doub
         require_device(*H);
         check_n(*H, n);
         hipStream_t st = H->dev->stream();
+        H->dev->begin(st);
         if (H->hx.n != (size_t)n) { H->hx.alloc(n); H->hb.alloc(n); }
         DRAUGR_HIP_CHECK(hipMemcpyAsync(H->hx.p, x, n * sizeof(double), hipMemcpyHostToDevice, st));
         DRAUGR_HIP_CHECK(hipMemcpyAsync(H->hb.p, b, n * sizeof(double), hipMemcpyHostToDevice, st));
@@ -244,6 +250,7 @@ static int32_t host_vector_call(int32_t handle, int32_t n, double* x, const doub
         else if (kind == 1) ret = H->dev->solve(H->hx.p, H->hb.p, tol, maxiter, o, st, &rr, cfg.verbose);
         else ret = H->dev->pcg(H->hx.p, H->hb.p, tol, maxiter, o, st, &rr, cfg.verbose);
         DRAUGR_HIP_CHECK(hipMemcpyAsync(x, H->hx.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        H->dev->end(st);
         DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
         if (relres) *relres = rr;
         if (kind != 0 && cfg.verbose >= 1)
@@ -281,9 +288,11 @@ static int32_t device_vector_call(int32_t handle, int32_t n, double* x, const do
         const CycleOpts o = opts_of(cfg);
         double rr = 0.0;
         int32_t ret = 0;
+        H->dev->begin(st);
         if (kind == 0) H->dev->cycle(x, b, zero_guess != 0, o, st);
         else if (kind == 1) ret = H->dev->solve(x, b, tol, maxiter, o, st, &rr, cfg.verbose);
         else ret = H->dev->pcg(x, b, tol, maxiter, o, st, &rr, cfg.verbose);
+        H->dev->end(st);
         if (relres) *relres = rr;
         return ret;
     });
@@ -432,7 +441,10 @@ int32_t draugr_hip_level_op(int32_t handle, int32_t level, int32_t op, int32_t p
         std::lock_guard<std::mutex> g(H->mu);
         require_device(*H);
         if (prec < 0 || prec > 2) throw Error("prec must be 0, 1 or 2");
-        H->dev->level_op(level, op, prec, in1, in2, out, out2, stream_of(*H, stream));
+        hipStream_t st = stream_of(*H, stream);
+        H->dev->begin(st);
+        H->dev->level_op(level, op, prec, in1, in2, out, out2, st);
+        H->dev->end(st);
         return 0;
     });
 }

@@ -336,6 +336,7 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
         throw Error("smoother not available in the HIP backend (supported: jacobi, l1_jacobi, colored_gs, "
                     "l1_colored_gs)");
     DRAUGR_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    DRAUGR_HIP_CHECK(hipEventCreateWithFlags(&last_ev_, hipEventDisableTiming));
     hipStream_t st = stream_;
     n0_ = h.levels.empty() ? h.A_coarsest.nrow : h.levels[0].A.nrow;
     L_.resize(h.levels.size());
@@ -407,7 +408,19 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
     if (fl) throw Error("coarsest-level matrix is singular (dense LU found a zero pivot)");
 }
 
+void DeviceHierarchy::begin(hipStream_t st) {
+    if (last_valid_ && last_st_ != st) DRAUGR_HIP_CHECK(hipStreamWaitEvent(st, last_ev_, 0));
+}
+
+void DeviceHierarchy::end(hipStream_t st) {
+    DRAUGR_HIP_CHECK(hipEventRecord(last_ev_, st));
+    last_st_ = st;
+    last_valid_ = true;
+}
+
 DeviceHierarchy::~DeviceHierarchy() {
+    if (last_valid_) (void)hipEventSynchronize(last_ev_);
+    if (last_ev_) (void)hipEventDestroy(last_ev_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (pinned_) (void)hipHostFree(pinned_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -514,6 +527,9 @@ void DeviceHierarchy::refresh_level_mirrors(int l, hipStream_t st) {
 
 void DeviceHierarchy::ensure_precision(Precision p) {
     if (p == Precision::FP64) return;
+    if (have32_ && (p != Precision::BF16 || have16_)) return;
+    // one-time creation of the mirrors: order against all work in flight
+    DRAUGR_HIP_CHECK(hipDeviceSynchronize());
     hipStream_t st = stream_;
     if (!have32_) {
         for (auto& D : L_) {
@@ -983,7 +999,7 @@ void DeviceHierarchy::level_op(int lvl, int op, int prec, const void* in1, const
 }
 
 void DeviceHierarchy::export_level_values(int lvl, int which, double* dst) {
-    DRAUGR_HIP_CHECK(hipStreamSynchronize(stream_));
+    DRAUGR_HIP_CHECK(hipDeviceSynchronize());
     if (which == 2) {
         DRAUGR_HIP_CHECK(hipMemcpy(dst, cinv64_.p, cinv64_.bytes(), hipMemcpyDeviceToHost));
         return;

@@ -86,6 +86,11 @@ public:
     int64_t bytes_resident() const;
     double cycle_bytes(const CycleOpts& o) const;  // algorithmic bytes of one V/W cycle (zero guess)
     void ensure_precision(Precision p);
+    // Stream ordering between calls: work a call enqueues on stream st waits
+    // for everything the previous call enqueued (possibly on another stream),
+    // since all calls share the hierarchy's workspaces.
+    void begin(hipStream_t st);
+    void end(hipStream_t st);
 
 private:
     template <typename TV, typename TX>
@@ -119,6 +124,9 @@ private:
     DBuf<uint32_t> counter_;
     double* pinned_ = nullptr;
     hipStream_t stream_ = nullptr;
+    hipEvent_t last_ev_ = nullptr;
+    hipStream_t last_st_ = nullptr;
+    bool last_valid_ = false;
 };
 
 }  // namespace draugr

@@ -60,6 +60,13 @@ def lib():
             raise ImportError(
                 "libdraugr_hip.so not found at %s: build it first (python -c 'import __graft_entry__ as g; "
                 "g.build()' or make -C draugr.jl_amd/csrc)" % LIB_PATH)
+        # One HIP runtime per process: when torch is present it carries its own
+        # libamdhip64.so.7; loading torch first makes the library bind to that
+        # same runtime (same soname) instead of a second copy from /opt/rocm.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -11,7 +11,7 @@
  * (HBM-resident) vectors with an explicit HIP stream, the FP64 PCG driver,
  * resetup from device values, and inspection / per-kernel hooks used by the
  * parity tests.  Plain pointers and sizes only; a stream is passed as void*
- * (a hipStream_t, or NULL for the hierarchy's own stream).
+ * (a hipStream_t; NULL is the default stream).
  *
  * Errors: every call returns -1 on failure and stores a message retrievable
  * with draugr_amg_last_error().  The message is per calling thread and valid
