@@ -17,7 +17,8 @@ using namespace dev;
 
 namespace {
 
-constexpr int kRedBlocks = 1024;
+constexpr int kRedBlocks = 4096;   // grid of the vector reductions
+constexpr int kCoarseLdsMax = 60 * 1024;
 
 template <class F>
 __global__ __launch_bounds__(kBlock) void k_map(int64_t n, F f) {
@@ -29,26 +30,16 @@ template <class F>
 void launch_map(int64_t n, F f, hipStream_t st) {
     if (n <= 0) return;
     int64_t nb = (n + kBlock - 1) / kBlock;
-    if (nb > 8192) nb = 8192;
+    if (nb > 16384) nb = 16384;
     hipLaunchKernelGGL(k_map<F>, dim3((unsigned)nb), dim3(kBlock), 0, st, n, f);
 }
 
-__device__ __forceinline__ void store_agent(double* p, double v) {
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_agent(const double* p) {
-    return __longlong_as_double((long long)__hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
-                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-
-// Block sum in LDS (fixed tree order) -> partial[block]; the last block to
-// arrive sums the partials in index order and calls fin(total) on one lane.
-// Deterministic for a fixed grid size.
-template <class FIN>
-__device__ __forceinline__ void block_reduce_finish(double v, double* sh, double* partial, uint32_t* counter,
-                                                    int nblocks, int block, FIN fin) {
-    __shared__ bool last;
+// Block sum in LDS (fixed tree order) -> partial[block] (plain store).  The
+// partials are summed by k_finalize, a one-workgroup kernel launched right
+// after: the kernel boundary makes them visible without per-block fences (an
+// agent-scope release fence per block writes back L2 on every XCD and costs
+// far more than the extra launch).  Deterministic for a fixed grid size.
+__device__ __forceinline__ void block_sum_store(double v, double* sh, double* partial, int block) {
     sh[threadIdx.x] = v;
     __syncthreads();
 #pragma unroll
@@ -56,17 +47,14 @@ __device__ __forceinline__ void block_reduce_finish(double v, double* sh, double
         if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        store_agent(&partial[block], sh[0]);
-        __threadfence();
-        const uint32_t t = atomicAdd(counter, 1u);
-        last = (t == (uint32_t)nblocks - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    if (threadIdx.x == 0) partial[block] = sh[0];
+}
+
+template <class FIN>
+__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partial, int nblocks, FIN fin) {
+    __shared__ double sh[kBlock];
     double acc = 0.0;
-    for (int i = threadIdx.x; i < nblocks; i += kBlock) acc += load_agent(&partial[i]);
+    for (int i = threadIdx.x; i < nblocks; i += kBlock) acc += partial[i];
     sh[threadIdx.x] = acc;
     __syncthreads();
 #pragma unroll
@@ -74,106 +62,62 @@ __device__ __forceinline__ void block_reduce_finish(double v, double* sh, double
         if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        fin(sh[0]);
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (threadIdx.x == 0) fin(sh[0]);
 }
 
-template <class F, class FIN>
-__global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, F f, double* partial, uint32_t* counter, FIN fin) {
+// Fused vector pass + global sum: every element i contributes f(i).
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, F f, double* partial) {
     __shared__ double sh[kBlock];
     double acc = 0.0;
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) acc += f(i);
-    block_reduce_finish(acc, sh, partial, counter, gridDim.x, blockIdx.x, fin);
+    block_sum_store(acc, sh, partial, blockIdx.x);
 }
 
 template <class F, class FIN>
-void launch_reduce(int64_t n, F f, FIN fin, double* partial, uint32_t* counter, hipStream_t st) {
+void launch_reduce(int64_t n, F f, FIN fin, double* partial, hipStream_t st) {
     int64_t nb = (n + kBlock - 1) / kBlock;
     nb = std::max<int64_t>(1, std::min<int64_t>(nb, kRedBlocks));
-    hipLaunchKernelGGL((k_reduce<F, FIN>), dim3((unsigned)nb), dim3(kBlock), 0, st, n, f, partial, counter, fin);
+    hipLaunchKernelGGL((k_reduce<F>), dim3((unsigned)nb), dim3(kBlock), 0, st, n, f, partial);
+    hipLaunchKernelGGL((k_finalize<FIN>), dim3(1), dim3(kBlock), 0, st, partial, (int)nb, fin);
 }
 
-// CSR row-group pass whose epilogue returns a contribution to a global sum
-// (e.g. q = A p fused with p.q).  Rows are split in contiguous chunks per block.
-template <int G, typename TV, typename TX, class XF, class EPI, class FIN>
-__global__ __launch_bounds__(kBlock) void k_rows_reduce(int nrows, int chunk, const int* __restrict__ rp,
-                                                        const int* __restrict__ cv, const TV* __restrict__ av,
-                                                        XF xf, EPI epi, double* partial, uint32_t* counter,
-                                                        FIN fin) {
+// CSR-stream pass whose per-row epilogue also returns a contribution to a
+// global sum (q = A p fused with p.q in PCG).
+template <typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_stream_reduce(const int* __restrict__ blk, const int* __restrict__ rp,
+                                                          const int* __restrict__ cv, const TV* __restrict__ av,
+                                                          XF xf, EPI epi, double* partial) {
+    __shared__ TX prod[kStreamNnz];
     __shared__ double sh[kBlock];
-    const int b = xcd_block();
-    const int lane = threadIdx.x % G, slot = threadIdx.x / G;
-    constexpr int kRows = kBlock / G;
-    const int r0 = b * chunk, r1 = min(nrows, r0 + chunk);
     double dot = 0.0;
-    for (int base = r0; base < r1; base += kRows) {
-        const int row = base + slot;
-        if (row < r1) {
-            const int s = rp[row], e = rp[row + 1];
-            TX acc = 0;
-            for (int k = s + lane; k < e; k += G) acc += ld<TX, TV>(av, k) * xf(ldi(cv, k));
-            acc = group_sum<G>(acc);
-            if (lane == 0) dot += epi(row, acc);
-        }
-    }
-    block_reduce_finish(dot, sh, partial, counter, gridDim.x, blockIdx.x, fin);
-}
-
-template <int G, typename TV, typename TX, class XF, class EPI, class FIN>
-void launch_rows_reduce_g(int nrows, const int* rp, const int* cv, const TV* av, XF xf, EPI epi, FIN fin,
-                          double* partial, uint32_t* counter, hipStream_t st) {
-    constexpr int kRows = kBlock / G;
-    int64_t nb = ((int64_t)nrows + kRows - 1) / kRows;
-    nb = std::max<int64_t>(1, std::min<int64_t>(nb, kRedBlocks));
-    int chunk = (int)(((int64_t)nrows + nb - 1) / nb);
-    chunk = ((chunk + kRows - 1) / kRows) * kRows;
-    nb = ((int64_t)nrows + chunk - 1) / chunk;
-    if (nb < 1) nb = 1;
-    hipLaunchKernelGGL((k_rows_reduce<G, TV, TX, XF, EPI, FIN>), dim3((unsigned)nb), dim3(kBlock), 0, st, nrows,
-                       chunk, rp, cv, av, xf, epi, partial, counter, fin);
+    stream_block<TV, TX>(
+        xcd_block(), blk, rp, cv, av, xf, prod,
+        [&](int row, const TX* seg, int len) { dot = epi(row, seq_sum(seg, len)); },
+        [&](int row, TX s) { dot = epi(row, s); });
+    block_sum_store(dot, sh, partial, blockIdx.x);
 }
 
 template <typename TV, typename TX, class XF, class EPI, class FIN>
-void launch_rows_reduce(int G, int nrows, const int* rp, const int* cv, const TV* av, XF xf, EPI epi, FIN fin,
-                        double* partial, uint32_t* counter, hipStream_t st) {
-    switch (G) {
-        case 1: launch_rows_reduce_g<1, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
-        case 2: launch_rows_reduce_g<2, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
-        case 4: launch_rows_reduce_g<4, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
-        case 8: launch_rows_reduce_g<8, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
-        case 16: launch_rows_reduce_g<16, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
-        case 32: launch_rows_reduce_g<32, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
-        default: launch_rows_reduce_g<64, TV, TX>(nrows, rp, cv, av, xf, epi, fin, partial, counter, st); break;
-    }
+void launch_stream_reduce(const DevCSR& M, const TV* av, XF xf, EPI epi, FIN fin, double* partial, hipStream_t st) {
+    hipLaunchKernelGGL((k_stream_reduce<TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.blk.p, M.rp.p,
+                       M.cv.p, av, xf, epi, partial);
+    hipLaunchKernelGGL((k_finalize<FIN>), dim3(1), dim3(kBlock), 0, st, partial, M.nblk, fin);
 }
 
 // Inverse diagonal for the smoothers: plain (src/smoothers.jl:47-63) or l1
-// (src/smoothers.jl:816-827).  One row group per row.
-template <int G, bool L1>
+// (src/smoothers.jl:816-827); one thread per row, row-order sums.
+template <bool L1>
 __global__ __launch_bounds__(kBlock) void k_invdiag(int n, const int* __restrict__ rp, const int* __restrict__ cv,
                                                     const double* __restrict__ av, double* __restrict__ d) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const int row = t / G, lane = t % G;
+    const int row = blockIdx.x * kBlock + threadIdx.x;
     if (row >= n) return;
-    double nrm = 0.0;
-    double dg = 0.0;
-    int dgpos = -1;
-    for (int k = rp[row] + lane; k < rp[row + 1]; k += G) {
+    double nrm = 0.0, dg = 0.0;
+    for (int k = rp[row]; k < rp[row + 1]; ++k) {
         nrm += fabs(av[k]);
-        if (cv[k] == row) { dg = av[k]; dgpos = k; }
+        if (cv[k] == row) dg = av[k];  // the last diagonal match wins, as in the reference
     }
-    nrm = group_sum<G>(nrm);
-    // the last diagonal match in row order wins
-#pragma unroll
-    for (int off = G >> 1; off > 0; off >>= 1) {
-        const int op = __shfl_xor(dgpos, off, G);
-        const double ov = __shfl_xor(dg, off, G);
-        if (op > dgpos) { dgpos = op; dg = ov; }
-    }
-    if (lane != 0) return;
     const double eps = 2.220446049250313e-16;
     if (L1) {
         d[row] = nrm > eps ? 1.0 / nrm : 0.0;
@@ -184,29 +128,25 @@ __global__ __launch_bounds__(kBlock) void k_invdiag(int n, const int* __restrict
 }
 
 // Galerkin numeric product into the fixed coarse pattern:
-// Ac[k] = sum_t P[pi_t] * A[ai_t] * P[pj_t] over the triples of nonzero k
-// (src/galerkin.jl:194-222).  One row group per coarse nonzero.
-template <int G>
+// Ac[k] = sum_t P[pi_t] * A[ai_t] * P[pj_t] over the triples of nonzero k in
+// generation order (src/galerkin.jl:194-222).  One thread per coarse nonzero.
 __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nnzc, const int64_t* __restrict__ off,
                                                      const int* __restrict__ pi, const int* __restrict__ ai,
                                                      const int* __restrict__ pj, const double* __restrict__ P,
                                                      const double* __restrict__ A, double* __restrict__ Ac) {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t k = t / G;
-    const int lane = (int)(t % G);
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (k >= nnzc) return;
     double acc = 0.0;
-    for (int64_t q = off[k] + lane; q < off[k + 1]; q += G) acc += (P[pi[q]] * A[ai[q]]) * P[pj[q]];
-    acc = group_sum<G>(acc);
-    if (lane == 0) Ac[k] = acc;
+    for (int64_t q = off[k]; q < off[k + 1]; ++q) acc += P[pi[q]] * A[ai[q]] * P[pj[q]];
+    Ac[k] = acc;
 }
 
-// Dense LU with partial pivoting of the coarsest operator and its inverse,
-// one workgroup (n_c is small: <= max_coarse_size normally).  Pivot: largest
-// |a| in the column, first index on ties (LAPACK getrf).  flag = 1 if singular.
-__global__ __launch_bounds__(kBlock) void k_dense_lu_inverse(int n, double* __restrict__ a,
-                                                             double* __restrict__ inv, int* __restrict__ piv,
-                                                             int* __restrict__ flag) {
+// Dense LU with partial pivoting of the coarsest operator, one workgroup
+// (n_c is small: <= max_coarse_size normally).  Pivot: largest |a| in the
+// column, first index on ties (LAPACK getf2); whole rows are swapped.
+// flag = 1 if a pivot is exactly zero.
+__global__ __launch_bounds__(kBlock) void k_dense_lu(int n, double* __restrict__ a, int* __restrict__ piv,
+                                                     int* __restrict__ flag) {
     __shared__ double sv[kBlock];
     __shared__ int si[kBlock];
     auto at = [&](int r, int c) -> double& { return a[(size_t)c * n + r]; };
@@ -255,39 +195,89 @@ __global__ __launch_bounds__(kBlock) void k_dense_lu_inverse(int n, double* __re
         }
         __syncthreads();
     }
-    // inverse columns: solve L U x = P e_c, one column per thread
-    for (int c = threadIdx.x; c < n; c += kBlock) {
-        double* x = inv + (size_t)c * n;
-        for (int r = 0; r < n; ++r) x[r] = (r == c) ? 1.0 : 0.0;
-        for (int k = 0; k < n; ++k) {
-            const int p = piv[k];
-            const double tmp = x[k];
-            x[k] = x[p];
-            x[p] = tmp;
-        }
-        for (int r = 0; r < n; ++r) {
-            double s = x[r];
-            for (int k = 0; k < r; ++k) s -= at(r, k) * x[k];
-            x[r] = s;
-        }
-        for (int r = n - 1; r >= 0; --r) {
-            double s = x[r];
-            for (int k = r + 1; k < n; ++k) s -= at(r, k) * x[k];
-            x[r] = s / at(r, r);
-        }
-    }
 }
 
-// x = Ainv b for the coarsest level (column-major inverse: lanes read
-// consecutive rows of one column, b[j] is a broadcast load).
+// x = A_c^-1 b from the LU factors in the order of the reference LAPACK
+// getrs for one right-hand side: row interchanges, then column-oriented unit
+// lower and upper triangular solves (the loops of reference BLAS trsm).  FP64
+// arithmetic; b and x in the cycle's vector type.  Factors staged in LDS when
+// they fit.
 template <typename TX>
-__global__ __launch_bounds__(kBlock) void k_coarse_apply(int n, const TX* __restrict__ inv, const TX* __restrict__ b,
-                                                         TX* __restrict__ x) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    TX acc = 0;
-    for (int j = 0; j < n; ++j) acc += inv[(size_t)j * n + i] * b[j];
-    x[i] = acc;
+__global__ __launch_bounds__(kBlock) void k_coarse_lu_solve(int n, const double* __restrict__ lu,
+                                                            const int* __restrict__ piv, const TX* __restrict__ b,
+                                                            TX* __restrict__ x, int stage) {
+    extern __shared__ double sm[];
+    double* xs = sm;
+    const double* L = lu;
+    if (stage) {
+        double* ls = sm + n;
+        for (int e = threadIdx.x; e < n * n; e += kBlock) ls[e] = lu[e];
+        L = ls;
+    }
+    for (int i = threadIdx.x; i < n; i += kBlock) xs[i] = (double)b[i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < n; ++k) {
+            const int p = piv[k];
+            const double t = xs[k];
+            xs[k] = xs[p];
+            xs[p] = t;
+        }
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        const double xj = xs[j];
+        for (int i = j + 1 + threadIdx.x; i < n; i += kBlock) xs[i] -= L[(size_t)j * n + i] * xj;
+        __syncthreads();
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        if (threadIdx.x == 0) xs[j] /= L[(size_t)j * n + j];
+        __syncthreads();
+        const double xj = xs[j];
+        for (int i = threadIdx.x; i < j; i += kBlock) xs[i] -= L[(size_t)j * n + i] * xj;
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += kBlock) x[i] = (TX)xs[i];
+}
+
+// The same solve for n_c <= 64 in one wavefront without barriers: lane i holds
+// row i of the factors in registers and x_i; x_j is broadcast with a shuffle
+// at step j.  Per-element operation order is that of k_coarse_lu_solve.
+template <typename TX>
+__global__ __launch_bounds__(64) void k_coarse_lu_solve_wave(int n, const double* __restrict__ lu,
+                                                             const int* __restrict__ piv, const TX* __restrict__ b,
+                                                             TX* __restrict__ x) {
+    __shared__ double xs[64];
+    const int i = threadIdx.x;
+    double row[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) row[j] = (j < n && i < n) ? lu[(size_t)j * n + i] : 0.0;
+    if (i < n) xs[i] = (double)b[i];
+    __syncthreads();
+    if (i == 0)
+        for (int k = 0; k < n; ++k) {
+            const int p = piv[k];
+            const double t = xs[k];
+            xs[k] = xs[p];
+            xs[p] = t;
+        }
+    __syncthreads();
+    double xi = i < n ? xs[i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {
+        if (j < n) {
+            const double xj = __shfl(xi, j, 64);
+            if (i > j && i < n) xi -= row[j] * xj;
+        }
+    }
+#pragma unroll
+    for (int j = 63; j >= 0; --j) {
+        if (j < n) {
+            if (i == j) xi /= row[j];
+            const double xj = __shfl(xi, j, 64);
+            if (i < j) xi -= row[j] * xj;
+        }
+    }
+    if (i < n) x[i] = (TX)xi;
 }
 
 template <typename TV> const TV* vals(const DevCSR& m);
@@ -315,6 +305,12 @@ template <typename TX> const TX* d_of(const DevLevel& l);
 template <> const double* d_of<double>(const DevLevel& l) { return l.d64.p; }
 template <> const float* d_of<float>(const DevLevel& l) { return l.d32.p; }
 
+// One CSR-stream pass of operator M.
+template <typename TV, typename TX, class XF, class EPI>
+void op(const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
+    launch_stream<TV, TX>(M.nblk, M.blk.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
+}
+
 void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
     d.nrow = h.nrow;
     d.ncol = h.ncol;
@@ -322,6 +318,10 @@ void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
     d.rp.upload(h.rp, st);
     d.cv.upload(h.cv, st);
     d.v64.upload(h.nz, st);
+    const std::vector<int32_t> blk = build_row_blocks(h.rp, stream_budget(h.nnz()));
+    d.nblk = (int)blk.size() - 1;
+    d.blk.upload(blk, st);
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // blk is pageable and local
 }
 
 bool is_l1(Smoother s) { return s == Smoother::L1Jacobi || s == Smoother::L1ColoredGS; }
@@ -340,25 +340,21 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
     hipStream_t st = stream_;
     n0_ = h.levels.empty() ? h.A_coarsest.nrow : h.levels[0].A.nrow;
     L_.resize(h.levels.size());
+    int max_blocks = kRedBlocks;
     for (size_t l = 0; l < h.levels.size(); ++l) {
         const HostLevel& H = h.levels[l];
         DevLevel& D = L_[l];
         upload_csr(D.A, H.A, st);
         upload_csr(D.P, H.P, st);
-        D.Pt.nrow = H.P.ncol;
-        D.Pt.ncol = H.P.nrow;
-        D.Pt.nnz = H.P.nnz();
-        D.Pt.rp.upload(H.Pt.offsets, st);
-        D.Pt.cv.upload(H.Pt.fine_rows, st);
+        CSR Pt;  // P^T as CSR with the values of P gathered through the transpose map
+        Pt.nrow = H.P.ncol;
+        Pt.ncol = H.P.nrow;
+        Pt.rp = H.Pt.offsets;
+        Pt.cv = H.Pt.fine_rows;
+        Pt.nz.resize(H.Pt.p_nz_idx.size());
+        for (size_t k = 0; k < Pt.nz.size(); ++k) Pt.nz[k] = H.P.nz[H.Pt.p_nz_idx[k]];
+        upload_csr(D.Pt, Pt, st);
         D.pt_pnz.upload(H.Pt.p_nz_idx, st);
-        std::vector<double> ptv(H.Pt.p_nz_idx.size());
-        for (size_t k = 0; k < ptv.size(); ++k) ptv[k] = H.P.nz[H.Pt.p_nz_idx[k]];
-        D.Pt.v64.upload(ptv, st);
-        DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // ptv is pageable and local
-        const int gs = cfg.group_size;
-        D.A.group = gs > 0 ? gs : pick_group(D.A.nnz, D.A.nrow);
-        D.P.group = gs > 0 ? gs : pick_group(D.P.nnz, D.P.nrow);
-        D.Pt.group = gs > 0 ? gs : pick_group(D.Pt.nnz, D.Pt.nrow);
         D.has_rmap = H.has_rmap;
         if (H.has_rmap) {
             D.ntriples = H.R.ntriples();
@@ -366,8 +362,6 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
             D.r_pi.upload(H.R.pi, st);
             D.r_ai.upload(H.R.ai, st);
             D.r_pj.upload(H.R.pj, st);
-            const int64_t nnzc = (int64_t)H.R.nz_offsets.size() - 1;
-            D.rap_group = pick_group(D.ntriples, nnzc);
         }
         if (is_gs(smoother_)) {
             D.ncolors = H.coloring.num_colors;
@@ -380,12 +374,12 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
         D.t64.alloc(n);
         D.xc64.alloc(nc);
         D.bc64.alloc(nc);
+        max_blocks = std::max(max_blocks, D.A.nblk);
     }
     upload_csr(Ac_, h.A_coarsest, st);
-    Ac_.group = pick_group(Ac_.nnz, Ac_.nrow);
+    max_blocks = std::max(max_blocks, Ac_.nblk);
     nc_ = h.n_coarse;
-    cdense_.alloc((size_t)nc_ * nc_);
-    cinv64_.alloc((size_t)nc_ * nc_);
+    clu_.alloc((size_t)nc_ * nc_);
     flag_.alloc(1 + nc_);
     const int64_t n0 = n0_;
     w_r_.alloc(n0);
@@ -395,9 +389,7 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
     w_z_.alloc(n0);
     w_x_.alloc(n0);
     scal_.alloc(S_COUNT_);
-    partial_.alloc(kRedBlocks);
-    counter_.alloc(1);
-    DRAUGR_HIP_CHECK(hipMemsetAsync(counter_.p, 0, sizeof(uint32_t), st));
+    partial_.alloc(max_blocks);
     DRAUGR_HIP_CHECK(hipMemsetAsync(scal_.p, 0, S_COUNT_ * sizeof(double), st));
     DRAUGR_HIP_CHECK(hipHostMalloc((void**)&pinned_, 16 * sizeof(double), hipHostMallocDefault));
     for (int l = 0; l < nlevels(); ++l) compute_dinv(l, st);
@@ -428,7 +420,9 @@ DeviceHierarchy::~DeviceHierarchy() {
 
 int64_t DeviceHierarchy::bytes_resident() const {
     int64_t b = 0;
-    auto csr = [&](const DevCSR& m) { b += m.rp.bytes() + m.cv.bytes() + m.v64.bytes() + m.v32.bytes() + m.v16.bytes(); };
+    auto csr = [&](const DevCSR& m) {
+        b += m.rp.bytes() + m.cv.bytes() + m.v64.bytes() + m.v32.bytes() + m.v16.bytes() + m.blk.bytes();
+    };
     for (auto& l : L_) {
         csr(l.A); csr(l.P); csr(l.Pt);
         b += l.pt_pnz.bytes() + l.r_off.bytes() + l.r_pi.bytes() + l.r_ai.bytes() + l.r_pj.bytes();
@@ -437,7 +431,7 @@ int64_t DeviceHierarchy::bytes_resident() const {
         b += l.r32.bytes() + l.t32.bytes() + l.xc32.bytes() + l.bc32.bytes();
     }
     csr(Ac_);
-    b += cdense_.bytes() + cinv64_.bytes() + cinv32_.bytes();
+    b += clu_.bytes();
     b += w_r_.bytes() + w_p0_.bytes() + w_p1_.bytes() + w_q_.bytes() + w_z_.bytes() + w_x_.bytes();
     b += w_r32_.bytes() + w_z32_.bytes();
     return b;
@@ -446,23 +440,13 @@ int64_t DeviceHierarchy::bytes_resident() const {
 void DeviceHierarchy::compute_dinv(int l, hipStream_t st) {
     DevLevel& D = L_[l];
     const int n = D.A.nrow;
-    const int G = D.A.group;
-    const int64_t nb = ((int64_t)n * G + kBlock - 1) / kBlock;
     if (n == 0) return;
-#define DRAUGR_INVDIAG(GG)                                                                                  \
-    case GG:                                                                                                \
-        if (is_l1(smoother_))                                                                               \
-            hipLaunchKernelGGL((k_invdiag<GG, true>), dim3((unsigned)nb), dim3(kBlock), 0, st, n, D.A.rp.p, \
-                               D.A.cv.p, D.A.v64.p, D.d64.p);                                                \
-        else                                                                                                \
-            hipLaunchKernelGGL((k_invdiag<GG, false>), dim3((unsigned)nb), dim3(kBlock), 0, st, n, D.A.rp.p, \
-                               D.A.cv.p, D.A.v64.p, D.d64.p);                                                \
-        break;
-    switch (G) {
-        DRAUGR_INVDIAG(1) DRAUGR_INVDIAG(2) DRAUGR_INVDIAG(4) DRAUGR_INVDIAG(8) DRAUGR_INVDIAG(16)
-        DRAUGR_INVDIAG(32) default : DRAUGR_INVDIAG(64)
-    }
-#undef DRAUGR_INVDIAG
+    const int nb = (n + kBlock - 1) / kBlock;
+    if (is_l1(smoother_))
+        hipLaunchKernelGGL(k_invdiag<true>, dim3(nb), dim3(kBlock), 0, st, n, D.A.rp.p, D.A.cv.p, D.A.v64.p, D.d64.p);
+    else
+        hipLaunchKernelGGL(k_invdiag<false>, dim3(nb), dim3(kBlock), 0, st, n, D.A.rp.p, D.A.cv.p, D.A.v64.p,
+                           D.d64.p);
     if (D.d32.p) {
         const double* s = D.d64.p;
         float* d = D.d32.p;
@@ -476,24 +460,17 @@ void DeviceHierarchy::galerkin(int l, hipStream_t st) {
     DevCSR& out = (l + 1 < nlevels()) ? L_[l + 1].A : Ac_;
     const int64_t nnzc = out.nnz;
     if (nnzc == 0) return;
-    const int G = D.rap_group;
-    const int64_t nb = (nnzc * G + kBlock - 1) / kBlock;
-#define DRAUGR_GAL(GG)                                                                                      \
-    case GG:                                                                                                \
-        hipLaunchKernelGGL((k_galerkin<GG>), dim3((unsigned)nb), dim3(kBlock), 0, st, nnzc, D.r_off.p,        \
-                           D.r_pi.p, D.r_ai.p, D.r_pj.p, D.P.v64.p, D.A.v64.p, out.v64.p);                  \
-        break;
-    switch (G) {
-        DRAUGR_GAL(1) DRAUGR_GAL(2) DRAUGR_GAL(4) DRAUGR_GAL(8) DRAUGR_GAL(16) DRAUGR_GAL(32)
-        default : DRAUGR_GAL(64)
-    }
-#undef DRAUGR_GAL
+    const int64_t nb = (nnzc + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_galerkin, dim3((unsigned)nb), dim3(kBlock), 0, st, nnzc, D.r_off.p, D.r_pi.p, D.r_ai.p,
+                       D.r_pj.p, D.P.v64.p, D.A.v64.p, out.v64.p);
 }
 
+// Coarsest operator: dense from its CSR values, then LU (src/setup.jl:177-209,
+// src/resetup.jl:123-158 and :44), all on the device.
 void DeviceHierarchy::coarse_factor(hipStream_t st) {
     const int n = nc_;
     if (n == 0) return;
-    double* dn = cdense_.p;
+    double* dn = clu_.p;
     launch_map((int64_t)n * n, [=] __device__(int64_t i) { dn[i] = 0.0; }, st);
     const int* rp = Ac_.rp.p;
     const int* cv = Ac_.cv.p;
@@ -503,13 +480,7 @@ void DeviceHierarchy::coarse_factor(hipStream_t st) {
     }, st);
     int* fl = flag_.p;
     launch_map(1, [=] __device__(int64_t) { fl[0] = 0; }, st);
-    hipLaunchKernelGGL(k_dense_lu_inverse, dim3(1), dim3(kBlock), 0, st, n, cdense_.p, cinv64_.p, flag_.p + 1,
-                       flag_.p);
-    if (cinv32_.p) {
-        const double* s = cinv64_.p;
-        float* d = cinv32_.p;
-        launch_map((int64_t)n * n, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
-    }
+    hipLaunchKernelGGL(k_dense_lu, dim3(1), dim3(kBlock), 0, st, n, clu_.p, flag_.p + 1, flag_.p);
 }
 
 void DeviceHierarchy::refresh_level_mirrors(int l, hipStream_t st) {
@@ -550,10 +521,6 @@ void DeviceHierarchy::ensure_precision(Precision p) {
             float* d = D.d32.p;
             launch_map(D.A.nrow, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
         }
-        cinv32_.alloc((size_t)nc_ * nc_);
-        const double* s = cinv64_.p;
-        float* d = cinv32_.p;
-        launch_map((int64_t)nc_ * nc_, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
         w_r32_.alloc(n0_);
         w_z32_.alloc(n0_);
         have32_ = true;
@@ -571,6 +538,9 @@ void DeviceHierarchy::ensure_precision(Precision p) {
 }
 
 // ── smoothing (src/smoothers.jl:78-124, 203-241, 296-313, 835-854) ──────────
+// Jacobi steps ping-pong between x and alt (no copy back: the caller tracks
+// which buffer holds the iterate).  A first step from a zero guess is
+// x1 = w d b, fused with the residual that follows it when fuse_resid.
 template <typename TV, typename TX>
 void DeviceHierarchy::smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bool& zero, bool fuse_resid,
                              bool& resid_done, hipStream_t st) {
@@ -588,16 +558,15 @@ void DeviceHierarchy::smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bo
             for (int c = 0; c < D.ncolors; ++c) {
                 const int off = D.color_off[c], cnt = D.color_off[c + 1] - off;
                 if (cnt == 0) continue;
-                launch_rows<TV, TX>(D.A.group, cnt, D.A.rp.p, D.A.cv.p, vals<TV>(D.A),
-                                    RowList{D.color_order.p + off}, XVec<TX>{x}, EpiGS<TX>{x, b, d}, st);
+                hipLaunchKernelGGL((k_gs_color<TV, TX>), dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cnt,
+                                   D.color_order.p + off, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), x, b, d);
             }
         return;
     }
     for (int s = 0; s < steps; ++s) {
         if (zero) {
             if (fuse_resid && steps == 1) {
-                launch_rows<TV, TX>(D.A.group, n, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{},
-                                    XZeroJacobi<TX>{d, b, w}, EpiZeroJacobiResid<TX>{b, d, alt, r_of<TX>(D), w}, st);
+                op<TV, TX>(D.A, XZeroJacobi<TX>{d, b, w}, EpiZeroJacobiResid<TX>{b, d, alt, r_of<TX>(D), w}, st);
                 resid_done = true;
             } else {
                 TX* o = alt;
@@ -605,8 +574,7 @@ void DeviceHierarchy::smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bo
             }
             zero = false;
         } else {
-            launch_rows<TV, TX>(D.A.group, n, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{}, XVec<TX>{x},
-                                EpiJacobi<TX>{x, b, d, alt, w}, st);
+            op<TV, TX>(D.A, XVec<TX>{x}, EpiJacobi<TX>{x, b, d, alt, w}, st);
         }
         std::swap(x, alt);
     }
@@ -621,11 +589,15 @@ template <typename TX>
 void DeviceHierarchy::coarse_solve(TX* x, const TX* b, hipStream_t st) {
     const int n = nc_;
     if (n == 0) return;
-    const TX* inv;
-    if constexpr (sizeof(TX) == 8) inv = (const TX*)cinv64_.p;
-    else inv = (const TX*)cinv32_.p;
-    const int nb = (n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_coarse_apply<TX>, dim3(nb), dim3(kBlock), 0, st, n, inv, b, x);
+    if (n <= 64) {
+        hipLaunchKernelGGL(k_coarse_lu_solve_wave<TX>, dim3(1), dim3(64), 0, st, n, clu_.p, flag_.p + 1, b, x);
+        return;
+    }
+    const size_t full = (size_t)(n + (size_t)n * n) * sizeof(double);
+    const int stage = full <= (size_t)kCoarseLdsMax ? 1 : 0;
+    const size_t lds = stage ? full : (size_t)n * sizeof(double);
+    if (lds > (size_t)kCoarseLdsMax) throw Error("coarsest level too large for the device direct solve");
+    hipLaunchKernelGGL(k_coarse_lu_solve<TX>, dim3(1), dim3(kBlock), lds, st, n, clu_.p, flag_.p + 1, b, x, stage);
 }
 
 // One visit of level l (src/cycle.jl:63-99).
@@ -640,19 +612,15 @@ void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const Cy
     TX* bc = bc_of<TX>(D);
     bool resid_done = false;
     smooth<TV, TX>(l, xcur, xalt, b, o.pre, zero, true, resid_done, st);
-    if (!resid_done)
-        launch_rows<TV, TX>(D.A.group, D.A.nrow, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{}, XVec<TX>{xcur},
-                            EpiResid<TX>{b, r}, st);
-    launch_rows<TP, TX>(D.Pt.group, D.Pt.nrow, D.Pt.rp.p, D.Pt.cv.p, vals<TP>(D.Pt), RowId{}, XVec<TX>{r},
-                        EpiStore<TX>{bc}, st);
+    if (!resid_done) op<TV, TX>(D.A, XVec<TX>{xcur}, EpiResid<TX>{b, r}, st);
+    op<TP, TX>(D.Pt, XVec<TX>{r}, EpiStore<TX>{bc}, st);
     if (l + 1 < nlevels()) {
         const int reps = o.w_cycle ? 2 : 1;
         for (int rep = 0; rep < reps; ++rep) cycle_level<TV, TX>(l + 1, xc, bc, rep == 0, o, st);
     } else {
         coarse_solve<TX>(xc, bc, st);
     }
-    launch_rows<TP, TX>(D.P.group, D.P.nrow, D.P.rp.p, D.P.cv.p, vals<TP>(D.P), RowId{}, XVec<TX>{xc},
-                        EpiAdd<TX>{xcur}, st);
+    op<TP, TX>(D.P, XVec<TX>{xc}, EpiAdd<TX>{xcur}, st);
     bool z = false, dummy = false;
     smooth<TV, TX>(l, xcur, xalt, b, o.post, z, false, dummy, st);
     if (xcur != x) {
@@ -684,10 +652,8 @@ void DeviceHierarchy::cycle(double* x, const double* b, bool zero, const CycleOp
         launch_map(n, [=] __device__(int64_t i) { r32[i] = (float)b[i]; }, st);
     } else {
         // correction form: x += V(0; b - A x), the same affine map in exact arithmetic
-        DevLevel& D = L_[0];
         double* r = w_r_.p;
-        launch_rows<double, double>(D.A.group, n, D.A.rp.p, D.A.cv.p, D.A.v64.p, RowId{}, XVec<double>{x},
-                                    EpiResid<double>{b, r}, st);
+        op<double, double>(L_[0].A, XVec<double>{x}, EpiResid<double>{b, r}, st);
         launch_map(n, [=] __device__(int64_t i) { r32[i] = (float)r[i]; }, st);
     }
     if (o.prec == Precision::FP32) cycle_top<float, float>(z32, r32, true, o, st);
@@ -710,7 +676,7 @@ int DeviceHierarchy::solve(double* x, const double* b, double tol, int maxiter, 
     const int n = n0_;
     double* scal = scal_.p;
     launch_reduce(n, [=] __device__(int64_t i) { return b[i] * b[i]; },
-                  [=] __device__(double s) { scal[S_BB] = s; }, partial_.p, counter_.p, st);
+                  [=] __device__(double s) { scal[S_BB] = s; }, partial_.p, st);
     const double bnorm = std::sqrt(reduce_to_host(st, S_BB));
     if (relres) *relres = 0.0;
     if (bnorm == 0.0) {
@@ -728,10 +694,9 @@ int DeviceHierarchy::solve(double* x, const double* b, double tol, int maxiter, 
     double rel = 1.0;
     for (int it = 1; it <= maxiter; ++it) {
         cycle(x, b, false, o, st);
-        launch_rows<double, double>(D.A.group, n, D.A.rp.p, D.A.cv.p, D.A.v64.p, RowId{}, XVec<double>{x},
-                                    EpiResid<double>{b, r}, st);
+        op<double, double>(D.A, XVec<double>{x}, EpiResid<double>{b, r}, st);
         launch_reduce(n, [=] __device__(int64_t i) { return r[i] * r[i]; },
-                      [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, counter_.p, st);
+                      [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, st);
         rel = std::sqrt(reduce_to_host(st, S_RR)) / bnorm;
         if (relres) *relres = rel;
         if (verbose >= 2) std::printf("  AMG iter %4d: relative residual = %.6e\n", it, rel);
@@ -756,12 +721,12 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
     double* z = w_z_.p;
     double* pa = w_p0_.p;
     double* pb = w_p1_.p;
-    const bool mixed = o.prec != Precision::FP64;
+    const bool mixed = o.prec != Precision::FP64 && nlevels() > 0;
     if (mixed) ensure_precision(o.prec);
     float* r32 = w_r32_.p;
     float* z32 = w_z32_.p;
     launch_reduce(n, [=] __device__(int64_t i) { return b[i] * b[i]; },
-                  [=] __device__(double s) { scal[S_BB] = s; }, partial_.p, counter_.p, st);
+                  [=] __device__(double s) { scal[S_BB] = s; }, partial_.p, st);
     const double bb = reduce_to_host(st, S_BB);
     if (relres) *relres = 0.0;
     if (bb == 0.0) {
@@ -769,21 +734,13 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
         DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
         return 0;
     }
-    // r = b - A x
-    if (nlevels() > 0) {
-        DevLevel& D = L_[0];
-        launch_rows<double, double>(D.A.group, n, D.A.rp.p, D.A.cv.p, D.A.v64.p, RowId{}, XVec<double>{x},
-                                    EpiResid<double>{b, r}, st);
-    } else {
-        launch_rows<double, double>(Ac_.group, n, Ac_.rp.p, Ac_.cv.p, Ac_.v64.p, RowId{}, XVec<double>{x},
-                                    EpiResid<double>{b, r}, st);
-    }
+    const DevCSR& A = nlevels() > 0 ? L_[0].A : Ac_;
+    op<double, double>(A, XVec<double>{x}, EpiResid<double>{b, r}, st);  // r = b - A x
     launch_reduce(n, [=] __device__(int64_t i) { return r[i] * r[i]; },
-                  [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, counter_.p, st);
+                  [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, st);
     double rel = std::sqrt(reduce_to_host(st, S_RR) / bb);
     if (relres) *relres = rel;
     if (rel < tol) return 0;
-    const DevCSR& A = nlevels() > 0 ? L_[0].A : Ac_;
     launch_map(n, [=] __device__(int64_t i) { pa[i] = 0.0; }, st);
     auto precondition = [&]() {
         if (mixed) {
@@ -798,16 +755,15 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
     // rz = r.z ; beta = 0 for the first direction
     if (mixed)
         launch_reduce(n, [=] __device__(int64_t i) { return r[i] * (double)z32[i]; },
-                      [=] __device__(double s) { scal[S_RZ] = s; scal[S_BETA] = 0.0; }, partial_.p, counter_.p, st);
+                      [=] __device__(double s) { scal[S_RZ] = s; scal[S_BETA] = 0.0; }, partial_.p, st);
     else
         launch_reduce(n, [=] __device__(int64_t i) { return r[i] * z[i]; },
-                      [=] __device__(double s) { scal[S_RZ] = s; scal[S_BETA] = 0.0; }, partial_.p, counter_.p, st);
+                      [=] __device__(double s) { scal[S_RZ] = s; scal[S_BETA] = 0.0; }, partial_.p, st);
     hipEvent_t ev;
     DRAUGR_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    int it = 0;
     int result = maxiter;
     try {
-        for (it = 1; it <= maxiter; ++it) {
+        for (int it = 1; it <= maxiter; ++it) {
             // p_new = z + beta p_old; q = A p_new; alpha = rz / (p_new . q)
             const double* betap = scal + S_BETA;
             auto fin_pq = [=] __device__(double pq) { scal[S_PQ] = pq; scal[S_ALPHA] = scal[S_RZ] / pq; };
@@ -819,8 +775,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                     q[i] = s;
                     return pn * s;
                 };
-                launch_rows_reduce<double, double>(A.group, n, A.rp.p, A.cv.p, A.v64.p, xf, epi, fin_pq, partial_.p,
-                                                   counter_.p, st);
+                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, partial_.p, st);
             } else {
                 auto xf = [=] __device__(int j) -> double { return z[j] + *betap * pa[j]; };
                 auto epi = [=] __device__(int i, double s) -> double {
@@ -829,8 +784,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                     q[i] = s;
                     return pn * s;
                 };
-                launch_rows_reduce<double, double>(A.group, n, A.rp.p, A.cv.p, A.v64.p, xf, epi, fin_pq, partial_.p,
-                                                   counter_.p, st);
+                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, partial_.p, st);
             }
             // x += alpha p; r -= alpha q; r.r  (and the FP32 copy of r)
             const double* alphap = scal + S_ALPHA;
@@ -844,7 +798,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                                   r32[i] = (float)ri;
                                   return ri * ri;
                               },
-                              [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, counter_.p, st);
+                              [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, st);
             else
                 launch_reduce(n,
                               [=] __device__(int64_t i) {
@@ -854,7 +808,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                                   r[i] = ri;
                                   return ri * ri;
                               },
-                              [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, counter_.p, st);
+                              [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, st);
             DRAUGR_HIP_CHECK(hipMemcpyAsync(pinned_ + S_RR, scal + S_RR, sizeof(double), hipMemcpyDeviceToHost, st));
             DRAUGR_HIP_CHECK(hipEventRecord(ev, st));
             // speculative: next preconditioned residual and beta
@@ -864,11 +818,9 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                 scal[S_RZ] = s;
             };
             if (mixed)
-                launch_reduce(n, [=] __device__(int64_t i) { return r[i] * (double)z32[i]; }, fin_rz, partial_.p,
-                              counter_.p, st);
+                launch_reduce(n, [=] __device__(int64_t i) { return r[i] * (double)z32[i]; }, fin_rz, partial_.p, st);
             else
-                launch_reduce(n, [=] __device__(int64_t i) { return r[i] * z[i]; }, fin_rz, partial_.p, counter_.p,
-                              st);
+                launch_reduce(n, [=] __device__(int64_t i) { return r[i] * z[i]; }, fin_rz, partial_.p, st);
             std::swap(pa, pb);
             DRAUGR_HIP_CHECK(hipEventSynchronize(ev));
             rel = std::sqrt(pinned_[S_RR] / bb);
@@ -936,25 +888,24 @@ double DeviceHierarchy::cycle_bytes(const CycleOpts& o) const {
         lv += D.Pt.nnz * (sp + 4) + (nc + 1) * 4 + n * sx + nc * sx;      // restrict
         lv += D.P.nnz * (sp + 4) + (n + 1) * 4 + nc * sx + 2 * n * sx;    // prolong
         total += visits * lv;
-        if (o.w_cycle && l > 0) visits *= 2;
-        else if (o.w_cycle && l == 0) visits *= 2;
+        if (o.w_cycle) visits *= 2;
     }
-    total += visits * ((double)nc_ * nc_ * sx + 2.0 * nc_ * sx);
+    total += visits * ((double)nc_ * nc_ * 8 + 2.0 * nc_ * sx);
     return total;
 }
 
 // ── per-kernel hooks for parity tests and measurement ───────────────────────
-void DeviceHierarchy::level_op(int lvl, int op, int prec, const void* in1, const void* in2, void* out, void* out2,
-                               hipStream_t st) {
-    if (op == 7) {
+void DeviceHierarchy::level_op(int lvl, int op_code, int prec, const void* in1, const void* in2, void* out,
+                               void* out2, hipStream_t st) {
+    if (op_code == 7) {
         if (prec == 0) coarse_solve<double>((double*)out, (const double*)in1, st);
-        else { ensure_precision(Precision::FP32); coarse_solve<float>((float*)out, (const float*)in1, st); }
+        else coarse_solve<float>((float*)out, (const float*)in1, st);
         return;
     }
     if (lvl < 0 || lvl >= nlevels()) throw Error("level index out of range");
     DevLevel& D = L_[lvl];
-    if (op == 8) { galerkin(lvl, st); return; }
-    if (op == 9) {
+    if (op_code == 8) { galerkin(lvl, st); return; }
+    if (op_code == 9) {
         compute_dinv(lvl, st);
         DRAUGR_HIP_CHECK(hipMemcpyAsync(out, D.d64.p, D.A.nrow * sizeof(double), hipMemcpyDeviceToDevice, st));
         return;
@@ -963,31 +914,25 @@ void DeviceHierarchy::level_op(int lvl, int op, int prec, const void* in1, const
         typedef decltype(tv_tag) TV;
         typedef decltype(tx_tag) TX;
         typedef typename TransferT<TV>::type TP;
-        const int n = D.A.nrow;
         const TX* d = d_of<TX>(D);
         const TX w = (TX)omega_;
-        switch (op) {
-            case 0: launch_rows<TV, TX>(D.A.group, n, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{},
-                                        XVec<TX>{(const TX*)in1}, EpiStore<TX>{(TX*)out}, st); break;
-            case 1: launch_rows<TV, TX>(D.A.group, n, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{},
-                                        XVec<TX>{(const TX*)in1}, EpiResid<TX>{(const TX*)in2, (TX*)out}, st); break;
-            case 2: launch_rows<TV, TX>(D.A.group, n, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{},
-                                        XVec<TX>{(const TX*)in1},
-                                        EpiJacobi<TX>{(const TX*)in1, (const TX*)in2, d, (TX*)out, w}, st); break;
-            case 3: launch_rows<TV, TX>(D.A.group, n, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), RowId{},
-                                        XZeroJacobi<TX>{d, (const TX*)in2, w},
-                                        EpiZeroJacobiResid<TX>{(const TX*)in2, d, (TX*)out, (TX*)out2, w}, st); break;
-            case 4: launch_rows<TP, TX>(D.Pt.group, D.Pt.nrow, D.Pt.rp.p, D.Pt.cv.p, vals<TP>(D.Pt), RowId{},
-                                        XVec<TX>{(const TX*)in1}, EpiStore<TX>{(TX*)out}, st); break;
-            case 5: launch_rows<TP, TX>(D.P.group, D.P.nrow, D.P.rp.p, D.P.cv.p, vals<TP>(D.P), RowId{},
-                                        XVec<TX>{(const TX*)in1}, EpiAdd<TX>{(TX*)out}, st); break;
+        const TX* a = (const TX*)in1;
+        const TX* bb = (const TX*)in2;
+        TX* o = (TX*)out;
+        switch (op_code) {
+            case 0: op<TV, TX>(D.A, XVec<TX>{a}, EpiStore<TX>{o}, st); break;
+            case 1: op<TV, TX>(D.A, XVec<TX>{a}, EpiResid<TX>{bb, o}, st); break;
+            case 2: op<TV, TX>(D.A, XVec<TX>{a}, EpiJacobi<TX>{a, bb, d, o, w}, st); break;
+            case 3: op<TV, TX>(D.A, XZeroJacobi<TX>{d, bb, w}, EpiZeroJacobiResid<TX>{bb, d, o, (TX*)out2, w}, st);
+                    break;
+            case 4: op<TP, TX>(D.Pt, XVec<TX>{a}, EpiStore<TX>{o}, st); break;
+            case 5: op<TP, TX>(D.P, XVec<TX>{a}, EpiAdd<TX>{o}, st); break;
             case 6:
                 for (int c = 0; c < D.ncolors; ++c) {
                     const int off = D.color_off[c], cnt = D.color_off[c + 1] - off;
                     if (cnt == 0) continue;
-                    launch_rows<TV, TX>(D.A.group, cnt, D.A.rp.p, D.A.cv.p, vals<TV>(D.A),
-                                        RowList{D.color_order.p + off}, XVec<TX>{(TX*)out},
-                                        EpiGS<TX>{(TX*)out, (const TX*)in2, d}, st);
+                    hipLaunchKernelGGL((k_gs_color<TV, TX>), dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                                       cnt, D.color_order.p + off, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), o, bb, d);
                 }
                 break;
             default: throw Error("unknown level op");
@@ -1001,7 +946,7 @@ void DeviceHierarchy::level_op(int lvl, int op, int prec, const void* in1, const
 void DeviceHierarchy::export_level_values(int lvl, int which, double* dst) {
     DRAUGR_HIP_CHECK(hipDeviceSynchronize());
     if (which == 2) {
-        DRAUGR_HIP_CHECK(hipMemcpy(dst, cinv64_.p, cinv64_.bytes(), hipMemcpyDeviceToHost));
+        DRAUGR_HIP_CHECK(hipMemcpy(dst, clu_.p, clu_.bytes(), hipMemcpyDeviceToHost));
         return;
     }
     if (which == 3) {

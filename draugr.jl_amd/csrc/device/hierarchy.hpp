@@ -20,7 +20,8 @@ struct DevCSR {
     DBuf<double> v64;
     DBuf<float> v32;
     DBuf<dev::bf16_t> v16;
-    int group = 1;  // lanes per row
+    DBuf<int32_t> blk;  // CSR-stream row blocks (offsets into rows)
+    int nblk = 0;
 };
 
 struct DevLevel {
@@ -30,7 +31,6 @@ struct DevLevel {
     DBuf<int32_t> pt_pnz;     // Pt entry -> P entry
     bool has_rmap = false;
     int64_t ntriples = 0;
-    int rap_group = 1;
     DBuf<int64_t> r_off;      // Galerkin triple map of A_{l+1}
     DBuf<int32_t> r_pi, r_ai, r_pj;
     DBuf<double> d64;         // smoother inverse diagonal (plain or l1)
@@ -82,7 +82,7 @@ public:
     // 9 inverse diagonal (returned in out)
     void level_op(int lvl, int op, int prec, const void* in1, const void* in2, void* out, void* out2,
                   hipStream_t st);
-    void export_level_values(int lvl, int which, double* host_dst);  // 0 A vals, 1 dinv, 2 coarse inverse
+    void export_level_values(int lvl, int which, double* host_dst);  // 0 A vals, 1 dinv, 2 coarse LU, 3 coarsest vals
     int64_t bytes_resident() const;
     double cycle_bytes(const CycleOpts& o) const;  // algorithmic bytes of one V/W cycle (zero guess)
     void ensure_precision(Precision p);
@@ -111,9 +111,8 @@ private:
     int32_t n0_ = 0;
     int32_t nc_ = 0;
     DevCSR Ac_;                        // coarsest operator (sparse; values by Galerkin)
-    DBuf<double> cdense_, cinv64_;
-    DBuf<float> cinv32_;
-    DBuf<int32_t> flag_;
+    DBuf<double> clu_;                 // dense LU factors of the coarsest operator (column-major)
+    DBuf<int32_t> flag_;               // [0] singular flag, [1..n] pivots
     Smoother smoother_;
     double omega_;
     bool have32_ = false, have16_ = false;
@@ -121,7 +120,6 @@ private:
     DBuf<double> w_r_, w_p0_, w_p1_, w_q_, w_z_, w_x_;
     DBuf<float> w_r32_, w_z32_;
     DBuf<double> scal_, partial_;
-    DBuf<uint32_t> counter_;
     double* pinned_ = nullptr;
     hipStream_t stream_ = nullptr;
     hipEvent_t last_ev_ = nullptr;

@@ -1,25 +1,36 @@
-// HIP kernels for gfx950 (CDNA4): every hot operation of the AMG cycle is a
-// CSR row-group pass with a fused epilogue.
+// HIP kernels for gfx950 (CDNA4) of the AMG cycle: every operator pass
+// (SpMV, residual, smoother step, restriction, prolongation) is one
+// CSR-stream launch with a fused per-row epilogue.
 //
-// A "row group" is G consecutive lanes of a 64-wide wavefront (G in
-// {1,2,4,8,16,32,64}); the group strides over the nonzeros of one row, so a
-// wavefront covers 64/G rows with coalesced loads of colval/nzval, and the
-// partial sums are combined with cross-lane shuffles.  G is picked per operator
-// from its mean row length (see pick_group()).  Blocks are remapped so that
-// each of the 8 XCDs sweeps a contiguous slice of rows: neighbouring rows share
-// x entries through that XCD's L2.
+// CSR-stream: the rows of an operator are cut, once at upload, into row
+// blocks of at most kStreamRows rows and kStreamNnz nonzeros.  A 256-thread
+// workgroup streams its block's contiguous nonzero range: every thread forms
+// products val[k] * x[col[k]] for k = tid, tid + 256, ... (coalesced loads of
+// col/val, several independent gathers in flight per thread) into LDS; then
+// one thread per row walks its row's products in row order and applies the
+// epilogue.  The row sums are therefore formed in exactly the reference's
+// sequential order (products rounded, then added left to right; device code is
+// built with -ffp-contract=off), so FP64 results equal the reference's bit for
+// bit.  A row longer than kStreamNnz gets a block of its own and is reduced by
+// the whole workgroup (tree order).  Blocks are remapped so that each of the 8
+// XCDs sweeps a contiguous slice of rows: neighbouring rows share x entries
+// through that XCD's L2.
 //
-// Value storage TV is double, float or bf16 (uint16_t); vectors / accumulation
-// TX is double (TV = double) or float (TV = float or bf16).
+// Value storage TV is double, float or bf16 (uint16_t); vectors and
+// accumulation TX are double (TV = double) or float (TV = float or bf16).
 #pragma once
 
 #include <hip/hip_runtime.h>
+
 #include <cstdint>
+#include <vector>
 
 namespace draugr {
 namespace dev {
 
 constexpr int kBlock = 256;
+constexpr int kStreamNnz = 2048;    // products staged in LDS per block
+constexpr int kStreamRows = kBlock; // one epilogue thread per row
 
 typedef uint16_t bf16_t;
 
@@ -52,20 +63,7 @@ __device__ __forceinline__ int xcd_block() {
     return x < r ? x * (q + 1) + t : r * (q + 1) + (x - r) * q + t;
 }
 
-template <int G, typename T> __device__ __forceinline__ T group_sum(T v) {
-#pragma unroll
-    for (int off = G >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
-    return v;
-}
-
-// ── row functors: list position -> row index ────────────────────────────────
-struct RowId { __device__ int operator()(int i) const { return i; } };
-struct RowList {
-    const int* list;
-    __device__ int operator()(int i) const { return list[i]; }
-};
-
-// ── x functors: value of the vector the row multiplies ──────────────────────
+// ── x functors: the vector entry a product multiplies ───────────────────────
 template <typename TX> struct XVec {
     const TX* x;
     __device__ TX operator()(int j) const { return x[j]; }
@@ -77,87 +75,153 @@ template <typename TX> struct XZeroJacobi {
     __device__ TX operator()(int j) const { return (w * d[j]) * b[j]; }
 };
 
-// ── epilogues ────────────────────────────────────────────────────────────────
-template <typename TX> struct EpiStore {          // y = A x
+// ── epilogues: seg = the row's products in row order (LDS), or a total when a
+//    long row was reduced by the whole block ───────────────────────────────
+template <typename TX> __device__ __forceinline__ TX seq_sum(const TX* seg, int len) {
+    TX s = 0;
+    for (int k = 0; k < len; ++k) s += seg[k];
+    return s;
+}
+
+template <typename TX> struct EpiStore {          // y = A x   (src/csr_matrix.jl:150-159)
     TX* y;
-    __device__ void operator()(int i, TX s) const { y[i] = s; }
+    __device__ void operator()(int i, const TX* seg, int len) const { y[i] = seq_sum(seg, len); }
+    __device__ void total(int i, TX s) const { y[i] = s; }
 };
-template <typename TX> struct EpiResid {          // r = b - A x
+template <typename TX> struct EpiResid {          // r = b - A x   (src/cycle.jl:32-43)
     const TX* b; TX* r;
-    __device__ void operator()(int i, TX s) const { r[i] = b[i] - s; }
+    __device__ void operator()(int i, const TX* seg, int len) const { r[i] = b[i] - seq_sum(seg, len); }
+    __device__ void total(int i, TX s) const { r[i] = b[i] - s; }
 };
-template <typename TX> struct EpiJacobi {         // xo = x + w d (b - A x)
+template <typename TX> struct EpiJacobi {         // xo = x + w d (b - A x)   (src/smoothers.jl:78-94)
     const TX* x; const TX* b; const TX* d; TX* xo; TX w;
-    __device__ void operator()(int i, TX s) const { xo[i] = x[i] + (w * d[i]) * (b[i] - s); }
+    __device__ void operator()(int i, const TX* seg, int len) const {
+        TX r = b[i];
+        for (int k = 0; k < len; ++k) r -= seg[k];  // residual by successive subtraction
+        xo[i] = x[i] + (w * d[i]) * r;
+    }
+    __device__ void total(int i, TX s) const { xo[i] = x[i] + (w * d[i]) * (b[i] - s); }
 };
 template <typename TX> struct EpiZeroJacobiResid {  // x1 = w d b, r = b - A x1
     const TX* b; const TX* d; TX* x1; TX* r; TX w;
-    __device__ void operator()(int i, TX s) const {
+    __device__ void operator()(int i, const TX* seg, int len) const {
+        x1[i] = (w * d[i]) * b[i];
+        r[i] = b[i] - seq_sum(seg, len);
+    }
+    __device__ void total(int i, TX s) const {
         x1[i] = (w * d[i]) * b[i];
         r[i] = b[i] - s;
     }
 };
-template <typename TX> struct EpiGS {             // x_i += d_i (b_i - A_i x), in place
-    TX* x; const TX* b; const TX* d;
-    __device__ void operator()(int i, TX s) const { x[i] += d[i] * (b[i] - s); }
-};
-template <typename TX> struct EpiAdd {            // x += P xc
+template <typename TX> struct EpiAdd {            // x += P xc, one product at a time (src/prolongation.jl:750-759)
     TX* x;
-    __device__ void operator()(int i, TX s) const { x[i] += s; }
+    __device__ void operator()(int i, const TX* seg, int len) const {
+        TX v = x[i];
+        for (int k = 0; k < len; ++k) v += seg[k];
+        x[i] = v;
+    }
+    __device__ void total(int i, TX s) const { x[i] += s; }
 };
 
-template <int G, bool XCD, typename TV, typename TX, class RowF, class XF, class EPI>
-__global__ __launch_bounds__(kBlock) void k_csr_rows(int nitems, const int* __restrict__ rp,
-                                                     const int* __restrict__ cv, const TV* __restrict__ av,
-                                                     RowF rowf, XF xf, EPI epi) {
-    const int b = XCD ? xcd_block() : (int)blockIdx.x;
-    const int t = b * kBlock + (int)threadIdx.x;
-    const int item = t / G, lane = t % G;
-    if (item >= nitems) return;
-    const int row = rowf(item);
-    const int s = rp[row], e = rp[row + 1];
-    TX acc = 0;
-    for (int k = s + lane; k < e; k += G) acc += ld<TX, TV>(av, k) * xf(ldi(cv, k));
-    acc = group_sum<G>(acc);
-    if (lane == 0) epi(row, acc);
-}
-
-template <int G, typename TV, typename TX, class RowF, class XF, class EPI>
-void launch_rows_g(int nitems, const int* rp, const int* cv, const TV* av, RowF rowf, XF xf, EPI epi,
-                   hipStream_t st) {
-    if (nitems <= 0) return;
-    const int64_t threads = (int64_t)nitems * G;
-    const int nb = (int)((threads + kBlock - 1) / kBlock);
-    if (nb >= 64)
-        hipLaunchKernelGGL((k_csr_rows<G, true, TV, TX, RowF, XF, EPI>), dim3(nb), dim3(kBlock), 0, st,
-                           nitems, rp, cv, av, rowf, xf, epi);
-    else
-        hipLaunchKernelGGL((k_csr_rows<G, false, TV, TX, RowF, XF, EPI>), dim3(nb), dim3(kBlock), 0, st,
-                           nitems, rp, cv, av, rowf, xf, epi);
-}
-
-template <typename TV, typename TX, class RowF, class XF, class EPI>
-void launch_rows(int G, int nitems, const int* rp, const int* cv, const TV* av, RowF rowf, XF xf, EPI epi,
-                 hipStream_t st) {
-    switch (G) {
-        case 1: launch_rows_g<1, TV, TX>(nitems, rp, cv, av, rowf, xf, epi, st); break;
-        case 2: launch_rows_g<2, TV, TX>(nitems, rp, cv, av, rowf, xf, epi, st); break;
-        case 4: launch_rows_g<4, TV, TX>(nitems, rp, cv, av, rowf, xf, epi, st); break;
-        case 8: launch_rows_g<8, TV, TX>(nitems, rp, cv, av, rowf, xf, epi, st); break;
-        case 16: launch_rows_g<16, TV, TX>(nitems, rp, cv, av, rowf, xf, epi, st); break;
-        case 32: launch_rows_g<32, TV, TX>(nitems, rp, cv, av, rowf, xf, epi, st); break;
-        default: launch_rows_g<64, TV, TX>(nitems, rp, cv, av, rowf, xf, epi, st); break;
+// One row block: stage the products, then call segf(row, seg, len) on the
+// row's thread, or totf(row, sum) on thread 0 for a long row.
+template <typename TV, typename TX, class XF, class SEGF, class TOTF>
+__device__ __forceinline__ void stream_block(int b, const int* __restrict__ blk, const int* __restrict__ rp,
+                                             const int* __restrict__ cv, const TV* __restrict__ av, XF xf,
+                                             TX* prod, SEGF segf, TOTF totf) {
+    const int r0 = blk[b], r1 = blk[b + 1];
+    const int z0 = rp[r0], z1 = rp[r1];
+    const int nz = z1 - z0;
+    if (nz <= kStreamNnz) {
+#pragma unroll 4
+        for (int k = threadIdx.x; k < nz; k += kBlock) prod[k] = ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
+        __syncthreads();
+        const int row = r0 + (int)threadIdx.x;
+        if (row < r1) {
+            const int s = rp[row] - z0;
+            segf(row, prod + s, rp[row + 1] - z0 - s);
+        }
+    } else {
+        TX acc = 0;
+        for (int k = threadIdx.x; k < nz; k += kBlock) acc += ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
+        prod[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) prod[threadIdx.x] += prod[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) totf(r0, prod[0]);
     }
 }
 
-// Lanes per row from the mean row length: the smallest power of two that
-// covers the mean, capped at a wavefront.
-inline int pick_group(int64_t nnz, int64_t nrows) {
-    if (nrows <= 0) return 1;
-    const double mean = (double)nnz / (double)nrows;
-    int g = 1;
-    while (g < 64 && g < mean) g <<= 1;
-    return g;
+template <bool XCD, typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_csr_stream(const int* __restrict__ blk, const int* __restrict__ rp,
+                                                       const int* __restrict__ cv, const TV* __restrict__ av,
+                                                       XF xf, EPI epi) {
+    __shared__ TX prod[kStreamNnz];
+    const int b = XCD ? xcd_block() : (int)blockIdx.x;
+    stream_block<TV, TX>(
+        b, blk, rp, cv, av, xf, prod, [&](int row, const TX* seg, int len) { epi(row, seg, len); },
+        [&](int row, TX s) { epi.total(row, s); });
+}
+
+template <typename TV, typename TX, class XF, class EPI>
+void launch_stream(int nblk, const int* blk, const int* rp, const int* cv, const TV* av, XF xf, EPI epi,
+                   hipStream_t st) {
+    if (nblk <= 0) return;
+    if (nblk >= 64)
+        hipLaunchKernelGGL((k_csr_stream<true, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk, rp, cv, av,
+                           xf, epi);
+    else
+        hipLaunchKernelGGL((k_csr_stream<false, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk, rp, cv,
+                           av, xf, epi);
+}
+
+// Nonzeros per row block: kStreamNnz for large operators; halved (down to
+// 256) while the operator would give fewer than ~8 blocks per CU, so that
+// small coarse levels still spread over the chip and each block's serial
+// phases stay short.
+inline int stream_budget(int64_t nnz) {
+    int budget = kStreamNnz;
+    while (budget > 256 && nnz / budget < 2048) budget >>= 1;
+    return budget;
+}
+
+// Row blocks of at most kStreamRows rows and `budget` nonzeros (a longer row
+// alone in its block).  Returns offsets: block b = rows [o[b], o[b+1]).
+inline std::vector<int32_t> build_row_blocks(const std::vector<int32_t>& rp, int budget = kStreamNnz) {
+    const int n = (int)rp.size() - 1;
+    std::vector<int32_t> o;
+    o.push_back(0);
+    int64_t acc = 0;
+    int rows = 0;
+    for (int i = 0; i < n; ++i) {
+        const int len = rp[i + 1] - rp[i];
+        if (rows > 0 && (acc + len > budget || rows == kStreamRows)) {
+            o.push_back(i);
+            acc = 0;
+            rows = 0;
+        }
+        acc += len;
+        ++rows;
+    }
+    if (n > 0) o.push_back(n);
+    return o;
+}
+
+// Coloured Gauss-Seidel update of the rows in one colour list, in place
+// (src/smoothers.jl:203-217): one thread per row, residual by successive
+// subtraction in row order.  Rows of one colour do not couple.
+template <typename TV, typename TX>
+__global__ __launch_bounds__(kBlock) void k_gs_color(int cnt, const int* __restrict__ list, const int* __restrict__ rp,
+                                                     const int* __restrict__ cv, const TV* __restrict__ av, TX* x,
+                                                     const TX* __restrict__ b, const TX* __restrict__ d) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= cnt) return;
+    const int i = list[t];
+    TX r = b[i];
+    for (int k = rp[i]; k < rp[i + 1]; ++k) r -= ld<TX, TV>(av, k) * x[cv[k]];
+    x[i] += d[i] * r;
 }
 
 }  // namespace dev

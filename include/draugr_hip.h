@@ -127,7 +127,7 @@ int32_t draugr_hip_level_op(int32_t handle, int32_t level, int32_t op, int32_t p
                             const void *in2, void *out, void *out2, void *stream);
 
 /* Device-side FP64 values to the host: which 0 A.nzval, 1 smoother inverse
- * diagonal, 2 coarse inverse (col-major), 3 coarsest operator values. */
+ * diagonal, 2 coarse LU factors (col-major), 3 coarsest operator values. */
 int32_t draugr_hip_export_device_values(int32_t handle, int32_t level, int32_t which, double *dst);
 
 /* Algorithmic bytes of one cycle from a zero guess under the given config,

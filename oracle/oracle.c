@@ -175,7 +175,10 @@ int32_t o_lu(int32_t n, double *a, int32_t *piv) {
     return 0;
 }
 
-/* x = A^-1 b from the LU factors (getrs). */
+/* x = A^-1 b from the LU factors, in the loop order of reference LAPACK
+ * getrs for one right-hand side: row interchanges (laswp), then the
+ * column-oriented unit-lower and upper solves of reference BLAS trsm
+ * ('L','L','N','U' and 'L','U','N','N'). */
 void o_lu_solve(int32_t n, const double *lu, const int32_t *piv, const double *b, double *x) {
     if (x != b) memcpy(x, b, (size_t)n * sizeof(double));
     for (int32_t k = 0; k < n; ++k) {
@@ -183,11 +186,11 @@ void o_lu_solve(int32_t n, const double *lu, const int32_t *piv, const double *b
         x[k] = x[piv[k]];
         x[piv[k]] = t;
     }
-    for (int32_t r = 0; r < n; ++r)
-        for (int32_t k = 0; k < r; ++k) x[r] -= lu[(size_t)k * n + r] * x[k];
-    for (int32_t r = n - 1; r >= 0; --r) {
-        for (int32_t k = r + 1; k < n; ++k) x[r] -= lu[(size_t)k * n + r] * x[k];
-        x[r] /= lu[(size_t)r * n + r];
+    for (int32_t j = 0; j < n; ++j)
+        for (int32_t i = j + 1; i < n; ++i) x[i] -= lu[(size_t)j * n + i] * x[j];
+    for (int32_t j = n - 1; j >= 0; --j) {
+        x[j] /= lu[(size_t)j * n + j];
+        for (int32_t i = 0; i < j; ++i) x[i] -= lu[(size_t)j * n + i] * x[j];
     }
 }
 

@@ -2,10 +2,12 @@
 FP64 / mixed-precision PCG and the partial resetup against the FP64 CPU
 oracle, on the same hierarchy and the same seeded inputs.
 
-Tolerances (relative 2-norm of the difference):
-  FP64 single kernels 1e-13 (summation order differs: row groups vs. the
-  reference's sequential row sums), FP64 cycle 1e-12, FP64 solutions 1e-10,
-  FP32 kernels 1e-5, BF16-value kernels 1e-2 (8-bit mantissa storage).
+Bar: FP64 kernels, cycles and resetup values are bit-identical to the oracle
+(both sum every row in the reference's sequential order, no FMA contraction).
+Solvers whose dot products are reduced in a different order (PCG,
+stationary norms) agree to 1e-10 relative with iteration counts within 1.
+FP32 kernels: 1e-5 relative 2-norm; BF16-value kernels: 1e-2 (8-bit
+mantissa storage).
 """
 import numpy as np
 import pytest
@@ -54,7 +56,18 @@ def _built(dh, orc, name):
     return _cache[name]
 
 
-TOL = {"fp64": 1e-13, "fp32": 1e-5, "bf16": 1e-2}
+TOL = {"fp32": 1e-5, "bf16": 1e-2}
+
+
+def same(got, ref, prec="fp64"):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    if prec == "fp64":
+        bad = np.flatnonzero(got != ref)
+        assert bad.size == 0, "%d of %d entries differ, first at %d: %r vs %r (rel err %.3e)" % (
+            bad.size, ref.size, bad[0], got[bad[0]], ref[bad[0]], rel_err(got, ref))
+    else:
+        assert rel_err(got, ref) < TOL[prec], rel_err(got, ref)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
@@ -63,7 +76,6 @@ def test_level_kernels(dh, orc, torch_cuda, name, prec):
     torch = torch_cuda
     A, cfg, sm, h, oh = _built(dh, orc, name)
     tdt = torch.float64 if prec == "fp64" else torch.float32
-    tol = TOL[prec]
     w = cfg.jacobi_omega
     rng = np.random.default_rng(SEED)
     for l in range(h.nlevels):
@@ -76,40 +88,43 @@ def test_level_kernels(dh, orc, torch_cuda, name, prec):
         rp, cv, nz = lv["A_rp"], lv["A_cv"], lv["A_nz"]
         print("case", name, prec, "level", l, "n", n, flush=True)
         h.level_op(l, 0, prec, dev(x), None, out)
-        assert rel_err(out.cpu().numpy(), orc.spmv(rp, cv, nz, x)) < tol
+        same(out.cpu().numpy(), orc.spmv(rp, cv, nz, x), prec)
         h.level_op(l, 1, prec, dev(x), dev(b), out)
-        assert rel_err(out.cpu().numpy(), orc.residual(rp, cv, nz, x, b)) < tol
+        same(out.cpu().numpy(), orc.residual(rp, cv, nz, x, b), prec)
         if sm in ("jacobi", "l1_jacobi"):
             h.level_op(l, 2, prec, dev(x), dev(b), out)
-            assert rel_err(out.cpu().numpy(), orc.jacobi_step(rp, cv, nz, x, b, d, w)) < tol
+            same(out.cpu().numpy(), orc.jacobi_step(rp, cv, nz, x, b, d, w), prec)
             h.level_op(l, 3, prec, None, dev(b), out, out2)
             x1 = orc.jacobi_step(rp, cv, nz, np.zeros(n), b, d, w)
-            assert rel_err(out.cpu().numpy(), x1) < tol
-            assert rel_err(out2.cpu().numpy(), orc.residual(rp, cv, nz, x1, b)) < tol
+            same(out.cpu().numpy(), x1, prec)
+            same(out2.cpu().numpy(), orc.residual(rp, cv, nz, x1, b), prec)
         else:
             xg = dev(x)
             h.level_op(l, 6, prec, None, dev(b), xg)
             ref = orc.gs_sweep(rp, cv, nz, x, b, d, lv["c_off"], lv["c_order"])
-            assert rel_err(xg.cpu().numpy(), ref) < tol
+            same(xg.cpu().numpy(), ref, prec)
         bc = torch.zeros(nc, dtype=tdt, device="cuda")
         h.level_op(l, 4, prec, dev(x), None, bc)
-        assert rel_err(bc.cpu().numpy(), orc.restrict(lv["Pt_off"], lv["Pt_rows"], lv["Pt_pnz"], lv["P_nz"], x)) < tol
+        same(bc.cpu().numpy(), orc.restrict(lv["Pt_off"], lv["Pt_rows"], lv["Pt_pnz"], lv["P_nz"], x), prec)
         xo = dev(x)
         h.level_op(l, 5, prec, dev(xc), None, xo)
-        assert rel_err(xo.cpu().numpy(), orc.prolong(lv["P_rp"], lv["P_cv"], lv["P_nz"], xc, x)) < tol
+        same(xo.cpu().numpy(), orc.prolong(lv["P_rp"], lv["P_cv"], lv["P_nz"], xc, x), prec)
     if prec == "fp64":
         for l in range(h.nlevels):
             n = oh.levels[l]["A_rp"].shape[0] - 1
             dd = torch.zeros(n, dtype=torch.float64, device="cuda")
             h.level_op(l, 9, 0, None, None, dd)
-            assert rel_err(dd.cpu().numpy(), oh.dinv(l)) < 1e-15
+            same(dd.cpu().numpy(), oh.dinv(l))
     # coarse direct solve
     c = h.coarsest()
     bc = rng.standard_normal(c["n"])
     out = torch.zeros(c["n"], dtype=tdt, device="cuda")
     h.level_op(0, 7, prec, torch.tensor(bc, dtype=tdt, device="cuda"), None, out)
-    ref = np.linalg.solve(c["dense"], bc)
-    assert rel_err(out.cpu().numpy(), ref) < (1e-11 if prec == "fp64" else 1e-3)
+    # pinned twice: the oracle's LAPACK-order LU solve (bit for bit in FP64) and
+    # numpy's LAPACK solve (to rounding)
+    assert rel_err(out.cpu().numpy(), np.linalg.solve(c["dense"], bc)) < (1e-11 if prec == "fp64" else 1e-3)
+    if prec == "fp64":
+        same(out.cpu().numpy(), orc.lu_solve_dense(c["dense"], bc))
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -121,10 +136,9 @@ def test_galerkin_matches_oracle(dh, orc, torch_cuda, name):
         nxt = h.level_info(l + 1)["nnz"] if l + 1 < h.nlevels else h.coarsest()["nnz"]
         h.level_op(l, 8, 0)
         got = h.device_values(l + 1, 0, nxt) if l + 1 < h.nlevels else h.device_values(0, 3, nxt)
-        assert rel_err(got, ref) < 1e-13
+        same(got, ref)
         # and the setup-time values are the same sums in the same order
-        setup_vals = oh.level_values(l + 1)
-        assert rel_err(setup_vals, ref) < 1e-14
+        same(oh.level_values(l + 1), ref)
 
 
 @pytest.mark.parametrize("cycle", ["v", "w"])
@@ -137,8 +151,7 @@ def test_cycle_fp64(dh, orc, name, cycle):
     x0, b = rng.standard_normal(A.n), rng.standard_normal(A.n)
     x = x0.copy()
     dh.amg_cycle_(x, b, h, c2)
-    ref = oh.cycle(x0, b)
-    assert rel_err(x, ref) < 1e-12
+    same(x, oh.cycle(x0, b))
     oh.set_options(cycle="v")
 
 
@@ -152,12 +165,12 @@ def test_cycle_pre_post_steps(dh, orc, steps):
     x0, b = rng.standard_normal(A.n), rng.standard_normal(A.n)
     x = x0.copy()
     dh.amg_cycle_(x, b, h, c2)
-    assert rel_err(x, oh.cycle(x0, b)) < 1e-12
+    same(x, oh.cycle(x0, b))
     # preconditioner form (zero guess) on device vectors
     import torch
     xd = torch.zeros(A.n, dtype=torch.float64, device="cuda")
     dh.amg_cycle_(xd, torch.tensor(b, device="cuda"), h, c2, zero_guess=True)
-    assert rel_err(xd.cpu().numpy(), oh.cycle(np.zeros(A.n), b)) < 1e-12
+    same(xd.cpu().numpy(), oh.cycle(np.zeros(A.n), b))
     oh.set_options(pre=1, post=1)
 
 
@@ -225,24 +238,21 @@ def test_resetup_matches_oracle(dh, orc, torch_cuda, name):
     dh.amg_resetup_(h, dh.problems.CSR(A.rowptr, A.colval, new), cfg)
     oh.resetup(new)
     for l in range(1, h.nlevels):
-        got = h.device_values(l, 0, h.level_info(l)["nnz"])
-        assert rel_err(got, oh.level_values(l)) < 1e-13
-    got = h.device_values(0, 3, h.coarsest()["nnz"])
-    assert rel_err(got, oh.coarse["nz"]) < 1e-13
+        same(h.device_values(l, 0, h.level_info(l)["nnz"]), oh.level_values(l))
+    same(h.device_values(0, 3, h.coarsest()["nnz"]), oh.coarse["nz"])
     for l in range(h.nlevels):
-        got = h.device_values(l, 1, h.level_info(l)["n"])
-        assert rel_err(got, oh.dinv(l)) < 1e-13
+        same(h.device_values(l, 1, h.level_info(l)["n"]), oh.dinv(l))
     b = rng.standard_normal(A.n)
     x = np.zeros(A.n)
     dh.amg_cycle_(x, b, h, cfg)
-    assert rel_err(x, oh.cycle(np.zeros(A.n), b)) < 1e-12
+    same(x, oh.cycle(np.zeros(A.n), b))
     # resetup from device values
     new2 = A.nzval * 2.0
     dh.amg_resetup_(h, torch.tensor(new2, device="cuda"), cfg)
     oh.resetup(new2)
     x = np.zeros(A.n)
     dh.amg_cycle_(x, b, h, cfg)
-    assert rel_err(x, oh.cycle(np.zeros(A.n), b)) < 1e-12
+    same(x, oh.cycle(np.zeros(A.n), b))
 
 
 def test_trivial_systems(dh):
