@@ -1,0 +1,71 @@
+"""Independent dense restatement of the AMG cycle for tiny problems (numpy).
+
+A second, deliberately different implementation of the cycle semantics of
+src/cycle.jl:63-99 (matrices as dense arrays, smoothing as matrix formulas,
+coarse solve by numpy.linalg.solve), used to pin the C oracle: a dropped term,
+a wrong sign or a transposed operand in either shows up as a mismatch.
+"""
+import numpy as np
+
+
+def dense(rp, cv, nz, ncol=None):
+    n = len(rp) - 1
+    M = np.zeros((n, ncol if ncol is not None else n))
+    for i in range(n):
+        for k in range(rp[i], rp[i + 1]):
+            M[i, cv[k]] += nz[k]
+    return M
+
+
+def invdiag(A, l1):
+    if l1:
+        s = np.abs(A).sum(axis=1)
+        return np.where(s > np.finfo(float).eps, 1.0 / np.where(s == 0, 1, s), 0.0)
+    d = np.diag(A).copy()
+    thr = np.finfo(float).eps * np.maximum(1.0, np.abs(A).sum(axis=1))
+    return np.where(np.abs(d) > thr, 1.0 / np.where(d == 0, 1, d), 0.0)
+
+
+class DenseAMG:
+    def __init__(self, levels, coarse, smoother, omega=2.0 / 3.0, cycle="v", pre=1, post=1):
+        self.A = [dense(l["A_rp"], l["A_cv"], l["A_nz"]) for l in levels]
+        self.P = [dense(l["P_rp"], l["P_cv"], l["P_nz"], len(l["Pt_off"]) - 1) for l in levels]
+        self.colors = [(l.get("c_off"), l.get("c_order")) for l in levels]
+        self.Ac = dense(coarse["rp"], coarse["cv"], coarse["nz"])
+        self.smoother = smoother
+        self.l1 = smoother.startswith("l1")
+        self.D = [invdiag(A, self.l1) for A in self.A]
+        self.omega, self.cycle_type, self.pre, self.post = omega, cycle, pre, post
+
+    def smooth(self, l, x, b, steps):
+        A, D = self.A[l], self.D[l]
+        if "gs" in self.smoother:
+            off, order = self.colors[l]
+            for _ in range(steps):
+                for c in range(len(off) - 1):
+                    rows = order[off[c]:off[c + 1]]
+                    r = b[rows] - A[rows] @ x
+                    x = x.copy()
+                    x[rows] += D[rows] * r
+            return x
+        for _ in range(steps):
+            x = x + self.omega * D * (b - A @ x)
+        return x
+
+    def descend(self, l, x, b):
+        x = self.smooth(l, x, b, self.pre)
+        r = b - self.A[l] @ x
+        bc = self.P[l].T @ r
+        xc = np.zeros(len(bc))
+        if l + 1 < len(self.A):
+            for _ in range(2 if self.cycle_type == "w" else 1):
+                xc = self.descend(l + 1, xc, bc)
+        else:
+            xc = np.linalg.solve(self.Ac, bc)
+        x = x + self.P[l] @ xc
+        return self.smooth(l, x, b, self.post)
+
+    def cycle(self, x, b):
+        if not self.A:
+            return np.linalg.solve(self.Ac, b)
+        return self.descend(0, np.array(x, dtype=float), np.asarray(b, dtype=float))

@@ -1,0 +1,244 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker:
+known answers from the reference's own tests (test/runtests.jl), scipy /
+numpy results, an independent dense restatement of the cycle
+(tests/dense_ref.py), and invariants the mathematics fixes.  No GPU needed."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import oracle_of, rel_err
+from dense_ref import DenseAMG
+
+SEED = 20261202
+
+
+def _sp(A):
+    return sp.csr_matrix((A.nzval, A.colval, A.rowptr), shape=(A.n, A.n))
+
+
+def _hier(dh, A, cfg):
+    return dh.amg_setup(A, cfg, upload=False)
+
+
+# ── known answers of the reference tests ────────────────────────────────────
+def test_spmv_known_answer(dh, orc):
+    """A*1 for 1-D Poisson is [1, 0, ..., 0, 1] (test/runtests.jl:52-94)."""
+    A = dh.problems.poisson1d(10)
+    y = orc.spmv(A.rowptr, A.colval, A.nzval, np.ones(10))
+    assert np.array_equal(y, np.r_[1.0, np.zeros(8), 1.0])
+
+
+def test_invdiag_known_answers(dh, orc):
+    A = dh.problems.poisson1d(10)
+    # Jacobi: 1/2 everywhere (test/runtests.jl:272-284)
+    assert np.array_equal(orc.invdiag(A.rowptr, A.colval, A.nzval, False), np.full(10, 0.5))
+    # l1: interior 1/4, boundary 1/3 (test/runtests.jl:696-706, 1857-1865)
+    d = orc.invdiag(A.rowptr, A.colval, A.nzval, True)
+    assert d[4] == 0.25 and d[0] == 1.0 / 3.0 and d[9] == 1.0 / 3.0
+
+
+def test_invdiag_safe_for_tiny_diagonal(dh, orc):
+    """Row with a 1e-20 diagonal gets 0, never Inf (test/runtests.jl:1784-1796)."""
+    rp = np.array([0, 2, 5, 7], np.int32)
+    cv = np.array([0, 1, 0, 1, 2, 1, 2], np.int32)
+    nz = np.array([2.0, -1.0, -1.0, 1e-20, -1.0, -1.0, 2.0])
+    d = orc.invdiag(rp, cv, nz, False)
+    assert np.all(np.isfinite(d)) and d[1] == 0.0 and d[0] == 0.5
+
+
+def test_trivial_system_known_answer(dh, orc):
+    """5 x = 3 -> 0.6 (test/runtests.jl:1798-1815)."""
+    A = dh.problems.CSR(np.array([0, 1], np.int32), np.array([0], np.int32), np.array([5.0]))
+    h = _hier(dh, A, dh.AMGConfig(max_coarse_size=10))
+    oh = oracle_of(h, orc, "jacobi")
+    x, it, _ = oh.solve(np.zeros(1), np.array([3.0]), 1e-10)
+    assert abs(x[0] - 0.6) < 1e-15 and it == 1
+
+
+# ── single kernels against scipy / numpy ────────────────────────────────────
+@pytest.fixture(scope="module")
+def agg_case(dh):
+    A = dh.problems.poisson2d(12)
+    cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType())
+    return A, cfg, _hier(dh, A, cfg)
+
+
+@pytest.fixture(scope="module")
+def hypre_case(dh):
+    P = dh.problems
+    A = P.heterogeneous3d(8, P.log_permeability(8, 1.5, SEED))
+    cfg = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())
+    return A, cfg, _hier(dh, A, cfg)
+
+
+def test_kernels_vs_scipy(dh, orc, agg_case, hypre_case):
+    rng = np.random.default_rng(SEED)
+    for A, cfg, h in (agg_case, hypre_case):
+        for l in range(h.nlevels):
+            lv = h.level(l)
+            As = sp.csr_matrix((lv["A_nz"], lv["A_cv"], lv["A_rp"]))
+            Ps = sp.csr_matrix((lv["P_nz"], lv["P_cv"], lv["P_rp"]), shape=(lv["n"], lv["n_coarse"]))
+            x, b, xc = rng.standard_normal(lv["n"]), rng.standard_normal(lv["n"]), rng.standard_normal(lv["n_coarse"])
+            assert rel_err(orc.spmv(lv["A_rp"], lv["A_cv"], lv["A_nz"], x), As @ x) < 1e-14
+            assert rel_err(orc.residual(lv["A_rp"], lv["A_cv"], lv["A_nz"], x, b), b - As @ x) < 1e-14
+            d = orc.invdiag(lv["A_rp"], lv["A_cv"], lv["A_nz"], True)
+            assert rel_err(d, 1.0 / abs(As).sum(axis=1).A1) < 1e-15
+            w = 2.0 / 3.0
+            assert rel_err(orc.jacobi_step(lv["A_rp"], lv["A_cv"], lv["A_nz"], x, b, d, w),
+                           x + w * d * (b - As @ x)) < 1e-14
+            assert rel_err(orc.restrict(lv["Pt_off"], lv["Pt_rows"], lv["Pt_pnz"], lv["P_nz"], x), Ps.T @ x) < 1e-14
+            assert rel_err(orc.prolong(lv["P_rp"], lv["P_cv"], lv["P_nz"], xc, x), x + Ps @ xc) < 1e-14
+
+
+def test_galerkin_vs_scipy_and_pattern(dh, orc, agg_case, hypre_case):
+    """Numeric RAP vs explicit P^T A P (test/runtests.jl:181-270); the coarse
+    pattern is the structural pattern of |P|^T |A| |P| (structural zeros kept)."""
+    for A, cfg, h in (agg_case, hypre_case):
+        for l in range(h.nlevels):
+            lv = h.level(l)
+            As = sp.csr_matrix((lv["A_nz"], lv["A_cv"], lv["A_rp"]))
+            Ps = sp.csr_matrix((lv["P_nz"], lv["P_cv"], lv["P_rp"]), shape=(lv["n"], lv["n_coarse"]))
+            vals = orc.galerkin(lv["R_off"], lv["R_pi"], lv["R_ai"], lv["R_pj"], lv["P_nz"], lv["A_nz"])
+            if l + 1 < h.nlevels:
+                nxt = h.level(l + 1)
+                rp, cv = nxt["A_rp"], nxt["A_cv"]
+            else:
+                c = h.coarsest()
+                rp, cv = c["rp"], c["cv"]
+            Ac = sp.csr_matrix((vals, cv, rp), shape=(lv["n_coarse"], lv["n_coarse"]))
+            ref = (Ps.T @ As @ Ps).toarray()
+            assert np.abs(Ac.toarray() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+            B = abs(Ps).T @ abs(As) @ abs(Ps)  # positive terms: no cancellation
+            B.sort_indices()
+            assert np.array_equal(B.indptr, rp) and np.array_equal(B.indices, cv)
+
+
+def test_lu_solve_vs_numpy(orc):
+    rng = np.random.default_rng(SEED)
+    for n in (1, 2, 7, 50):
+        M = rng.standard_normal((n, n)) + n * np.eye(n)
+        b = rng.standard_normal(n)
+        assert rel_err(orc.lu_solve_dense(M, b), np.linalg.solve(M, b)) < 1e-13
+    with pytest.raises(ZeroDivisionError):
+        orc.lu_solve_dense(np.zeros((3, 3)), np.ones(3))
+
+
+# ── the cycle against the independent dense restatement ──────────────────────
+@pytest.mark.parametrize("smoother,cycle,pre,post", [
+    ("jacobi", "v", 1, 1), ("jacobi", "w", 2, 1), ("l1_jacobi", "v", 1, 1), ("l1_jacobi", "v", 0, 2),
+    ("colored_gs", "v", 1, 1), ("l1_colored_gs", "w", 1, 2)])
+def test_cycle_vs_dense(dh, orc, smoother, cycle, pre, post):
+    P = dh.problems
+    A = P.heterogeneous3d(7, P.log_permeability(7, 1.0, SEED + 4))
+    sm = {"jacobi": dh.JacobiSmootherType(), "l1_jacobi": dh.L1JacobiSmootherType(),
+          "colored_gs": dh.ColoredGaussSeidelType(), "l1_colored_gs": dh.L1ColoredGaussSeidelType()}[smoother]
+    cfg = dh.AMGConfig(coarsening=dh.HMISCoarsening(0.25, dh.ExtendedIInterpolation(0.3)), smoother=sm,
+                       max_coarse_size=20)
+    h = _hier(dh, A, cfg)
+    assert h.nlevels >= 2
+    oh = oracle_of(h, orc, smoother, cycle=cycle, pre=pre, post=post)
+    levels = [h.level(l) for l in range(h.nlevels)]
+    ref = DenseAMG(levels, h.coarsest(), smoother, cycle=cycle, pre=pre, post=post)
+    rng = np.random.default_rng(SEED)
+    x0, b = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    assert rel_err(oh.cycle(x0, b), ref.cycle(x0, b)) < 1e-12
+    assert rel_err(oh.cycle(np.zeros(A.n), b), ref.cycle(np.zeros(A.n), b)) < 1e-12
+
+
+def test_dense_check_catches_planted_mistakes(dh, orc, hypre_case):
+    """The comparison above is sensitive: a restatement with a wrong sign, a
+    dropped smoothing step, the wrong diagonal, no coarse correction or a
+    transfer operator off by 1e-6 is told apart from the oracle."""
+    A, cfg, h = hypre_case
+    oh = oracle_of(h, orc, "l1_jacobi")
+    levels = [h.level(l) for l in range(h.nlevels)]
+    rng = np.random.default_rng(SEED + 1)
+    x0, b = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    good = oh.cycle(x0, b)
+    assert rel_err(good, DenseAMG(levels, h.coarsest(), "l1_jacobi").cycle(x0, b)) < 1e-12
+    bads = [DenseAMG(levels, h.coarsest(), "l1_jacobi", omega=-2.0 / 3.0),   # wrong sign
+            DenseAMG(levels, h.coarsest(), "l1_jacobi", post=0),              # dropped step
+            DenseAMG(levels, h.coarsest(), "jacobi")]                         # wrong diagonal
+    no_coarse = DenseAMG(levels, h.coarsest(), "l1_jacobi")
+    no_coarse.Ac = 1e30 * no_coarse.Ac                                          # dropped coarse correction
+    bads.append(no_coarse)
+    perturbed = DenseAMG(levels, h.coarsest(), "l1_jacobi")
+    perturbed.P[0] = perturbed.P[0] * (1.0 + 1e-6)                              # slightly wrong transfer
+    bads.append(perturbed)
+    for bad in bads:
+        assert rel_err(good, bad.cycle(x0, b)) > 1e-9
+
+
+def test_vcycle_jacobi_is_symmetric(dh, orc, agg_case):
+    """V(1,1) with Jacobi is a symmetric operator: <u, M v> = <M u, v>."""
+    A, cfg, h = agg_case
+    oh = oracle_of(h, orc, "jacobi")
+    rng = np.random.default_rng(SEED)
+    u, v = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    Mu, Mv = oh.cycle(np.zeros(A.n), u), oh.cycle(np.zeros(A.n), v)
+    assert abs(u @ Mv - Mu @ v) < 1e-12 * abs(u @ Mv)
+
+
+# ── solvers: reference invariants and scipy ──────────────────────────────────
+def test_stationary_solve_reference_thresholds(dh, orc):
+    """Relative residual < 1e-8 in < 200 cycles on 2-D Poisson 10x10
+    (test/runtests.jl:332-344); W-cycle needs no more cycles than V
+    (test/runtests.jl:1828-1851)."""
+    A = dh.problems.poisson2d(10)
+    b = np.random.default_rng(SEED).random(A.n)
+    its = {}
+    for cyc in ("v", "w"):
+        cfg = dh.AMGConfig(cycle_type=cyc.upper(), pre_smoothing_steps=2, post_smoothing_steps=2)
+        h = _hier(dh, A, cfg)
+        oh = oracle_of(h, orc, "l1_colored_gs", cycle=cyc, pre=2, post=2)
+        x, it, rel = oh.solve(np.zeros(A.n), b, 1e-8, 200)
+        assert rel < 1e-8 and it < 200
+        assert np.linalg.norm(b - _sp(A) @ x) / np.linalg.norm(b) < 1e-8
+        its[cyc] = it
+    assert its["w"] <= its["v"]
+
+
+def test_pcg_matches_scipy_cg(dh, orc, hypre_case):
+    """The oracle PCG is textbook preconditioned CG: same iterations (+-1) and
+    solution as scipy's CG with the oracle cycle as preconditioner."""
+    A, cfg, h = hypre_case
+    oh = oracle_of(h, orc, "l1_jacobi")
+    b = dh.problems.splitmix_normal(SEED, A.n)
+    x, it, rel = oh.pcg(np.zeros(A.n), b, 1e-8, 200)
+    M = spla.LinearOperator((A.n, A.n), matvec=lambda r: oh.cycle(np.zeros(A.n), r))
+    count = [0]
+    xs, info = spla.cg(_sp(A), b, rtol=1e-8, atol=0.0, maxiter=200, M=M,
+                       callback=lambda xk: count.__setitem__(0, count[0] + 1))
+    assert info == 0 and abs(count[0] - it) <= 1
+    assert rel_err(x, xs) < 1e-7
+    assert np.linalg.norm(b - _sp(A) @ x) / np.linalg.norm(b) < 1.01e-8
+
+
+def test_resetup_vs_scipy(dh, orc, hypre_case):
+    """Partial resetup keeps P and recomputes every coarse operator as P^T A P
+    of the new values (src/resetup.jl:27-45)."""
+    A, cfg, h = hypre_case
+    oh = oracle_of(h, orc, "l1_jacobi")
+    rng = np.random.default_rng(SEED)
+    s = np.exp(0.5 * rng.standard_normal(A.n))
+    rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+    new = A.nzval * s[rows] * s[A.colval]
+    oh.resetup(new)
+    Al = sp.csr_matrix((new, A.colval, A.rowptr), shape=(A.n, A.n))
+    for l in range(h.nlevels):
+        lv = h.level(l)
+        Ps = sp.csr_matrix((lv["P_nz"], lv["P_cv"], lv["P_rp"]), shape=(lv["n"], lv["n_coarse"]))
+        Ac = (Ps.T @ Al @ Ps).tocsr()
+        Ac.sort_indices()
+        got = oh.level_values(l + 1)
+        if l + 1 < h.nlevels:
+            pat = h.level(l + 1)
+            rp, cv = pat["A_rp"], pat["A_cv"]
+        else:
+            c = h.coarsest()
+            rp, cv = c["rp"], c["cv"]
+        G = sp.csr_matrix((got, cv, rp), shape=Ac.shape).toarray()
+        assert np.abs(G - Ac.toarray()).max() < 1e-12 * np.abs(Ac).max()
+        Al = sp.csr_matrix((got, cv, rp), shape=Ac.shape)
+        assert rel_err(oh.dinv(l), 1.0 / abs(sp.csr_matrix((oh.level_values(l), lv["A_cv"], lv["A_rp"]))).sum(axis=1).A1) < 1e-15

@@ -1,0 +1,171 @@
+"""The product's host setup (C++, draugr.jl_amd/csrc/host) against the
+oracle's pure-Python restatements (oracle/setup_ref.py) bit for bit, and
+against the setup invariants the reference tests assert.  No GPU needed."""
+import numpy as np
+import pytest
+
+from oracle import setup_ref as ref
+
+SEED = 20261202
+
+
+def _anisotropic(dh, n, kx=1e4, ky=1e-2):
+    """test/runtests.jl:1681-1694 style anisotropic 5-point operator."""
+    A = dh.problems.poisson2d(n)
+    rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+    d = np.abs(A.colval - rows)
+    nz = np.where(d == 1, -kx, np.where(d == n, -ky, 0.0))
+    diag = np.zeros(A.n)
+    np.add.at(diag, rows, -nz)
+    nz = np.where(d == 0, diag[rows] + 1e-3, nz)
+    return dh.problems.CSR(A.rowptr, A.colval, nz)
+
+
+def _random_spd(dh, n, seed):
+    rng = np.random.default_rng(seed)
+    import scipy.sparse as sp
+    M = sp.random(n, n, density=4.0 / n, random_state=seed, format="csr")
+    M = abs(M + M.T)
+    M.setdiag(0)
+    M.eliminate_zeros()
+    A = -M
+    A = A + sp.diags(np.asarray(M.sum(axis=1)).ravel() + 0.1 + rng.random(n))
+    A = A.tocsr()
+    A.sort_indices()
+    return dh.problems.CSR(A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy())
+
+
+def _mats(dh):
+    P = dh.problems
+    return {"poisson1d": P.poisson1d(30), "poisson2d": P.poisson2d(13), "aniso": _anisotropic(dh, 11),
+            "random": _random_spd(dh, 300, SEED), "hetero3d": P.heterogeneous3d(6, P.log_permeability(6, 1.5, 3))}
+
+
+@pytest.mark.parametrize("name", ["poisson1d", "poisson2d", "aniso", "random", "hetero3d"])
+def test_aggregation_matches_restatement(dh, name):
+    A = _mats(dh)[name]
+    for theta in (0.25, 0.08):
+        cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(theta), smoother=dh.JacobiSmootherType(),
+                           max_coarse_size=4)
+        h = dh.amg_setup(A, cfg, upload=False)
+        lv = h.level(0)
+        agg, nc = ref.coarsen_aggregation(A.rowptr, A.colval, A.nzval, theta)
+        assert lv["n_coarse"] == nc
+        assert np.array_equal(lv["P_cv"], np.asarray(agg, np.int32))
+        assert np.array_equal(lv["P_nz"], np.ones(A.n))
+
+
+@pytest.mark.parametrize("name", ["poisson2d", "aniso", "random", "hetero3d"])
+@pytest.mark.parametrize("config", ["aggregation", "hypre", "pmis_direct", "rs_standard"])
+def test_coarse_sparsity_transpose_map_match_restatement(dh, name, config):
+    """Coarse pattern, values (same summation order), triple map and transpose
+    map of every level equal the restatement of src/galerkin.jl:11-183 and
+    src/prolongation.jl:767-797 exactly."""
+    A = _mats(dh)[name]
+    cfg = {"aggregation": dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType(),
+                                       max_coarse_size=8),
+           "hypre": dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), max_coarse_size=8),
+           "pmis_direct": dh.AMGConfig(coarsening=dh.PMISCoarsening(0.25), smoother=dh.JacobiSmootherType(),
+                                       max_coarse_size=8),
+           "rs_standard": dh.AMGConfig(coarsening=dh.RSCoarsening(0.25, dh.StandardInterpolation(0.2)),
+                                       smoother=dh.JacobiSmootherType(), max_coarse_size=8)}[config]
+    h = dh.amg_setup(A, cfg, upload=False)
+    for l in range(h.nlevels):
+        lv = h.level(l)
+        Arp, Acv, Anz = lv["A_rp"], lv["A_cv"], lv["A_nz"]
+        P = (lv["P_rp"], lv["P_cv"], lv["P_nz"])
+        (rp, cv, vals), rm = ref.coarse_sparsity((Arp, Acv, Anz), P, lv["n_coarse"])
+        if l + 1 < h.nlevels:
+            nxt = h.level(l + 1)
+            grp, gcv, gnz = nxt["A_rp"], nxt["A_cv"], nxt["A_nz"]
+        else:
+            c = h.coarsest()
+            grp, gcv, gnz = c["rp"], c["cv"], c["nz"]
+        assert np.array_equal(grp, rp) and np.array_equal(gcv, cv)
+        assert np.array_equal(gnz, vals)
+        for k in ("R_off", "R_pi", "R_ai", "R_pj"):
+            assert np.array_equal(lv[k], rm[k]), k
+        off, rows, pnz = ref.transpose_map(P, lv["n_coarse"])
+        assert np.array_equal(lv["Pt_off"], off) and np.array_equal(lv["Pt_rows"], rows)
+        assert np.array_equal(lv["Pt_pnz"], pnz)
+
+
+@pytest.mark.parametrize("name", ["poisson2d", "random", "hetero3d"])
+def test_coloring_matches_restatement_and_is_valid(dh, name):
+    A = _mats(dh)[name]
+    cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.ColoredGaussSeidelType(),
+                       max_coarse_size=8)
+    h = dh.amg_setup(A, cfg, upload=False)
+    for l in range(h.nlevels):
+        lv = h.level(l)
+        colors, off, order = ref.greedy_coloring(lv["A_rp"], lv["A_cv"])
+        assert np.array_equal(lv["c_off"], off) and np.array_equal(lv["c_order"], order)
+        col = np.empty(lv["n"], np.int32)
+        for c in range(len(off) - 1):
+            col[order[off[c]:off[c + 1]]] = c
+        rows = np.repeat(np.arange(lv["n"]), np.diff(lv["A_rp"]))
+        offd = rows != lv["A_cv"]
+        assert not np.any(col[rows[offd]] == col[lv["A_cv"][offd]])
+
+
+def test_aggregation_prolongation_known_answers(dh, orc):
+    """P*1 = 1 and sum(P^T r) = sum(r) for aggregation (test/runtests.jl:138-179)."""
+    A = dh.problems.poisson1d(10)
+    h = dh.amg_setup(A, dh.AMGConfig(coarsening=dh.AggregationCoarsening(), smoother=dh.JacobiSmootherType(),
+                                     max_coarse_size=2), upload=False)
+    lv = h.level(0)
+    assert np.array_equal(np.diff(lv["P_rp"]), np.ones(10, np.int32))
+    xf = orc.prolong(lv["P_rp"], lv["P_cv"], lv["P_nz"], np.ones(lv["n_coarse"]), np.zeros(10))
+    assert np.array_equal(xf, np.ones(10))
+    r = np.random.default_rng(SEED).standard_normal(10)
+    bc = orc.restrict(lv["Pt_off"], lv["Pt_rows"], lv["Pt_pnz"], lv["P_nz"], r)
+    assert abs(bc.sum() - r.sum()) < 1e-13
+    assert lv["Pt_off"][-1] == 10
+
+
+def test_hmis_hierarchy_shape(dh):
+    """HMIS with ext+i on 2-D Poisson: <= 10 levels at 50x50 and a first
+    coarsening ratio < 0.6 at 20x20 (test/runtests.jl:967-987)."""
+    cfg = dh.AMGConfig(coarsening=dh.HMISCoarsening(0.5, dh.ExtendedIInterpolation()))
+    h = dh.amg_setup(dh.problems.poisson2d(50), cfg, upload=False)
+    assert h.nlevels + 1 <= 10
+    h2 = dh.amg_setup(dh.problems.poisson2d(20), cfg, upload=False)
+    assert h2.level(0)["n_coarse"] / 400 < 0.6
+
+
+@pytest.mark.parametrize("config", ["hypre", "pmis_direct", "hmis_ext"])
+def test_cf_interpolation_invariants(dh, config):
+    """C rows of P are unit rows; every row of P has an entry; ext+i rows of a
+    zero-row-sum interior sum to 1 (interpolation of constants)."""
+    P_ = dh.problems
+    A = P_.heterogeneous3d(8, np.zeros(512))  # homogeneous: interior rows sum to 0
+    cfg = {"hypre": dh.hypre_default_config(),
+           "pmis_direct": dh.AMGConfig(coarsening=dh.PMISCoarsening(0.25)),
+           "hmis_ext": dh.AMGConfig(coarsening=dh.HMISCoarsening(0.25, dh.ExtendedIInterpolation()))}[config]
+    h = dh.amg_setup(A, cfg, upload=False)
+    lv = h.level(0)
+    lens = np.diff(lv["P_rp"])
+    assert np.all(lens >= 1)
+    rows = np.repeat(np.arange(A.n), lens)
+    rowsum = np.zeros(A.n)
+    np.add.at(rowsum, rows, lv["P_nz"])
+    Arows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+    arow = np.zeros(A.n)
+    np.add.at(arow, Arows, A.nzval)
+    interior = np.abs(arow) < 1e-12
+    if config != "pmis_direct":
+        assert np.allclose(rowsum[interior], 1.0, atol=1e-12)
+    # unit rows: a row with a single entry 1.0 is a C point; there are n_coarse of them at least
+    unit = (lens == 1) & np.isclose(lv["P_nz"][lv["P_rp"][:-1]], 1.0)
+    assert unit.sum() >= lv["n_coarse"]
+
+
+def test_setup_is_deterministic_for_a_seed(dh):
+    A = dh.problems.heterogeneous3d(10, dh.problems.log_permeability(10, 1.5, 9))
+    cfg = dh.hypre_default_config()
+    a = dh.amg_setup(A, cfg, upload=False).level(0)
+    b = dh.amg_setup(A, cfg, upload=False).level(0)
+    for k in a:
+        assert np.array_equal(a[k], b[k])
+    c = dh.amg_setup(A, dh.hypre_default_config(seed=12345), upload=False).level(0)
+    assert c["n_coarse"] > 0
