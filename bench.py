@@ -106,6 +106,7 @@ def main():
     log("FP64 cycle: %.4f s/solve, iterations %s, relres %.2e" % (t64, its64, rel64))
     t32, its32, rel32, x32 = time_solves(dh, torch, h, cfg32, b, args.steps, args.warmup, args.tol)
     log("%s cycle: %.4f s/solve, iterations %s, relres %.2e" % (args.precision, t32, its32, rel32))
+    log(h.tune_report())
     xr = x64.cpu().numpy()
     x3 = x32.cpu().numpy()
     true32 = float(np.linalg.norm(b_host - A.matvec(x3)) / np.linalg.norm(b_host))

@@ -471,6 +471,64 @@ int32_t draugr_hip_cycle_bytes(int32_t handle, int32_t config_handle, double* by
     });
 }
 
+int32_t draugr_hip_assemble_7pt(int32_t handle, int32_t m, const double* k, double* nzval, void* stream) {
+    return guarded([&]() -> int32_t {
+        auto H = get_hier(handle);
+        if (!H) throw Error("Invalid hierarchy handle: " + std::to_string(handle));
+        std::lock_guard<std::mutex> g(H->mu);
+        require_device(*H);
+        const int64_t n = (int64_t)m * m * m;
+        if (m < 1 || n != H->dev->n_fine() || H->dev->finest_nnz() != 7 * n - 6 * (int64_t)m * m)
+            throw Error("DimensionMismatch: the hierarchy's finest operator is not the 7-point operator on m^3 cells");
+        if (!k || !nzval) throw Error("ArgumentError: null pointer");
+        hipStream_t st = stream_of(*H, stream);
+        H->dev->begin(st);
+        assemble_7pt(m, k, H->dev->finest_rp(), nzval, st);
+        H->dev->end(st);
+        return 0;
+    });
+}
+
+int32_t draugr_hip_matrix_grad(int32_t handle, const double* x, const double* lambda, double* grad, void* stream) {
+    return guarded([&]() -> int32_t {
+        auto H = get_hier(handle);
+        if (!H) throw Error("Invalid hierarchy handle: " + std::to_string(handle));
+        std::lock_guard<std::mutex> g(H->mu);
+        require_device(*H);
+        if (!x || !lambda || !grad) throw Error("ArgumentError: null pointer");
+        hipStream_t st = stream_of(*H, stream);
+        H->dev->begin(st);
+        matrix_grad(H->dev->n_fine(), H->dev->finest_rp(), H->dev->finest_cv(), x, lambda, grad, st);
+        H->dev->end(st);
+        return 0;
+    });
+}
+
+int32_t draugr_hip_grad_logk_7pt(int32_t m, const double* k, const double* x, const double* lambda, double* grad,
+                                 void* stream) {
+    return guarded([&]() -> int32_t {
+        if (m < 0) throw Error("ArgumentError: m must be non-negative");
+        if (m > 0 && (!k || !x || !lambda || !grad)) throw Error("ArgumentError: null pointer");
+        grad_logk_7pt(m, k, x, lambda, grad, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+int32_t draugr_hip_tune_report(int32_t handle, char* buf, int32_t buflen) {
+    return guarded([&]() -> int32_t {
+        auto H = get_hier(handle);
+        if (!H) throw Error("Invalid hierarchy handle: " + std::to_string(handle));
+        require_device(*H);
+        const std::string s = H->dev->tune_report();
+        if (buf && buflen > 0) {
+            const size_t m = std::min<size_t>(s.size(), (size_t)buflen - 1);
+            std::memcpy(buf, s.data(), m);
+            buf[m] = 0;
+        }
+        return (int32_t)s.size();
+    });
+}
+
 int32_t draugr_hip_resident_bytes(int32_t handle, int64_t* bytes) {
     return guarded([&]() -> int32_t {
         auto H = get_hier(handle);

@@ -94,7 +94,11 @@ __global__ __launch_bounds__(kBlock) void k_stream_reduce(const int* __restrict_
     double dot = 0.0;
     stream_block<TV, TX>(
         xcd_block(), blk, rp, cv, av, xf, prod,
-        [&](int row, const TX* seg, int len) { dot = epi(row, seq_sum(seg, len)); },
+        [&](int row, const TX* seg, int len) {
+            TX a = 0;
+            for (int k = 0; k < len; ++k) a += seg[k];
+            dot = epi(row, a);
+        },
         [&](int row, TX s) { dot = epi(row, s); });
     block_sum_store(dot, sh, partial, blockIdx.x);
 }
@@ -195,6 +199,16 @@ __global__ __launch_bounds__(kBlock) void k_dense_lu(int n, double* __restrict__
         }
         __syncthreads();
     }
+    // the row interchanges as one permutation: (P b)[i] = b[perm[i]]
+    if (threadIdx.x == 0) {
+        int* perm = piv + n;
+        for (int i = 0; i < n; ++i) perm[i] = i;
+        for (int k = 0; k < n; ++k) {
+            const int t = perm[k];
+            perm[k] = perm[piv[k]];
+            perm[piv[k]] = t;
+        }
+    }
 }
 
 // x = A_c^-1 b from the LU factors in the order of the reference LAPACK
@@ -240,32 +254,28 @@ __global__ __launch_bounds__(kBlock) void k_coarse_lu_solve(int n, const double*
 }
 
 // The same solve for n_c <= 64 in one wavefront without barriers: lane i holds
-// row i of the factors in registers and x_i; x_j is broadcast with a shuffle
-// at step j.  Per-element operation order is that of k_coarse_lu_solve.
+// row i of the factors in registers and x_i; x_j is broadcast from lane j with
+// readlane at step j.  Per-element operation order is that of
+// k_coarse_lu_solve; the row interchanges are applied as one permutation.
+__device__ __forceinline__ double bcast_lane(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
 template <typename TX>
 __global__ __launch_bounds__(64) void k_coarse_lu_solve_wave(int n, const double* __restrict__ lu,
-                                                             const int* __restrict__ piv, const TX* __restrict__ b,
-                                                             TX* __restrict__ x) {
-    __shared__ double xs[64];
+                                                             const int* __restrict__ perm,
+                                                             const TX* __restrict__ b, TX* __restrict__ x) {
     const int i = threadIdx.x;
     double row[64];
 #pragma unroll
     for (int j = 0; j < 64; ++j) row[j] = (j < n && i < n) ? lu[(size_t)j * n + i] : 0.0;
-    if (i < n) xs[i] = (double)b[i];
-    __syncthreads();
-    if (i == 0)
-        for (int k = 0; k < n; ++k) {
-            const int p = piv[k];
-            const double t = xs[k];
-            xs[k] = xs[p];
-            xs[p] = t;
-        }
-    __syncthreads();
-    double xi = i < n ? xs[i] : 0.0;
+    double xi = i < n ? (double)b[perm[i]] : 0.0;
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
         if (j < n) {
-            const double xj = __shfl(xi, j, 64);
+            const double xj = bcast_lane(xi, j);
             if (i > j && i < n) xi -= row[j] * xj;
         }
     }
@@ -273,7 +283,7 @@ __global__ __launch_bounds__(64) void k_coarse_lu_solve_wave(int n, const double
     for (int j = 63; j >= 0; --j) {
         if (j < n) {
             if (i == j) xi /= row[j];
-            const double xj = __shfl(xi, j, 64);
+            const double xj = bcast_lane(xi, j);
             if (i < j) xi -= row[j] * xj;
         }
     }
@@ -305,10 +315,22 @@ template <typename TX> const TX* d_of(const DevLevel& l);
 template <> const double* d_of<double>(const DevLevel& l) { return l.d64.p; }
 template <> const float* d_of<float>(const DevLevel& l) { return l.d32.p; }
 
-// One CSR-stream pass of operator M.
+template <typename TV> constexpr int pidx();
+template <> constexpr int pidx<double>() { return 0; }
+template <> constexpr int pidx<float>() { return 1; }
+template <> constexpr int pidx<bf16_t>() { return 2; }
+
+template <typename TV, typename TX, class XF, class EPI>
+void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
+    if (kind == kOpRowSeq) launch_rowseq<TV, TX>(M.nrow, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
+    else launch_stream<TV, TX>(M.nblk, M.blk.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
+}
+
+// One pass of operator M with the kernel the autotuner picked for this
+// value type (CSR-stream until tuned).
 template <typename TV, typename TX, class XF, class EPI>
 void op(const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
-    launch_stream<TV, TX>(M.nblk, M.blk.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
+    op_kind<TV, TX>(M.kind[pidx<TV>()], M, xf, epi, st);
 }
 
 void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
@@ -380,7 +402,7 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
     max_blocks = std::max(max_blocks, Ac_.nblk);
     nc_ = h.n_coarse;
     clu_.alloc((size_t)nc_ * nc_);
-    flag_.alloc(1 + nc_);
+    flag_.alloc(1 + 2 * (size_t)nc_);  // [0] singular flag, [1..n] pivots, [n+1..2n] permutation
     const int64_t n0 = n0_;
     w_r_.alloc(n0);
     w_p0_.alloc(n0);
@@ -565,7 +587,7 @@ void DeviceHierarchy::smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bo
     }
     for (int s = 0; s < steps; ++s) {
         if (zero) {
-            if (fuse_resid && steps == 1) {
+            if (fuse_resid && steps == 1 && D.fuse_zero[pidx<TV>()]) {
                 op<TV, TX>(D.A, XZeroJacobi<TX>{d, b, w}, EpiZeroJacobiResid<TX>{b, d, alt, r_of<TX>(D), w}, st);
                 resid_done = true;
             } else {
@@ -590,7 +612,7 @@ void DeviceHierarchy::coarse_solve(TX* x, const TX* b, hipStream_t st) {
     const int n = nc_;
     if (n == 0) return;
     if (n <= 64) {
-        hipLaunchKernelGGL(k_coarse_lu_solve_wave<TX>, dim3(1), dim3(64), 0, st, n, clu_.p, flag_.p + 1, b, x);
+        hipLaunchKernelGGL(k_coarse_lu_solve_wave<TX>, dim3(1), dim3(64), 0, st, n, clu_.p, flag_.p + 1 + n, b, x);
         return;
     }
     const size_t full = (size_t)(n + (size_t)n * n) * sizeof(double);
@@ -598,6 +620,88 @@ void DeviceHierarchy::coarse_solve(TX* x, const TX* b, hipStream_t st) {
     const size_t lds = stage ? full : (size_t)n * sizeof(double);
     if (lds > (size_t)kCoarseLdsMax) throw Error("coarsest level too large for the device direct solve");
     hipLaunchKernelGGL(k_coarse_lu_solve<TX>, dim3(1), dim3(kBlock), lds, st, n, clu_.p, flag_.p + 1, b, x, stage);
+}
+
+// Autotuner: for every level, time the candidate kernels of each operator
+// pass (CSR-stream with LDS-staged row blocks vs thread-per-row) and the two
+// forms of the zero-guess smoothing step (fused with the residual vs a map
+// followed by the residual), on the level's real operators, and keep the
+// fastest.  All candidates compute the same values in the same order, so the
+// choice never changes a result.  Runs once per cycle precision, on first use.
+template <typename TV, typename TX>
+void DeviceHierarchy::tune() {
+    typedef typename TransferT<TV>::type TP;
+    const int pi = pidx<TV>();
+    DRAUGR_HIP_CHECK(hipDeviceSynchronize());
+    hipStream_t st = stream_;
+    hipEvent_t e0, e1;
+    DRAUGR_HIP_CHECK(hipEventCreate(&e0));
+    DRAUGR_HIP_CHECK(hipEventCreate(&e1));
+    auto timeit = [&](auto&& fn) {
+        fn();
+        float best = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {
+            DRAUGR_HIP_CHECK(hipEventRecord(e0, st));
+            fn();
+            DRAUGR_HIP_CHECK(hipEventRecord(e1, st));
+            DRAUGR_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            DRAUGR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, ms);
+        }
+        return best;
+    };
+    TX* s1 = reinterpret_cast<TX*>(w_q_.p);  // scratch (finest size, >= any level)
+    TX* s2 = reinterpret_cast<TX*>(w_z_.p);
+    for (int l = 0; l < nlevels(); ++l) {
+        DevLevel& D = L_[l];
+        const int n = D.A.nrow, nc = D.P.ncol;
+        TX* xin = t_of<TX>(D);
+        TX* bin = r_of<TX>(D);
+        TX* xcv = xc_of<TX>(D);
+        launch_map(n, [=] __device__(int64_t i) { xin[i] = (TX)1; bin[i] = (TX)1; s1[i] = (TX)0; s2[i] = (TX)0; }, st);
+        launch_map(nc, [=] __device__(int64_t i) { xcv[i] = (TX)1; }, st);
+        const TX* d = d_of<TX>(D);
+        const TX w = (TX)omega_;
+        float tA[2], tP[2], tPt[2];
+        for (int k = 0; k < 2; ++k) {
+            tA[k] = timeit([&] { op_kind<TV, TX>(k, D.A, XVec<TX>{xin}, EpiJacobi<TX>{xin, bin, d, s1, w}, st); });
+            tPt[k] = timeit([&] { op_kind<TP, TX>(k, D.Pt, XVec<TX>{xin}, EpiStore<TX>{s2}, st); });
+            tP[k] = timeit([&] { op_kind<TP, TX>(k, D.P, XVec<TX>{xcv}, EpiAdd<TX>{s2}, st); });
+        }
+        D.A.kind[pi] = tA[1] < tA[0] ? kOpRowSeq : kOpStream;
+        D.Pt.kind[pi] = tPt[1] < tPt[0] ? kOpRowSeq : kOpStream;
+        D.P.kind[pi] = tP[1] < tP[0] ? kOpRowSeq : kOpStream;
+        const float fused = timeit([&] {
+            op<TV, TX>(D.A, XZeroJacobi<TX>{d, bin, w}, EpiZeroJacobiResid<TX>{bin, d, s1, s2, w}, st);
+        });
+        const float split = timeit([&] {
+            launch_map(n, [=] __device__(int64_t i) { s1[i] = (w * d[i]) * bin[i]; }, st);
+            op<TV, TX>(D.A, XVec<TX>{s1}, EpiResid<TX>{bin, s2}, st);
+        });
+        D.fuse_zero[pi] = fused < split;
+    }
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+    DRAUGR_HIP_CHECK(hipEventDestroy(e0));
+    DRAUGR_HIP_CHECK(hipEventDestroy(e1));
+    tuned_[pi] = true;
+}
+
+std::string DeviceHierarchy::tune_report() const {
+    std::string out;
+    const char* names[3] = {"fp64", "fp32", "bf16"};
+    char buf[160];
+    for (int p = 0; p < 3; ++p) {
+        if (!tuned_[p]) continue;
+        for (int l = 0; l < nlevels(); ++l) {
+            const DevLevel& D = L_[l];
+            std::snprintf(buf, sizeof buf, "%s level %d: A=%s Pt=%s P=%s zero-guess=%s\n", names[p], l,
+                          D.A.kind[p] ? "rowseq" : "stream", D.Pt.kind[p] ? "rowseq" : "stream",
+                          D.P.kind[p] ? "rowseq" : "stream", D.fuse_zero[p] ? "fused" : "split");
+            out += buf;
+        }
+    }
+    return out;
 }
 
 // One visit of level l (src/cycle.jl:63-99).
@@ -631,6 +735,7 @@ void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const Cy
 
 template <typename TV, typename TX>
 void DeviceHierarchy::cycle_top(TX* x, const TX* b, bool zero, const CycleOpts& o, hipStream_t st) {
+    if (!tuned_[pidx<TV>()]) tune<TV, TX>();
     if (nlevels() == 0) {
         coarse_solve<TX>(x, b, st);
         return;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "../host/types.hpp"
@@ -22,6 +23,7 @@ struct DevCSR {
     DBuf<dev::bf16_t> v16;
     DBuf<int32_t> blk;  // CSR-stream row blocks (offsets into rows)
     int nblk = 0;
+    int kind[3] = {0, 0, 0};  // kernel per cycle precision (dev::OpKernel), set by the autotuner
 };
 
 struct DevLevel {
@@ -38,6 +40,7 @@ struct DevLevel {
     int ncolors = 0;
     std::vector<int32_t> color_off;  // host copy (loop control)
     DBuf<int32_t> color_order;
+    bool fuse_zero[3] = {false, false, false};  // zero-guess step fused with its residual (autotuned)
     // workspaces: residual, ping-pong smoother buffer, coarse x and b
     DBuf<double> r64, t64, xc64, bc64;
     DBuf<float> r32, t32, xc32, bc32;
@@ -54,12 +57,21 @@ struct CycleOpts {
     Precision prec = Precision::FP64;
 };
 
+// Differentiation kernels (adjoint.hip)
+void assemble_7pt(int m, const double* k, const int* rp, double* nz, hipStream_t st);
+void matrix_grad(int n, const int* rp, const int* cv, const double* x, const double* lam, double* g, hipStream_t st);
+void grad_logk_7pt(int m, const double* k, const double* x, const double* lam, double* g, hipStream_t st);
+
 class DeviceHierarchy {
 public:
     DeviceHierarchy(const HostHierarchy& h, const Config& cfg);
     ~DeviceHierarchy();
 
     int32_t n_fine() const { return n0_; }
+    // finest operator (device): row pointer, columns, nnz
+    const int32_t* finest_rp() const { return L_.empty() ? Ac_.rp.p : L_[0].A.rp.p; }
+    const int32_t* finest_cv() const { return L_.empty() ? Ac_.cv.p : L_[0].A.cv.p; }
+    int64_t finest_nnz() const { return L_.empty() ? Ac_.nnz : L_[0].A.nnz; }
     int nlevels() const { return (int)L_.size(); }
     hipStream_t stream() const { return stream_; }
 
@@ -76,6 +88,9 @@ public:
     void resetup(const double* nzval_dev, hipStream_t st);
 
     // ── test / measurement hooks ─────────────────────────────────────────────
+    // Autotuner report: per level and precision the kernel picked for A, P^T, P
+    // and the zero-guess fusion, as text.
+    std::string tune_report() const;
     // op: 0 y=A x, 1 r=b-Ax, 2 jacobi step xo=x+w d(b-Ax), 3 zero-guess jacobi
     // + residual (x1, r), 4 restrict bc=P^T r, 5 prolong x+=P xc, 6 GS sweep
     // (in place, all colours), 7 coarse solve, 8 galerkin (A_{l+1} values),
@@ -102,6 +117,7 @@ private:
     template <typename TV, typename TX> void cycle_top(TX* x, const TX* b, bool zero_guess, const CycleOpts& o,
                                                        hipStream_t st);
     void refresh_level_mirrors(int l, hipStream_t st);
+    template <typename TV, typename TX> void tune();
     void compute_dinv(int l, hipStream_t st);
     void galerkin(int l, hipStream_t st);
     void coarse_factor(hipStream_t st);
@@ -112,10 +128,11 @@ private:
     int32_t nc_ = 0;
     DevCSR Ac_;                        // coarsest operator (sparse; values by Galerkin)
     DBuf<double> clu_;                 // dense LU factors of the coarsest operator (column-major)
-    DBuf<int32_t> flag_;               // [0] singular flag, [1..n] pivots
+    DBuf<int32_t> flag_;               // [0] singular flag, [1..n] pivots, [n+1..2n] permutation
     Smoother smoother_;
     double omega_;
     bool have32_ = false, have16_ = false;
+    bool tuned_[3] = {false, false, false};
     // solver workspaces at the finest size
     DBuf<double> w_r_, w_p0_, w_p1_, w_q_, w_z_, w_x_;
     DBuf<float> w_r32_, w_z32_;

@@ -75,56 +75,51 @@ template <typename TX> struct XZeroJacobi {
     __device__ TX operator()(int j) const { return (w * d[j]) * b[j]; }
 };
 
-// ── epilogues: seg = the row's products in row order (LDS), or a total when a
-//    long row was reduced by the whole block ───────────────────────────────
-template <typename TX> __device__ __forceinline__ TX seq_sum(const TX* seg, int len) {
-    TX s = 0;
-    for (int k = 0; k < len; ++k) s += seg[k];
-    return s;
-}
-
+// ── epilogues.  A row's products p_k arrive in row order; every epilogue is
+//    a = init(i); a = acc(a, p_k) for each k; fin(i, a) — the reference's own
+//    accumulation for that operation.  total(i, s) finishes a long row whose
+//    products were summed by the whole block (tree order). ────────────────────
 template <typename TX> struct EpiStore {          // y = A x   (src/csr_matrix.jl:150-159)
     TX* y;
-    __device__ void operator()(int i, const TX* seg, int len) const { y[i] = seq_sum(seg, len); }
+    __device__ TX init(int) const { return 0; }
+    __device__ TX acc(TX a, TX p) const { return a + p; }
+    __device__ void fin(int i, TX a) const { y[i] = a; }
     __device__ void total(int i, TX s) const { y[i] = s; }
 };
 template <typename TX> struct EpiResid {          // r = b - A x   (src/cycle.jl:32-43)
     const TX* b; TX* r;
-    __device__ void operator()(int i, const TX* seg, int len) const { r[i] = b[i] - seq_sum(seg, len); }
+    __device__ TX init(int) const { return 0; }
+    __device__ TX acc(TX a, TX p) const { return a + p; }
+    __device__ void fin(int i, TX a) const { r[i] = b[i] - a; }
     __device__ void total(int i, TX s) const { r[i] = b[i] - s; }
 };
 template <typename TX> struct EpiJacobi {         // xo = x + w d (b - A x)   (src/smoothers.jl:78-94)
     const TX* x; const TX* b; const TX* d; TX* xo; TX w;
-    __device__ void operator()(int i, const TX* seg, int len) const {
-        TX r = b[i];
-        for (int k = 0; k < len; ++k) r -= seg[k];  // residual by successive subtraction
-        xo[i] = x[i] + (w * d[i]) * r;
-    }
+    __device__ TX init(int i) const { return b[i]; }  // residual by successive subtraction
+    __device__ TX acc(TX a, TX p) const { return a - p; }
+    __device__ void fin(int i, TX a) const { xo[i] = x[i] + (w * d[i]) * a; }
     __device__ void total(int i, TX s) const { xo[i] = x[i] + (w * d[i]) * (b[i] - s); }
 };
 template <typename TX> struct EpiZeroJacobiResid {  // x1 = w d b, r = b - A x1
     const TX* b; const TX* d; TX* x1; TX* r; TX w;
-    __device__ void operator()(int i, const TX* seg, int len) const {
+    __device__ TX init(int) const { return 0; }
+    __device__ TX acc(TX a, TX p) const { return a + p; }
+    __device__ void fin(int i, TX a) const {
         x1[i] = (w * d[i]) * b[i];
-        r[i] = b[i] - seq_sum(seg, len);
+        r[i] = b[i] - a;
     }
-    __device__ void total(int i, TX s) const {
-        x1[i] = (w * d[i]) * b[i];
-        r[i] = b[i] - s;
-    }
+    __device__ void total(int i, TX s) const { fin(i, s); }
 };
 template <typename TX> struct EpiAdd {            // x += P xc, one product at a time (src/prolongation.jl:750-759)
     TX* x;
-    __device__ void operator()(int i, const TX* seg, int len) const {
-        TX v = x[i];
-        for (int k = 0; k < len; ++k) v += seg[k];
-        x[i] = v;
-    }
+    __device__ TX init(int i) const { return x[i]; }
+    __device__ TX acc(TX a, TX p) const { return a + p; }
+    __device__ void fin(int i, TX a) const { x[i] = a; }
     __device__ void total(int i, TX s) const { x[i] += s; }
 };
 
-// One row block: stage the products, then call segf(row, seg, len) on the
-// row's thread, or totf(row, sum) on thread 0 for a long row.
+// One row block: stage the products, then finish each row on its own thread
+// (segf(row, seg, len)), or a long row on thread 0 (totf(row, sum)).
 template <typename TV, typename TX, class XF, class SEGF, class TOTF>
 __device__ __forceinline__ void stream_block(int b, const int* __restrict__ blk, const int* __restrict__ rp,
                                              const int* __restrict__ cv, const TV* __restrict__ av, XF xf,
@@ -132,15 +127,13 @@ __device__ __forceinline__ void stream_block(int b, const int* __restrict__ blk,
     const int r0 = blk[b], r1 = blk[b + 1];
     const int z0 = rp[r0], z1 = rp[r1];
     const int nz = z1 - z0;
+    const int row = r0 + (int)threadIdx.x;
     if (nz <= kStreamNnz) {
+        const int rs = row < r1 ? rp[row] : 0, re = row < r1 ? rp[row + 1] : 0;  // issued before the barrier
 #pragma unroll 4
         for (int k = threadIdx.x; k < nz; k += kBlock) prod[k] = ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
         __syncthreads();
-        const int row = r0 + (int)threadIdx.x;
-        if (row < r1) {
-            const int s = rp[row] - z0;
-            segf(row, prod + s, rp[row + 1] - z0 - s);
-        }
+        if (row < r1) segf(row, prod + (rs - z0), re - rs);
     } else {
         TX acc = 0;
         for (int k = threadIdx.x; k < nz; k += kBlock) acc += ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
@@ -154,6 +147,13 @@ __device__ __forceinline__ void stream_block(int b, const int* __restrict__ blk,
     }
 }
 
+template <class EPI, typename TX>
+__device__ __forceinline__ void epi_segment(const EPI& epi, int row, const TX* seg, int len) {
+    TX a = epi.init(row);
+    for (int k = 0; k < len; ++k) a = epi.acc(a, seg[k]);
+    epi.fin(row, a);
+}
+
 template <bool XCD, typename TV, typename TX, class XF, class EPI>
 __global__ __launch_bounds__(kBlock) void k_csr_stream(const int* __restrict__ blk, const int* __restrict__ rp,
                                                        const int* __restrict__ cv, const TV* __restrict__ av,
@@ -161,9 +161,26 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(const int* __restrict__ b
     __shared__ TX prod[kStreamNnz];
     const int b = XCD ? xcd_block() : (int)blockIdx.x;
     stream_block<TV, TX>(
-        b, blk, rp, cv, av, xf, prod, [&](int row, const TX* seg, int len) { epi(row, seg, len); },
+        b, blk, rp, cv, av, xf, prod, [&](int row, const TX* seg, int len) { epi_segment(epi, row, seg, len); },
         [&](int row, TX s) { epi.total(row, s); });
 }
+
+// Thread-per-row variant without LDS: fewer dependent round trips, for the
+// small coarse levels where a kernel's latency, not bandwidth, is the cost.
+// Same per-row operation order as the stream kernel.
+template <typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_csr_rowseq(int nrows, const int* __restrict__ rp,
+                                                       const int* __restrict__ cv, const TV* __restrict__ av,
+                                                       XF xf, EPI epi) {
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    if (row >= nrows) return;
+    TX a = epi.init(row);
+    const int e = rp[row + 1];
+    for (int k = rp[row]; k < e; ++k) a = epi.acc(a, ld<TX, TV>(av, k) * xf(ldi(cv, k)));
+    epi.fin(row, a);
+}
+
+enum OpKernel : int { kOpStream = 0, kOpRowSeq = 1 };
 
 template <typename TV, typename TX, class XF, class EPI>
 void launch_stream(int nblk, const int* blk, const int* rp, const int* cv, const TV* av, XF xf, EPI epi,
@@ -175,6 +192,13 @@ void launch_stream(int nblk, const int* blk, const int* rp, const int* cv, const
     else
         hipLaunchKernelGGL((k_csr_stream<false, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk, rp, cv,
                            av, xf, epi);
+}
+
+template <typename TV, typename TX, class XF, class EPI>
+void launch_rowseq(int nrows, const int* rp, const int* cv, const TV* av, XF xf, EPI epi, hipStream_t st) {
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL((k_csr_rowseq<TV, TX, XF, EPI>), dim3((nrows + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                       nrows, rp, cv, av, xf, epi);
 }
 
 // Nonzeros per row block: kStreamNnz for large operators; halved (down to

@@ -162,6 +162,13 @@ class AMGHierarchy:
         check(_lib.lib().draugr_hip_cycle_bytes(self.handle, c.handle, ctypes.byref(v)), "draugr_hip_cycle_bytes")
         return v.value
 
+    def tune_report(self) -> str:
+        L = _lib.lib()
+        m = check(L.draugr_hip_tune_report(self.handle, None, 0), "draugr_hip_tune_report")
+        buf = ctypes.create_string_buffer(m + 1)
+        L.draugr_hip_tune_report(self.handle, buf, m + 1)
+        return buf.value.decode()
+
     def resident_bytes(self) -> int:
         v = ctypes.c_int64(0)
         check(_lib.lib().draugr_hip_resident_bytes(self.handle, ctypes.byref(v)), "draugr_hip_resident_bytes")

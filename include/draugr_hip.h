@@ -135,6 +135,27 @@ int32_t draugr_hip_export_device_values(int32_t handle, int32_t level, int32_t w
 int32_t draugr_hip_cycle_bytes(int32_t handle, int32_t config_handle, double *bytes);
 int32_t draugr_hip_resident_bytes(int32_t handle, int64_t *bytes);
 
+/* ── Differentiation of x = A(k)^-1 b (device pointers) ─────────────────── */
+
+/* Values of the 7-point cell-centred diffusion operator on an m^3 grid
+ * (cell c = i + m (j + m z)) from cell permeabilities k: harmonic-mean face
+ * transmissibilities, Dirichlet faces through a half-cell term 2 k.  Written
+ * in the row layout of the hierarchy's finest operator (columns sorted), ready
+ * for draugr_hip_resetup_device.  Requires n == m^3 and that pattern. */
+int32_t draugr_hip_assemble_7pt(int32_t handle, int32_t m, const double *k, double *nzval, void *stream);
+
+/* dL/dA_ij = -lambda_i x_j on the finest operator's pattern (grad has nnz). */
+int32_t draugr_hip_matrix_grad(int32_t handle, const double *x, const double *lambda, double *grad, void *stream);
+
+/* Chain rule of dL/dA to d L / d log k for the operator of
+ * draugr_hip_assemble_7pt; grad has m^3 entries. */
+int32_t draugr_hip_grad_logk_7pt(int32_t m, const double *k, const double *x, const double *lambda, double *grad,
+                                 void *stream);
+
+/* Kernel choices of the autotuner (per level and cycle precision) as text;
+ * returns the length.  The tuner runs on the first cycle of each precision. */
+int32_t draugr_hip_tune_report(int32_t handle, char *buf, int32_t buflen);
+
 #ifdef __cplusplus
 }
 #endif
