@@ -93,8 +93,8 @@ __global__ __launch_bounds__(kBlock) void k_stream_reduce(const int* __restrict_
     __shared__ double sh[kBlock];
     double dot = 0.0;
     stream_block<TV, TX>(
-        xcd_block(), blk, rp, cv, av, xf, prod,
-        [&](int row, const TX* seg, int len) {
+        xcd_block(), blk, rp, cv, av, xf, prod, [](int) { return NoPre{}; },
+        [&](int row, const TX* seg, int len, NoPre) {
             TX a = 0;
             for (int k = 0; k < len; ++k) a += seg[k];
             dot = epi(row, a);

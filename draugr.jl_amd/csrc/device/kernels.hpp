@@ -76,64 +76,93 @@ template <typename TX> struct XZeroJacobi {
 };
 
 // ── epilogues.  A row's products p_k arrive in row order; every epilogue is
-//    a = init(i); a = acc(a, p_k) for each k; fin(i, a) — the reference's own
+//    q = pre(i) (the row's own operands, loaded before the products are
+//    staged so that their latency overlaps the gathers); a = init(q);
+//    a = acc(a, p_k) for each k; fin(i, a, q) — the reference's own
 //    accumulation for that operation.  total(i, s) finishes a long row whose
 //    products were summed by the whole block (tree order). ────────────────────
+struct NoPre {};
 template <typename TX> struct EpiStore {          // y = A x   (src/csr_matrix.jl:150-159)
     TX* y;
-    __device__ TX init(int) const { return 0; }
+    typedef NoPre Pre;
+    __device__ Pre pre(int) const { return {}; }
+    __device__ TX init(const Pre&) const { return 0; }
     __device__ TX acc(TX a, TX p) const { return a + p; }
-    __device__ void fin(int i, TX a) const { y[i] = a; }
+    __device__ void fin(int i, TX a, const Pre&) const { y[i] = a; }
     __device__ void total(int i, TX s) const { y[i] = s; }
 };
 template <typename TX> struct EpiResid {          // r = b - A x   (src/cycle.jl:32-43)
     const TX* b; TX* r;
-    __device__ TX init(int) const { return 0; }
+    struct Pre { TX b; };
+    __device__ Pre pre(int i) const { return {b[i]}; }
+    __device__ TX init(const Pre&) const { return 0; }
     __device__ TX acc(TX a, TX p) const { return a + p; }
-    __device__ void fin(int i, TX a) const { r[i] = b[i] - a; }
+    __device__ void fin(int i, TX a, const Pre& q) const { r[i] = q.b - a; }
     __device__ void total(int i, TX s) const { r[i] = b[i] - s; }
 };
 template <typename TX> struct EpiJacobi {         // xo = x + w d (b - A x)   (src/smoothers.jl:78-94)
     const TX* x; const TX* b; const TX* d; TX* xo; TX w;
-    __device__ TX init(int i) const { return b[i]; }  // residual by successive subtraction
+    struct Pre { TX x, b, d; };
+    __device__ Pre pre(int i) const { return {x[i], b[i], d[i]}; }
+    __device__ TX init(const Pre& q) const { return q.b; }  // residual by successive subtraction
     __device__ TX acc(TX a, TX p) const { return a - p; }
-    __device__ void fin(int i, TX a) const { xo[i] = x[i] + (w * d[i]) * a; }
+    __device__ void fin(int i, TX a, const Pre& q) const { xo[i] = q.x + (w * q.d) * a; }
     __device__ void total(int i, TX s) const { xo[i] = x[i] + (w * d[i]) * (b[i] - s); }
 };
 template <typename TX> struct EpiZeroJacobiResid {  // x1 = w d b, r = b - A x1
     const TX* b; const TX* d; TX* x1; TX* r; TX w;
-    __device__ TX init(int) const { return 0; }
+    struct Pre { TX b, d; };
+    __device__ Pre pre(int i) const { return {b[i], d[i]}; }
+    __device__ TX init(const Pre&) const { return 0; }
     __device__ TX acc(TX a, TX p) const { return a + p; }
-    __device__ void fin(int i, TX a) const {
-        x1[i] = (w * d[i]) * b[i];
-        r[i] = b[i] - a;
+    __device__ void fin(int i, TX a, const Pre& q) const {
+        x1[i] = (w * q.d) * q.b;
+        r[i] = q.b - a;
     }
-    __device__ void total(int i, TX s) const { fin(i, s); }
+    __device__ void total(int i, TX s) const { fin(i, s, pre(i)); }
 };
 template <typename TX> struct EpiAdd {            // x += P xc, one product at a time (src/prolongation.jl:750-759)
     TX* x;
-    __device__ TX init(int i) const { return x[i]; }
+    struct Pre { TX x; };
+    __device__ Pre pre(int i) const { return {x[i]}; }
+    __device__ TX init(const Pre& q) const { return q.x; }
     __device__ TX acc(TX a, TX p) const { return a + p; }
-    __device__ void fin(int i, TX a) const { x[i] = a; }
+    __device__ void fin(int i, TX a, const Pre&) const { x[i] = a; }
     __device__ void total(int i, TX s) const { x[i] += s; }
 };
 
 // One row block: stage the products, then finish each row on its own thread
-// (segf(row, seg, len)), or a long row on thread 0 (totf(row, sum)).
-template <typename TV, typename TX, class XF, class SEGF, class TOTF>
+// (segf(row, seg, len, pref(row))), or a long row on thread 0 (totf(row, sum)).
+// The block's column indices are all loaded before any gather is issued (at
+// most kStreamNnz / kBlock per thread), so every thread keeps that many
+// independent loads in flight.
+template <typename TV, typename TX, class XF, class PREF, class SEGF, class TOTF>
 __device__ __forceinline__ void stream_block(int b, const int* __restrict__ blk, const int* __restrict__ rp,
                                              const int* __restrict__ cv, const TV* __restrict__ av, XF xf,
-                                             TX* prod, SEGF segf, TOTF totf) {
+                                             TX* prod, PREF pref, SEGF segf, TOTF totf) {
     const int r0 = blk[b], r1 = blk[b + 1];
     const int z0 = rp[r0], z1 = rp[r1];
     const int nz = z1 - z0;
     const int row = r0 + (int)threadIdx.x;
     if (nz <= kStreamNnz) {
-        const int rs = row < r1 ? rp[row] : 0, re = row < r1 ? rp[row + 1] : 0;  // issued before the barrier
-#pragma unroll 4
-        for (int k = threadIdx.x; k < nz; k += kBlock) prod[k] = ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
+        const bool own = row < r1;
+        const int rs = own ? rp[row] : 0, re = own ? rp[row + 1] : 0;  // issued before the barrier
+        decltype(pref(0)) q{};
+        if (own) q = pref(row);
+        constexpr int kPer = kStreamNnz / kBlock;
+        int cj[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int k = (int)threadIdx.x + u * kBlock;
+            cj[u] = k < nz ? ldi(cv, z0 + k) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int k = (int)threadIdx.x + u * kBlock;
+            if (k < nz) prod[k] = ld<TX, TV>(av, z0 + k) * xf(cj[u]);
+        }
         __syncthreads();
-        if (row < r1) segf(row, prod + (rs - z0), re - rs);
+        if (own) segf(row, prod + (rs - z0), re - rs, q);
     } else {
         TX acc = 0;
         for (int k = threadIdx.x; k < nz; k += kBlock) acc += ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
@@ -148,10 +177,11 @@ __device__ __forceinline__ void stream_block(int b, const int* __restrict__ blk,
 }
 
 template <class EPI, typename TX>
-__device__ __forceinline__ void epi_segment(const EPI& epi, int row, const TX* seg, int len) {
-    TX a = epi.init(row);
+__device__ __forceinline__ void epi_segment(const EPI& epi, int row, const TX* seg, int len,
+                                            const typename EPI::Pre& q) {
+    TX a = epi.init(q);
     for (int k = 0; k < len; ++k) a = epi.acc(a, seg[k]);
-    epi.fin(row, a);
+    epi.fin(row, a, q);
 }
 
 template <bool XCD, typename TV, typename TX, class XF, class EPI>
@@ -161,7 +191,8 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(const int* __restrict__ b
     __shared__ TX prod[kStreamNnz];
     const int b = XCD ? xcd_block() : (int)blockIdx.x;
     stream_block<TV, TX>(
-        b, blk, rp, cv, av, xf, prod, [&](int row, const TX* seg, int len) { epi_segment(epi, row, seg, len); },
+        b, blk, rp, cv, av, xf, prod, [&](int row) { return epi.pre(row); },
+        [&](int row, const TX* seg, int len, const typename EPI::Pre& q) { epi_segment(epi, row, seg, len, q); },
         [&](int row, TX s) { epi.total(row, s); });
 }
 
@@ -174,10 +205,11 @@ __global__ __launch_bounds__(kBlock) void k_csr_rowseq(int nrows, const int* __r
                                                        XF xf, EPI epi) {
     const int row = blockIdx.x * kBlock + threadIdx.x;
     if (row >= nrows) return;
-    TX a = epi.init(row);
+    const typename EPI::Pre q = epi.pre(row);
+    TX a = epi.init(q);
     const int e = rp[row + 1];
     for (int k = rp[row]; k < e; ++k) a = epi.acc(a, ld<TX, TV>(av, k) * xf(ldi(cv, k)));
-    epi.fin(row, a);
+    epi.fin(row, a, q);
 }
 
 enum OpKernel : int { kOpStream = 0, kOpRowSeq = 1 };

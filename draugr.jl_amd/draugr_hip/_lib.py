@@ -9,7 +9,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libdraugr_hip.so")
+LIB_PATH = os.environ.get("DRAUGR_HIP_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libdraugr_hip.so")
 
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
