@@ -85,15 +85,16 @@ void launch_reduce(int64_t n, F f, FIN fin, double* partial, hipStream_t st) {
 
 // CSR-stream pass whose per-row epilogue also returns a contribution to a
 // global sum (q = A p fused with p.q in PCG).
-template <typename TV, typename TX, class XF, class EPI>
-__global__ __launch_bounds__(kBlock) void k_stream_reduce(const int* __restrict__ blk, const int* __restrict__ rp,
-                                                          const int* __restrict__ cv, const TV* __restrict__ av,
-                                                          XF xf, EPI epi, double* partial) {
+template <int PER, typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_stream_reduce(const int* __restrict__ blk, const int* __restrict__ bz,
+                                                          const int* __restrict__ rp, const int* __restrict__ cv,
+                                                          const TV* __restrict__ av, XF xf, EPI epi,
+                                                          double* partial) {
     __shared__ TX prod[kStreamNnz];
     __shared__ double sh[kBlock];
     double dot = 0.0;
-    stream_block<TV, TX>(
-        xcd_block(), blk, rp, cv, av, xf, prod, [](int) { return NoPre{}; },
+    stream_block<PER, TV, TX>(
+        xcd_block(), blk, bz, rp, cv, av, xf, prod, [](int) { return NoPre{}; },
         [&](int row, const TX* seg, int len, NoPre) {
             TX a = 0;
             for (int k = 0; k < len; ++k) a += seg[k];
@@ -105,8 +106,16 @@ __global__ __launch_bounds__(kBlock) void k_stream_reduce(const int* __restrict_
 
 template <typename TV, typename TX, class XF, class EPI, class FIN>
 void launch_stream_reduce(const DevCSR& M, const TV* av, XF xf, EPI epi, FIN fin, double* partial, hipStream_t st) {
-    hipLaunchKernelGGL((k_stream_reduce<TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.blk.p, M.rp.p,
-                       M.cv.p, av, xf, epi, partial);
+#define DRAUGR_SR(PER)                                                                                           \
+    hipLaunchKernelGGL((k_stream_reduce<PER, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.blk.p, M.bz.p, \
+                       M.rp.p, M.cv.p, av, xf, epi, partial)
+    switch (M.per) {
+        case 1: DRAUGR_SR(1); break;
+        case 2: DRAUGR_SR(2); break;
+        case 4: DRAUGR_SR(4); break;
+        default: DRAUGR_SR(8); break;
+    }
+#undef DRAUGR_SR
     hipLaunchKernelGGL((k_finalize<FIN>), dim3(1), dim3(kBlock), 0, st, partial, M.nblk, fin);
 }
 
@@ -323,7 +332,7 @@ template <> constexpr int pidx<bf16_t>() { return 2; }
 template <typename TV, typename TX, class XF, class EPI>
 void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
     if (kind == kOpRowSeq) launch_rowseq<TV, TX>(M.nrow, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
-    else launch_stream<TV, TX>(M.nblk, M.blk.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
+    else launch_stream<TV, TX>(M.nblk, M.per, M.blk.p, M.bz.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
 }
 
 // One pass of operator M with the kernel the autotuner picked for this
@@ -340,9 +349,14 @@ void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
     d.rp.upload(h.rp, st);
     d.cv.upload(h.cv, st);
     d.v64.upload(h.nz, st);
-    const std::vector<int32_t> blk = build_row_blocks(h.rp, stream_budget(h.nnz()));
+    const int budget = stream_budget(h.nnz());
+    const std::vector<int32_t> blk = build_row_blocks(h.rp, budget);
+    std::vector<int32_t> bz(blk.size());
+    for (size_t b = 0; b < blk.size(); ++b) bz[b] = h.rp[blk[b]];
     d.nblk = (int)blk.size() - 1;
+    d.per = budget / kBlock;
     d.blk.upload(blk, st);
+    d.bz.upload(bz, st);
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // blk is pageable and local
 }
 

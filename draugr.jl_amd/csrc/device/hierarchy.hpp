@@ -22,7 +22,9 @@ struct DevCSR {
     DBuf<float> v32;
     DBuf<dev::bf16_t> v16;
     DBuf<int32_t> blk;  // CSR-stream row blocks (offsets into rows)
+    DBuf<int32_t> bz;   // nonzero offset of each block boundary (rp[blk[b]])
     int nblk = 0;
+    int per = 8;        // block budget / kBlock: products per thread
     int kind[3] = {0, 0, 0};  // kernel per cycle precision (dev::OpKernel), set by the autotuner
 };
 

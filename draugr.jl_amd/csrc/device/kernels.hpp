@@ -133,46 +133,60 @@ template <typename TX> struct EpiAdd {            // x += P xc, one product at a
 
 // One row block: stage the products, then finish each row on its own thread
 // (segf(row, seg, len, pref(row))), or a long row on thread 0 (totf(row, sum)).
-// The block's column indices are all loaded before any gather is issued (at
-// most kStreamNnz / kBlock per thread), so every thread keeps that many
-// independent loads in flight.
-template <typename TV, typename TX, class XF, class PREF, class SEGF, class TOTF>
-__device__ __forceinline__ void stream_block(int b, const int* __restrict__ blk, const int* __restrict__ rp,
-                                             const int* __restrict__ cv, const TV* __restrict__ av, XF xf,
-                                             TX* prod, PREF pref, SEGF segf, TOTF totf) {
+// The block table holds both row and nonzero offsets (blk, bz), so a block's
+// range is known after one round of loads.  A block of at most PER * kBlock
+// nonzeros (PER = the operator's budget / kBlock) is loaded without branches:
+// every thread issues its PER column and value loads together (indices past
+// the block's end are clamped to its last entry and the result discarded),
+// then its PER gathers together, so each wave keeps 2 PER + PER loads in
+// flight instead of waiting once per product.
+template <int PER, typename TV, typename TX, class XF, class PREF, class SEGF, class TOTF>
+__device__ __forceinline__ void stream_block(int b, const int* __restrict__ blk, const int* __restrict__ bz,
+                                             const int* __restrict__ rp, const int* __restrict__ cv,
+                                             const TV* __restrict__ av, XF xf, TX* prod, PREF pref, SEGF segf,
+                                             TOTF totf) {
     const int r0 = blk[b], r1 = blk[b + 1];
-    const int z0 = rp[r0], z1 = rp[r1];
+    const int z0 = bz[b], z1 = bz[b + 1];
     const int nz = z1 - z0;
-    const int row = r0 + (int)threadIdx.x;
+    const int t = (int)threadIdx.x;
+    const int row = r0 + t;
     if (nz <= kStreamNnz) {
         const bool own = row < r1;
         const int rs = own ? rp[row] : 0, re = own ? rp[row + 1] : 0;  // issued before the barrier
         decltype(pref(0)) q{};
         if (own) q = pref(row);
-        constexpr int kPer = kStreamNnz / kBlock;
-        int cj[kPer];
+        if (nz <= PER * kBlock) {
+            if (nz > 0) {
+                int cj[PER];
+                TX v[PER], g[PER];
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int k = (int)threadIdx.x + u * kBlock;
-            cj[u] = k < nz ? ldi(cv, z0 + k) : 0;
-        }
+                for (int u = 0; u < PER; ++u) {
+                    const int k = min(t + u * kBlock, nz - 1);
+                    cj[u] = ldi(cv, z0 + k);
+                    v[u] = ld<TX, TV>(av, z0 + k);
+                }
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int k = (int)threadIdx.x + u * kBlock;
-            if (k < nz) prod[k] = ld<TX, TV>(av, z0 + k) * xf(cj[u]);
+                for (int u = 0; u < PER; ++u) g[u] = xf(cj[u]);
+#pragma unroll
+                for (int u = 0; u < PER; ++u)
+                    if (t + u * kBlock < nz) prod[t + u * kBlock] = v[u] * g[u];
+            }
+        } else {  // a single row longer than the budget (still summed in row order)
+#pragma unroll 4
+            for (int k = t; k < nz; k += kBlock) prod[k] = ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
         }
         __syncthreads();
         if (own) segf(row, prod + (rs - z0), re - rs, q);
     } else {
         TX acc = 0;
-        for (int k = threadIdx.x; k < nz; k += kBlock) acc += ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
-        prod[threadIdx.x] = acc;
+        for (int k = t; k < nz; k += kBlock) acc += ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
+        prod[t] = acc;
         __syncthreads();
         for (int s = kBlock / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) prod[threadIdx.x] += prod[threadIdx.x + s];
+            if (t < s) prod[t] += prod[t + s];
             __syncthreads();
         }
-        if (threadIdx.x == 0) totf(r0, prod[0]);
+        if (t == 0) totf(r0, prod[0]);
     }
 }
 
@@ -184,14 +198,14 @@ __device__ __forceinline__ void epi_segment(const EPI& epi, int row, const TX* s
     epi.fin(row, a, q);
 }
 
-template <bool XCD, typename TV, typename TX, class XF, class EPI>
-__global__ __launch_bounds__(kBlock) void k_csr_stream(const int* __restrict__ blk, const int* __restrict__ rp,
-                                                       const int* __restrict__ cv, const TV* __restrict__ av,
-                                                       XF xf, EPI epi) {
+template <bool XCD, int PER, typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_csr_stream(const int* __restrict__ blk, const int* __restrict__ bz,
+                                                       const int* __restrict__ rp, const int* __restrict__ cv,
+                                                       const TV* __restrict__ av, XF xf, EPI epi) {
     __shared__ TX prod[kStreamNnz];
     const int b = XCD ? xcd_block() : (int)blockIdx.x;
-    stream_block<TV, TX>(
-        b, blk, rp, cv, av, xf, prod, [&](int row) { return epi.pre(row); },
+    stream_block<PER, TV, TX>(
+        b, blk, bz, rp, cv, av, xf, prod, [&](int row) { return epi.pre(row); },
         [&](int row, const TX* seg, int len, const typename EPI::Pre& q) { epi_segment(epi, row, seg, len, q); },
         [&](int row, TX s) { epi.total(row, s); });
 }
@@ -214,16 +228,28 @@ __global__ __launch_bounds__(kBlock) void k_csr_rowseq(int nrows, const int* __r
 
 enum OpKernel : int { kOpStream = 0, kOpRowSeq = 1 };
 
-template <typename TV, typename TX, class XF, class EPI>
-void launch_stream(int nblk, const int* blk, const int* rp, const int* cv, const TV* av, XF xf, EPI epi,
-                   hipStream_t st) {
-    if (nblk <= 0) return;
+template <int PER, typename TV, typename TX, class XF, class EPI>
+void launch_stream_per(int nblk, const int* blk, const int* bz, const int* rp, const int* cv, const TV* av, XF xf,
+                       EPI epi, hipStream_t st) {
     if (nblk >= 64)
-        hipLaunchKernelGGL((k_csr_stream<true, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk, rp, cv, av,
-                           xf, epi);
+        hipLaunchKernelGGL((k_csr_stream<true, PER, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk, bz, rp,
+                           cv, av, xf, epi);
     else
-        hipLaunchKernelGGL((k_csr_stream<false, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk, rp, cv,
-                           av, xf, epi);
+        hipLaunchKernelGGL((k_csr_stream<false, PER, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk, bz,
+                           rp, cv, av, xf, epi);
+}
+
+// per = the operator's block budget / kBlock (1, 2, 4 or 8)
+template <typename TV, typename TX, class XF, class EPI>
+void launch_stream(int nblk, int per, const int* blk, const int* bz, const int* rp, const int* cv, const TV* av,
+                   XF xf, EPI epi, hipStream_t st) {
+    if (nblk <= 0) return;
+    switch (per) {
+        case 1: launch_stream_per<1, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
+        case 2: launch_stream_per<2, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
+        case 4: launch_stream_per<4, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
+        default: launch_stream_per<8, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
+    }
 }
 
 template <typename TV, typename TX, class XF, class EPI>
