@@ -97,6 +97,7 @@ CycleOpts opts_of(const Config& c) {
     o.pre = c.pre_smoothing_steps;
     o.post = c.post_smoothing_steps;
     o.prec = c.precision;
+    o.graphs = c.graphs;
     return o;
 }
 

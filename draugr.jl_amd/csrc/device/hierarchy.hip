@@ -448,6 +448,8 @@ void DeviceHierarchy::end(hipStream_t st) {
 
 DeviceHierarchy::~DeviceHierarchy() {
     if (last_valid_) (void)hipEventSynchronize(last_ev_);
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    if (cap_) (void)hipStreamDestroy(cap_);
     if (last_ev_) (void)hipEventDestroy(last_ev_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (pinned_) (void)hipHostFree(pinned_);
@@ -757,6 +759,41 @@ void DeviceHierarchy::cycle_top(TX* x, const TX* b, bool zero, const CycleOpts& 
     cycle_level<TV, TX>(0, x, b, zero, o, st);
 }
 
+// The graph holds the cycle's launches with their arguments (buffers of this
+// hierarchy and the caller's x, b), so it stays valid across resetups, which
+// rewrite values in place.  The tuner and the precision mirrors must exist
+// before capture (both synchronize).  The cycle itself contains no memset
+// nodes (buffers are cleared by fill kernels).
+template <typename TV, typename TX>
+void DeviceHierarchy::precondition(TX* x, const TX* b, const CycleOpts& o, hipStream_t st) {
+    if (!tuned_[pidx<TV>()]) tune<TV, TX>();
+    if (!o.graphs || nlevels() == 0) {
+        cycle_top<TV, TX>(x, b, true, o, st);
+        return;
+    }
+    const auto key = std::make_tuple(pidx<TV>(), (int)o.w_cycle, o.pre, o.post, (const void*)x, (const void*)b);
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+        if (!cap_) DRAUGR_HIP_CHECK(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
+        hipGraph_t g = nullptr;
+        DRAUGR_HIP_CHECK(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
+        try {
+            cycle_top<TV, TX>(x, b, true, o, cap_);
+        } catch (...) {
+            (void)hipStreamEndCapture(cap_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        DRAUGR_HIP_CHECK(hipStreamEndCapture(cap_, &g));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        DRAUGR_HIP_CHECK(e);
+        it = graphs_.emplace(key, ex).first;
+    }
+    DRAUGR_HIP_CHECK(hipGraphLaunch(it->second, st));
+}
+
 void DeviceHierarchy::cycle(double* x, const double* b, bool zero, const CycleOpts& o, hipStream_t st) {
     if (o.prec == Precision::FP64 || nlevels() == 0) {
         // no AMG levels: the cycle is the direct solve (src/cycle.jl:18-24), kept in FP64
@@ -775,8 +812,8 @@ void DeviceHierarchy::cycle(double* x, const double* b, bool zero, const CycleOp
         op<double, double>(L_[0].A, XVec<double>{x}, EpiResid<double>{b, r}, st);
         launch_map(n, [=] __device__(int64_t i) { r32[i] = (float)r[i]; }, st);
     }
-    if (o.prec == Precision::FP32) cycle_top<float, float>(z32, r32, true, o, st);
-    else cycle_top<bf16_t, float>(z32, r32, true, o, st);
+    if (o.prec == Precision::FP32) precondition<float, float>(z32, r32, o, st);
+    else precondition<bf16_t, float>(z32, r32, o, st);
     if (zero) launch_map(n, [=] __device__(int64_t i) { x[i] = (double)z32[i]; }, st);
     else launch_map(n, [=] __device__(int64_t i) { x[i] += (double)z32[i]; }, st);
 }
@@ -861,16 +898,16 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
     if (relres) *relres = rel;
     if (rel < tol) return 0;
     launch_map(n, [=] __device__(int64_t i) { pa[i] = 0.0; }, st);
-    auto precondition = [&]() {
+    auto apply_prec = [&]() {
         if (mixed) {
-            if (o.prec == Precision::FP32) cycle_top<float, float>(z32, r32, true, o, st);
-            else cycle_top<bf16_t, float>(z32, r32, true, o, st);
+            if (o.prec == Precision::FP32) precondition<float, float>(z32, r32, o, st);
+            else precondition<bf16_t, float>(z32, r32, o, st);
         } else {
-            cycle_top<double, double>(z, r, true, o, st);
+            precondition<double, double>(z, r, o, st);
         }
     };
     if (mixed) launch_map(n, [=] __device__(int64_t i) { r32[i] = (float)r[i]; }, st);
-    precondition();
+    apply_prec();
     // rz = r.z ; beta = 0 for the first direction
     if (mixed)
         launch_reduce(n, [=] __device__(int64_t i) { return r[i] * (double)z32[i]; },
@@ -931,7 +968,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
             DRAUGR_HIP_CHECK(hipMemcpyAsync(pinned_ + S_RR, scal + S_RR, sizeof(double), hipMemcpyDeviceToHost, st));
             DRAUGR_HIP_CHECK(hipEventRecord(ev, st));
             // speculative: next preconditioned residual and beta
-            precondition();
+            apply_prec();
             auto fin_rz = [=] __device__(double s) {
                 scal[S_BETA] = s / scal[S_RZ];
                 scal[S_RZ] = s;

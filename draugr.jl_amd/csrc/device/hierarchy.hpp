@@ -4,7 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <memory>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -57,6 +59,7 @@ struct CycleOpts {
     bool w_cycle = false;
     int pre = 1, post = 1;
     Precision prec = Precision::FP64;
+    bool graphs = true;  // zero-guess cycles on fixed buffers replayed from a hipGraph
 };
 
 // Differentiation kernels (adjoint.hip)
@@ -118,6 +121,9 @@ private:
     template <typename TX> void coarse_solve(TX* x, const TX* b, hipStream_t st);
     template <typename TV, typename TX> void cycle_top(TX* x, const TX* b, bool zero_guess, const CycleOpts& o,
                                                        hipStream_t st);
+    // zero-guess cycle x = V(0; b) on buffers that stay fixed for the
+    // hierarchy's life: captured once into a hipGraph, then replayed
+    template <typename TV, typename TX> void precondition(TX* x, const TX* b, const CycleOpts& o, hipStream_t st);
     void refresh_level_mirrors(int l, hipStream_t st);
     template <typename TV, typename TX> void tune();
     void compute_dinv(int l, hipStream_t st);
@@ -142,6 +148,8 @@ private:
     double* pinned_ = nullptr;
     hipStream_t stream_ = nullptr;
     hipEvent_t last_ev_ = nullptr;
+    hipStream_t cap_ = nullptr;  // capture stream for the graphs
+    std::map<std::tuple<int, int, int, int, const void*, const void*>, hipGraphExec_t> graphs_;
     hipStream_t last_st_ = nullptr;
     bool last_valid_ = false;
 };

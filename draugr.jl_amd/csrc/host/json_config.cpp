@@ -1,7 +1,7 @@
 // JSON configuration for the C-ABI: the keys, defaults, value coercions and
 // error messages of draugr_amg_config_from_json (src/cfunction_interface.jl:30-221,
 // c_api/draugr_amg.h:58-89), plus this backend's own keys ("precision",
-// "seed", "group_size").  A small recursive-descent parser; unknown keys are
+// "seed", "graphs").  A small recursive-descent parser; unknown keys are
 // ignored as in the reference.
 #include "types.hpp"
 
@@ -264,7 +264,7 @@ Config config_from_json(const char* json) {
     // backend extensions
     if (const JVal* v = get("precision")) cfg.precision = lookup(kPrecision, to_string(*v), "precision");
     if (const JVal* v = get("seed")) cfg.seed = (uint64_t)std::llround(to_float(*v));
-    if (const JVal* v = get("group_size")) cfg.group_size = to_int(*v);
+    if (const JVal* v = get("graphs")) cfg.graphs = to_bool(*v);
     if (const JVal* v = get("filtering")) {
         cfg.coarsening.filtering = to_bool(*v);
         cfg.initial_coarsening.filtering = cfg.coarsening.filtering;

@@ -75,7 +75,7 @@ struct Config {
     // ── extensions of this backend (no counterpart in the reference config) ──
     Precision precision = Precision::FP64;  // arithmetic of the V/W cycle
     uint64_t seed = 0x5EEDull;              // seed of the setup RNG (PMIS/HMIS measures)
-    int group_size = 0;                     // lanes per CSR row, 0 = autotune
+    bool graphs = true;                     // replay the preconditioner cycle from a captured hipGraph
 };
 
 // ── hierarchy (host copy; uploaded once to HBM) ──────────────────────────────

@@ -3,7 +3,7 @@
 Type tags keep the reference's names, field meaning and defaults; the
 configuration is handed to the C-ABI as the JSON document that
 ``draugr_amg_config_from_json`` accepts (src/cfunction_interface.jl:178-221),
-extended with this backend's keys (``precision``, ``seed``, ``group_size``,
+extended with this backend's keys (``precision``, ``seed``, ``graphs``,
 ``initial_theta``, ``initial_interpolation``, ``sa_omega``, ``filtering``,
 ``filter_tol``, ``aggressive_base``).
 """
@@ -169,7 +169,7 @@ class AMGConfig:
     # backend extensions
     precision: str = "fp64"
     seed: int = 0x5EED
-    group_size: int = 0
+    graphs: bool = True
 
     def __post_init__(self):
         if str(self.cycle_type).upper() not in ("V", "W"):
@@ -201,7 +201,7 @@ class AMGConfig:
             "initial_theta": ic.θ,
             "precision": self.precision,
             "seed": self.seed,
-            "group_size": self.group_size,
+            "graphs": self.graphs,
         }
         if hasattr(c, "interpolation"):
             d["interpolation"] = c.interpolation.to_json()

@@ -37,7 +37,8 @@ const char *draugr_amg_last_error(void);
 /* Config from JSON; all keys optional, unknown keys ignored, defaults of
  * AMGConfig() (src/types.jl:514-536).  Keys of c_api/draugr_amg.h:58-89 plus
  * "precision" ("fp64" | "fp32" | "bf16": arithmetic of the V/W cycle),
- * "seed" (setup RNG seed), "group_size" (lanes per CSR row, 0 = auto),
+ * "seed" (setup RNG seed), "graphs" (true: the PCG / mixed-precision preconditioner cycle is
+ * captured once into a hipGraph and replayed; false: launched eagerly),
  * "filtering", "filter_tol".  Returns a handle > 0, or -1.
  * Replaces c_api/draugr_amg.h:94. */
 int32_t draugr_amg_config_from_json(const char *json_str);
