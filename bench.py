@@ -10,7 +10,10 @@ solve time, the speedup, iteration counts, V-cycle time and bandwidth, the
 roofline of the dominant kernel and a CPU baseline (the oracle's OpenMP
 restatement of the same solve on a bounded sample) ride along in the line.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--precision fp32|bf16]
+       python bench.py --kernel-only   (setup, a 256 MiB calibration copy and the
+                                        dominant-kernel launches only: the run the
+                                        rocprofv3 --pmc passes profile)
 """
 import argparse
 import json
@@ -41,7 +44,64 @@ def parse():
     ap.add_argument("--cpu-sample-iters", type=int, default=3,
                     help="PCG iterations of the CPU baseline sample (its time is scaled to a full solve)")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only the calibration copy and the dominant-kernel launches (for counter runs)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "stream_jacobi_level1_128cubed_traffic.md"),
+                    help="summary of the FETCH_SIZE / WRITE_SIZE passes holding the measured HBM bytes per launch")
     return ap.parse_args()
+
+
+def measured_traffic(path, kernel_tag):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (a line 'traffic_bytes_per_launch <tag> <bytes>'), or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel_tag:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
+def dominant_kernel(torch, h, prec, reps):
+    """The largest launch of the cycle: one Jacobi / l1-Jacobi step on level 1
+    (the densest operator of the hierarchy), in the cycle's precision, timed
+    with events on torch's current stream, which is the stream the library
+    launches it on.  Returns (seconds per launch, algorithmic bytes, tag)."""
+    li = h.level_info(1)
+    n, nnz = li["n"], li["nnz"]
+    xa = torch.randn(n, dtype=torch.float32, device="cuda")
+    ba = torch.randn(n, dtype=torch.float32, device="cuda")
+    oa = torch.empty(n, dtype=torch.float32, device="cuda")
+    fn = lambda: h.level_op(1, 2, prec, xa, ba, oa)
+    fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for e0, e1 in ev:  # one event pair per launch: the kernel's own duration, not the gap between launches
+        e0.record()
+        fn()
+        e1.record()
+    torch.cuda.synchronize()
+    t = float(np.median([e0.elapsed_time(e1) for e0, e1 in ev])) / 1e3
+    sv = 4 if prec == "fp32" else 2
+    nbytes = nnz * (sv + 4) + (n + 1) * 4 + 4 * n * 4  # A_1 once; x, b, dinv read, x_out written
+    return t, nbytes, "level1_jacobi_%s_grid%d" % (prec, h_blocks(h, 1))
+
+
+def h_blocks(h, l):
+    return int(h.level_info(l)["nblk"])
+
+
+def calibration_copy(torch):
+    """A plain 256 MiB device-to-device copy: the counter runs scale
+    FETCH_SIZE / WRITE_SIZE by what they read for these known bytes."""
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
 
 
 def log(*a):
@@ -102,6 +162,12 @@ def main():
     log("setup + upload %.2fs, resident %.1f MB" % (t_setup, h.resident_bytes() / 1e6))
 
     b = torch.tensor(b_host, device="cuda")
+    if args.kernel_only:
+        calibration_copy(torch)
+        t_k, kbytes, tag = dominant_kernel(torch, h, args.precision, 20)
+        print(json.dumps({"kernel": tag, "bytes_per_launch": kbytes, "avg_launch_us": t_k * 1e6,
+                          "achieved_GBps": kbytes / t_k / 1e9}))
+        return
     t64, its64, rel64, x64 = time_solves(dh, torch, h, cfg64, b, args.steps, args.warmup, args.tol)
     log("FP64 cycle: %.4f s/solve, iterations %s, relres %.2e" % (t64, its64, rel64))
     t32, its32, rel32, x32 = time_solves(dh, torch, h, cfg32, b, args.steps, args.warmup, args.tol)
@@ -118,17 +184,10 @@ def main():
     t_cyc64 = event_time(torch, lambda: dh.amg_cycle_(z, b, h, cfg64, zero_guess=True), 20)
     bytes32, bytes64 = h.cycle_bytes(cfg32), h.cycle_bytes(cfg64)
 
-    # dominant kernel: level-0 Jacobi step in the cycle's precision, timed live
-    tdt = torch.float32
-    n0, nnz0 = A.n, A.nnz
-    xa = torch.randn(n0, dtype=tdt, device="cuda")
-    ba = torch.randn(n0, dtype=tdt, device="cuda")
-    oa = torch.empty(n0, dtype=tdt, device="cuda")
-    prec_code = args.precision
-    t_k = event_time(torch, lambda: h.level_op(0, 2, prec_code, xa, ba, oa), 50)
-    sv = 4 if args.precision == "fp32" else 2
-    kbytes = nnz0 * (sv + 4) + (n0 + 1) * 4 + 4 * n0 * 4
+    # dominant kernel, timed live
+    t_k, kbytes, tag = dominant_kernel(torch, h, args.precision, 50)
     ach = kbytes / t_k / 1e9
+    traffic = measured_traffic(args.traffic, tag)
 
     out = {
         "metric": METRIC,
@@ -160,8 +219,9 @@ def main():
         "vcycle_bytes": {"mixed": bytes32, "fp64": bytes64},
         "setup_s": t_setup,
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "level-0 %s Jacobi step (k_csr_rows EpiJacobi)" % args.precision,
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "%s: k_csr_stream EpiJacobi, l1-Jacobi step on level 1 (%d rows, %d nnz)"
+                               % (tag, h.level_info(1)["n"], h.level_info(1)["nnz"]),
                      "bytes_per_launch": kbytes, "avg_launch_us": t_k * 1e6},
     }
 

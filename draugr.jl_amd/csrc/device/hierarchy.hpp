@@ -78,6 +78,7 @@ public:
     const int32_t* finest_cv() const { return L_.empty() ? Ac_.cv.p : L_[0].A.cv.p; }
     int64_t finest_nnz() const { return L_.empty() ? Ac_.nnz : L_[0].A.nnz; }
     int nlevels() const { return (int)L_.size(); }
+    int level_blocks(int l) const { return l < nlevels() ? L_[l].A.nblk : Ac_.nblk; }
     hipStream_t stream() const { return stream_; }
 
     // One V/W cycle on device vectors x (in/out) and b; zero_guess asserts x == 0

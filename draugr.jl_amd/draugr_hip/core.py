@@ -107,7 +107,7 @@ class AMGHierarchy:
     def level_info(self, l: int) -> dict:
         out = (ctypes.c_int64 * 8)()
         m = check(_lib.lib().draugr_hip_level_info(self.handle, l, out, 8), "draugr_hip_level_info")
-        keys = ["n", "nnz", "n_coarse", "nnz_P", "ntriples", "num_colors", "has_rmap"]
+        keys = ["n", "nnz", "n_coarse", "nnz_P", "ntriples", "num_colors", "has_rmap", "nblk"]
         return dict(zip(keys, [int(out[i]) for i in range(m)]))
 
     def _export(self, l, which, count, dtype):

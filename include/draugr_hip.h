@@ -88,7 +88,8 @@ int32_t draugr_hip_upload(int32_t handle);
  * total operator nnz, total rows.  Returns the number of values written. */
 int32_t draugr_hip_hierarchy_info(int32_t handle, int64_t *out, int32_t nout);
 
-/* out[0..]: n, nnz(A), n_coarse, nnz(P), ntriples, num_colors, has_rmap. */
+/* out[0..]: n, nnz(A), n_coarse, nnz(P), ntriples, num_colors, has_rmap,
+ * row blocks of A on the device (0 before upload). */
 int32_t draugr_hip_level_info(int32_t handle, int32_t level, int64_t *out, int32_t nout);
 
 /* Copy a host array of level `level` (0-based) into dst, 0-based indices:
