@@ -5,6 +5,7 @@
 // x in place.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -13,6 +14,7 @@
 #include <string>
 
 #include "../../../include/draugr_hip.h"
+#include "batch.hpp"
 #include "hierarchy.hpp"
 
 namespace draugr {
@@ -29,8 +31,22 @@ struct Hier {
     std::mutex mu;
 };
 
+struct Batch {
+    Config cfg;
+    std::unique_ptr<BatchedHierarchy> dev;
+    double setup_seconds = 0.0;
+    hipEvent_t last = nullptr;   // end of the previous call's work (calls share workspaces)
+    bool last_valid = false;
+    std::mutex mu;
+    ~Batch() {
+        if (last_valid) (void)hipEventSynchronize(last);
+        if (last) (void)hipEventDestroy(last);
+    }
+};
+
 std::mutex g_lock;
 std::map<int32_t, std::shared_ptr<Hier>> g_hier;
+std::map<int32_t, std::shared_ptr<Batch>> g_batch;
 std::map<int32_t, Config> g_cfg;
 int32_t g_next = 1;
 
@@ -126,6 +142,22 @@ int32_t setup_impl(int32_t n, int32_t nnz, const int32_t* rp, const int32_t* cv,
     const int32_t h = new_handle();
     g_hier[h] = H;
     return h;
+}
+
+std::shared_ptr<Batch> get_batch(int32_t h) {
+    std::lock_guard<std::mutex> g(g_lock);
+    auto it = g_batch.find(h);
+    if (it == g_batch.end()) throw Error("Invalid batch handle: " + std::to_string(h));
+    return it->second;
+}
+
+void batch_begin(Batch& B, hipStream_t st) {
+    if (B.last_valid) DRAUGR_HIP_CHECK(hipStreamWaitEvent(st, B.last, 0));
+}
+void batch_end(Batch& B, hipStream_t st) {
+    if (!B.last) DRAUGR_HIP_CHECK(hipEventCreateWithFlags(&B.last, hipEventDisableTiming));
+    DRAUGR_HIP_CHECK(hipEventRecord(B.last, st));
+    B.last_valid = true;
 }
 
 // The caller's stream as given: NULL is HIP's default (null) stream.
@@ -536,6 +568,108 @@ int32_t draugr_hip_resident_bytes(int32_t handle, int64_t* bytes) {
         if (!H) throw Error("Invalid hierarchy handle: " + std::to_string(handle));
         require_device(*H);
         *bytes = H->dev->bytes_resident();
+        return 0;
+    });
+}
+
+// ── batches of independent systems ──────────────────────────────────────────
+int32_t draugr_hip_batch_setup(int32_t nsys, const int32_t* n, const int32_t* nnz, const int32_t* const* rowptr,
+                               const int32_t* const* colval, const double* const* nzval, int32_t config_handle,
+                               int32_t index_base, int32_t nrhs) {
+    return guarded([&]() -> int32_t {
+        Config cfg;
+        if (!get_cfg(config_handle, cfg)) throw Error("Invalid config handle: " + std::to_string(config_handle));
+        if (nsys < 1 || !n || !nnz || !rowptr || !colval || !nzval)
+            throw Error("ArgumentError: a batch needs nsys >= 1 and non-null per-system arrays");
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<HostHierarchy> host(nsys);
+        for (int32_t s = 0; s < nsys; ++s) {
+            CSR A = csr_from_raw(n[s], nnz[s], rowptr[s], colval[s], nzval[s], index_base);
+            host[s] = amg_setup_host(A, cfg, false);
+        }
+        std::vector<const HostHierarchy*> hp(nsys);
+        for (int32_t s = 0; s < nsys; ++s) hp[s] = &host[s];
+        auto B = std::make_shared<Batch>();
+        B->cfg = cfg;
+        B->dev = std::make_unique<BatchedHierarchy>(hp, cfg, nrhs);
+        B->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::lock_guard<std::mutex> g(g_lock);
+        const int32_t h = new_handle();
+        g_batch[h] = B;
+        return h;
+    });
+}
+
+int32_t draugr_hip_batch_info(int32_t batch, int64_t* out, int32_t nout) {
+    return guarded([&]() -> int32_t {
+        auto B = get_batch(batch);
+        std::lock_guard<std::mutex> g(B->mu);
+        const BatchedHierarchy& D = *B->dev;
+        const int64_t v[] = {D.nsys(), D.nrhs(), D.rows(), D.nlevels_max(), (int64_t)(B->setup_seconds * 1e6),
+                             D.bytes_resident()};
+        const int32_t m = std::min<int32_t>(nout, (int32_t)(sizeof v / sizeof v[0]));
+        for (int32_t i = 0; i < m; ++i) out[i] = v[i];
+        return m;
+    });
+}
+
+int32_t draugr_hip_batch_system_info(int32_t batch, int32_t sys, int64_t* out, int32_t nout) {
+    return guarded([&]() -> int32_t {
+        auto B = get_batch(batch);
+        const BatchedHierarchy& D = *B->dev;
+        if (sys < 0 || sys >= D.nsys()) throw Error("system index out of range");
+        const int64_t v[] = {D.row_offset(sys + 1) - D.row_offset(sys), D.row_offset(sys), D.sys_levels(sys),
+                             D.sys_coarse(sys)};
+        const int32_t m = std::min<int32_t>(nout, (int32_t)(sizeof v / sizeof v[0]));
+        for (int32_t i = 0; i < m; ++i) out[i] = v[i];
+        return m;
+    });
+}
+
+int32_t draugr_hip_batch_cycle_device(int32_t batch, double* x, const double* b, int32_t config_handle,
+                                      void* stream) {
+    return guarded([&]() -> int32_t {
+        auto B = get_batch(batch);
+        Config cfg;
+        if (!get_cfg(config_handle, cfg)) throw Error("Invalid config handle: " + std::to_string(config_handle));
+        if (!x || !b) throw Error("ArgumentError: null vector");
+        std::lock_guard<std::mutex> g(B->mu);
+        hipStream_t st = (hipStream_t)stream;
+        batch_begin(*B, st);
+        B->dev->cycle(x, b, opts_of(cfg), st);
+        batch_end(*B, st);
+        return 0;
+    });
+}
+
+int32_t draugr_hip_batch_pcg_device(int32_t batch, double* x, const double* b, int32_t config_handle, double tol,
+                                    int32_t maxiter, void* stream, int32_t* iters, double* relres) {
+    return guarded([&]() -> int32_t {
+        auto B = get_batch(batch);
+        Config cfg;
+        if (!get_cfg(config_handle, cfg)) throw Error("Invalid config handle: " + std::to_string(config_handle));
+        if (!x || !b) throw Error("ArgumentError: null vector");
+        if (maxiter < 0) throw Error("ArgumentError: maxiter must be non-negative");
+        std::lock_guard<std::mutex> g(B->mu);
+        hipStream_t st = (hipStream_t)stream;
+        batch_begin(*B, st);
+        const int it = B->dev->pcg(x, b, tol, maxiter, opts_of(cfg), st, iters, relres);
+        batch_end(*B, st);
+        return it;
+    });
+}
+
+int32_t draugr_hip_batch_free(int32_t batch) {
+    return guarded([&]() -> int32_t {
+        std::shared_ptr<Batch> B;
+        {
+            std::lock_guard<std::mutex> g(g_lock);
+            auto it = g_batch.find(batch);
+            if (it == g_batch.end()) throw Error("Invalid batch handle: " + std::to_string(batch));
+            B = it->second;
+            g_batch.erase(it);
+        }
+        std::lock_guard<std::mutex> g(B->mu);
         return 0;
     });
 }

@@ -505,6 +505,18 @@ void DeviceHierarchy::galerkin(int l, hipStream_t st) {
 
 // Coarsest operator: dense from its CSR values, then LU (src/setup.jl:177-209,
 // src/resetup.jl:123-158 and :44), all on the device.
+void dense_lu_device(int n, double* a, int* piv, int* flag, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_dense_lu, dim3(1), dim3(kBlock), 0, st, n, a, piv, flag);
+}
+
+void invdiag_device(int n, const int* rp, const int* cv, const double* av, double* d, bool l1, hipStream_t st) {
+    if (n <= 0) return;
+    const int nb = (n + kBlock - 1) / kBlock;
+    if (l1) hipLaunchKernelGGL(k_invdiag<true>, dim3(nb), dim3(kBlock), 0, st, n, rp, cv, av, d);
+    else hipLaunchKernelGGL(k_invdiag<false>, dim3(nb), dim3(kBlock), 0, st, n, rp, cv, av, d);
+}
+
 void DeviceHierarchy::coarse_factor(hipStream_t st) {
     const int n = nc_;
     if (n == 0) return;

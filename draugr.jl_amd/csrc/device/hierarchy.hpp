@@ -62,6 +62,13 @@ struct CycleOpts {
     bool graphs = true;  // zero-guess cycles on fixed buffers replayed from a hipGraph
 };
 
+// Kernels shared with the batched path (batch.hip):
+// dense LU with partial pivoting in place (column-major n x n; piv[0..n) the
+// LAPACK pivots, piv[n..2n) the row permutation; *flag = 1 if singular), and
+// the plain / l1 inverse diagonal of a CSR operator.
+void dense_lu_device(int n, double* a, int* piv, int* flag, hipStream_t st);
+void invdiag_device(int n, const int* rp, const int* cv, const double* av, double* d, bool l1, hipStream_t st);
+
 // Differentiation kernels (adjoint.hip)
 void assemble_7pt(int m, const double* k, const int* rp, double* nz, hipStream_t st);
 void matrix_grad(int n, const int* rp, const int* cv, const double* x, const double* lam, double* g, hipStream_t st);

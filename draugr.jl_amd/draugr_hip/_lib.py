@@ -48,6 +48,12 @@ SIGNATURES = {
     "draugr_hip_assemble_7pt": (_i32, [_i32, _i32, _vp, _vp, _vp]),
     "draugr_hip_matrix_grad": (_i32, [_i32, _vp, _vp, _vp, _vp]),
     "draugr_hip_grad_logk_7pt": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp]),
+    "draugr_hip_batch_setup": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32]),
+    "draugr_hip_batch_info": (_i32, [_i32, _i64p, _i32]),
+    "draugr_hip_batch_system_info": (_i32, [_i32, _i32, _i64p, _i32]),
+    "draugr_hip_batch_cycle_device": (_i32, [_i32, _vp, _vp, _i32, _vp]),
+    "draugr_hip_batch_pcg_device": (_i32, [_i32, _vp, _vp, _i32, _f64, _i32, _vp, _vp, _vp]),
+    "draugr_hip_batch_free": (_i32, [_i32]),
 }
 
 _lib = None

@@ -154,6 +154,43 @@ int32_t draugr_hip_matrix_grad(int32_t handle, const double *x, const double *la
 int32_t draugr_hip_grad_logk_7pt(int32_t m, const double *k, const double *x, const double *lambda, double *grad,
                                  void *stream);
 
+/* ── Batches of independent systems (no counterpart in the reference, which
+ *    solves one system and one right-hand side per call: src/cycle.jl:12, :110,
+ *    c_api/draugr_amg.h:167-180) ──────────────────────────────────────────── */
+
+/* Set up nsys systems (system s: n[s] x n[s] CSR with nnz[s] entries, the
+ * arrays of c_api/draugr_amg.h:115-120 per system) with one config, each with
+ * its own hierarchy, and pack them for nrhs right-hand sides (a power of two,
+ * 1..256).  The smoother must be jacobi or l1_jacobi and every system must have
+ * at least one AMG level.  Returns a batch handle > 0, or -1. */
+int32_t draugr_hip_batch_setup(int32_t nsys, const int32_t *n, const int32_t *nnz, const int32_t *const *rowptr,
+                               const int32_t *const *colval, const double *const *nzval, int32_t config_handle,
+                               int32_t index_base, int32_t nrhs);
+
+/* out[0..]: nsys, nrhs, total rows, deepest hierarchy (AMG levels), setup
+ * microseconds, resident bytes.  Returns the number of values written. */
+int32_t draugr_hip_batch_info(int32_t batch, int64_t *out, int32_t nout);
+
+/* out[0..]: n, row offset in the packed vectors, AMG levels, coarsest size. */
+int32_t draugr_hip_batch_system_info(int32_t batch, int32_t sys, int64_t *out, int32_t nout);
+
+/* Vectors of a batch are packed and in HBM: system s occupies rows
+ * [offset_s, offset_s + n_s), row i holds its nrhs values contiguously
+ * (entry (i, j) at i * nrhs + j).  x = V(0; b), one V(1,1) cycle per system and
+ * right-hand side, in the config's precision ("fp64" or "fp32"). */
+int32_t draugr_hip_batch_cycle_device(int32_t batch, double *x, const double *b, int32_t config_handle,
+                                      void *stream);
+
+/* FP64 PCG with the batched cycle as preconditioner, each (system, rhs)
+ * stopping on its own ||r||/||b|| < tol; x is the initial guess and the result.
+ * iters[s * nrhs + j], relres[s * nrhs + j] (may be NULL) receive each pair's
+ * iteration count and final recurrence residual.  Returns the largest
+ * iteration count, or -1. */
+int32_t draugr_hip_batch_pcg_device(int32_t batch, double *x, const double *b, int32_t config_handle, double tol,
+                                    int32_t maxiter, void *stream, int32_t *iters, double *relres);
+
+int32_t draugr_hip_batch_free(int32_t batch);
+
 /* Kernel choices of the autotuner (per level and cycle precision) as text;
  * returns the length.  The tuner runs on the first cycle of each precision. */
 int32_t draugr_hip_tune_report(int32_t handle, char *buf, int32_t buflen);

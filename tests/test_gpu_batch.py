@@ -1,0 +1,195 @@
+"""GPU parity of the batched path (many uneven systems, K right-hand sides).
+
+* A batched V(1,1) cycle equals, for every system and right-hand side, the
+  single-system cycle of that system on that right-hand side bit for bit,
+  in FP64 and FP32 (same per-row sums, same epilogues, same coarse solve),
+  and in FP64 it equals the oracle's cycle bit for bit.
+* The batched FP64 PCG agrees with the oracle's PCG per (system, rhs) to
+  1e-10 relative with iteration counts within 1; the FP32-cycle PCG reaches
+  the tolerance (true residual) per pair.
+* The graph-replayed PCG equals the eagerly launched one bit for bit.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from conftest import oracle_of, rel_err
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261202
+EDGES = [6, 13, 9, 16, 11]  # uneven systems: different depths and coarsest sizes
+
+
+def _systems(dh, edges, sigma=1.0):
+    P = dh.problems
+    return [P.heterogeneous3d(m, P.log_permeability(m, sigma, SEED + 400 + m)) for m in edges]
+
+
+def _cfg(dh, **kw):
+    kw.setdefault("max_coarse_size", 12)
+    return dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), **kw)
+
+
+def _rhs(dh, batch, seed):
+    P = dh.problems
+    return [P.splitmix_normal(seed + s, n * batch.nrhs).reshape(n, batch.nrhs) for s, n in enumerate(batch.sizes)]
+
+
+_cache = {}
+
+
+def _setup(dh, edges, K, **kw):
+    key = (tuple(edges), K, tuple(sorted(kw.items())))
+    if key not in _cache:
+        mats = _systems(dh, edges)
+        cfg = _cfg(dh, **kw)
+        batch = dh.amg_batch_setup(mats, cfg, nrhs=K)
+        singles = [dh.amg_setup(A, cfg) for A in mats]
+        _cache[key] = (mats, cfg, batch, singles)
+    return _cache[key]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("K", [1, 4])
+def test_batched_cycle_equals_single_cycles(dh, torch_cuda, prec, K):
+    torch = torch_cuda
+    mats, cfg, batch, singles = _setup(dh, EDGES, K)
+    assert len({h.nlevels for h in singles}) > 1  # uneven depths exercised
+    c = dataclasses.replace(cfg, precision=prec)
+    bs = _rhs(dh, batch, SEED + 500)
+    b = batch.pack(bs)
+    x = torch.zeros_like(b)
+    batch.cycle_(x, b, c)
+    torch.cuda.synchronize()
+    got = batch.unpack(x)
+    for s, h in enumerate(singles):
+        for j in range(K):
+            bj = torch.tensor(np.ascontiguousarray(bs[s][:, j]), device="cuda")
+            xj = torch.zeros_like(bj)
+            dh.amg_cycle_(xj, bj, h, c, zero_guess=True)
+            torch.cuda.synchronize()
+            assert np.array_equal(got[s][:, j], xj.cpu().numpy()), (prec, K, s, j)
+
+
+def test_batched_cycle_equals_oracle(dh, orc, torch_cuda):
+    torch = torch_cuda
+    mats, cfg, batch, singles = _setup(dh, EDGES, 4)
+    bs = _rhs(dh, batch, SEED + 510)
+    b = batch.pack(bs)
+    x = torch.zeros_like(b)
+    batch.cycle_(x, b, dataclasses.replace(cfg, precision="fp64"))
+    got = batch.unpack(x)
+    for s, h in enumerate(singles):
+        oh = oracle_of(h, orc, "l1_jacobi")
+        for j in range(4):
+            ref = oh.cycle(np.zeros(batch.sizes[s]), bs[s][:, j])
+            assert np.array_equal(got[s][:, j], ref), (s, j)
+
+
+def test_batched_cycle_large_coarsest(dh, torch_cuda):
+    """Coarsest levels above 64 unknowns take the one-thread substitution."""
+    torch = torch_cuda
+    mats, cfg, batch, singles = _setup(dh, [12, 16, 14], 2, max_coarse_size=150)
+    assert any(batch.system_info(s)["n_coarse"] > 64 for s in range(batch.nsys))
+    bs = _rhs(dh, batch, SEED + 520)
+    b = batch.pack(bs)
+    x = torch.zeros_like(b)
+    batch.cycle_(x, b, dataclasses.replace(cfg, precision="fp64"))
+    got = batch.unpack(x)
+    for s, h in enumerate(singles):
+        for j in range(2):
+            bj = torch.tensor(np.ascontiguousarray(bs[s][:, j]), device="cuda")
+            xj = torch.zeros_like(bj)
+            dh.amg_cycle_(xj, bj, h, dataclasses.replace(cfg, precision="fp64"), zero_guess=True)
+            torch.cuda.synchronize()
+            assert np.array_equal(got[s][:, j], xj.cpu().numpy()), (s, j)
+
+
+def test_batched_pcg_fp64_matches_oracle(dh, orc, torch_cuda):
+    torch = torch_cuda
+    mats, cfg, batch, singles = _setup(dh, EDGES, 4)
+    bs = _rhs(dh, batch, SEED + 530)
+    b = batch.pack(bs)
+    x = torch.zeros_like(b)
+    _, its, rel = batch.pcg_solve_(x, b, dataclasses.replace(cfg, precision="fp64"), tol=1e-10, maxiter=300)
+    got = batch.unpack(x)
+    for s, h in enumerate(singles):
+        oh = oracle_of(h, orc, "l1_jacobi")
+        for j in range(4):
+            xo, ito, _ = oh.pcg(np.zeros(batch.sizes[s]), bs[s][:, j], tol=1e-10, maxiter=300)
+            assert abs(int(its[s, j]) - ito) <= 1, (s, j, its[s, j], ito)
+            assert rel_err(got[s][:, j], xo) < 1e-10, (s, j)
+            assert rel[s, j] < 1e-10
+
+
+def test_batched_pcg_mixed_reaches_tolerance(dh, torch_cuda):
+    torch = torch_cuda
+    mats, cfg, batch, singles = _setup(dh, EDGES, 4)
+    bs = _rhs(dh, batch, SEED + 540)
+    b = batch.pack(bs)
+    x64 = torch.zeros_like(b)
+    _, its64, _ = batch.pcg_solve_(x64, b, dataclasses.replace(cfg, precision="fp64"), tol=1e-8)
+    x32 = torch.zeros_like(b)
+    _, its32, rel32 = batch.pcg_solve_(x32, b, dataclasses.replace(cfg, precision="fp32"), tol=1e-8)
+    got = batch.unpack(x32)
+    for s, A in enumerate(mats):
+        for j in range(4):
+            r = bs[s][:, j] - A.matvec(got[s][:, j])
+            assert np.linalg.norm(r) / np.linalg.norm(bs[s][:, j]) < 1.5e-8, (s, j)
+            assert its32[s, j] <= its64[s, j] + 1, (s, j, its32[s, j], its64[s, j])
+    assert (rel32 < 1e-8).all()
+
+
+def test_batched_pcg_graph_equals_eager(dh, torch_cuda):
+    torch = torch_cuda
+    mats, cfg, batch, singles = _setup(dh, EDGES, 4)
+    bs = _rhs(dh, batch, SEED + 550)
+    b = batch.pack(bs)
+    for prec in ("fp64", "fp32"):
+        for rep in range(2):
+            xg = torch.zeros_like(b)
+            _, itg, relg = batch.pcg_solve_(xg, b, dataclasses.replace(cfg, precision=prec, graphs=True), tol=1e-9)
+            torch.cuda.synchronize()
+            xe = torch.zeros_like(b)
+            _, ite, rele = batch.pcg_solve_(xe, b, dataclasses.replace(cfg, precision=prec, graphs=False), tol=1e-9)
+            torch.cuda.synchronize()
+            assert np.array_equal(itg, ite) and np.array_equal(relg, rele), (prec, rep)
+            assert torch.equal(xg, xe), (prec, rep)
+
+
+def test_batched_pcg_zero_rhs_and_initial_guess(dh, torch_cuda):
+    """A zero right-hand side gives x = 0 in 0 iterations; a converged initial
+    guess stops at once."""
+    torch = torch_cuda
+    mats, cfg, batch, singles = _setup(dh, EDGES, 4)
+    bs = _rhs(dh, batch, SEED + 560)
+    bs[1][:, 2] = 0.0
+    b = batch.pack(bs)
+    x = torch.ones_like(b)
+    _, its, rel = batch.pcg_solve_(x, b, dataclasses.replace(cfg, precision="fp64"), tol=1e-10, maxiter=300)
+    got = batch.unpack(x)
+    assert its[1, 2] == 0 and not got[1][:, 2].any()
+    _, its2, _ = batch.pcg_solve_(x, b, dataclasses.replace(cfg, precision="fp64"), tol=1e-8, maxiter=300)
+    assert (its2 == 0).all()
+
+
+def test_batch_errors(dh, torch_cuda):
+    mats = _systems(dh, [8, 10])
+    with pytest.raises(dh.DraugrError, match="jacobi"):
+        dh.amg_batch_setup(mats, dh.AMGConfig(smoother=dh.ColoredGaussSeidelType()), nrhs=2)
+    with pytest.raises(dh.DraugrError, match="power of two"):
+        dh.amg_batch_setup(mats, _cfg(dh), nrhs=3)
+    with pytest.raises(dh.DraugrError, match="no AMG level"):
+        dh.amg_batch_setup([dh.problems.poisson1d(5)] + mats, _cfg(dh), nrhs=1)
+    batch = dh.amg_batch_setup(mats, _cfg(dh), nrhs=2)
+    b = batch.pack(_rhs(dh, batch, SEED))
+    with pytest.raises(dh.DraugrError, match="fp64 and fp32"):
+        batch.cycle_(b.clone(), b, _cfg(dh, precision="bf16"))
+    with pytest.raises(dh.DraugrError, match="V\\(1,1\\)"):
+        batch.cycle_(b.clone(), b, _cfg(dh, cycle_type="w"))
+    batch.free()
+    with pytest.raises(dh.DraugrError, match="Invalid batch handle"):
+        batch.handle = 99999
+        batch.info()
