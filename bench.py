@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--cpu-sample-iters", type=int, default=3,
                     help="PCG iterations of the CPU baseline sample (its time is scaled to a full solve)")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--workload", default="hetero128",
+                    choices=["hetero128", "poisson256", "resetup192", "batched48", "adjoint96"],
+                    help="hetero128 (default): the headline line; the others print their own JSON line")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the calibration copy and the dominant-kernel launches (for counter runs)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "stream_jacobi_level1_128cubed_traffic.md"),
@@ -137,6 +140,164 @@ def event_time(torch, fn, reps):
     return e0.elapsed_time(e1) / 1e3 / reps
 
 
+def timed(torch, fn):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, out
+
+
+def workload_poisson256(args, torch, dh):
+    """2-D 5-point Poisson 256^2, aggregation (theta 0.25), Jacobi (2/3) V(1,1),
+    max_coarse_size 50, RHS U(-1,1); FP64 PCG to 1e-8 on the GPU and the
+    oracle's CPU PCG on the same hierarchy."""
+    P = dh.problems
+    A = P.poisson2d(256)
+    b_host = 2.0 * P.splitmix_uniform(SEED + 0, A.n) - 1.0
+    cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType(),
+                       jacobi_omega=2.0 / 3.0, max_coarse_size=50)
+    h = dh.amg_setup(A, cfg)
+    b = torch.tensor(b_host, device="cuda")
+    t, its, rel, x = time_solves(dh, torch, h, cfg, b, args.steps, args.warmup, args.tol)
+    out = {"workload": "poisson256", "n": A.n, "nnz": A.nnz, "levels": h.nlevels, "gpu_s_per_solve": t,
+           "iterations": its[-1], "relres": rel}
+    if not args.skip_cpu:
+        from oracle import oracle as orc
+        levels = [h.level(l, with_rmap=False) for l in range(h.nlevels)]
+        c = h.coarsest()
+        oh = orc.OracleHierarchy(levels, c["rp"], c["cv"], c["nz"], smoother="jacobi")
+        t0 = time.perf_counter()
+        xo, ito, _ = oh.pcg(np.zeros(A.n), b_host, args.tol, 500)
+        out.update({"cpu_oracle_s_per_solve": time.perf_counter() - t0, "cpu_iterations": ito,
+                    "rel_err_gpu_vs_cpu": float(np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo))})
+    print(json.dumps(out))
+
+
+def workload_resetup192(args, torch, dh):
+    """3-D heterogeneous 7-point 192^3 (sigma 1.5): one setup, then 10 partial
+    resetups with k <- k exp(N(0, 0.3^2)) assembled on the device; per resetup
+    the FP64-, FP32- and BF16-cycle PCG solves to 1e-8 with their true
+    residuals and errors against the FP64 solution."""
+    from draugr_hip import adjoint as dadj
+    P = dh.problems
+    m = 192
+    logk = P.log_permeability(m, 1.5, SEED + 3)
+    A = P.heterogeneous3d(m, logk)
+    cfg = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())
+    t0 = time.perf_counter()
+    h = dh.amg_setup(A, cfg)
+    t_setup = time.perf_counter() - t0
+    log(h.summary())
+    b = torch.tensor(P.splitmix_normal(SEED + 13, A.n), device="cuda")
+    lk = torch.tensor(logk, device="cuda")
+    rows = []
+    for r in range(10):
+        eps = torch.tensor(0.3 * P.splitmix_normal(SEED + 100 + r, A.n), device="cuda")
+        lk = lk + eps
+        k = torch.exp(lk)
+        t_res, _ = timed(torch, lambda: dh.amg_resetup_(h, dadj.assemble_7pt_(h, m, k), cfg))
+        row = {"assemble_and_resetup_s": t_res}
+        xs = {}
+        for prec in ("fp64", "fp32", "bf16"):
+            c = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), precision=prec)
+            x = torch.zeros_like(b)
+            dh.pcg_solve_(x, b, h, c, tol=args.tol, maxiter=500)  # warm (tuning, mirrors, graphs)
+            x.zero_()
+            t, (_, it, rel) = timed(torch, lambda: dh.pcg_solve_(x, b, h, c, tol=args.tol, maxiter=500))
+            xs[prec] = x
+            row[prec] = {"s": t, "iterations": it, "relres": rel}
+        nz = dadj.assemble_7pt_(h, m, k)
+        import scipy.sparse as sp
+        Ah = sp.csr_matrix((nz.cpu().numpy(), A.colval, A.rowptr), shape=(A.n, A.n))
+        bh = b.cpu().numpy()
+        x64 = xs["fp64"].cpu().numpy()
+        for prec in ("fp64", "fp32", "bf16"):
+            xp = xs[prec].cpu().numpy()
+            row[prec]["true_relres"] = float(np.linalg.norm(bh - Ah @ xp) / np.linalg.norm(bh))
+            row[prec]["rel_err_vs_fp64"] = float(np.linalg.norm(xp - x64) / np.linalg.norm(x64))
+        log("resetup %d: %s" % (r, json.dumps(row)))
+        rows.append(row)
+    avg = lambda key: float(np.mean([r_[key]["s"] for r_ in rows]))
+    print(json.dumps({"workload": "resetup192", "n": A.n, "nnz": A.nnz, "levels": h.nlevels, "setup_s": t_setup,
+                      "assemble_and_resetup_s_mean": float(np.mean([r_["assemble_and_resetup_s"] for r_ in rows])),
+                      "pcg_s_mean": {p_: avg(p_) for p_ in ("fp64", "fp32", "bf16")},
+                      "speedup_fp32": avg("fp64") / avg("fp32"), "speedup_bf16": avg("fp64") / avg("bf16"),
+                      "per_resetup": rows}))
+
+
+def workload_batched48(args, torch, dh):
+    """48 independent 3-D heterogeneous 7-point systems, edges uniform in
+    {32..96} (sigma 1.0), 16 right-hand sides N(0,1) each; batched FP32-cycle
+    (and FP64-cycle) PCG to 1e-8, one graph replay per cycle."""
+    P = dh.problems
+    u = P.splitmix_uniform(SEED + 4, 48)
+    edges = [32 + int(v * 65) for v in u]
+    mats = [P.heterogeneous3d(m, P.log_permeability(m, 1.0, SEED + 1000 + s)) for s, m in enumerate(edges)]
+    K = 16
+    cfg = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())
+    t0 = time.perf_counter()
+    batch = dh.amg_batch_setup(mats, cfg, nrhs=K)
+    t_setup = time.perf_counter() - t0
+    info = batch.info()
+    log("batch: %d systems, %d rows, %d rhs, deepest %d levels, setup %.1fs, resident %.1f GB"
+        % (info["nsys"], info["rows"], K, info["max_levels"], t_setup, info["resident_bytes"] / 1e9))
+    b = torch.tensor(P.splitmix_normal(SEED + 14, info["rows"] * K), device="cuda")
+    out = {"workload": "batched48", "systems": 48, "rhs": K, "rows": info["rows"],
+           "edges": edges, "setup_s": t_setup}
+    for prec in ("fp64", "fp32"):
+        c = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), precision=prec)
+        x = torch.zeros_like(b)
+        batch.pcg_solve_(x, b, c, tol=args.tol, maxiter=500)
+        ts = []
+        for _ in range(max(1, args.steps // 3)):
+            x.zero_()
+            t, (_, its, rel) = timed(torch, lambda: batch.pcg_solve_(x, b, c, tol=args.tol, maxiter=500))
+            ts.append(t)
+        t = float(np.median(ts))
+        out[prec] = {"s_per_batched_solve": t, "solves_per_s": 48 * K / t, "max_iterations": int(its.max()),
+                     "mean_iterations": float(its.mean()), "max_relres": float(rel.max())}
+        if prec == "fp32":
+            xs = batch.unpack(x)
+            bs = batch.unpack(b)
+            worst = 0.0
+            for s_ in range(0, 48, 8):
+                for j in range(0, K, 5):
+                    r = bs[s_][:, j] - mats[s_].matvec(xs[s_][:, j])
+                    worst = max(worst, float(np.linalg.norm(r) / np.linalg.norm(bs[s_][:, j])))
+            out[prec]["true_relres_worst_of_sample"] = worst
+    out["speedup_fp32"] = out["fp64"]["s_per_batched_solve"] / out["fp32"]["s_per_batched_solve"]
+    print(json.dumps(out))
+
+
+def workload_adjoint96(args, torch, dh):
+    """Loss 1/2 |x - x*|^2, A(k) x = b on 96^3 (sigma 1.5): time of one
+    forward (assembly, resetup, PCG) plus backward (adjoint PCG, gradient
+    kernels) evaluation, FP64 and FP32 cycle."""
+    from draugr_hip import adjoint as dadj
+    P = dh.problems
+    m = 96
+    logk = P.log_permeability(m, 1.5, SEED + 5)
+    cfg = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())
+    h = dh.amg_setup(P.heterogeneous3d(m, logk), cfg)
+    b = torch.tensor(P.splitmix_normal(SEED + 105, m ** 3), device="cuda")
+    xs = torch.tensor(P.splitmix_normal(SEED + 205, m ** 3), device="cuda")
+    out = {"workload": "adjoint96", "n": m ** 3}
+    for prec in ("fp64", "fp32"):
+        c = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), precision=prec)
+
+        def evaluate():
+            lk = torch.tensor(logk, device="cuda").requires_grad_(True)
+            x = dadj.solve_7pt(lk, b, h, m, c, tol=1e-10)
+            (0.5 * ((x - xs) ** 2).sum()).backward()
+            return lk.grad
+
+        evaluate()
+        ts = [timed(torch, evaluate)[0] for _ in range(3)]
+        out[prec] = {"s_per_loss_and_gradient": float(np.median(ts))}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -145,6 +306,11 @@ def main():
         sys.exit(2)
     import torch
     import draugr_hip as dh
+    other = {"poisson256": workload_poisson256, "resetup192": workload_resetup192,
+             "batched48": workload_batched48, "adjoint96": workload_adjoint96}
+    if args.workload in other:
+        other[args.workload](args, torch, dh)
+        return
 
     P = dh.problems
     m = args.size
