@@ -78,16 +78,9 @@ def dominant_kernel(torch, h, prec, reps):
     xa = torch.randn(n, dtype=torch.float32, device="cuda")
     ba = torch.randn(n, dtype=torch.float32, device="cuda")
     oa = torch.empty(n, dtype=torch.float32, device="cuda")
-    fn = lambda: h.level_op(1, 2, prec, xa, ba, oa)
-    fn()
-    torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-    for e0, e1 in ev:  # one event pair per launch: the kernel's own duration, not the gap between launches
-        e0.record()
-        fn()
-        e1.record()
-    torch.cuda.synchronize()
-    t = float(np.median([e0.elapsed_time(e1) for e0, e1 in ev])) / 1e3
+    # back-to-back launches between two events: the per-launch time a cycle sees
+    # (an event pair around each single launch adds the events' own overhead)
+    t = event_time(torch, lambda: h.level_op(1, 2, prec, xa, ba, oa), reps)
     sv = 4 if prec == "fp32" else 2
     nbytes = nnz * (sv + 4) + (n + 1) * 4 + 4 * n * 4  # A_1 once; x, b, dinv read, x_out written
     return t, nbytes, "level1_jacobi_%s_grid%d" % (prec, h_blocks(h, 1))
