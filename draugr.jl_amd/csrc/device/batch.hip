@@ -474,7 +474,7 @@ BatchedHierarchy::BatchedHierarchy(const std::vector<const HostHierarchy*>& hs, 
         up(D.P, D.nA, L_[l + 1].nV, prp, pcv, pnz);
         up(D.Pt, L_[l + 1].nV, D.nA, trp, tcv, tnz);
         D.d64.alloc(D.nA);
-        invdiag_device(D.nA, D.A.rp.p, D.A.cv.p, D.A.v64.p, D.d64.p, l1_, st);
+        invdiag_device(D.nA, D.A.rp.p, D.A.cv.p, D.A.v64.p, D.d64.p, l1_ ? 1 : 0, st);
         upload_cast(D.d32, D.d64, D.nA, st);
     }
     // coarsest levels: dense LU per system on the device

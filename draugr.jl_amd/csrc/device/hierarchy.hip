@@ -121,19 +121,22 @@ void launch_stream_reduce(const DevCSR& M, const TV* av, XF xf, EPI epi, FIN fin
 
 // Inverse diagonal for the smoothers: plain (src/smoothers.jl:47-63) or l1
 // (src/smoothers.jl:816-827); one thread per row, row-order sums.
-template <bool L1>
+template <int MODE>  // 0 plain, 1 l1, 2 SPAI(0): m_i = a_ii / sum_j a_ij^2 (src/smoothers.jl:517-532)
 __global__ __launch_bounds__(kBlock) void k_invdiag(int n, const int* __restrict__ rp, const int* __restrict__ cv,
                                                     const double* __restrict__ av, double* __restrict__ d) {
     const int row = blockIdx.x * kBlock + threadIdx.x;
     if (row >= n) return;
     double nrm = 0.0, dg = 0.0;
     for (int k = rp[row]; k < rp[row + 1]; ++k) {
-        nrm += fabs(av[k]);
-        if (cv[k] == row) dg = av[k];  // the last diagonal match wins, as in the reference
+        const double v = av[k];
+        nrm += MODE == 2 ? v * v : fabs(v);
+        if (cv[k] == row) dg = v;  // the last diagonal match wins, as in the reference
     }
     const double eps = 2.220446049250313e-16;
-    if (L1) {
+    if (MODE == 1) {
         d[row] = nrm > eps ? 1.0 / nrm : 0.0;
+    } else if (MODE == 2) {
+        d[row] = nrm > 0.0 ? dg / nrm : 0.0;
     } else {
         const double thr = eps * fmax(1.0, nrm);
         d[row] = fabs(dg) > thr ? 1.0 / dg : 0.0;
@@ -367,10 +370,11 @@ bool is_gs(Smoother s) { return s == Smoother::ColoredGS || s == Smoother::L1Col
 
 // ─────────────────────────────────────────────────────────────────────────────
 DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
-    : smoother_(cfg.smoother), omega_(cfg.jacobi_omega) {
-    if (!(smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || is_gs(smoother_)))
+    : smoother_(cfg.smoother), omega_(cfg.jacobi_omega), seed_(cfg.seed) {
+    if (!(smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || is_gs(smoother_) ||
+          smoother_ == Smoother::SPAI0 || smoother_ == Smoother::Chebyshev))
         throw Error("smoother not available in the HIP backend (supported: jacobi, l1_jacobi, colored_gs, "
-                    "l1_colored_gs)");
+                    "l1_colored_gs, spai0, chebyshev)");
     DRAUGR_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     DRAUGR_HIP_CHECK(hipEventCreateWithFlags(&last_ev_, hipEventDisableTiming));
     hipStream_t st = stream_;
@@ -405,6 +409,26 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
             D.color_order.upload(H.coloring.color_order, st);
         }
         const int32_t n = H.A.nrow, nc = H.P.ncol;
+        if (smoother_ == Smoother::Chebyshev) {
+            // start vector of the power iteration: N(0,1) from the setup seed, normalised
+            std::vector<double> v(n);
+            Rng rng(seed_ + 0x632BE59BD9B4E019ull * (uint64_t)(l + 1));
+            for (int32_t i = 0; i < n; i += 2) {
+                const double u1 = rng.uniform(), u2 = rng.uniform();
+                const double rad = std::sqrt(-2.0 * std::log1p(-u1));
+                v[i] = rad * std::cos(6.283185307179586 * u2);
+                if (i + 1 < n) v[i + 1] = rad * std::sin(6.283185307179586 * u2);
+            }
+            double nrm = 0.0;
+            for (double a : v) nrm += a * a;
+            nrm = std::sqrt(nrm);
+            for (double& a : v) a /= nrm;
+            D.pv0.upload(v, st);
+            DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+            D.pv.alloc(n);
+            D.pw.alloc(n);
+            D.cd64.alloc(n);
+        }
         D.d64.alloc(n);
         D.r64.alloc(n);
         D.t64.alloc(n);
@@ -429,6 +453,7 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
     DRAUGR_HIP_CHECK(hipMemsetAsync(scal_.p, 0, S_COUNT_ * sizeof(double), st));
     DRAUGR_HIP_CHECK(hipHostMalloc((void**)&pinned_, 16 * sizeof(double), hipHostMallocDefault));
     for (int l = 0; l < nlevels(); ++l) compute_dinv(l, st);
+    if (smoother_ == Smoother::Chebyshev) chebyshev_bounds(st);
     coarse_factor(st);
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
     int fl = 0;
@@ -479,16 +504,46 @@ void DeviceHierarchy::compute_dinv(int l, hipStream_t st) {
     DevLevel& D = L_[l];
     const int n = D.A.nrow;
     if (n == 0) return;
-    const int nb = (n + kBlock - 1) / kBlock;
-    if (is_l1(smoother_))
-        hipLaunchKernelGGL(k_invdiag<true>, dim3(nb), dim3(kBlock), 0, st, n, D.A.rp.p, D.A.cv.p, D.A.v64.p, D.d64.p);
-    else
-        hipLaunchKernelGGL(k_invdiag<false>, dim3(nb), dim3(kBlock), 0, st, n, D.A.rp.p, D.A.cv.p, D.A.v64.p,
-                           D.d64.p);
+    invdiag_device(n, D.A.rp.p, D.A.cv.p, D.A.v64.p, D.d64.p, is_l1(smoother_) ? 1 : (smoother_ == Smoother::SPAI0 ? 2 : 0),
+                   st);
     if (D.d32.p) {
         const double* s = D.d64.p;
         float* d = D.d32.p;
         launch_map(n, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
+    }
+}
+
+// Spectral radius of D^-1 A by 10 power-iteration steps from the level's
+// fixed random unit vector (src/smoothers.jl:866-885): w = (A v) .* dinv,
+// lambda = ||w||, v = w / lambda while lambda > eps.  FP64, on the device; the
+// norm is the deterministic two-kernel sum of squares.
+double DeviceHierarchy::estimate_rho(int l, hipStream_t st) {
+    DevLevel& D = L_[l];
+    const int n = D.A.nrow;
+    if (n == 0) return 0.0;
+    double* v = D.pv.p;
+    double* w = D.pw.p;
+    double* scal = scal_.p;
+    DRAUGR_HIP_CHECK(hipMemcpyAsync(v, D.pv0.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    for (int it = 0; it < 10; ++it) {
+        op<double, double>(D.A, XVec<double>{v}, EpiStoreScaled<double>{w, D.d64.p}, st);
+        launch_reduce(n, [=] __device__(int64_t i) { return w[i] * w[i]; },
+                      [=] __device__(double s) { scal[S_RHO] = sqrt(s); }, partial_.p, st);
+        launch_map(n, [=] __device__(int64_t i) {
+            const double lam = scal[S_RHO];
+            if (lam > 2.220446049250313e-16) v[i] = w[i] / lam;
+        }, st);
+    }
+    return reduce_to_host(st, S_RHO);
+}
+
+// Chebyshev bounds per level: lambda_max = 1.1 rho, lambda_min = lambda_max / 30
+// (src/smoothers.jl:897-903, 910-920).
+void DeviceHierarchy::chebyshev_bounds(hipStream_t st) {
+    for (int l = 0; l < nlevels(); ++l) {
+        const double rho = estimate_rho(l, st);
+        L_[l].cheb_lmax = 1.1 * rho;
+        L_[l].cheb_lmin = L_[l].cheb_lmax / 30.0;
     }
 }
 
@@ -510,11 +565,12 @@ void dense_lu_device(int n, double* a, int* piv, int* flag, hipStream_t st) {
     hipLaunchKernelGGL(k_dense_lu, dim3(1), dim3(kBlock), 0, st, n, a, piv, flag);
 }
 
-void invdiag_device(int n, const int* rp, const int* cv, const double* av, double* d, bool l1, hipStream_t st) {
+void invdiag_device(int n, const int* rp, const int* cv, const double* av, double* d, int mode, hipStream_t st) {
     if (n <= 0) return;
     const int nb = (n + kBlock - 1) / kBlock;
-    if (l1) hipLaunchKernelGGL(k_invdiag<true>, dim3(nb), dim3(kBlock), 0, st, n, rp, cv, av, d);
-    else hipLaunchKernelGGL(k_invdiag<false>, dim3(nb), dim3(kBlock), 0, st, n, rp, cv, av, d);
+    if (mode == 1) hipLaunchKernelGGL(k_invdiag<1>, dim3(nb), dim3(kBlock), 0, st, n, rp, cv, av, d);
+    else if (mode == 2) hipLaunchKernelGGL(k_invdiag<2>, dim3(nb), dim3(kBlock), 0, st, n, rp, cv, av, d);
+    else hipLaunchKernelGGL(k_invdiag<0>, dim3(nb), dim3(kBlock), 0, st, n, rp, cv, av, d);
 }
 
 void DeviceHierarchy::coarse_factor(hipStream_t st) {
@@ -562,6 +618,7 @@ void DeviceHierarchy::ensure_precision(Precision p) {
             D.t32.alloc(D.A.nrow);
             D.xc32.alloc(D.P.ncol);
             D.bc32.alloc(D.P.ncol);
+            if (smoother_ == Smoother::Chebyshev) D.cd32.alloc(D.A.nrow);
             for (DevCSR* m : {&D.A, &D.P, &D.Pt}) {
                 const double* s = m->v64.p;
                 float* d = m->v32.p;
@@ -597,7 +654,11 @@ void DeviceHierarchy::smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bo
     DevLevel& D = L_[l];
     const int n = D.A.nrow;
     const TX* d = d_of<TX>(D);
-    const TX w = (TX)omega_;
+    const TX w = (TX)weight();
+    if (smoother_ == Smoother::Chebyshev) {
+        chebyshev<TV, TX>(l, x, alt, b, steps, zero, st);
+        return;
+    }
     if (is_gs(smoother_)) {
         if (zero) {
             TX* xx = x;
@@ -690,7 +751,7 @@ void DeviceHierarchy::tune() {
         launch_map(n, [=] __device__(int64_t i) { xin[i] = (TX)1; bin[i] = (TX)1; s1[i] = (TX)0; s2[i] = (TX)0; }, st);
         launch_map(nc, [=] __device__(int64_t i) { xcv[i] = (TX)1; }, st);
         const TX* d = d_of<TX>(D);
-        const TX w = (TX)omega_;
+        const TX w = (TX)weight();
         float tA[2], tP[2], tPt[2];
         for (int k = 0; k < 2; ++k) {
             tA[k] = timeit([&] { op_kind<TV, TX>(k, D.A, XVec<TX>{xin}, EpiJacobi<TX>{xin, bin, d, s1, w}, st); });
@@ -730,6 +791,39 @@ std::string DeviceHierarchy::tune_report() const {
         }
     }
     return out;
+}
+
+// Chebyshev smoothing (src/smoothers.jl:952-990): per step, degree 3 passes
+// of r = b - A x fused with the three-term update of d and x (out of place:
+// x and alt are swapped after each pass).  The recurrence scalars are formed
+// in FP64 as the reference forms them, then rounded to the cycle precision.
+template <typename TV, typename TX>
+void DeviceHierarchy::chebyshev(int l, TX*& x, TX*& alt, const TX* b, int steps, bool& zero, hipStream_t st) {
+    DevLevel& D = L_[l];
+    const int n = D.A.nrow;
+    if (zero) {
+        TX* xx = x;
+        launch_map(n, [=] __device__(int64_t i) { xx[i] = 0; }, st);
+        zero = false;
+    }
+    const TX* dv = d_of<TX>(D);
+    TX* dd = sizeof(TX) == 8 ? (TX*)D.cd64.p : (TX*)D.cd32.p;
+    const double theta = (D.cheb_lmax + D.cheb_lmin) / 2.0;
+    const double delta = (D.cheb_lmax - D.cheb_lmin) / 2.0;
+    constexpr int kDegree = 3;  // build_chebyshev_smoother's default (src/smoothers.jl:890)
+    for (int s = 0; s < steps; ++s) {
+        op<TV, TX>(D.A, XVec<TX>{x}, EpiCheb0<TX>{x, b, dv, dd, alt, (TX)(1.0 / theta)}, st);
+        std::swap(x, alt);
+        double sigma_old = theta / delta;
+        for (int k = 1; k < kDegree; ++k) {
+            const double sigma_new = 1.0 / (2.0 * theta / delta - sigma_old);
+            const double cr = 2.0 * sigma_new / delta;
+            const double cd = sigma_new * sigma_old;
+            op<TV, TX>(D.A, XVec<TX>{x}, EpiChebK<TX>{x, b, dv, dd, alt, (TX)cr, (TX)cd}, st);
+            std::swap(x, alt);
+            sigma_old = sigma_new;
+        }
+    }
 }
 
 // One visit of level l (src/cycle.jl:63-99).
@@ -1021,6 +1115,7 @@ void DeviceHierarchy::resetup(const double* nzval, hipStream_t st) {
             if (l + 1 < nlevels()) compute_dinv(l + 1, st);
         }
         for (int l = 0; l < nlevels(); ++l) refresh_level_mirrors(l, st);
+        if (smoother_ == Smoother::Chebyshev) chebyshev_bounds(st);
     }
     coarse_factor(st);
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
@@ -1044,6 +1139,10 @@ double DeviceHierarchy::cycle_bytes(const CycleOpts& o) const {
         if (is_gs(smoother_)) {
             lv += n * sx;  // zero fill
             lv += (o.pre + o.post) * (BA + 4 * n * sx);
+            lv += BA + 3 * n * sx;  // residual
+        } else if (smoother_ == Smoother::Chebyshev) {
+            // zero fill, then 3 passes per step: x, b, dinv (and d) read, d and x_out written
+            lv += n * sx + (o.pre + o.post) * (3 * BA + 5 * n * sx + 2 * 6 * n * sx);
             lv += BA + 3 * n * sx;  // residual
         } else {
             // pre-smoothing from zero (fused with the residual when pre == 1)
@@ -1083,7 +1182,7 @@ void DeviceHierarchy::level_op(int lvl, int op_code, int prec, const void* in1, 
         typedef decltype(tx_tag) TX;
         typedef typename TransferT<TV>::type TP;
         const TX* d = d_of<TX>(D);
-        const TX w = (TX)omega_;
+        const TX w = (TX)weight();
         const TX* a = (const TX*)in1;
         const TX* bb = (const TX*)in2;
         TX* o = (TX*)out;
@@ -1103,6 +1202,18 @@ void DeviceHierarchy::level_op(int lvl, int op_code, int prec, const void* in1, 
                                        cnt, D.color_order.p + off, D.A.rp.p, D.A.cv.p, vals<TV>(D.A), o, bb, d);
                 }
                 break;
+            case 10: {  // one Chebyshev smoothing step in place on out (out holds x on entry)
+                if (smoother_ != Smoother::Chebyshev) throw Error("level op 10 needs the chebyshev smoother");
+                TX* xp = o;
+                TX* ap = t_of<TX>(D);
+                bool z = false;
+                chebyshev<TV, TX>(lvl, xp, ap, bb, 1, z, st);
+                if (xp != o) {
+                    const TX* s = xp;
+                    launch_map(D.A.nrow, [=] __device__(int64_t i) { o[i] = s[i]; }, st);
+                }
+                break;
+            }
             default: throw Error("unknown level op");
         }
     };
@@ -1125,7 +1236,11 @@ void DeviceHierarchy::export_level_values(int lvl, int which, double* dst) {
     const DevLevel& D = L_[lvl];
     if (which == 0) DRAUGR_HIP_CHECK(hipMemcpy(dst, D.A.v64.p, D.A.v64.bytes(), hipMemcpyDeviceToHost));
     else if (which == 1) DRAUGR_HIP_CHECK(hipMemcpy(dst, D.d64.p, D.d64.bytes(), hipMemcpyDeviceToHost));
-    else throw Error("unknown export selector");
+    else if (which == 4) { dst[0] = D.cheb_lmin; dst[1] = D.cheb_lmax; }
+    else if (which == 5) {
+        if (!D.pv0.p) throw Error("no power-iteration start vector (smoother is not chebyshev)");
+        DRAUGR_HIP_CHECK(hipMemcpy(dst, D.pv0.p, D.pv0.bytes(), hipMemcpyDeviceToHost));
+    } else throw Error("unknown export selector");
 }
 
 }  // namespace draugr

@@ -35,6 +35,11 @@ struct DevLevel {
     DevCSR P;                 // prolongation n_l x n_{l+1}
     DevCSR Pt;                // P^T as CSR (values gathered from P)
     DBuf<int32_t> pt_pnz;     // Pt entry -> P entry
+    // Chebyshev smoother: eigenvalue bounds of D^-1 A, power-iteration vectors,
+    // the direction vector d of the recurrence per precision
+    double cheb_lmin = 0.0, cheb_lmax = 0.0;
+    DBuf<double> pv0, pv, pw, cd64;
+    DBuf<float> cd32;
     bool has_rmap = false;
     int64_t ntriples = 0;
     DBuf<int64_t> r_off;      // Galerkin triple map of A_{l+1}
@@ -52,7 +57,7 @@ struct DevLevel {
 
 // Device scalars used by the fused reductions of PCG / stationary solve.
 enum Scalar : int {
-    S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_COUNT_ = 16
+    S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_RHO, S_COUNT_ = 16
 };
 
 struct CycleOpts {
@@ -67,7 +72,9 @@ struct CycleOpts {
 // LAPACK pivots, piv[n..2n) the row permutation; *flag = 1 if singular), and
 // the plain / l1 inverse diagonal of a CSR operator.
 void dense_lu_device(int n, double* a, int* piv, int* flag, hipStream_t st);
-void invdiag_device(int n, const int* rp, const int* cv, const double* av, double* d, bool l1, hipStream_t st);
+// mode 0: 1 / a_ii (src/smoothers.jl:47-63), 1: 1 / sum |a_ij| (:816-827),
+// 2: SPAI(0) a_ii / sum a_ij^2 (:502-532).
+void invdiag_device(int n, const int* rp, const int* cv, const double* av, double* d, int mode, hipStream_t st);
 
 // Differentiation kernels (adjoint.hip)
 void assemble_7pt(int m, const double* k, const int* rp, double* nz, hipStream_t st);
@@ -135,6 +142,11 @@ private:
     void refresh_level_mirrors(int l, hipStream_t st);
     template <typename TV, typename TX> void tune();
     void compute_dinv(int l, hipStream_t st);
+    double estimate_rho(int l, hipStream_t st);  // power iteration on D^-1 A (FP64)
+    void chebyshev_bounds(hipStream_t st);
+    template <typename TV, typename TX>
+    void chebyshev(int l, TX*& x, TX*& alt, const TX* b, int steps, bool& zero, hipStream_t st);
+    double weight() const { return smoother_ == Smoother::SPAI0 ? 1.0 : omega_; }
     void galerkin(int l, hipStream_t st);
     void coarse_factor(hipStream_t st);
     double reduce_to_host(hipStream_t st, int slot);
@@ -147,6 +159,7 @@ private:
     DBuf<int32_t> flag_;               // [0] singular flag, [1..n] pivots, [n+1..2n] permutation
     Smoother smoother_;
     double omega_;
+    uint64_t seed_ = 0;
     bool have32_ = false, have16_ = false;
     bool tuned_[3] = {false, false, false};
     // solver workspaces at the finest size

@@ -131,6 +131,47 @@ template <typename TX> struct EpiAdd {            // x += P xc, one product at a
     __device__ void total(int i, TX s) const { x[i] += s; }
 };
 
+// Chebyshev smoother passes (src/smoothers.jl:925-990): with r = b - A x
+// (row sum, then subtraction), the first pass of a step sets
+// d = (dinv r) / theta, the others d = (c_r dinv) r + c_d d; both write
+// x_out = x + d.  Out of place (the caller swaps x and x_out).
+template <typename TX> struct EpiCheb0 {
+    const TX* x; const TX* b; const TX* dv; TX* d; TX* xo; TX inv_theta;
+    struct Pre { TX x, b, dv; };
+    __device__ Pre pre(int i) const { return {x[i], b[i], dv[i]}; }
+    __device__ TX init(const Pre&) const { return 0; }
+    __device__ TX acc(TX a, TX p) const { return a + p; }
+    __device__ void fin(int i, TX a, const Pre& q) const {
+        const TX dn = q.dv * (q.b - a) * inv_theta;
+        d[i] = dn;
+        xo[i] = q.x + dn;
+    }
+    __device__ void total(int i, TX s) const { fin(i, s, pre(i)); }
+};
+template <typename TX> struct EpiChebK {
+    const TX* x; const TX* b; const TX* dv; TX* d; TX* xo; TX cr, cd;
+    struct Pre { TX x, b, dv, d; };
+    __device__ Pre pre(int i) const { return {x[i], b[i], dv[i], d[i]}; }
+    __device__ TX init(const Pre&) const { return 0; }
+    __device__ TX acc(TX a, TX p) const { return a + p; }
+    __device__ void fin(int i, TX a, const Pre& q) const {
+        const TX dn = cr * q.dv * (q.b - a) + cd * q.d;
+        d[i] = dn;
+        xo[i] = q.x + dn;
+    }
+    __device__ void total(int i, TX s) const { fin(i, s, pre(i)); }
+};
+// w = (A v) .* dinv: one power-iteration step of src/smoothers.jl:866-885
+template <typename TX> struct EpiStoreScaled {
+    TX* y; const TX* dv;
+    struct Pre { TX dv; };
+    __device__ Pre pre(int i) const { return {dv[i]}; }
+    __device__ TX init(const Pre&) const { return 0; }
+    __device__ TX acc(TX a, TX p) const { return a + p; }
+    __device__ void fin(int i, TX a, const Pre& q) const { y[i] = a * q.dv; }
+    __device__ void total(int i, TX s) const { y[i] = s * dv[i]; }
+};
+
 // One row block: stage the products, then finish each row on its own thread
 // (segf(row, seg, len, pref(row))), or a long row on thread 0 (totf(row, sum)).
 // The block table holds both row and nonzero offsets (blk, bz), so a block's

@@ -124,12 +124,15 @@ int32_t draugr_hip_resetup_device(int32_t handle, int64_t nnz, const double *nzv
  *  op 4 out = P^T in1 (restriction)  op 5 out += P in1 (prolongation)
  *  op 6 out: one coloured Gauss-Seidel sweep in place, rhs in2
  *  op 7 out = A_coarse^-1 in1        op 8 Galerkin values of level+1
- *  op 9 out = inverse diagonal of the smoother (f64) */
+ *  op 9 out = inverse diagonal of the smoother (f64; SPAI(0): a_ii / sum a_ij^2)
+ *  op 10 out: one Chebyshev smoothing step in place (out holds x), rhs in2 */
 int32_t draugr_hip_level_op(int32_t handle, int32_t level, int32_t op, int32_t prec, const void *in1,
                             const void *in2, void *out, void *out2, void *stream);
 
 /* Device-side FP64 values to the host: which 0 A.nzval, 1 smoother inverse
- * diagonal, 2 coarse LU factors (col-major), 3 coarsest operator values. */
+ * diagonal, 2 coarse LU factors (col-major), 3 coarsest operator values,
+ * 4 Chebyshev bounds (lambda_min, lambda_max) of the level, 5 the level's
+ * power-iteration start vector (chebyshev smoother). */
 int32_t draugr_hip_export_device_values(int32_t handle, int32_t level, int32_t which, double *dst);
 
 /* Algorithmic bytes of one cycle from a zero guess under the given config,

@@ -34,6 +34,8 @@ typedef struct {
     const int32_t *c_off, *c_order; /* colouring (may be NULL) */
     double *dinv;                  /* smoother inverse diagonal (n) */
     double *r, *tmp, *xc, *bc;     /* workspaces: n, n, nc, nc */
+    double cheb_lmin, cheb_lmax;   /* Chebyshev bounds of D^-1 A (input) */
+    double *cheb_d;                /* Chebyshev direction vector (n) */
 } OLevel;
 
 typedef struct {
@@ -45,7 +47,7 @@ typedef struct {
     double *coarse_b, *coarse_x;
     const int32_t *ac_rp, *ac_cv;  /* coarsest sparse operator */
     double *ac_nz;
-    int32_t smoother;              /* 0 jacobi, 1 colored gs, 5 l1 jacobi, 6 l1 colored gs */
+    int32_t smoother;              /* 0 jacobi, 1 colored gs, 3 spai0, 5 l1 jacobi, 6 l1 colored gs, 8 chebyshev */
     double omega;
     int32_t w_cycle, pre, post;
 } OHier;
@@ -73,16 +75,18 @@ void o_residual(int32_t n, const int32_t *rp, const int32_t *cv, const double *n
 
 /* Smoother inverse diagonal: plain (src/smoothers.jl:47-63) or l1
  * (src/smoothers.jl:816-827). */
-void o_invdiag(int32_t n, const int32_t *rp, const int32_t *cv, const double *nz, int32_t l1, double *d) {
+void o_invdiag(int32_t n, const int32_t *rp, const int32_t *cv, const double *nz, int32_t mode, double *d) {
 #pragma omp parallel for schedule(static)
     for (int32_t i = 0; i < n; ++i) {
         double nrm = 0.0, dg = 0.0;
         for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
-            nrm += fabs(nz[k]);
+            nrm += mode == 2 ? nz[k] * nz[k] : fabs(nz[k]);
             if (cv[k] == i) dg = nz[k];
         }
-        if (l1) {
+        if (mode == 1) {
             d[i] = nrm > EPS64 ? 1.0 / nrm : 0.0;
+        } else if (mode == 2) { /* SPAI(0), src/smoothers.jl:517-532 */
+            d[i] = nrm > 0.0 ? dg / nrm : 0.0;
         } else {
             double thr = EPS64 * (nrm > 1.0 ? nrm : 1.0);
             d[i] = fabs(dg) > thr ? 1.0 / dg : 0.0;
@@ -194,7 +198,41 @@ void o_lu_solve(int32_t n, const double *lu, const int32_t *piv, const double *b
     }
 }
 
+/* Chebyshev smoothing, degree 3 (src/smoothers.jl:952-990): per step
+ * r = b - A x; d = dinv r / theta; x += d; then twice
+ * sigma' = 1 / (2 theta / delta - sigma); d = (2 sigma' / delta) dinv r + sigma' sigma d; x += d. */
+void o_cheb_smooth(int32_t n, const int32_t *rp, const int32_t *cv, const double *nz, double *x, const double *b,
+                   const double *dinv, double lmin, double lmax, int32_t steps, double *r, double *d) {
+    const double theta = (lmax + lmin) / 2.0, delta = (lmax - lmin) / 2.0;
+    for (int32_t s = 0; s < steps; ++s) {
+        o_residual(n, rp, cv, nz, x, b, r);
+        const double inv_theta = 1.0 / theta;
+#pragma omp parallel for schedule(static)
+        for (int32_t i = 0; i < n; ++i) {
+            d[i] = dinv[i] * r[i] * inv_theta;
+            x[i] += d[i];
+        }
+        double sigma_old = theta / delta;
+        for (int k = 1; k < 3; ++k) {
+            o_residual(n, rp, cv, nz, x, b, r);
+            const double sigma_new = 1.0 / (2.0 * theta / delta - sigma_old);
+            const double cr = 2.0 * sigma_new / delta, cd = sigma_new * sigma_old;
+#pragma omp parallel for schedule(static)
+            for (int32_t i = 0; i < n; ++i) {
+                d[i] = cr * dinv[i] * r[i] + cd * d[i];
+                x[i] += d[i];
+            }
+            sigma_old = sigma_new;
+        }
+    }
+}
+
 static void o_smooth(const OHier *h, OLevel *L, double *x, const double *b, int32_t steps) {
+    if (h->smoother == 8) {
+        o_cheb_smooth(L->n, L->a_rp, L->a_cv, L->a_nz, x, b, L->dinv, L->cheb_lmin, L->cheb_lmax, steps, L->tmp,
+                      L->cheb_d);
+        return;
+    }
     const int gs = (h->smoother == 1 || h->smoother == 6);
     if (gs) {
         for (int32_t s = 0; s < steps; ++s)
@@ -203,8 +241,9 @@ static void o_smooth(const OHier *h, OLevel *L, double *x, const double *b, int3
     }
     /* ping-pong with a copy back on odd step counts (src/smoothers.jl:103-124) */
     double *src = x, *dst = L->tmp;
+    const double w = h->smoother == 3 ? 1.0 : h->omega; /* SPAI(0): x + m r (src/smoothers.jl:547-558) */
     for (int32_t s = 0; s < steps; ++s) {
-        o_jacobi_step(L->n, L->a_rp, L->a_cv, L->a_nz, src, b, L->dinv, h->omega, dst);
+        o_jacobi_step(L->n, L->a_rp, L->a_cv, L->a_nz, src, b, L->dinv, w, dst);
         double *t = src;
         src = dst;
         dst = t;
@@ -332,10 +371,10 @@ int32_t o_pcg(const OHier *h, int32_t n, const int32_t *rp, const int32_t *cv, c
  * values already in lv[0].a_nz; recompute dinv, Galerkin level by level, the
  * coarsest operator and its LU.  Returns 0 or a zero-pivot code. */
 int32_t o_resetup(OHier *h) {
-    const int l1 = (h->smoother == 5 || h->smoother == 6);
+    const int mode = (h->smoother == 5 || h->smoother == 6) ? 1 : (h->smoother == 3 ? 2 : 0);
     for (int32_t l = 0; l < h->nlevels; ++l) {
         OLevel *L = &h->lv[l];
-        o_invdiag(L->n, L->a_rp, L->a_cv, L->a_nz, l1, L->dinv);
+        o_invdiag(L->n, L->a_rp, L->a_cv, L->a_nz, mode, L->dinv);
         double *out = (l + 1 < h->nlevels) ? h->lv[l + 1].a_nz : h->ac_nz;
         const int32_t nrow_next = (l + 1 < h->nlevels) ? h->lv[l + 1].n : h->ncoarse;
         const int32_t *rp_next = (l + 1 < h->nlevels) ? h->lv[l + 1].a_rp : h->ac_rp;
@@ -350,9 +389,9 @@ int32_t o_resetup(OHier *h) {
 
 /* Inverse diagonals of all levels (the smoother build of src/setup.jl:150). */
 void o_prepare(OHier *h) {
-    const int l1 = (h->smoother == 5 || h->smoother == 6);
+    const int mode = (h->smoother == 5 || h->smoother == 6) ? 1 : (h->smoother == 3 ? 2 : 0);
     for (int32_t l = 0; l < h->nlevels; ++l) {
         OLevel *L = &h->lv[l];
-        o_invdiag(L->n, L->a_rp, L->a_cv, L->a_nz, l1, L->dinv);
+        o_invdiag(L->n, L->a_rp, L->a_cv, L->a_nz, mode, L->dinv);
     }
 }

@@ -26,7 +26,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
-SMOOTHER_CODES = {"jacobi": 0, "colored_gs": 1, "l1_jacobi": 5, "l1_colored_gs": 6}
+SMOOTHER_CODES = {"jacobi": 0, "colored_gs": 1, "spai0": 3, "l1_jacobi": 5, "l1_colored_gs": 6, "chebyshev": 8}
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -42,6 +42,7 @@ class _OLevel(ctypes.Structure):
         ("r_off", _i64p), ("r_pi", _i32p), ("r_ai", _i32p), ("r_pj", _i32p),
         ("ncolors", ctypes.c_int32), ("c_off", _i32p), ("c_order", _i32p),
         ("dinv", _f64p), ("r", _f64p), ("tmp", _f64p), ("xc", _f64p), ("bc", _f64p),
+        ("cheb_lmin", ctypes.c_double), ("cheb_lmax", ctypes.c_double), ("cheb_d", _f64p),
     ]
 
 
@@ -106,11 +107,41 @@ def residual(rp, cv, nz, x, b):
 
 
 def invdiag(rp, cv, nz, l1):
+    """mode (l1): 0 plain 1 / a_ii, 1 l1, 2 SPAI(0) a_ii / sum a_ij^2."""
     rp, cv, nz = _i32(rp), _i32(cv), _f64(nz)
     d = np.empty(len(rp) - 1)
     lib().o_invdiag(ctypes.c_int32(len(d)), _p(rp, _i32p), _p(cv, _i32p), _p(nz, _f64p), ctypes.c_int32(int(l1)),
                     _p(d, _f64p))
     return d
+
+
+def cheb_smooth(rp, cv, nz, x, b, dinv, lmin, lmax, steps=1):
+    """Chebyshev smoothing of x (degree 3) with the given bounds; returns the new x."""
+    x = _f64(x).copy()
+    n = len(x)
+    rp, cv, nz, b, dinv = _i32(rp), _i32(cv), _f64(nz), _f64(b), _f64(dinv)
+    r, d = np.zeros(n), np.zeros(n)
+    lib().o_cheb_smooth(ctypes.c_int32(n), _p(rp, _i32p), _p(cv, _i32p), _p(nz, _f64p), _p(x, _f64p),
+                        _p(b, _f64p), _p(dinv, _f64p), ctypes.c_double(lmin), ctypes.c_double(lmax),
+                        ctypes.c_int32(steps), _p(r, _f64p), _p(d, _f64p))
+    return x
+
+
+def power_rho(rp, cv, nz, dinv, v0, niter=10):
+    """Spectral-radius estimate of D^-1 A by power iteration from the unit vector
+    v0 (src/smoothers.jl:866-885): w = (A v) .* dinv, lambda = ||w||,
+    v = w / lambda while lambda > eps.  numpy restatement."""
+    import scipy.sparse as sp
+    n = len(rp) - 1
+    A = sp.csr_matrix((_f64(nz), _i32(cv), _i32(rp)), shape=(n, n))
+    v = _f64(v0).copy()
+    lam = 1.0
+    for _ in range(niter):
+        w = (A @ v) * dinv
+        lam = float(np.sqrt(np.dot(w, w)))
+        if lam > np.finfo(float).eps:
+            v = w / lam
+    return lam
 
 
 def jacobi_step(rp, cv, nz, x, b, d, w):
@@ -175,7 +206,7 @@ class OracleHierarchy:
     """A fixed hierarchy on which the oracle runs cycles, solves and resetups."""
 
     def __init__(self, levels, coarse_rp, coarse_cv, coarse_nz, smoother="l1_jacobi", omega=2.0 / 3.0,
-                 cycle="v", pre=1, post=1):
+                 cycle="v", pre=1, post=1, cheb=None):
         self._keep = []
         keep = self._keep.append
         L = len(levels)
@@ -207,6 +238,10 @@ class OracleHierarchy:
             o.c_off, o.c_order = _p(d["c_off"], _i32p), _p(d["c_order"], _i32p)
             o.dinv, o.r, o.tmp = _p(d["dinv"], _f64p), _p(d["r"], _f64p), _p(d["tmp"], _f64p)
             o.xc, o.bc = _p(d["xc"], _f64p), _p(d["bc"], _f64p)
+            d["cheb_d"] = np.zeros(n)
+            o.cheb_d = _p(d["cheb_d"], _f64p)
+            if cheb is not None:
+                o.cheb_lmin, o.cheb_lmax = float(cheb[l][0]), float(cheb[l][1])
         self._arr = arr
         nco = len(coarse_rp) - 1
         self.coarse = {"rp": _i32(coarse_rp), "cv": _i32(coarse_cv), "nz": _f64(coarse_nz).copy()}
@@ -240,6 +275,11 @@ class OracleHierarchy:
         code = lib().o_lu(ctypes.c_int32(nco), _p(self.coarse_lu, _f64p), _p(self.coarse_piv, _i32p))
         if code:
             raise ZeroDivisionError("coarsest matrix singular")
+
+    def set_cheb(self, cheb):
+        """Chebyshev bounds (lambda_min, lambda_max) per level."""
+        for l, (lo, hi) in enumerate(cheb):
+            self._arr[l].cheb_lmin, self._arr[l].cheb_lmax = float(lo), float(hi)
 
     def set_options(self, cycle=None, pre=None, post=None):
         if cycle is not None:

@@ -17,7 +17,10 @@ def dense(rp, cv, nz, ncol=None):
     return M
 
 
-def invdiag(A, l1):
+def invdiag(A, l1, spai0=False):
+    if spai0:
+        s = (A * A).sum(axis=1)
+        return np.where(s > 0, np.diag(A) / np.where(s == 0, 1, s), 0.0)
     if l1:
         s = np.abs(A).sum(axis=1)
         return np.where(s > np.finfo(float).eps, 1.0 / np.where(s == 0, 1, s), 0.0)
@@ -27,18 +30,36 @@ def invdiag(A, l1):
 
 
 class DenseAMG:
-    def __init__(self, levels, coarse, smoother, omega=2.0 / 3.0, cycle="v", pre=1, post=1):
+    def __init__(self, levels, coarse, smoother, omega=2.0 / 3.0, cycle="v", pre=1, post=1, cheb=None):
         self.A = [dense(l["A_rp"], l["A_cv"], l["A_nz"]) for l in levels]
         self.P = [dense(l["P_rp"], l["P_cv"], l["P_nz"], len(l["Pt_off"]) - 1) for l in levels]
         self.colors = [(l.get("c_off"), l.get("c_order")) for l in levels]
         self.Ac = dense(coarse["rp"], coarse["cv"], coarse["nz"])
         self.smoother = smoother
         self.l1 = smoother.startswith("l1")
-        self.D = [invdiag(A, self.l1) for A in self.A]
+        self.D = [invdiag(A, self.l1, smoother == "spai0") for A in self.A]
+        self.cheb = cheb
         self.omega, self.cycle_type, self.pre, self.post = omega, cycle, pre, post
 
     def smooth(self, l, x, b, steps):
         A, D = self.A[l], self.D[l]
+        if self.smoother == "chebyshev":
+            lo, hi = self.cheb[l]
+            th, de = (hi + lo) / 2, (hi - lo) / 2
+            for _ in range(steps):
+                d = D * (b - A @ x) / th
+                x = x + d
+                so = th / de
+                for _k in range(2):
+                    sn = 1 / (2 * th / de - so)
+                    d = 2 * sn / de * D * (b - A @ x) + sn * so * d
+                    x = x + d
+                    so = sn
+            return x
+        if self.smoother == "spai0":
+            for _ in range(steps):
+                x = x + D * (b - A @ x)
+            return x
         if "gs" in self.smoother:
             off, order = self.colors[l]
             for _ in range(steps):

@@ -242,3 +242,90 @@ def test_resetup_vs_scipy(dh, orc, hypre_case):
         assert np.abs(G - Ac.toarray()).max() < 1e-12 * np.abs(Ac).max()
         Al = sp.csr_matrix((got, cv, rp), shape=Ac.shape)
         assert rel_err(oh.dinv(l), 1.0 / abs(sp.csr_matrix((oh.level_values(l), lv["A_cv"], lv["A_rp"]))).sum(axis=1).A1) < 1e-15
+
+
+# ── SPAI(0) and Chebyshev smoothers ──────────────────────────────────────────
+def _cheb_bounds(h, orc, seed=SEED + 77):
+    """lambda bounds per level from the numpy power iteration on D^-1 A."""
+    out = []
+    for l in range(h.nlevels):
+        lv = h.level(l, with_rmap=False)
+        d = orc.invdiag(lv["A_rp"], lv["A_cv"], lv["A_nz"], 0)
+        v0 = np.random.default_rng(seed + l).standard_normal(len(d))
+        v0 /= np.linalg.norm(v0)
+        rho = orc.power_rho(lv["A_rp"], lv["A_cv"], lv["A_nz"], d, v0)
+        out.append((1.1 * rho / 30.0, 1.1 * rho))
+    return out
+
+
+def test_spai0_known_answers(dh, orc):
+    """Interior 1/3 and boundary 2/5 of 1-D Poisson (test/runtests.jl:754-762);
+    ten SPAI(0) steps from zero reduce the residual (:764-773)."""
+    A = dh.problems.poisson1d(10)
+    m = orc.invdiag(A.rowptr, A.colval, A.nzval, 2)
+    assert m[4] == 2.0 / 6.0 and m[0] == 2.0 / 5.0
+    x = np.zeros(10)
+    for _ in range(10):
+        x = orc.jacobi_step(A.rowptr, A.colval, A.nzval, x, np.ones(10), m, 1.0)
+    assert np.linalg.norm(np.ones(10) - A.matvec(x)) < np.linalg.norm(np.ones(10))
+
+
+def test_power_iteration_bounds(dh, orc):
+    """The estimate never exceeds the true spectral radius of D^-1 A (1-D
+    Poisson with Jacobi: 1 + cos(pi / (n + 1))) and approaches it."""
+    n = 64
+    A = dh.problems.poisson1d(n)
+    d = orc.invdiag(A.rowptr, A.colval, A.nzval, 0)
+    v0 = np.random.default_rng(1).standard_normal(n)
+    v0 /= np.linalg.norm(v0)
+    rho = orc.power_rho(A.rowptr, A.colval, A.nzval, d, v0)
+    exact = 1.0 + np.cos(np.pi / (n + 1))
+    assert 0.9 * exact < rho <= exact * (1 + 1e-12)
+    more = orc.power_rho(A.rowptr, A.colval, A.nzval, d, v0, niter=200)
+    assert rho <= more <= exact * (1 + 1e-12)
+
+
+def test_chebyshev_smoothing_known_answer(dh, orc):
+    """Five Chebyshev steps from zero reduce the residual (test/runtests.jl:1924-1933);
+    the oracle's smoothing equals the dense restatement's."""
+    A = dh.problems.poisson1d(10)
+    d = orc.invdiag(A.rowptr, A.colval, A.nzval, 0)
+    v0 = np.random.default_rng(3).standard_normal(10)
+    v0 /= np.linalg.norm(v0)
+    lmax = 1.1 * orc.power_rho(A.rowptr, A.colval, A.nzval, d, v0)
+    x = orc.cheb_smooth(A.rowptr, A.colval, A.nzval, np.zeros(10), np.ones(10), d, lmax / 30, lmax, steps=5)
+    assert np.linalg.norm(np.ones(10) - A.matvec(x)) < np.linalg.norm(np.ones(10))
+    from dense_ref import dense
+    Ad = dense(A.rowptr, A.colval, A.nzval)
+    th, de = (lmax + lmax / 30) / 2, (lmax - lmax / 30) / 2
+    xd = np.zeros(10)
+    for _ in range(5):
+        dd = d * (1 - Ad @ xd) / th
+        xd = xd + dd
+        so = th / de
+        for _k in range(2):
+            sn = 1 / (2 * th / de - so)
+            dd = 2 * sn / de * d * (1 - Ad @ xd) + sn * so * dd
+            xd = xd + dd
+            so = sn
+    assert rel_err(x, xd) < 1e-13
+
+
+@pytest.mark.parametrize("smoother", ["spai0", "chebyshev"])
+def test_cycle_vs_dense_new_smoothers(dh, orc, smoother):
+    P = dh.problems
+    A = P.heterogeneous3d(8, P.log_permeability(8, 1.0, SEED + 5))
+    st = dh.SPAI0SmootherType() if smoother == "spai0" else dh.ChebyshevSmootherType()
+    cfg = dh.AMGConfig(coarsening=dh.HMISCoarsening(0.5, dh.ExtendedIInterpolation(0.3)), smoother=st,
+                       max_coarse_size=10)
+    h = _hier(dh, A, cfg)
+    levels = [h.level(l) for l in range(h.nlevels)]
+    c = h.coarsest()
+    cheb = _cheb_bounds(h, orc) if smoother == "chebyshev" else None
+    oh = orc.OracleHierarchy(levels, c["rp"], c["cv"], c["nz"], smoother=smoother, cheb=cheb, pre=2, post=2)
+    dn = DenseAMG(levels, c, smoother, cheb=cheb, pre=2, post=2)
+    b = P.splitmix_normal(SEED + 6, A.n)
+    x0 = P.splitmix_normal(SEED + 7, A.n)
+    assert rel_err(oh.cycle(x0, b), dn.cycle(x0, b)) < 1e-12
+    x, it, rel = oh.solve(np.zeros(A.n), b, 1e-8, 200)
+    assert rel < 1e-8 and it < 200
