@@ -5,6 +5,7 @@
 // x in place.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -670,6 +671,23 @@ int32_t draugr_hip_batch_free(int32_t batch) {
             g_batch.erase(it);
         }
         std::lock_guard<std::mutex> g(B->mu);
+        return 0;
+    });
+}
+
+int32_t draugr_hip_ilu0_factor(int32_t n, const int32_t* rowptr, const int32_t* colval, const double* nzval,
+                               int32_t* diag, double* L, double* U) {
+    return guarded([&]() -> int32_t {
+        if (!diag || !L || !U) throw Error("ArgumentError: null output array");
+        const int32_t nnz = rowptr ? rowptr[n] : 0;
+        CSR A = csr_from_raw(n, nnz, rowptr, colval, nzval, 0);
+        IluStructure s;
+        ilu0_structure(A, s);
+        std::vector<double> Lv, Uv;
+        ilu0_factorize(A, s.diag_idx, Lv, Uv);
+        std::copy(s.diag_idx.begin(), s.diag_idx.end(), diag);
+        std::copy(Lv.begin(), Lv.end(), L);
+        std::copy(Uv.begin(), Uv.end(), U);
         return 0;
     });
 }

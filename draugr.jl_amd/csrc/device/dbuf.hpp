@@ -32,7 +32,10 @@ struct DBuf {
         return *this;
     }
     ~DBuf() { release(); }
+    // keeps the allocation when the size is unchanged: captured graphs hold
+    // these addresses, so a same-size re-upload must not move the buffer
     void alloc(size_t count) {
+        if (p && count == n) return;
         release();
         n = count;
         if (count) DRAUGR_HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));

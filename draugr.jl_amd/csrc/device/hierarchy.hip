@@ -143,6 +143,33 @@ __global__ __launch_bounds__(kBlock) void k_invdiag(int n, const int* __restrict
     }
 }
 
+// ILU(0) triangular solves by level schedule (src/smoothers.jl:1383-1411):
+// rows of one level are independent; each row subtracts in column order.
+template <typename TF, typename TX>
+__global__ __launch_bounds__(kBlock) void k_ilu_fwd(int cnt, const int* __restrict__ list, const int* __restrict__ rp,
+                                                    const int* __restrict__ cv, const int* __restrict__ diag,
+                                                    const TF* __restrict__ L, TX* t) {
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= cnt) return;
+    const int i = list[q];
+    TX acc = t[i];
+    for (int k = rp[i]; k < diag[i]; ++k) acc -= (TX)L[k] * t[cv[k]];
+    t[i] = acc;
+}
+template <typename TF, typename TX>
+__global__ __launch_bounds__(kBlock) void k_ilu_bwd(int cnt, const int* __restrict__ list, const int* __restrict__ rp,
+                                                    const int* __restrict__ cv, const int* __restrict__ diag,
+                                                    const TF* __restrict__ U, TX* t, TX eps) {
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= cnt) return;
+    const int i = list[q];
+    TX acc = t[i];
+    const int dk = diag[i];
+    for (int k = dk + 1; k < rp[i + 1]; ++k) acc -= (TX)U[k] * t[cv[k]];
+    const TX u = (TX)U[dk];
+    t[i] = fabs(u) > eps ? acc / u : (TX)0;
+}
+
 // Galerkin numeric product into the fixed coarse pattern:
 // Ac[k] = sum_t P[pi_t] * A[ai_t] * P[pj_t] over the triples of nonzero k in
 // generation order (src/galerkin.jl:194-222).  One thread per coarse nonzero.
@@ -372,9 +399,9 @@ bool is_gs(Smoother s) { return s == Smoother::ColoredGS || s == Smoother::L1Col
 DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
     : smoother_(cfg.smoother), omega_(cfg.jacobi_omega), seed_(cfg.seed) {
     if (!(smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || is_gs(smoother_) ||
-          smoother_ == Smoother::SPAI0 || smoother_ == Smoother::Chebyshev))
+          smoother_ == Smoother::SPAI0 || smoother_ == Smoother::Chebyshev || smoother_ == Smoother::ILU0))
         throw Error("smoother not available in the HIP backend (supported: jacobi, l1_jacobi, colored_gs, "
-                    "l1_colored_gs, spai0, chebyshev)");
+                    "l1_colored_gs, spai0, chebyshev, ilu0)");
     DRAUGR_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     DRAUGR_HIP_CHECK(hipEventCreateWithFlags(&last_ev_, hipEventDisableTiming));
     hipStream_t st = stream_;
@@ -409,6 +436,15 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
             D.color_order.upload(H.coloring.color_order, st);
         }
         const int32_t n = H.A.nrow, nc = H.P.ncol;
+        if (smoother_ == Smoother::ILU0) {
+            ilu0_structure(H.A, D.ilu);
+            D.h_rp = H.A.rp;
+            D.h_cv = H.A.cv;
+            D.ilu_diag.upload(D.ilu.diag_idx, st);
+            D.ilu_fwd.upload(D.ilu.fwd_order, st);
+            D.ilu_bwd.upload(D.ilu.bwd_order, st);
+            ilu_factor((int)l, H.A.nz, st);
+        }
         if (smoother_ == Smoother::Chebyshev) {
             // start vector of the power iteration: N(0,1) from the setup seed, normalised
             std::vector<double> v(n);
@@ -619,6 +655,15 @@ void DeviceHierarchy::ensure_precision(Precision p) {
             D.xc32.alloc(D.P.ncol);
             D.bc32.alloc(D.P.ncol);
             if (smoother_ == Smoother::Chebyshev) D.cd32.alloc(D.A.nrow);
+            if (smoother_ == Smoother::ILU0) {
+                D.ilu_L32.alloc(D.A.nnz);
+                D.ilu_U32.alloc(D.A.nnz);
+                for (auto pr : {std::make_pair(D.ilu_L64.p, D.ilu_L32.p), std::make_pair(D.ilu_U64.p, D.ilu_U32.p)}) {
+                    const double* s = pr.first;
+                    float* d = pr.second;
+                    launch_map(D.A.nnz, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
+                }
+            }
             for (DevCSR* m : {&D.A, &D.P, &D.Pt}) {
                 const double* s = m->v64.p;
                 float* d = m->v32.p;
@@ -657,6 +702,10 @@ void DeviceHierarchy::smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bo
     const TX w = (TX)weight();
     if (smoother_ == Smoother::Chebyshev) {
         chebyshev<TV, TX>(l, x, alt, b, steps, zero, st);
+        return;
+    }
+    if (smoother_ == Smoother::ILU0) {
+        ilu_smooth<TV, TX>(l, x, alt, b, steps, zero, st);
         return;
     }
     if (is_gs(smoother_)) {
@@ -791,6 +840,63 @@ std::string DeviceHierarchy::tune_report() const {
         }
     }
     return out;
+}
+
+// ILU(0) smoothing (src/smoothers.jl:1356-1433): per step tmp = b - A x (row
+// sum, then subtraction), forward substitution with L, backward with U by
+// level schedule, x += tmp where tmp is finite.  In place on x.
+template <typename TV, typename TX>
+void DeviceHierarchy::ilu_smooth(int l, TX* x, TX* tmp, const TX* b, int steps, bool& zero, hipStream_t st) {
+    typedef typename TransferT<TV>::type TF;
+    DevLevel& D = L_[l];
+    const int n = D.A.nrow;
+    if (zero) {
+        launch_map(n, [=] __device__(int64_t i) { x[i] = 0; }, st);
+        zero = false;
+    }
+    const TF* Lf = sizeof(TX) == 8 ? (const TF*)D.ilu_L64.p : (const TF*)D.ilu_L32.p;
+    const TF* Uf = sizeof(TX) == 8 ? (const TF*)D.ilu_U64.p : (const TF*)D.ilu_U32.p;
+    const TX eps = sizeof(TX) == 8 ? (TX)2.220446049250313e-16 : (TX)1.1920928955078125e-07;
+    for (int s = 0; s < steps; ++s) {
+        op<TV, TX>(D.A, XVec<TX>{x}, EpiResid<TX>{b, tmp}, st);
+        const auto& I = D.ilu;
+        for (size_t c = 1; c < I.fwd_offsets.size(); ++c) {
+            const int off = I.fwd_offsets[c - 1], cnt = I.fwd_offsets[c] - off;
+            hipLaunchKernelGGL((k_ilu_fwd<TF, TX>), dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cnt,
+                               D.ilu_fwd.p + off, D.A.rp.p, D.A.cv.p, D.ilu_diag.p, Lf, tmp);
+        }
+        for (size_t c = 1; c < I.bwd_offsets.size(); ++c) {
+            const int off = I.bwd_offsets[c - 1], cnt = I.bwd_offsets[c] - off;
+            hipLaunchKernelGGL((k_ilu_bwd<TF, TX>), dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cnt,
+                               D.ilu_bwd.p + off, D.A.rp.p, D.A.cv.p, D.ilu_diag.p, Uf, tmp, eps);
+        }
+        const TX* t = tmp;
+        launch_map(n, [=] __device__(int64_t i) {
+            const TX v = t[i];
+            if (isfinite(v)) x[i] += v;
+        }, st);
+    }
+}
+
+void DeviceHierarchy::ilu_factor(int l, const std::vector<double>& values, hipStream_t st) {
+    DevLevel& D = L_[l];
+    CSR A;
+    A.nrow = A.ncol = D.A.nrow;
+    A.rp = D.h_rp;
+    A.cv = D.h_cv;
+    A.nz = values;
+    std::vector<double> Lv, Uv;
+    ilu0_factorize(A, D.ilu.diag_idx, Lv, Uv);
+    D.ilu_L64.upload(Lv, st);
+    D.ilu_U64.upload(Uv, st);
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // host vectors are local
+    if (D.ilu_L32.p) {
+        for (auto pr : {std::make_pair(D.ilu_L64.p, D.ilu_L32.p), std::make_pair(D.ilu_U64.p, D.ilu_U32.p)}) {
+            const double* s = pr.first;
+            float* d = pr.second;
+            launch_map(D.A.nnz, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
+        }
+    }
 }
 
 // Chebyshev smoothing (src/smoothers.jl:952-990): per step, degree 3 passes
@@ -1116,6 +1222,15 @@ void DeviceHierarchy::resetup(const double* nzval, hipStream_t st) {
         }
         for (int l = 0; l < nlevels(); ++l) refresh_level_mirrors(l, st);
         if (smoother_ == Smoother::Chebyshev) chebyshev_bounds(st);
+        if (smoother_ == Smoother::ILU0) {
+            // the factorization runs on the host, as the reference's update_smoother! (src/smoothers.jl:1331-1337)
+            DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+            for (int l = 0; l < nlevels(); ++l) {
+                std::vector<double> v(L_[l].A.nnz);
+                DRAUGR_HIP_CHECK(hipMemcpy(v.data(), L_[l].A.v64.p, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+                ilu_factor(l, v, st);
+            }
+        }
     }
     coarse_factor(st);
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
@@ -1139,6 +1254,11 @@ double DeviceHierarchy::cycle_bytes(const CycleOpts& o) const {
         if (is_gs(smoother_)) {
             lv += n * sx;  // zero fill
             lv += (o.pre + o.post) * (BA + 4 * n * sx);
+            lv += BA + 3 * n * sx;  // residual
+        } else if (smoother_ == Smoother::ILU0) {
+            // zero fill, then per step: residual, both triangular sweeps (each reads A's pattern and a factor), update
+            const double BT = D.A.nnz * (sv + 4) + (n + 1) * 4 + n * 4;  // one sweep's factor, pattern, schedule
+            lv += n * sx + (o.pre + o.post) * (BA + 3 * n * sx + 2 * BT + 4 * n * sx + 3 * n * sx);
             lv += BA + 3 * n * sx;  // residual
         } else if (smoother_ == Smoother::Chebyshev) {
             // zero fill, then 3 passes per step: x, b, dinv (and d) read, d and x_out written
@@ -1214,6 +1334,12 @@ void DeviceHierarchy::level_op(int lvl, int op_code, int prec, const void* in1, 
                 }
                 break;
             }
+            case 11: {  // one ILU(0) smoothing step in place on out (out holds x), rhs in2
+                if (smoother_ != Smoother::ILU0) throw Error("level op 11 needs the ilu0 smoother");
+                bool z = false;
+                ilu_smooth<TV, TX>(lvl, o, t_of<TX>(D), bb, 1, z, st);
+                break;
+            }
             default: throw Error("unknown level op");
         }
     };
@@ -1237,6 +1363,11 @@ void DeviceHierarchy::export_level_values(int lvl, int which, double* dst) {
     if (which == 0) DRAUGR_HIP_CHECK(hipMemcpy(dst, D.A.v64.p, D.A.v64.bytes(), hipMemcpyDeviceToHost));
     else if (which == 1) DRAUGR_HIP_CHECK(hipMemcpy(dst, D.d64.p, D.d64.bytes(), hipMemcpyDeviceToHost));
     else if (which == 4) { dst[0] = D.cheb_lmin; dst[1] = D.cheb_lmax; }
+    else if (which == 6 || which == 7) {
+        const DBuf<double>& f = which == 6 ? D.ilu_L64 : D.ilu_U64;
+        if (!f.p) throw Error("no ILU(0) factors (smoother is not ilu0)");
+        DRAUGR_HIP_CHECK(hipMemcpy(dst, f.p, f.bytes(), hipMemcpyDeviceToHost));
+    }
     else if (which == 5) {
         if (!D.pv0.p) throw Error("no power-iteration start vector (smoother is not chebyshev)");
         DRAUGR_HIP_CHECK(hipMemcpy(dst, D.pv0.p, D.pv0.bytes(), hipMemcpyDeviceToHost));

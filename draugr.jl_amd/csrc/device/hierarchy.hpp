@@ -40,6 +40,13 @@ struct DevLevel {
     double cheb_lmin = 0.0, cheb_lmax = 0.0;
     DBuf<double> pv0, pv, pw, cd64;
     DBuf<float> cd32;
+    // ILU(0) smoother: level schedules (host offsets, device orders), diagonal
+    // positions, factors on A's pattern, the host pattern for refactorization
+    IluStructure ilu;
+    DBuf<int32_t> ilu_diag, ilu_fwd, ilu_bwd;
+    DBuf<double> ilu_L64, ilu_U64;
+    DBuf<float> ilu_L32, ilu_U32;
+    std::vector<int32_t> h_rp, h_cv;
     bool has_rmap = false;
     int64_t ntriples = 0;
     DBuf<int64_t> r_off;      // Galerkin triple map of A_{l+1}
@@ -144,6 +151,9 @@ private:
     void compute_dinv(int l, hipStream_t st);
     double estimate_rho(int l, hipStream_t st);  // power iteration on D^-1 A (FP64)
     void chebyshev_bounds(hipStream_t st);
+    void ilu_factor(int l, const std::vector<double>& values, hipStream_t st);  // host factorization, upload
+    template <typename TV, typename TX>
+    void ilu_smooth(int l, TX* x, TX* tmp, const TX* b, int steps, bool& zero, hipStream_t st);
     template <typename TV, typename TX>
     void chebyshev(int l, TX*& x, TX*& alt, const TX* b, int steps, bool& zero, hipStream_t st);
     double weight() const { return smoother_ == Smoother::SPAI0 ? 1.0 : omega_; }

@@ -151,6 +151,18 @@ void coarse_sparsity(const CSR& A, const CSR& P, int32_t nc, CSR& Ac, Restrictio
 void build_transpose_map(const CSR& P, TransposeMap& t);
 void greedy_coloring(const CSR& A, Coloring& c);
 
+// ILU(0) (src/smoothers.jl:1016-1098, 1228-1307): diagonal positions and
+// the forward / backward level schedules of a CSR operator, and its factors
+// L (strict lower, unit diagonal implied) and U (upper with diagonal) on A's
+// pattern.
+struct IluStructure {
+    std::vector<int32_t> diag_idx;
+    std::vector<int32_t> fwd_order, fwd_offsets, bwd_order, bwd_offsets;
+};
+void ilu0_structure(const CSR& A, IluStructure& s);
+void ilu0_factorize(const CSR& A, const std::vector<int32_t>& diag_idx, std::vector<double>& L,
+                    std::vector<double>& U);
+
 HostHierarchy amg_setup_host(const CSR& A, const Config& cfg, bool allow_partial_resetup);
 std::string hierarchy_summary(const HostHierarchy& h, const Config& cfg, int64_t n_finest);
 

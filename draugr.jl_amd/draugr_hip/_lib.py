@@ -54,6 +54,7 @@ SIGNATURES = {
     "draugr_hip_batch_cycle_device": (_i32, [_i32, _vp, _vp, _i32, _vp]),
     "draugr_hip_batch_pcg_device": (_i32, [_i32, _vp, _vp, _i32, _f64, _i32, _vp, _vp, _vp]),
     "draugr_hip_batch_free": (_i32, [_i32]),
+    "draugr_hip_ilu0_factor": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -125,14 +125,16 @@ int32_t draugr_hip_resetup_device(int32_t handle, int64_t nnz, const double *nzv
  *  op 6 out: one coloured Gauss-Seidel sweep in place, rhs in2
  *  op 7 out = A_coarse^-1 in1        op 8 Galerkin values of level+1
  *  op 9 out = inverse diagonal of the smoother (f64; SPAI(0): a_ii / sum a_ij^2)
- *  op 10 out: one Chebyshev smoothing step in place (out holds x), rhs in2 */
+ *  op 10 out: one Chebyshev smoothing step in place (out holds x), rhs in2
+ *  op 11 out: one ILU(0) smoothing step in place (out holds x), rhs in2 */
 int32_t draugr_hip_level_op(int32_t handle, int32_t level, int32_t op, int32_t prec, const void *in1,
                             const void *in2, void *out, void *out2, void *stream);
 
 /* Device-side FP64 values to the host: which 0 A.nzval, 1 smoother inverse
  * diagonal, 2 coarse LU factors (col-major), 3 coarsest operator values,
  * 4 Chebyshev bounds (lambda_min, lambda_max) of the level, 5 the level's
- * power-iteration start vector (chebyshev smoother). */
+ * power-iteration start vector (chebyshev smoother), 6 / 7 the ILU(0)
+ * factors L / U on the level's pattern (ilu0 smoother). */
 int32_t draugr_hip_export_device_values(int32_t handle, int32_t level, int32_t which, double *dst);
 
 /* Algorithmic bytes of one cycle from a zero guess under the given config,
@@ -193,6 +195,13 @@ int32_t draugr_hip_batch_pcg_device(int32_t batch, double *x, const double *b, i
                                     int32_t maxiter, void *stream, int32_t *iters, double *relres);
 
 int32_t draugr_hip_batch_free(int32_t batch);
+
+/* ILU(0) of a CSR operator on the host (0-based, columns sorted), as the
+ * ilu0 smoother factorizes each level (src/smoothers.jl:1016-1098): diag[n]
+ * receives the diagonal positions, L[nnz] / U[nnz] the factors on A's
+ * pattern.  Needs no GPU. */
+int32_t draugr_hip_ilu0_factor(int32_t n, const int32_t *rowptr, const int32_t *colval, const double *nzval,
+                               int32_t *diag, double *L, double *U);
 
 /* Kernel choices of the autotuner (per level and cycle precision) as text;
  * returns the length.  The tuner runs on the first cycle of each precision. */

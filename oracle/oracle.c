@@ -36,6 +36,8 @@ typedef struct {
     double *r, *tmp, *xc, *bc;     /* workspaces: n, n, nc, nc */
     double cheb_lmin, cheb_lmax;   /* Chebyshev bounds of D^-1 A (input) */
     double *cheb_d;                /* Chebyshev direction vector (n) */
+    int32_t *ilu_diag;             /* ILU(0): diagonal positions (n), factors on A's pattern (nnz) */
+    double *ilu_l, *ilu_u;
 } OLevel;
 
 typedef struct {
@@ -47,7 +49,7 @@ typedef struct {
     double *coarse_b, *coarse_x;
     const int32_t *ac_rp, *ac_cv;  /* coarsest sparse operator */
     double *ac_nz;
-    int32_t smoother;              /* 0 jacobi, 1 colored gs, 3 spai0, 5 l1 jacobi, 6 l1 colored gs, 8 chebyshev */
+    int32_t smoother;              /* 0 jacobi, 1 colored gs, 3 spai0, 5 l1 jacobi, 6 l1 colored gs, 8 chebyshev, 9 ilu0 */
     double omega;
     int32_t w_cycle, pre, post;
 } OHier;
@@ -227,7 +229,69 @@ void o_cheb_smooth(int32_t n, const int32_t *rp, const int32_t *cv, const double
     }
 }
 
+/* ILU(0) factorization on A's pattern (src/smoothers.jl:1016-1079): IKJ order,
+ * a pivot below eps max(1, ||a_k||_1) zeroes the entry, multipliers clamped
+ * to 1e8 in magnitude, tiny diagonals raised to the threshold.  diag: the
+ * first stored diagonal position of each row. */
+void o_ilu0_factor(int32_t n, const int32_t *rp, const int32_t *cv, const double *nz, const int32_t *diag, double *l,
+                   double *u) {
+    const int32_t nnz = rp[n];
+    double *rn = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    for (int32_t k = 0; k < nnz; ++k) { u[k] = nz[k]; l[k] = 0.0; }
+    for (int32_t i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s += fabs(nz[k]);
+        rn[i] = s;
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        for (int32_t q = rp[i]; q < diag[i]; ++q) {
+            const int32_t k = cv[q];
+            const double ukk = u[diag[k]];
+            if (fabs(ukk) < EPS64 * (rn[k] > 1.0 ? rn[k] : 1.0)) { l[q] = 0.0; u[q] = 0.0; continue; }
+            double lik = u[q] / ukk;
+            if (fabs(lik) > 1e8) lik = lik * (1e8 / fabs(lik));
+            l[q] = lik;
+            u[q] = 0.0;
+            for (int32_t qk = diag[k] + 1; qk < rp[k + 1]; ++qk) {
+                /* (i, cv[qk]) in row i, by bisection (columns sorted) */
+                int32_t lo = rp[i], hi = rp[i + 1] - 1, hit = -1;
+                while (lo <= hi) {
+                    const int32_t mid = (lo + hi) >> 1;
+                    if (cv[mid] == cv[qk]) { hit = mid; break; }
+                    if (cv[mid] < cv[qk]) lo = mid + 1; else hi = mid - 1;
+                }
+                if (hit >= 0) u[hit] -= lik * u[qk];
+            }
+        }
+        const double thr = EPS64 * (rn[i] > 1.0 ? rn[i] : 1.0);
+        if (fabs(u[diag[i]]) < thr) u[diag[i]] = thr;
+    }
+    free(rn);
+}
+
+/* ILU(0) smoothing (src/smoothers.jl:1356-1433) with sequential substitutions
+ * (any order that respects the dependencies gives the same values). */
+void o_ilu0_smooth(int32_t n, const int32_t *rp, const int32_t *cv, const double *nz, const int32_t *diag,
+                   const double *l, const double *u, double *x, const double *b, double *t, int32_t steps) {
+    for (int32_t s = 0; s < steps; ++s) {
+        o_residual(n, rp, cv, nz, x, b, t);
+        for (int32_t i = 0; i < n; ++i)
+            for (int32_t k = rp[i]; k < diag[i]; ++k) t[i] -= l[k] * t[cv[k]];
+        for (int32_t i = n - 1; i >= 0; --i) {
+            for (int32_t k = diag[i] + 1; k < rp[i + 1]; ++k) t[i] -= u[k] * t[cv[k]];
+            const double uii = u[diag[i]];
+            t[i] = fabs(uii) > EPS64 ? t[i] / uii : 0.0;
+        }
+        for (int32_t i = 0; i < n; ++i)
+            if (isfinite(t[i])) x[i] += t[i];
+    }
+}
+
 static void o_smooth(const OHier *h, OLevel *L, double *x, const double *b, int32_t steps) {
+    if (h->smoother == 9) {
+        o_ilu0_smooth(L->n, L->a_rp, L->a_cv, L->a_nz, L->ilu_diag, L->ilu_l, L->ilu_u, x, b, L->tmp, steps);
+        return;
+    }
     if (h->smoother == 8) {
         o_cheb_smooth(L->n, L->a_rp, L->a_cv, L->a_nz, x, b, L->dinv, L->cheb_lmin, L->cheb_lmax, steps, L->tmp,
                       L->cheb_d);
@@ -375,6 +439,7 @@ int32_t o_resetup(OHier *h) {
     for (int32_t l = 0; l < h->nlevels; ++l) {
         OLevel *L = &h->lv[l];
         o_invdiag(L->n, L->a_rp, L->a_cv, L->a_nz, mode, L->dinv);
+        if (h->smoother == 9) o_ilu0_factor(L->n, L->a_rp, L->a_cv, L->a_nz, L->ilu_diag, L->ilu_l, L->ilu_u);
         double *out = (l + 1 < h->nlevels) ? h->lv[l + 1].a_nz : h->ac_nz;
         const int32_t nrow_next = (l + 1 < h->nlevels) ? h->lv[l + 1].n : h->ncoarse;
         const int32_t *rp_next = (l + 1 < h->nlevels) ? h->lv[l + 1].a_rp : h->ac_rp;
@@ -393,5 +458,6 @@ void o_prepare(OHier *h) {
     for (int32_t l = 0; l < h->nlevels; ++l) {
         OLevel *L = &h->lv[l];
         o_invdiag(L->n, L->a_rp, L->a_cv, L->a_nz, mode, L->dinv);
+        if (h->smoother == 9) o_ilu0_factor(L->n, L->a_rp, L->a_cv, L->a_nz, L->ilu_diag, L->ilu_l, L->ilu_u);
     }
 }

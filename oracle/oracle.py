@@ -26,7 +26,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
-SMOOTHER_CODES = {"jacobi": 0, "colored_gs": 1, "spai0": 3, "l1_jacobi": 5, "l1_colored_gs": 6, "chebyshev": 8}
+SMOOTHER_CODES = {"jacobi": 0, "colored_gs": 1, "spai0": 3, "l1_jacobi": 5, "l1_colored_gs": 6, "chebyshev": 8,
+                  "ilu0": 9}
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -43,6 +44,7 @@ class _OLevel(ctypes.Structure):
         ("ncolors", ctypes.c_int32), ("c_off", _i32p), ("c_order", _i32p),
         ("dinv", _f64p), ("r", _f64p), ("tmp", _f64p), ("xc", _f64p), ("bc", _f64p),
         ("cheb_lmin", ctypes.c_double), ("cheb_lmax", ctypes.c_double), ("cheb_d", _f64p),
+        ("ilu_diag", _i32p), ("ilu_l", _f64p), ("ilu_u", _f64p),
     ]
 
 
@@ -124,6 +126,39 @@ def cheb_smooth(rp, cv, nz, x, b, dinv, lmin, lmax, steps=1):
     lib().o_cheb_smooth(ctypes.c_int32(n), _p(rp, _i32p), _p(cv, _i32p), _p(nz, _f64p), _p(x, _f64p),
                         _p(b, _f64p), _p(dinv, _f64p), ctypes.c_double(lmin), ctypes.c_double(lmax),
                         ctypes.c_int32(steps), _p(r, _f64p), _p(d, _f64p))
+    return x
+
+
+def ilu0_diag(rp, cv):
+    """First stored diagonal position of every row (-1 when absent)."""
+    rp, cv = _i32(rp), _i32(cv)
+    n = len(rp) - 1
+    d = np.full(n, -1, dtype=np.int32)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    hit = np.nonzero(cv == rows)[0]
+    first = np.unique(rows[hit], return_index=True)
+    d[first[0]] = hit[first[1]]
+    return d
+
+
+def ilu0_factor(rp, cv, nz):
+    """ILU(0) factors (L strict lower, U upper with diagonal) on A's pattern."""
+    rp, cv, nz = _i32(rp), _i32(cv), _f64(nz)
+    diag = ilu0_diag(rp, cv)
+    L, U = np.zeros(len(nz)), np.zeros(len(nz))
+    lib().o_ilu0_factor(ctypes.c_int32(len(rp) - 1), _p(rp, _i32p), _p(cv, _i32p), _p(nz, _f64p), _p(diag, _i32p),
+                        _p(L, _f64p), _p(U, _f64p))
+    return diag, L, U
+
+
+def ilu0_smooth(rp, cv, nz, x, b, steps=1):
+    """ILU(0) smoothing of x with the oracle's own factors; returns the new x."""
+    rp, cv, nz, b = _i32(rp), _i32(cv), _f64(nz), _f64(b)
+    diag, L, U = ilu0_factor(rp, cv, nz)
+    x = _f64(x).copy()
+    t = np.zeros(len(x))
+    lib().o_ilu0_smooth(ctypes.c_int32(len(x)), _p(rp, _i32p), _p(cv, _i32p), _p(nz, _f64p), _p(diag, _i32p),
+                        _p(L, _f64p), _p(U, _f64p), _p(x, _f64p), _p(b, _f64p), _p(t, _f64p), ctypes.c_int32(steps))
     return x
 
 
@@ -240,6 +275,10 @@ class OracleHierarchy:
             o.xc, o.bc = _p(d["xc"], _f64p), _p(d["bc"], _f64p)
             d["cheb_d"] = np.zeros(n)
             o.cheb_d = _p(d["cheb_d"], _f64p)
+            d["ilu_diag"] = ilu0_diag(d["A_rp"], d["A_cv"])
+            d["ilu_l"], d["ilu_u"] = np.zeros(len(d["A_cv"])), np.zeros(len(d["A_cv"]))
+            o.ilu_diag = _p(d["ilu_diag"], _i32p)
+            o.ilu_l, o.ilu_u = _p(d["ilu_l"], _f64p), _p(d["ilu_u"], _f64p)
             if cheb is not None:
                 o.cheb_lmin, o.cheb_lmax = float(cheb[l][0]), float(cheb[l][1])
         self._arr = arr

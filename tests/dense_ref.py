@@ -29,6 +29,36 @@ def invdiag(A, l1, spai0=False):
     return np.where(np.abs(d) > thr, 1.0 / np.where(d == 0, 1, d), 0.0)
 
 
+def ilu0(A):
+    """ILU(0) of a dense matrix on its own nonzero pattern (IKJ order, the
+    reference's safeguards: src/smoothers.jl:1016-1079)."""
+    n = A.shape[0]
+    pat = A != 0
+    np.fill_diagonal(pat, True)
+    U = A.astype(float).copy()
+    L = np.zeros_like(U)
+    eps = np.finfo(float).eps
+    rn = np.abs(A).sum(axis=1)
+    for i in range(n):
+        for k in np.nonzero(pat[i, :i])[0]:
+            if abs(U[k, k]) < eps * max(1.0, rn[k]):
+                L[i, k] = 0.0
+                U[i, k] = 0.0
+                continue
+            lik = U[i, k] / U[k, k]
+            if abs(lik) > 1e8:
+                lik = lik * (1e8 / abs(lik))
+            L[i, k] = lik
+            U[i, k] = 0.0
+            for j in np.nonzero(pat[k, k + 1:])[0] + k + 1:
+                if pat[i, j]:
+                    U[i, j] -= lik * U[k, j]
+        thr = eps * max(1.0, rn[i])
+        if abs(U[i, i]) < thr:
+            U[i, i] = thr
+    return L, np.triu(U)
+
+
 class DenseAMG:
     def __init__(self, levels, coarse, smoother, omega=2.0 / 3.0, cycle="v", pre=1, post=1, cheb=None):
         self.A = [dense(l["A_rp"], l["A_cv"], l["A_nz"]) for l in levels]
@@ -38,6 +68,7 @@ class DenseAMG:
         self.smoother = smoother
         self.l1 = smoother.startswith("l1")
         self.D = [invdiag(A, self.l1, smoother == "spai0") for A in self.A]
+        self.LU = [ilu0(A) for A in self.A] if smoother == "ilu0" else None
         self.cheb = cheb
         self.omega, self.cycle_type, self.pre, self.post = omega, cycle, pre, post
 
@@ -55,6 +86,12 @@ class DenseAMG:
                     d = 2 * sn / de * D * (b - A @ x) + sn * so * d
                     x = x + d
                     so = sn
+            return x
+        if self.smoother == "ilu0":
+            Lf, Uf = self.LU[l]
+            M = (np.eye(len(x)) + Lf) @ Uf
+            for _ in range(steps):
+                x = x + np.linalg.solve(M, b - A @ x)
             return x
         if self.smoother == "spai0":
             for _ in range(steps):

@@ -143,3 +143,56 @@ def test_chebyshev_resetup_recomputes_bounds(dh, orc, torch_cuda):
     x = torch.zeros_like(z)
     _, it, rel = dh.pcg_solve_(x, torch.tensor(b, device="cuda"), h, cfg, tol=1e-8, maxiter=200)
     assert rel < 1e-8
+
+
+# ── ILU(0), level-scheduled ─────────────────────────────────────────────────
+def _ilu_built(dh):
+    if "ilu" not in _cache:
+        P = dh.problems
+        A = P.heterogeneous3d(12, P.log_permeability(12, 1.0, SEED + 700))
+        cfg = dh.AMGConfig(coarsening=dh.HMISCoarsening(0.5, dh.ExtendedIInterpolation(0.3)),
+                           smoother=dh.ILU0SmootherType())
+        _cache["ilu"] = (A, cfg, dh.amg_setup(A, cfg))
+    return _cache["ilu"]
+
+
+def test_ilu0_factors_and_step(dh, orc, torch_cuda):
+    torch = torch_cuda
+    A, cfg, h = _ilu_built(dh)
+    for l in range(h.nlevels):
+        lv = h.level(l, with_rmap=False)
+        n, nnz = lv["n"], len(lv["A_cv"])
+        _, L, U = orc.ilu0_factor(lv["A_rp"], lv["A_cv"], lv["A_nz"])
+        assert np.array_equal(h.device_values(l, 6, nnz), L) and np.array_equal(h.device_values(l, 7, nnz), U), l
+        x0 = dh.problems.splitmix_normal(SEED + 710 + l, n)
+        b = dh.problems.splitmix_normal(SEED + 720 + l, n)
+        out = torch.tensor(x0, device="cuda")
+        h.level_op(l, 11, "fp64", in2=torch.tensor(b, device="cuda"), out=out)
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), orc.ilu0_smooth(lv["A_rp"], lv["A_cv"], lv["A_nz"], x0, b)), l
+
+
+def test_ilu0_cycle_pcg_and_resetup(dh, orc, torch_cuda):
+    torch = torch_cuda
+    P = dh.problems
+    A, cfg, h = _ilu_built(dh)
+    oh = oracle_of(h, orc, "ilu0")
+    b = P.splitmix_normal(SEED + 730, A.n)
+    z = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+    dh.amg_cycle_(z, torch.tensor(b, device="cuda"), h, cfg, zero_guess=True)
+    assert np.array_equal(z.cpu().numpy(), oh.cycle(np.zeros(A.n), b))
+    x = torch.zeros_like(z)
+    _, it, rel = dh.pcg_solve_(x, torch.tensor(b, device="cuda"), h, cfg, tol=1e-10, maxiter=200)
+    xo, ito, _ = oh.pcg(np.zeros(A.n), b, 1e-10, 200)
+    assert abs(it - ito) <= 1 and rel_err(x.cpu().numpy(), xo) < 1e-10
+    x32 = torch.zeros_like(z)
+    _, it32, rel32 = dh.pcg_solve_(x32, torch.tensor(b, device="cuda"), h, dataclasses.replace(cfg, precision="fp32"),
+                                  tol=1e-8, maxiter=200)
+    assert rel32 < 1e-8
+    # resetup refactorizes every level (on the host, as the reference)
+    A2 = P.heterogeneous3d(12, P.log_permeability(12, 1.0, SEED + 700) + 0.4 * P.splitmix_normal(SEED + 731, A.n))
+    dh.amg_resetup_(h, A2, cfg)
+    oh.resetup(A2.nzval)
+    z.zero_()
+    dh.amg_cycle_(z, torch.tensor(b, device="cuda"), h, cfg, zero_guess=True)
+    assert np.array_equal(z.cpu().numpy(), oh.cycle(np.zeros(A.n), b))

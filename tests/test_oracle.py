@@ -329,3 +329,51 @@ def test_cycle_vs_dense_new_smoothers(dh, orc, smoother):
     assert rel_err(oh.cycle(x0, b), dn.cycle(x0, b)) < 1e-12
     x, it, rel = oh.solve(np.zeros(A.n), b, 1e-8, 200)
     assert rel < 1e-8 and it < 200
+
+
+# ── ILU(0) ───────────────────────────────────────────────────────────────────
+def test_ilu0_tridiagonal_is_exact(dh, orc):
+    """On a tridiagonal matrix ILU(0) is the exact LU: one step solves."""
+    A = dh.problems.poisson1d(50)
+    b = dh.problems.splitmix_normal(SEED + 90, 50)
+    x = orc.ilu0_smooth(A.rowptr, A.colval, A.nzval, np.zeros(50), b)
+    assert np.linalg.norm(b - A.matvec(x)) < 1e-12 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("name", ["poisson2d", "hetero3d"])
+def test_ilu0_factors_vs_dense_and_host(dh, orc, name):
+    """The oracle's factors equal the dense restatement's and the library's
+    host factorization (the one the ilu0 smoother uploads) bit for bit."""
+    import ctypes
+    from dense_ref import dense, ilu0
+    P = dh.problems
+    A = P.poisson2d(9) if name == "poisson2d" else P.heterogeneous3d(5, P.log_permeability(5, 1.5, SEED + 91))
+    diag, L, U = orc.ilu0_factor(A.rowptr, A.colval, A.nzval)
+    Ld, Ud = ilu0(dense(A.rowptr, A.colval, A.nzval))
+    rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+    low = A.colval < rows
+    assert np.array_equal(L[low], Ld[rows[low], A.colval[low]]) and not L[~low].any()
+    up = A.colval >= rows
+    assert np.array_equal(U[up], Ud[rows[up], A.colval[up]]) and not U[~up].any()
+    d2, L2, U2 = np.zeros(A.n, np.int32), np.zeros(A.nnz), np.zeros(A.nnz)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    dh._lib.check(dh.lib().draugr_hip_ilu0_factor(A.n, vp(A.rowptr), vp(A.colval), vp(A.nzval), vp(d2), vp(L2),
+                                                  vp(U2)))
+    assert np.array_equal(d2, diag) and np.array_equal(L2, L) and np.array_equal(U2, U)
+
+
+def test_cycle_vs_dense_ilu0(dh, orc):
+    P = dh.problems
+    A = P.heterogeneous3d(8, P.log_permeability(8, 1.0, SEED + 92))
+    cfg = dh.AMGConfig(coarsening=dh.HMISCoarsening(0.5, dh.ExtendedIInterpolation(0.3)),
+                       smoother=dh.ILU0SmootherType(), max_coarse_size=10)
+    h = _hier(dh, A, cfg)
+    levels = [h.level(l) for l in range(h.nlevels)]
+    c = h.coarsest()
+    oh = orc.OracleHierarchy(levels, c["rp"], c["cv"], c["nz"], smoother="ilu0")
+    dn = DenseAMG(levels, c, "ilu0")
+    b = P.splitmix_normal(SEED + 93, A.n)
+    x0 = P.splitmix_normal(SEED + 94, A.n)
+    assert rel_err(oh.cycle(x0, b), dn.cycle(x0, b)) < 1e-11
+    x, it, rel = oh.solve(np.zeros(A.n), b, 1e-8, 200)
+    assert rel < 1e-8 and it < 50
