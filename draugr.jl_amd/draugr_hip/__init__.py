@@ -12,6 +12,7 @@ from .config import (AbsoluteStrength, AggregationCoarsening, AggressiveCoarseni
                      SerialGaussSeidelType, SerialILU0SmootherType, SignedStrength, SmoothedAggregationCoarsening,
                      SPAI0SmootherType, SPAI1SmootherType, StandardInterpolation, hypre_default_config)
 from .core import (AMGHierarchy, DraugrPreconditioner, amg_cycle_, amg_resetup_, amg_setup, amg_solve_, as_csr,
+                   aslinearoperator,
                    device_count, ldiv_, pcg_solve_, preconditioner_apply_, preconditioner_update_, upload)
 from . import adjoint, batch, problems
 from .batch import AMGBatch, amg_batch_setup

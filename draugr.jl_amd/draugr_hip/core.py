@@ -324,6 +324,22 @@ class DraugrPreconditioner:
         return self.apply_(x, b)
 
 
+def aslinearoperator(prec: DraugrPreconditioner):
+    """The preconditioner as a ``scipy.sparse.linalg.LinearOperator`` (z = V(0; r)
+    on host vectors), the role ldiv! plays for LinearSolve.jl
+    (ext/DraugrLinearSolveExt.jl:37-39) and apply! for Jutul
+    (ext/DraugrJutulExt.jl:172-176): ``scipy.sparse.linalg.cg(A, b, M=...)``."""
+    import scipy.sparse.linalg as spla
+    n = prec.nrows()
+
+    def mv(r):
+        z = np.zeros(n)
+        prec.apply_(z, np.ascontiguousarray(np.asarray(r, dtype=np.float64).ravel()))
+        return z
+
+    return spla.LinearOperator((n, n), matvec=mv, dtype=np.float64)
+
+
 preconditioner_update_ = DraugrPreconditioner.update_
 preconditioner_apply_ = DraugrPreconditioner.apply_
 ldiv_ = DraugrPreconditioner.ldiv_

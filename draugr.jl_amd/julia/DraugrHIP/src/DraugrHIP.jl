@@ -23,7 +23,7 @@ using ChainRulesCore
 using LinearAlgebra
 using SparseArrays
 
-export HIPHierarchy, hip_setup, pcg_solve!, solve_7pt
+export HIPHierarchy, HIPPreconditioner, hip_setup, pcg_solve!, solve_7pt
 
 const libdraugr_hip = get(ENV, "DRAUGR_HIP_LIB",
                           normpath(joinpath(@__DIR__, "..", "..", "..", "lib", "libdraugr_hip.so")))
@@ -201,6 +201,42 @@ function LinearAlgebra.ldiv!(z::ROCVector{Float64}, h::HIPHierarchy, r::ROCVecto
                 h.handle, h.n, pointer(z), pointer(r), c.h, 1, _stream()))
     return z
 end
+
+# ── preconditioner adapters (ext/DraugrLinearSolveExt.jl:18-62,
+#    ext/DraugrJutulExt.jl:146-185) on the HIP hierarchy ─────────────────────
+"""
+    HIPPreconditioner(config; precision=:fp32)
+
+AMG preconditioner on the GPU for LinearSolve.jl (`ldiv!`) and Jutul
+(`update_preconditioner!` / `apply!` / `operator_nrows`): the first update
+sets the hierarchy up, later updates reset it up (partial), apply is z = 0
+and one cycle, zero-copy on `ROCVector`s.
+"""
+mutable struct HIPPreconditioner
+    config::AMGConfig
+    precision::Symbol
+    hierarchy::Union{Nothing, HIPHierarchy}
+end
+HIPPreconditioner(config::AMGConfig=AMGConfig(); precision::Symbol=:fp32) =
+    HIPPreconditioner(config, precision, nothing)
+
+function update!(P::HIPPreconditioner, A::SparseMatrixCSC)
+    if P.hierarchy === nothing
+        P.hierarchy = hip_setup(A, P.config; precision=P.precision)
+    else
+        Draugr.amg_resetup!(P.hierarchy, A, P.config)
+    end
+    return P
+end
+
+LinearAlgebra.ldiv!(z::ROCVector{Float64}, P::HIPPreconditioner, r::ROCVector{Float64}) =
+    LinearAlgebra.ldiv!(z, P.hierarchy, r)
+
+# Jutul's interface, when Jutul is loaded alongside (its methods are added
+# by the package extension that wires this type in):
+#   Jutul.update_preconditioner!(P::HIPPreconditioner, A, b, context, executor) = update!(P, A)
+#   Jutul.apply!(x, P::HIPPreconditioner, y) = ldiv!(x, P, y)
+#   Jutul.operator_nrows(P::HIPPreconditioner) = P.hierarchy === nothing ? 0 : P.hierarchy.n
 
 # ── differentiation of x = A(k)^-1 b for the 7-point operator ──────────────
 function _assemble!(nz::ROCVector{Float64}, h::HIPHierarchy, m::Int, k::ROCVector{Float64})

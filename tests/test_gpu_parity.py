@@ -294,3 +294,24 @@ def test_gpu_error_paths(dh):
     # full resetup rebuilds
     dh.amg_resetup_(h_np, dh.problems.poisson2d(12), cfg, partial=False)
     assert h_np.n == 144 and h_np.info()["n_fine"] == 144
+
+
+def test_preconditioner_adapter_with_scipy_cg(dh, torch_cuda):
+    """DraugrPreconditioner (src/preconditioner.jl:29-102) as the M of scipy's
+    CG: update_ sets up, then resets up; CG converges in about the PCG's
+    iteration count."""
+    import scipy.sparse.linalg as spla
+    P = dh.problems
+    A = P.heterogeneous3d(16, P.log_permeability(16, 1.0, SEED + 800))
+    prec = dh.DraugrPreconditioner(dh.hypre_default_config(smoother=dh.L1JacobiSmootherType()))
+    prec.update_(A)
+    b = P.splitmix_normal(SEED + 801, A.n)
+    its = []
+    x, info = spla.cg(A.to_scipy(), b, rtol=1e-8, maxiter=200, M=dh.aslinearoperator(prec),
+                      callback=lambda xk: its.append(1))
+    assert info == 0 and np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b) < 1e-7
+    assert len(its) < 40
+    A2 = P.heterogeneous3d(16, P.log_permeability(16, 1.0, SEED + 802))
+    prec.update_(A2)  # resetup path
+    x2, info2 = spla.cg(A2.to_scipy(), b, rtol=1e-8, maxiter=200, M=dh.aslinearoperator(prec))
+    assert info2 == 0
