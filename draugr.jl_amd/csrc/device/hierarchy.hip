@@ -487,7 +487,9 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
     scal_.alloc(S_COUNT_);
     partial_.alloc(max_blocks);
     DRAUGR_HIP_CHECK(hipMemsetAsync(scal_.p, 0, S_COUNT_ * sizeof(double), st));
-    DRAUGR_HIP_CHECK(hipHostMalloc((void**)&pinned_, 16 * sizeof(double), hipHostMallocDefault));
+    DRAUGR_HIP_CHECK(hipHostMalloc((void**)&pinned_, 16 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+    DRAUGR_HIP_CHECK(hipHostGetDevicePointer((void**)&pinned_dev_, pinned_, 0));
+    for (int i = 0; i < 16; ++i) pinned_[i] = 0.0;
     for (int l = 0; l < nlevels(); ++l) compute_dinv(l, st);
     if (smoother_ == Smoother::Chebyshev) chebyshev_bounds(st);
     coarse_factor(st);
@@ -1154,8 +1156,17 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                 };
                 launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, partial_.p, st);
             }
-            // x += alpha p; r -= alpha q; r.r  (and the FP32 copy of r)
+            // x += alpha p; r -= alpha q; r.r  (and the FP32 copy of r).  The
+            // finalize also writes r.r and the iteration number straight into
+            // mapped pinned host memory, so no copy is enqueued for the check.
             const double* alphap = scal + S_ALPHA;
+            double* hp = pinned_dev_;
+            const double itd = (double)it;
+            auto fin_rr = [=] __device__(double s) {
+                scal[S_RR] = s;
+                hp[kPinRR] = s;
+                hp[kPinIter] = itd;
+            };
             if (mixed)
                 launch_reduce(n,
                               [=] __device__(int64_t i) {
@@ -1166,7 +1177,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                                   r32[i] = (float)ri;
                                   return ri * ri;
                               },
-                              [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, st);
+                              fin_rr, partial_.p, st);
             else
                 launch_reduce(n,
                               [=] __device__(int64_t i) {
@@ -1176,8 +1187,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                                   r[i] = ri;
                                   return ri * ri;
                               },
-                              [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, st);
-            DRAUGR_HIP_CHECK(hipMemcpyAsync(pinned_ + S_RR, scal + S_RR, sizeof(double), hipMemcpyDeviceToHost, st));
+                              fin_rr, partial_.p, st);
             DRAUGR_HIP_CHECK(hipEventRecord(ev, st));
             // speculative: next preconditioned residual and beta
             apply_prec();
@@ -1191,7 +1201,10 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                 launch_reduce(n, [=] __device__(int64_t i) { return r[i] * z[i]; }, fin_rz, partial_.p, st);
             std::swap(pa, pb);
             DRAUGR_HIP_CHECK(hipEventSynchronize(ev));
-            rel = std::sqrt(pinned_[S_RR] / bb);
+            double rr = pinned_[kPinRR];
+            if (pinned_[kPinIter] != (double)it)  // not visible yet: read the device copy
+                DRAUGR_HIP_CHECK(hipMemcpy(&rr, scal + S_RR, sizeof(double), hipMemcpyDeviceToHost));
+            rel = std::sqrt(rr / bb);
             if (relres) *relres = rel;
             if (verbose >= 2) std::printf("  PCG iter %4d: relative residual = %.6e\n", it, rel);
             if (rel < tol) { result = it; break; }

@@ -176,7 +176,9 @@ private:
     DBuf<double> w_r_, w_p0_, w_p1_, w_q_, w_z_, w_x_;
     DBuf<float> w_r32_, w_z32_;
     DBuf<double> scal_, partial_;
-    double* pinned_ = nullptr;
+    double* pinned_ = nullptr;      // mapped, coherent host memory
+    double* pinned_dev_ = nullptr;  // its device address
+    static constexpr int kPinRR = 8, kPinIter = 9;
     hipStream_t stream_ = nullptr;
     hipEvent_t last_ev_ = nullptr;
     hipStream_t cap_ = nullptr;  // capture stream for the graphs
