@@ -362,6 +362,7 @@ template <> constexpr int pidx<bf16_t>() { return 2; }
 template <typename TV, typename TX, class XF, class EPI>
 void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
     if (kind == kOpRowSeq) launch_rowseq<TV, TX>(M.nrow, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
+    else if (kind == kOpPipe) launch_pipe<TV, TX>(M.nblk, M.per, M.blk.p, M.bz.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
     else launch_stream<TV, TX>(M.nblk, M.per, M.blk.p, M.bz.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
 }
 
@@ -397,7 +398,7 @@ bool is_gs(Smoother s) { return s == Smoother::ColoredGS || s == Smoother::L1Col
 
 // ─────────────────────────────────────────────────────────────────────────────
 DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
-    : smoother_(cfg.smoother), omega_(cfg.jacobi_omega), seed_(cfg.seed) {
+    : smoother_(cfg.smoother), omega_(cfg.jacobi_omega), seed_(cfg.seed), forced_kind_(cfg.kernel) {
     if (!(smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || is_gs(smoother_) ||
           smoother_ == Smoother::SPAI0 || smoother_ == Smoother::Chebyshev || smoother_ == Smoother::ILU0))
         throw Error("smoother not available in the HIP backend (supported: jacobi, l1_jacobi, colored_gs, "
@@ -772,6 +773,14 @@ template <typename TV, typename TX>
 void DeviceHierarchy::tune() {
     typedef typename TransferT<TV>::type TP;
     const int pi = pidx<TV>();
+    if (forced_kind_ >= 0) {
+        for (auto& D : L_) {
+            D.A.kind[pi] = D.Pt.kind[pi] = D.P.kind[pi] = forced_kind_;
+            D.fuse_zero[pi] = true;
+        }
+        tuned_[pi] = true;
+        return;
+    }
     DRAUGR_HIP_CHECK(hipDeviceSynchronize());
     hipStream_t st = stream_;
     hipEvent_t e0, e1;
@@ -803,15 +812,16 @@ void DeviceHierarchy::tune() {
         launch_map(nc, [=] __device__(int64_t i) { xcv[i] = (TX)1; }, st);
         const TX* d = d_of<TX>(D);
         const TX w = (TX)weight();
-        float tA[2], tP[2], tPt[2];
-        for (int k = 0; k < 2; ++k) {
+        float tA[3], tP[3], tPt[3];
+        for (int k = 0; k < 3; ++k) {
             tA[k] = timeit([&] { op_kind<TV, TX>(k, D.A, XVec<TX>{xin}, EpiJacobi<TX>{xin, bin, d, s1, w}, st); });
             tPt[k] = timeit([&] { op_kind<TP, TX>(k, D.Pt, XVec<TX>{xin}, EpiStore<TX>{s2}, st); });
             tP[k] = timeit([&] { op_kind<TP, TX>(k, D.P, XVec<TX>{xcv}, EpiAdd<TX>{s2}, st); });
         }
-        D.A.kind[pi] = tA[1] < tA[0] ? kOpRowSeq : kOpStream;
-        D.Pt.kind[pi] = tPt[1] < tPt[0] ? kOpRowSeq : kOpStream;
-        D.P.kind[pi] = tP[1] < tP[0] ? kOpRowSeq : kOpStream;
+        auto best = [](const float* t) { return (int)(std::min_element(t, t + 3) - t); };
+        D.A.kind[pi] = best(tA);
+        D.Pt.kind[pi] = best(tPt);
+        D.P.kind[pi] = best(tP);
         const float fused = timeit([&] {
             op<TV, TX>(D.A, XZeroJacobi<TX>{d, bin, w}, EpiZeroJacobiResid<TX>{bin, d, s1, s2, w}, st);
         });
@@ -835,9 +845,9 @@ std::string DeviceHierarchy::tune_report() const {
         if (!tuned_[p]) continue;
         for (int l = 0; l < nlevels(); ++l) {
             const DevLevel& D = L_[l];
+            const char* kn[3] = {"stream", "rowseq", "pipe"};
             std::snprintf(buf, sizeof buf, "%s level %d: A=%s Pt=%s P=%s zero-guess=%s\n", names[p], l,
-                          D.A.kind[p] ? "rowseq" : "stream", D.Pt.kind[p] ? "rowseq" : "stream",
-                          D.P.kind[p] ? "rowseq" : "stream", D.fuse_zero[p] ? "fused" : "split");
+                          kn[D.A.kind[p]], kn[D.Pt.kind[p]], kn[D.P.kind[p]], D.fuse_zero[p] ? "fused" : "split");
             out += buf;
         }
     }

@@ -170,6 +170,7 @@ private:
     Smoother smoother_;
     double omega_;
     uint64_t seed_ = 0;
+    int forced_kind_ = -1;  // config "kernel": every sparse pass uses this kernel (no tuning)
     bool have32_ = false, have16_ = false;
     bool tuned_[3] = {false, false, false};
     // solver workspaces at the finest size

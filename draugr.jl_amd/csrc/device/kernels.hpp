@@ -22,6 +22,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -267,7 +268,120 @@ __global__ __launch_bounds__(kBlock) void k_csr_rowseq(int nrows, const int* __r
     epi.fin(row, a, q);
 }
 
-enum OpKernel : int { kOpStream = 0, kOpRowSeq = 1 };
+// Software-pipelined CSR-stream: workgroup w walks a contiguous run of row
+// blocks; while block b's gathers, LDS stage and epilogue run, the column and
+// value loads of block b+1 are already in flight, so each wave keeps two
+// blocks' worth of memory requests outstanding (the plain kernel waits for
+// one block's loads before it can issue its gathers).  Same per-row
+// operation order as k_csr_stream.  Blocks longer than PER * kBlock nonzeros
+// fall back to the plain kernel's loop / tree paths.
+template <bool XCD, int PER, typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_csr_pipe(int nblk, const int* __restrict__ blk,
+                                                     const int* __restrict__ bz, const int* __restrict__ rp,
+                                                     const int* __restrict__ cv, const TV* __restrict__ av, XF xf,
+                                                     EPI epi) {
+    __shared__ TX prod[kStreamNnz];
+    const int w = XCD ? xcd_block() : (int)blockIdx.x;
+    const int per_wg = (nblk + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int b0 = w * per_wg, b1 = min(nblk, b0 + per_wg);
+    if (b0 >= b1) return;
+    const int t = (int)threadIdx.x;
+    int cj[PER];
+    TX v[PER];
+    auto load = [&](int z0, int nz) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int k = min(t + u * kBlock, nz - 1);
+            cj[u] = ldi(cv, z0 + k);
+            v[u] = ld<TX, TV>(av, z0 + k);
+        }
+    };
+    int r0 = blk[b0], r1 = blk[b0 + 1], z0 = bz[b0], nz = bz[b0 + 1] - z0;
+    if (nz > 0 && nz <= PER * kBlock) load(z0, nz);
+    for (int b = b0; b < b1; ++b) {
+        const int row = r0 + t;
+        const bool own = row < r1;
+        const int rs = own ? rp[row] : 0, re = own ? rp[row + 1] : 0;
+        typename EPI::Pre q{};
+        if (own) q = epi.pre(row);
+        // next block's header (the loads themselves are issued after this block's gathers)
+        int nr0 = 0, nr1 = 0, nz0 = 0, nnz = 0;
+        if (b + 1 < b1) {
+            nr0 = blk[b + 1];
+            nr1 = blk[b + 2];
+            nz0 = bz[b + 1];
+            nnz = bz[b + 2] - nz0;
+        }
+        if (nz <= PER * kBlock) {
+            if (nz > 0) {
+                TX g[PER];
+#pragma unroll
+                for (int u = 0; u < PER; ++u) g[u] = xf(cj[u]);
+                TX vv[PER];
+#pragma unroll
+                for (int u = 0; u < PER; ++u) vv[u] = v[u];
+                if (nnz > 0 && nnz <= PER * kBlock) load(nz0, nnz);
+#pragma unroll
+                for (int u = 0; u < PER; ++u)
+                    if (t + u * kBlock < nz) prod[t + u * kBlock] = vv[u] * g[u];
+            } else if (nnz > 0 && nnz <= PER * kBlock) {
+                load(nz0, nnz);
+            }
+            __syncthreads();
+            if (own) epi_segment(epi, row, prod + (rs - z0), re - rs, q);
+        } else if (nz <= kStreamNnz) {  // one row longer than the budget: row order, no pipelining
+            for (int k = t; k < nz; k += kBlock) prod[k] = ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
+            __syncthreads();
+            if (own) epi_segment(epi, row, prod + (rs - z0), re - rs, q);
+            if (nnz > 0 && nnz <= PER * kBlock) load(nz0, nnz);
+        } else {  // a row longer than the LDS stage: tree sum
+            TX acc = 0;
+            for (int k = t; k < nz; k += kBlock) acc += ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
+            prod[t] = acc;
+            __syncthreads();
+            for (int s = kBlock / 2; s > 0; s >>= 1) {
+                if (t < s) prod[t] += prod[t + s];
+                __syncthreads();
+            }
+            if (t == 0) epi.total(r0, prod[0]);
+            if (nnz > 0 && nnz <= PER * kBlock) load(nz0, nnz);
+        }
+        __syncthreads();  // the stage is reused by the next block
+        r0 = nr0;
+        r1 = nr1;
+        z0 = nz0;
+        nz = nnz;
+    }
+}
+
+// Resident workgroups of the pipelined kernel: about 8 per CU on 256 CUs.
+constexpr int kPipeGrid = 2048;
+
+template <int PER, typename TV, typename TX, class XF, class EPI>
+void launch_pipe_per(int nblk, const int* blk, const int* bz, const int* rp, const int* cv, const TV* av, XF xf,
+                     EPI epi, hipStream_t st) {
+    const int grid = std::min(nblk, kPipeGrid);
+    if (grid >= 64)
+        hipLaunchKernelGGL((k_csr_pipe<true, PER, TV, TX, XF, EPI>), dim3(grid), dim3(kBlock), 0, st, nblk, blk, bz,
+                           rp, cv, av, xf, epi);
+    else
+        hipLaunchKernelGGL((k_csr_pipe<false, PER, TV, TX, XF, EPI>), dim3(grid), dim3(kBlock), 0, st, nblk, blk,
+                           bz, rp, cv, av, xf, epi);
+}
+
+template <typename TV, typename TX, class XF, class EPI>
+void launch_pipe(int nblk, int per, const int* blk, const int* bz, const int* rp, const int* cv, const TV* av, XF xf,
+                 EPI epi, hipStream_t st) {
+    if (nblk <= 0) return;
+    switch (per) {
+        case 1: launch_pipe_per<1, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
+        case 2: launch_pipe_per<2, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
+        case 4: launch_pipe_per<4, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
+        default: launch_pipe_per<8, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
+    }
+}
+
+enum OpKernel : int { kOpStream = 0, kOpRowSeq = 1, kOpPipe = 2 };
 
 template <int PER, typename TV, typename TX, class XF, class EPI>
 void launch_stream_per(int nblk, const int* blk, const int* bz, const int* rp, const int* cv, const TV* av, XF xf,

@@ -84,14 +84,22 @@ HostHierarchy amg_setup_host(const CSR& A_in, const Config& cfg, bool allow_part
         if (n <= cfg.max_coarse_size) break;
         const CoarsenSpec& alg = lvl <= cfg.initial_coarsening_levels ? cfg.initial_coarsening : cfg.coarsening;
         int32_t nc = 0;
+        auto ta = std::chrono::steady_clock::now();
         CSR P = coarsen_with_fallback(A, alg, cfg, rng, nc);
         if (nc >= n || nc == 0) break;
         HostLevel L;
         CSR Ac;
+        auto tb = std::chrono::steady_clock::now();
         coarse_sparsity(A, P, nc, Ac, allow_partial_resetup ? &L.R : nullptr);
         L.has_rmap = allow_partial_resetup;
+        auto tc = std::chrono::steady_clock::now();
         build_transpose_map(P, L.Pt);
         if (needs_coloring(cfg.smoother)) greedy_coloring(A, L.coloring);
+        if (cfg.verbose >= 2) {
+            const auto sec = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+            std::printf("  setup level %d: n=%d coarsening+P %.3fs, coarse operator %.3fs, maps %.3fs\n", lvl, n,
+                        sec(ta, tb), sec(tb, tc), sec(tc, std::chrono::steady_clock::now()));
+        }
         L.A = std::move(A);
         L.P = std::move(P);
         h.levels.push_back(std::move(L));

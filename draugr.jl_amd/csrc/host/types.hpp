@@ -76,6 +76,7 @@ struct Config {
     Precision precision = Precision::FP64;  // arithmetic of the V/W cycle
     uint64_t seed = 0x5EEDull;              // seed of the setup RNG (PMIS/HMIS measures)
     bool graphs = true;                     // replay the preconditioner cycle from a captured hipGraph
+    int kernel = -1;                        // sparse-pass kernel: -1 autotuned, 0 stream, 1 row-sequential, 2 pipelined
 };
 
 // ── hierarchy (host copy; uploaded once to HBM) ──────────────────────────────

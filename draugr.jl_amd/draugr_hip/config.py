@@ -170,6 +170,7 @@ class AMGConfig:
     precision: str = "fp64"
     seed: int = 0x5EED
     graphs: bool = True
+    kernel: str = "auto"  # sparse-pass kernel: "auto" (autotuned), "stream", "rowseq", "pipe"
 
     def __post_init__(self):
         if str(self.cycle_type).upper() not in ("V", "W"):
@@ -202,6 +203,7 @@ class AMGConfig:
             "precision": self.precision,
             "seed": self.seed,
             "graphs": self.graphs,
+            "kernel": self.kernel,
         }
         if hasattr(c, "interpolation"):
             d["interpolation"] = c.interpolation.to_json()

@@ -315,3 +315,26 @@ def test_preconditioner_adapter_with_scipy_cg(dh, torch_cuda):
     prec.update_(A2)  # resetup path
     x2, info2 = spla.cg(A2.to_scipy(), b, rtol=1e-8, maxiter=200, M=dh.aslinearoperator(prec))
     assert info2 == 0
+
+
+@pytest.mark.parametrize("kernel", ["stream", "rowseq", "pipe"])
+@pytest.mark.parametrize("name", ["hypre3d", "agg2d"])
+def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
+    """Each of the three sparse-pass kernels the autotuner chooses among,
+    forced on every level, gives the oracle's FP64 cycle bit for bit and FP32
+    cycles equal to each other to rounding."""
+    import dataclasses
+    torch = torch_cuda
+    A, cfg, sm = _case(dh, name)
+    c = dataclasses.replace(cfg, kernel=kernel)
+    h = dh.amg_setup(A, c)
+    oh = oracle_of(h, orc, sm)
+    b = dh.problems.splitmix_normal(SEED + 900, A.n)
+    x0 = dh.problems.splitmix_normal(SEED + 901, A.n)
+    x = torch.tensor(x0, device="cuda")
+    dh.amg_cycle_(x, torch.tensor(b, device="cuda"), h, c)
+    assert np.array_equal(x.cpu().numpy(), oh.cycle(x0, b))
+    z = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+    _, it, rel = dh.pcg_solve_(z, torch.tensor(b, device="cuda"), h, dataclasses.replace(c, precision="fp32"),
+                              tol=1e-8)
+    assert rel < 1e-8
