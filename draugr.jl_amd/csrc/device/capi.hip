@@ -16,6 +16,7 @@
 
 #include "../../../include/draugr_hip.h"
 #include "batch.hpp"
+#include "galerkin.hpp"
 #include "hierarchy.hpp"
 
 namespace draugr {
@@ -136,7 +137,7 @@ int32_t setup_impl(int32_t n, int32_t nnz, const int32_t* rp, const int32_t* cv,
     auto H = std::make_shared<Hier>();
     H->cfg = cfg;
     H->allow_partial = allow != 0;
-    H->host = amg_setup_host(A, cfg, H->allow_partial);
+    H->host = amg_setup_host(A, cfg, H->allow_partial, upload && cfg.gpu_galerkin ? galerkin_device : nullptr);
     if (upload) H->dev = std::make_unique<DeviceHierarchy>(H->host, cfg);
     if (cfg.verbose >= 1) std::printf("%s", hierarchy_summary(H->host, cfg, n).c_str());
     std::lock_guard<std::mutex> g(g_lock);
@@ -236,7 +237,7 @@ int32_t draugr_amg_resetup(int32_t handle, int32_t n, int32_t nnz, const int32_t
             H->cfg = cfg;
             H->allow_partial = allow_partial_resetup != 0;
             H->dev.reset();
-            H->host = amg_setup_host(A, cfg, H->allow_partial);
+            H->host = amg_setup_host(A, cfg, H->allow_partial, cfg.gpu_galerkin ? galerkin_device : nullptr);
             H->dev = std::make_unique<DeviceHierarchy>(H->host, cfg);
             if (cfg.verbose >= 1) std::printf("%s", hierarchy_summary(H->host, cfg, n).c_str());
         }
@@ -431,6 +432,12 @@ int32_t draugr_hip_level_export(int32_t handle, int32_t level, int32_t which, vo
         }
         if (level < 0 || level >= (int32_t)H->host.levels.size()) throw Error("level index out of range");
         const HostLevel& L = H->host.levels[level];
+        // triple map left in HBM by the GPU setup: copied out
+        auto rm = [&]() { return std::static_pointer_cast<DeviceRMap>(L.R.device); };
+        auto get_dev = [&](const auto* src, size_t count) {
+            if (count) DRAUGR_HIP_CHECK(hipMemcpy(dst, src, count * sizeof(src[0]), hipMemcpyDeviceToHost));
+            return 0;
+        };
         switch (which) {
             case 0: return put(L.A.rp);
             case 1: return put(L.A.cv);
@@ -441,10 +448,10 @@ int32_t draugr_hip_level_export(int32_t handle, int32_t level, int32_t which, vo
             case 6: return put(L.Pt.offsets);
             case 7: return put(L.Pt.fine_rows);
             case 8: return put(L.Pt.p_nz_idx);
-            case 9: return put(L.R.nz_offsets);
-            case 10: return put(L.R.pi);
-            case 11: return put(L.R.ai);
-            case 12: return put(L.R.pj);
+            case 9: return L.R.device ? get_dev(rm()->off.p, rm()->off.n) : put(L.R.nz_offsets);
+            case 10: return L.R.device ? get_dev(rm()->pi.p, (size_t)rm()->ntriples) : put(L.R.pi);
+            case 11: return L.R.device ? get_dev(rm()->ai.p, (size_t)rm()->ntriples) : put(L.R.ai);
+            case 12: return L.R.device ? get_dev(rm()->pj.p, (size_t)rm()->ntriples) : put(L.R.pj);
             case 13: return put(L.coloring.color_offsets);
             case 14: return put(L.coloring.color_order);
             default: throw Error("unknown export selector");
@@ -586,7 +593,7 @@ int32_t draugr_hip_batch_setup(int32_t nsys, const int32_t* n, const int32_t* nn
         std::vector<HostHierarchy> host(nsys);
         for (int32_t s = 0; s < nsys; ++s) {
             CSR A = csr_from_raw(n[s], nnz[s], rowptr[s], colval[s], nzval[s], index_base);
-            host[s] = amg_setup_host(A, cfg, false);
+            host[s] = amg_setup_host(A, cfg, false, cfg.gpu_galerkin ? galerkin_device : nullptr);
         }
         std::vector<const HostHierarchy*> hp(nsys);
         for (int32_t s = 0; s < nsys; ++s) hp[s] = &host[s];

@@ -424,13 +424,7 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
         upload_csr(D.Pt, Pt, st);
         D.pt_pnz.upload(H.Pt.p_nz_idx, st);
         D.has_rmap = H.has_rmap;
-        if (H.has_rmap) {
-            D.ntriples = H.R.ntriples();
-            D.r_off.upload(H.R.nz_offsets, st);
-            D.r_pi.upload(H.R.pi, st);
-            D.r_ai.upload(H.R.ai, st);
-            D.r_pj.upload(H.R.pj, st);
-        }
+        if (H.has_rmap) D.rmap = device_rmap(H.R, st);
         if (is_gs(smoother_)) {
             D.ncolors = H.coloring.num_colors;
             D.color_off = H.coloring.color_offsets;
@@ -527,7 +521,8 @@ int64_t DeviceHierarchy::bytes_resident() const {
     };
     for (auto& l : L_) {
         csr(l.A); csr(l.P); csr(l.Pt);
-        b += l.pt_pnz.bytes() + l.r_off.bytes() + l.r_pi.bytes() + l.r_ai.bytes() + l.r_pj.bytes();
+        b += l.pt_pnz.bytes();
+        if (l.rmap) b += l.rmap->off.bytes() + l.rmap->pi.bytes() + l.rmap->ai.bytes() + l.rmap->pj.bytes();
         b += l.d64.bytes() + l.d32.bytes() + l.color_order.bytes();
         b += l.r64.bytes() + l.t64.bytes() + l.xc64.bytes() + l.bc64.bytes();
         b += l.r32.bytes() + l.t32.bytes() + l.xc32.bytes() + l.bc32.bytes();
@@ -588,13 +583,14 @@ void DeviceHierarchy::chebyshev_bounds(hipStream_t st) {
 
 void DeviceHierarchy::galerkin(int l, hipStream_t st) {
     DevLevel& D = L_[l];
-    if (!D.has_rmap) throw Error("partial resetup needs the restriction maps (setup with allow_partial_resetup=1)");
+    if (!D.has_rmap || !D.rmap) throw Error("partial resetup needs the restriction maps (setup with allow_partial_resetup=1)");
     DevCSR& out = (l + 1 < nlevels()) ? L_[l + 1].A : Ac_;
     const int64_t nnzc = out.nnz;
     if (nnzc == 0) return;
     const int64_t nb = (nnzc + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_galerkin, dim3((unsigned)nb), dim3(kBlock), 0, st, nnzc, D.r_off.p, D.r_pi.p, D.r_ai.p,
-                       D.r_pj.p, D.P.v64.p, D.A.v64.p, out.v64.p);
+    const DeviceRMap& R = *D.rmap;
+    hipLaunchKernelGGL(k_galerkin, dim3((unsigned)nb), dim3(kBlock), 0, st, nnzc, R.off.p, R.pi.p, R.ai.p, R.pj.p,
+                       D.P.v64.p, D.A.v64.p, out.v64.p);
 }
 
 // Coarsest operator: dense from its CSR values, then LU (src/setup.jl:177-209,

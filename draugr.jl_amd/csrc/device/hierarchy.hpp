@@ -12,6 +12,7 @@
 
 #include "../host/types.hpp"
 #include "dbuf.hpp"
+#include "galerkin.hpp"
 #include "kernels.hpp"
 
 namespace draugr {
@@ -48,9 +49,7 @@ struct DevLevel {
     DBuf<float> ilu_L32, ilu_U32;
     std::vector<int32_t> h_rp, h_cv;
     bool has_rmap = false;
-    int64_t ntriples = 0;
-    DBuf<int64_t> r_off;      // Galerkin triple map of A_{l+1}
-    DBuf<int32_t> r_pi, r_ai, r_pj;
+    std::shared_ptr<DeviceRMap> rmap;  // Galerkin triple map of A_{l+1} (shared with the host level when built on the GPU)
     DBuf<double> d64;         // smoother inverse diagonal (plain or l1)
     DBuf<float> d32;
     int ncolors = 0;

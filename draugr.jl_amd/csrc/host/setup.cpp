@@ -24,6 +24,7 @@ CSR coarsen_and_build_p(const CSR& A, const CoarsenSpec& alg, const Config& cfg,
     const CSR& As = weaken ? Aw : A;  // matrix the splitting / aggregation sees
     if (uses_cf(alg.type)) {
         std::vector<int8_t> cf;
+        const auto t0 = std::chrono::steady_clock::now();
         switch (alg.type) {
             case Coarsen::PMIS: coarsen_pmis(As, alg.theta, rng, cf, nc); break;
             case Coarsen::HMIS: coarsen_hmis(As, alg.theta, rng, cf, nc); break;
@@ -32,8 +33,15 @@ CSR coarsen_and_build_p(const CSR& A, const CoarsenSpec& alg, const Config& cfg,
             case Coarsen::AggressiveHMIS: coarsen_aggressive_cf(As, alg.theta, true, rng, cf, nc); break;
             default: break;
         }
+        const auto t1 = std::chrono::steady_clock::now();
         CSR P = interpolation_cf(A, cf, nc, alg.interp, alg.theta);
+        const auto t2 = std::chrono::steady_clock::now();
         if (alg.interp.trunc_factor > 0) P = filter_prolongation(P, alg.interp.trunc_factor);
+        if (cfg.verbose >= 3) {
+            const auto sec = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+            std::printf("    C/F splitting %.3fs, interpolation %.3fs, truncation %.3fs\n", sec(t0, t1), sec(t1, t2),
+                        sec(t2, std::chrono::steady_clock::now()));
+        }
         return P;
     }
     std::vector<int32_t> agg;
@@ -74,7 +82,7 @@ bool needs_coloring(Smoother s) { return s == Smoother::ColoredGS || s == Smooth
 
 // Level loop of src/setup.jl:123-166 and the dense coarse operator of
 // src/setup.jl:177-209.
-HostHierarchy amg_setup_host(const CSR& A_in, const Config& cfg, bool allow_partial_resetup) {
+HostHierarchy amg_setup_host(const CSR& A_in, const Config& cfg, bool allow_partial_resetup, GalerkinFn galerkin) {
     auto t0 = std::chrono::steady_clock::now();
     HostHierarchy h;
     Rng rng(cfg.seed);
@@ -90,10 +98,16 @@ HostHierarchy amg_setup_host(const CSR& A_in, const Config& cfg, bool allow_part
         HostLevel L;
         CSR Ac;
         auto tb = std::chrono::steady_clock::now();
-        coarse_sparsity(A, P, nc, Ac, allow_partial_resetup ? &L.R : nullptr);
+        auto tc = tb;
+        if (galerkin) {
+            galerkin(A, P, nc, Ac, allow_partial_resetup ? &L.R : nullptr, L.Pt);
+            tc = std::chrono::steady_clock::now();
+        } else {
+            coarse_sparsity(A, P, nc, Ac, allow_partial_resetup ? &L.R : nullptr);
+            tc = std::chrono::steady_clock::now();
+            build_transpose_map(P, L.Pt);
+        }
         L.has_rmap = allow_partial_resetup;
-        auto tc = std::chrono::steady_clock::now();
-        build_transpose_map(P, L.Pt);
         if (needs_coloring(cfg.smoother)) greedy_coloring(A, L.coloring);
         if (cfg.verbose >= 2) {
             const auto sec = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };

@@ -8,6 +8,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -77,6 +78,7 @@ struct Config {
     uint64_t seed = 0x5EEDull;              // seed of the setup RNG (PMIS/HMIS measures)
     bool graphs = true;                     // replay the preconditioner cycle from a captured hipGraph
     int kernel = -1;                        // sparse-pass kernel: -1 autotuned, 0 stream, 1 row-sequential, 2 pipelined
+    bool gpu_galerkin = true;               // coarse operators, triple maps and transpose maps built on the GPU
 };
 
 // ── hierarchy (host copy; uploaded once to HBM) ──────────────────────────────
@@ -89,7 +91,11 @@ struct TransposeMap {            // src/types.jl:394-398
 struct RestrictionMap {          // src/types.jl:413-418
     std::vector<int64_t> nz_offsets; // nnz(A_coarse) + 1
     std::vector<int32_t> pi, ai, pj; // ntriples each
-    int64_t ntriples() const { return (int64_t)pi.size(); }
+    // built on the GPU: the map stays in HBM (an opaque device object shared
+    // with the device hierarchy) and the host vectors above stay empty
+    std::shared_ptr<void> device;
+    int64_t device_ntriples = 0;
+    int64_t ntriples() const { return device ? device_ntriples : (int64_t)pi.size(); }
 };
 
 struct Coloring {                // src/types.jl:206-228
@@ -164,7 +170,11 @@ void ilu0_structure(const CSR& A, IluStructure& s);
 void ilu0_factorize(const CSR& A, const std::vector<int32_t>& diag_idx, std::vector<double>& L,
                     std::vector<double>& U);
 
-HostHierarchy amg_setup_host(const CSR& A, const Config& cfg, bool allow_partial_resetup);
+// Builder of a level's coarse operator Ac = P^T A P, its triple map (when
+// rmap is not null) and the transpose map of P.  Null = the host's
+// coarse_sparsity + build_transpose_map.
+using GalerkinFn = void (*)(const CSR& A, const CSR& P, int32_t nc, CSR& Ac, RestrictionMap* rmap, TransposeMap& T);
+HostHierarchy amg_setup_host(const CSR& A, const Config& cfg, bool allow_partial_resetup, GalerkinFn galerkin = nullptr);
 std::string hierarchy_summary(const HostHierarchy& h, const Config& cfg, int64_t n_finest);
 
 // Dense LU with partial pivoting (column-major), and its explicit inverse.

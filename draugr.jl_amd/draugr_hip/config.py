@@ -3,7 +3,7 @@
 Type tags keep the reference's names, field meaning and defaults; the
 configuration is handed to the C-ABI as the JSON document that
 ``draugr_amg_config_from_json`` accepts (src/cfunction_interface.jl:178-221),
-extended with this backend's keys (``precision``, ``seed``, ``graphs``,
+extended with this backend's keys (``precision``, ``seed``, ``graphs``, ``gpu_galerkin``,
 ``initial_theta``, ``initial_interpolation``, ``sa_omega``, ``filtering``,
 ``filter_tol``, ``aggressive_base``).
 """
@@ -171,6 +171,7 @@ class AMGConfig:
     seed: int = 0x5EED
     graphs: bool = True
     kernel: str = "auto"  # sparse-pass kernel: "auto" (autotuned), "stream", "rowseq", "pipe"
+    gpu_galerkin: bool = True  # coarse operators / triple maps of the setup built on the GPU
 
     def __post_init__(self):
         if str(self.cycle_type).upper() not in ("V", "W"):
@@ -204,6 +205,7 @@ class AMGConfig:
             "seed": self.seed,
             "graphs": self.graphs,
             "kernel": self.kernel,
+            "gpu_galerkin": self.gpu_galerkin,
         }
         if hasattr(c, "interpolation"):
             d["interpolation"] = c.interpolation.to_json()

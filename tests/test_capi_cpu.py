@@ -35,7 +35,8 @@ def test_config_json_defaults_and_errors(dh):
     assert L.draugr_amg_config_from_json(b"{}") > 0
     # unknown keys ignored, string-encoded numbers accepted, names case-insensitive
     h = L.draugr_amg_config_from_json(b'{"foo": 1, "theta": "0.3", "max_levels": "7", "smoother": "JACOBI",'
-                                      b' "cycle": "W", "coarse_solve_on_cpu": "true"}')
+                                      b' "cycle": "W", "coarse_solve_on_cpu": "true", "gpu_galerkin": true,'
+                                      b' "kernel": "pipe", "graphs": false}')
     assert h > 0
     assert L.draugr_amg_config_free(h) == 0
     assert L.draugr_amg_config_free(h) == -1
@@ -46,6 +47,7 @@ def test_config_json_defaults_and_errors(dh):
                      (b'{"cycle": "x"}', "Unknown cycle"),
                      (b'{"strength": "weird"}', "Unknown strength"),
                      (b'{"precision": "fp8"}', "Unknown precision"),
+                     (b'{"kernel": "csr"}', "Unknown kernel"),
                      (b'{"theta": "abc"}', "cannot parse"),
                      (b'{"theta": 0.5', "JSON parse error"),
                      (b'[1, 2]', "top level")]:

@@ -1,0 +1,105 @@
+"""Setup on the GPU: the coarse operators, triple maps and transpose maps
+built in HBM (config key ``gpu_galerkin``) against the host's.
+
+The host's coarse_sparsity / build_transpose_map are pinned on the CPU (the
+oracle's Galerkin product and dense P^T A P, tests/test_setup_host.py); the
+GPU build must reproduce them bit for bit: same pattern, same triples in the
+same order, same values.  Every level of every hierarchy is compared, with
+and without triple maps, then a resetup and a PCG solve on both hierarchies
+must agree to the last bit.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261204
+
+
+def _problems(dh):
+    P = dh.problems
+    m = 40
+    A3 = P.heterogeneous3d(m, P.log_permeability(m, 1.5, SEED))
+    return [
+        ("hypre_default_hetero40", A3, dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())),
+        ("pmis_direct_poisson2d_96", P.poisson2d(96),
+         dh.AMGConfig(coarsening=dh.PMISCoarsening(0.25), smoother=dh.JacobiSmootherType())),
+        ("aggregation_poisson2d_128", P.poisson2d(128),
+         dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType())),
+        ("smoothed_aggregation_hetero40", A3,
+         dh.AMGConfig(coarsening=dh.SmoothedAggregationCoarsening(0.08), smoother=dh.L1JacobiSmootherType())),
+    ]
+
+
+def _same(a, b):
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    return a.dtype == b.dtype and a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+def _compare(h_host, h_gpu, name):
+    assert h_host.nlevels == h_gpu.nlevels, name
+    for l in range(h_host.nlevels):
+        a, b = h_host.level(l), h_gpu.level(l)
+        assert sorted(a) == sorted(b), (name, l)
+        for k in a:
+            if isinstance(a[k], np.ndarray):
+                assert _same(a[k], b[k]), (name, l, k)
+            else:
+                assert a[k] == b[k], (name, l, k)
+    ca, cb = h_host.coarsest(), h_gpu.coarsest()
+    for k in ("rp", "cv", "nz", "dense"):
+        assert _same(ca[k], cb[k]), (name, "coarsest", k)
+
+
+@pytest.mark.parametrize("allow_partial", [True, False])
+def test_gpu_galerkin_matches_host(dh, torch_cuda, allow_partial):
+    for name, A, cfg in _problems(dh):
+        print("case", name, "allow_partial", allow_partial, flush=True)
+        h_host = dh.amg_setup(A, cfg, allow_partial_resetup=allow_partial)
+        h_gpu = dh.amg_setup(A, dataclasses.replace(cfg, gpu_galerkin=True), allow_partial_resetup=allow_partial)
+        assert h_host.nlevels >= 3, name
+        _compare(h_host, h_gpu, name)
+
+
+def test_gpu_galerkin_resetup_and_solve_bitwise(dh, torch_cuda):
+    torch = torch_cuda
+    P = dh.problems
+    m = 48
+    A = P.heterogeneous3d(m, P.log_permeability(m, 1.5, SEED + 1))
+    cfg = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())
+    hs = [dh.amg_setup(A, cfg), dh.amg_setup(A, dataclasses.replace(cfg, gpu_galerkin=True))]
+    A2 = P.heterogeneous3d(m, P.log_permeability(m, 1.5, SEED + 2))
+    b = torch.tensor(P.splitmix_normal(SEED + 3, A.n), device="cuda")
+    out = []
+    for h in hs:
+        dh.amg_resetup_(h, torch.tensor(A2.nzval, device="cuda"), cfg)
+        x = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+        _, it, rel = dh.pcg_solve_(x, b, h, cfg, tol=1e-8, maxiter=200)
+        out.append((it, rel, x.cpu().numpy()))
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1] and _same(out[0][2], out[1][2])
+    _compare(hs[0], hs[1], "after resetup")
+
+
+def test_gpu_galerkin_full_resetup_and_batch(dh, torch_cuda):
+    P = dh.problems
+    cfg = dh.AMGConfig(coarsening=dh.HMISCoarsening(0.5, dh.ExtendedIInterpolation(0.3)),
+                       smoother=dh.L1JacobiSmootherType())
+    cg = dataclasses.replace(cfg, gpu_galerkin=True)
+    A = P.poisson2d(80)
+    B = P.heterogeneous3d(24, P.log_permeability(24, 1.0, SEED + 4))
+    h1, h2 = dh.amg_setup(A, cfg), dh.amg_setup(A, cg)
+    dh.amg_resetup_(h1, B, cfg, partial=False)  # full resetup = setup of the new matrix
+    dh.amg_resetup_(h2, B, cg, partial=False)
+    _compare(h1, h2, "full resetup")
+    b1 = dh.amg_batch_setup([A, B], cfg, nrhs=2)
+    b2 = dh.amg_batch_setup([A, B], cg, nrhs=2)
+    import torch
+    rhs = b1.pack([np.stack([P.splitmix_normal(SEED + 5 + s + 7 * j, n) for j in range(2)], axis=1)
+                   for s, n in enumerate(b1.sizes)])
+    x1 = torch.zeros_like(rhs)
+    x2 = torch.zeros_like(rhs)
+    _, it1, rel1 = b1.pcg_solve_(x1, rhs, tol=1e-8)
+    _, it2, rel2 = b2.pcg_solve_(x2, rhs, tol=1e-8)
+    assert np.array_equal(it1, it2) and _same(rel1, rel2) and _same(x1.cpu().numpy(), x2.cpu().numpy())
