@@ -17,6 +17,7 @@
 #include "../../../include/draugr_hip.h"
 #include "batch.hpp"
 #include "galerkin.hpp"
+#include "interp.hpp"
 #include "hierarchy.hpp"
 
 namespace draugr {
@@ -129,6 +130,13 @@ void check_n(Hier& H, int32_t n) {
                     std::to_string(H.dev->n_fine()));
 }
 
+// Setup phases run on the GPU (config gpu_setup): interpolation and the
+// Galerkin product; null = everything on the host.
+const SetupHooks* gpu_hooks(const Config& cfg) {
+    static const SetupHooks hooks{galerkin_device, interpolation_device};
+    return cfg.gpu_setup ? &hooks : nullptr;
+}
+
 int32_t setup_impl(int32_t n, int32_t nnz, const int32_t* rp, const int32_t* cv, const double* nz, int32_t ch,
                    int32_t base, int32_t allow, bool upload) {
     Config cfg;
@@ -137,7 +145,7 @@ int32_t setup_impl(int32_t n, int32_t nnz, const int32_t* rp, const int32_t* cv,
     auto H = std::make_shared<Hier>();
     H->cfg = cfg;
     H->allow_partial = allow != 0;
-    H->host = amg_setup_host(A, cfg, H->allow_partial, upload && cfg.gpu_galerkin ? galerkin_device : nullptr);
+    H->host = amg_setup_host(A, cfg, H->allow_partial, upload ? gpu_hooks(cfg) : nullptr);
     if (upload) H->dev = std::make_unique<DeviceHierarchy>(H->host, cfg);
     if (cfg.verbose >= 1) std::printf("%s", hierarchy_summary(H->host, cfg, n).c_str());
     std::lock_guard<std::mutex> g(g_lock);
@@ -237,7 +245,7 @@ int32_t draugr_amg_resetup(int32_t handle, int32_t n, int32_t nnz, const int32_t
             H->cfg = cfg;
             H->allow_partial = allow_partial_resetup != 0;
             H->dev.reset();
-            H->host = amg_setup_host(A, cfg, H->allow_partial, cfg.gpu_galerkin ? galerkin_device : nullptr);
+            H->host = amg_setup_host(A, cfg, H->allow_partial, gpu_hooks(cfg));
             H->dev = std::make_unique<DeviceHierarchy>(H->host, cfg);
             if (cfg.verbose >= 1) std::printf("%s", hierarchy_summary(H->host, cfg, n).c_str());
         }
@@ -593,7 +601,7 @@ int32_t draugr_hip_batch_setup(int32_t nsys, const int32_t* n, const int32_t* nn
         std::vector<HostHierarchy> host(nsys);
         for (int32_t s = 0; s < nsys; ++s) {
             CSR A = csr_from_raw(n[s], nnz[s], rowptr[s], colval[s], nzval[s], index_base);
-            host[s] = amg_setup_host(A, cfg, false, cfg.gpu_galerkin ? galerkin_device : nullptr);
+            host[s] = amg_setup_host(A, cfg, false, gpu_hooks(cfg));
         }
         std::vector<const HostHierarchy*> hp(nsys);
         for (int32_t s = 0; s < nsys; ++s) hp[s] = &host[s];

@@ -168,116 +168,132 @@ CSR interp_standard(const CSR& A, const std::vector<int8_t>& cf, const std::vect
     return rb.P;
 }
 
-// Extended+i interpolation with in-place truncation; src/prolongation.jl:467-690.
+}  // namespace
+
+// Extended+i interpolation of one F row i (src/prolongation.jl:467-690): the
+// row's entries (coarse column, weight) in the reference's keep order (the
+// caller sorts them by column).  w.pmark / w.fmark are n-long work arrays
+// holding -1 where unused; pmark is restored on return.
+void ext_i_row(const CSR& A, const std::vector<int8_t>& cf, const std::vector<int32_t>& cmap,
+               const std::vector<uint8_t>& s, const InterpSpec& ip, int32_t i, ExtIWork& w,
+               std::vector<std::pair<int32_t, double>>& out) {
+    std::vector<int32_t>& pmark = w.pmark;
+    std::vector<int32_t>& fmark = w.fmark;
+    std::vector<int32_t>& chat = w.chat;
+    std::vector<double>& pdata = w.pdata;
+    std::vector<int32_t>& keep = w.keep;
+    out.clear();
+    chat.clear();
+    // phase 1: C-hat = strong C neighbours and strong C neighbours of strong F neighbours
+    for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+        const int32_t j = A.cv[k];
+        if (!(j != i && s[k])) continue;
+        if (cf[j] == 1) {
+            if (pmark[j] < 0) { pmark[j] = (int32_t)chat.size(); chat.push_back(j); }
+        } else if (cf[j] == -1) {
+            fmark[j] = i;
+            for (int32_t k2 = A.rp[j]; k2 < A.rp[j + 1]; ++k2) {
+                const int32_t m = A.cv[k2];
+                if (m != j && s[k2] && cf[m] == 1 && pmark[m] < 0) {
+                    pmark[m] = (int32_t)chat.size();
+                    chat.push_back(m);
+                }
+            }
+        }
+    }
+    const int32_t nch = (int32_t)chat.size();
+    if (nch == 0) {
+        out.emplace_back(nearest_coarse(A, i, cf, cmap), 1.0);
+        return;
+    }
+    // phase 2: weights
+    pdata.assign(nch, 0.0);
+    double diag = 0.0;
+    for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+        const int32_t j = A.cv[k];
+        const double aij = A.nz[k];
+        if (j == i) { diag += aij; continue; }
+        if (pmark[j] >= 0) {
+            pdata[pmark[j]] += aij;
+        } else if (fmark[j] == i) {
+            const double dj = diag_first(A, j);
+            const double sg = dj < 0 ? -1.0 : 1.0;
+            double sum = 0.0;
+            for (int32_t t = A.rp[j]; t < A.rp[j + 1]; ++t) {
+                const int32_t m = A.cv[t];
+                if (m == j) continue;
+                if (sg * A.nz[t] < 0 && (pmark[m] >= 0 || m == i)) sum += A.nz[t];
+            }
+            if (std::fabs(sum) > kEps) {
+                const double dist = aij / sum;
+                for (int32_t t = A.rp[j]; t < A.rp[j + 1]; ++t) {
+                    const int32_t m = A.cv[t];
+                    if (m == j) continue;
+                    const double ajm = A.nz[t];
+                    if (sg * ajm < 0) {
+                        if (pmark[m] >= 0) pdata[pmark[m]] += dist * ajm;
+                        else if (m == i) diag += dist * ajm;
+                    }
+                }
+            } else {
+                diag += aij;
+            }
+        } else {
+            diag += aij;
+        }
+    }
+    // phase 3: normalise
+    if (std::fabs(diag) > kEps)
+        for (auto& v : pdata) v /= -diag;
+    // phase 4: truncation by factor, then by element count
+    keep.resize(nch);
+    std::iota(keep.begin(), keep.end(), 0);
+    auto mag = [&](int32_t idx) { return std::pow(std::fabs(pdata[idx]), (double)ip.norm_p); };
+    if (ip.trunc_factor > 0) {
+        double mx = 0.0;
+        for (int32_t idx = 0; idx < nch; ++idx) mx = std::max(mx, mag(idx));
+        const double thr = ip.trunc_factor * mx;
+        keep.clear();
+        for (int32_t idx = 0; idx < nch; ++idx)
+            if (mag(idx) >= thr) keep.push_back(idx);
+    }
+    if (ip.max_elements > 0 && (int32_t)keep.size() > ip.max_elements) {
+        std::stable_sort(keep.begin(), keep.end(), [&](int32_t a, int32_t b) {
+            return std::fabs(pdata[a]) > std::fabs(pdata[b]);
+        });
+        keep.resize(ip.max_elements);
+    }
+    double scale = 1.0;
+    if (ip.rescale && (int32_t)keep.size() < nch) {
+        std::vector<uint8_t> kept(nch, 0);
+        for (auto idx : keep) kept[idx] = 1;
+        double removed = 0.0;
+        for (int32_t idx = 0; idx < nch; ++idx)
+            if (!kept[idx]) removed += pdata[idx];
+        const double den = 1.0 - removed;
+        if (std::fabs(den) > 1e-12) scale = 1.0 / den;
+    }
+    for (auto idx : keep) out.emplace_back(cmap[chat[idx]], pdata[idx] * scale);
+    for (auto c : chat) pmark[c] = -1;
+}
+
+namespace {
+
 CSR interp_ext_i(const CSR& A, const std::vector<int8_t>& cf, const std::vector<int32_t>& cmap,
                  int32_t nc, const std::vector<uint8_t>& s, const InterpSpec& ip) {
     const int32_t n = A.nrow;
     RowBuilder rb(n, nc);
-    std::vector<int32_t> pmark(n, -1);   // position in C-hat, or -1
-    std::vector<int32_t> fmark(n, -1);   // row id for which node is a strong F neighbour
-    std::vector<int32_t> chat;
-    std::vector<double> pdata;
-    std::vector<int32_t> keep;
+    ExtIWork w(n);
+    std::vector<std::pair<int32_t, double>> out;
     for (int32_t i = 0; i < n; ++i) {
         if (cf[i] == 1) {
             rb.add(cmap[i], 1.0);
             rb.end_row();
             continue;
         }
-        chat.clear();
-        // phase 1: C-hat = strong C neighbours and strong C neighbours of strong F neighbours
-        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-            const int32_t j = A.cv[k];
-            if (!(j != i && s[k])) continue;
-            if (cf[j] == 1) {
-                if (pmark[j] < 0) { pmark[j] = (int32_t)chat.size(); chat.push_back(j); }
-            } else if (cf[j] == -1) {
-                fmark[j] = i;
-                for (int32_t k2 = A.rp[j]; k2 < A.rp[j + 1]; ++k2) {
-                    const int32_t m = A.cv[k2];
-                    if (m != j && s[k2] && cf[m] == 1 && pmark[m] < 0) {
-                        pmark[m] = (int32_t)chat.size();
-                        chat.push_back(m);
-                    }
-                }
-            }
-        }
-        const int32_t nch = (int32_t)chat.size();
-        if (nch == 0) {
-            rb.add(nearest_coarse(A, i, cf, cmap), 1.0);
-            rb.end_row();
-            continue;
-        }
-        // phase 2: weights
-        pdata.assign(nch, 0.0);
-        double diag = 0.0;
-        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-            const int32_t j = A.cv[k];
-            const double aij = A.nz[k];
-            if (j == i) { diag += aij; continue; }
-            if (pmark[j] >= 0) {
-                pdata[pmark[j]] += aij;
-            } else if (fmark[j] == i) {
-                const double dj = diag_first(A, j);
-                const double sg = dj < 0 ? -1.0 : 1.0;
-                double sum = 0.0;
-                for (int32_t t = A.rp[j]; t < A.rp[j + 1]; ++t) {
-                    const int32_t m = A.cv[t];
-                    if (m == j) continue;
-                    if (sg * A.nz[t] < 0 && (pmark[m] >= 0 || m == i)) sum += A.nz[t];
-                }
-                if (std::fabs(sum) > kEps) {
-                    const double dist = aij / sum;
-                    for (int32_t t = A.rp[j]; t < A.rp[j + 1]; ++t) {
-                        const int32_t m = A.cv[t];
-                        if (m == j) continue;
-                        const double ajm = A.nz[t];
-                        if (sg * ajm < 0) {
-                            if (pmark[m] >= 0) pdata[pmark[m]] += dist * ajm;
-                            else if (m == i) diag += dist * ajm;
-                        }
-                    }
-                } else {
-                    diag += aij;
-                }
-            } else {
-                diag += aij;
-            }
-        }
-        // phase 3: normalise
-        if (std::fabs(diag) > kEps)
-            for (auto& v : pdata) v /= -diag;
-        // phase 4: truncation by factor, then by element count
-        keep.resize(nch);
-        std::iota(keep.begin(), keep.end(), 0);
-        auto mag = [&](int32_t idx) { return std::pow(std::fabs(pdata[idx]), (double)ip.norm_p); };
-        if (ip.trunc_factor > 0) {
-            double mx = 0.0;
-            for (int32_t idx = 0; idx < nch; ++idx) mx = std::max(mx, mag(idx));
-            const double thr = ip.trunc_factor * mx;
-            keep.clear();
-            for (int32_t idx = 0; idx < nch; ++idx)
-                if (mag(idx) >= thr) keep.push_back(idx);
-        }
-        if (ip.max_elements > 0 && (int32_t)keep.size() > ip.max_elements) {
-            std::stable_sort(keep.begin(), keep.end(), [&](int32_t a, int32_t b) {
-                return std::fabs(pdata[a]) > std::fabs(pdata[b]);
-            });
-            keep.resize(ip.max_elements);
-        }
-        double scale = 1.0;
-        if (ip.rescale && (int32_t)keep.size() < nch) {
-            std::vector<uint8_t> kept(nch, 0);
-            for (auto idx : keep) kept[idx] = 1;
-            double removed = 0.0;
-            for (int32_t idx = 0; idx < nch; ++idx)
-                if (!kept[idx]) removed += pdata[idx];
-            const double den = 1.0 - removed;
-            if (std::fabs(den) > 1e-12) scale = 1.0 / den;
-        }
-        for (auto idx : keep) rb.add(cmap[chat[idx]], pdata[idx] * scale);
+        ext_i_row(A, cf, cmap, s, ip, i, w, out);
+        for (auto& e : out) rb.add(e.first, e.second);
         rb.end_row();
-        for (auto c : chat) pmark[c] = -1;
     }
     return rb.P;
 }

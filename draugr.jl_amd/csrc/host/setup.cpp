@@ -17,7 +17,8 @@ bool uses_cf(Coarsen c) {
 }
 
 // Coarsen once and build P; src/setup.jl:256-327.
-CSR coarsen_and_build_p(const CSR& A, const CoarsenSpec& alg, const Config& cfg, Rng& rng, int32_t& nc) {
+CSR coarsen_and_build_p(const CSR& A, const CoarsenSpec& alg, const Config& cfg, Rng& rng, int32_t& nc,
+                        const SetupHooks& hooks) {
     const bool weaken = cfg.max_row_sum < 1.0;
     CSR Aw;
     if (weaken) Aw = apply_max_row_sum(A, cfg.max_row_sum);
@@ -34,9 +35,12 @@ CSR coarsen_and_build_p(const CSR& A, const CoarsenSpec& alg, const Config& cfg,
             default: break;
         }
         const auto t1 = std::chrono::steady_clock::now();
-        CSR P = interpolation_cf(A, cf, nc, alg.interp, alg.theta);
+        const bool truncate = alg.interp.trunc_factor > 0;
+        CSR P;
+        if (hooks.interp) P = hooks.interp(A, cf, nc, alg.interp, alg.theta, truncate, cfg.gpu_setup_min_rows);
+        else P = interpolation_cf(A, cf, nc, alg.interp, alg.theta);
         const auto t2 = std::chrono::steady_clock::now();
-        if (alg.interp.trunc_factor > 0) P = filter_prolongation(P, alg.interp.trunc_factor);
+        if (truncate && !hooks.interp) P = filter_prolongation(P, alg.interp.trunc_factor);
         if (cfg.verbose >= 3) {
             const auto sec = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
             std::printf("    C/F splitting %.3fs, interpolation %.3fs, truncation %.3fs\n", sec(t0, t1), sec(t1, t2),
@@ -58,15 +62,16 @@ CSR coarsen_and_build_p(const CSR& A, const CoarsenSpec& alg, const Config& cfg,
 
 // Retry with theta halved (up to 3 times) when n_coarse >= 0.8 n;
 // src/setup.jl:219-237.
-CSR coarsen_with_fallback(const CSR& A, const CoarsenSpec& alg, const Config& cfg, Rng& rng, int32_t& nc) {
+CSR coarsen_with_fallback(const CSR& A, const CoarsenSpec& alg, const Config& cfg, Rng& rng, int32_t& nc,
+                          const SetupHooks& hooks) {
     const int32_t n = A.nrow;
-    CSR P = coarsen_and_build_p(A, alg, cfg, rng, nc);
+    CSR P = coarsen_and_build_p(A, alg, cfg, rng, nc, hooks);
     if (nc < 0.8 * n || n <= cfg.max_coarse_size) return P;
     for (int attempt = 1; attempt <= 3; ++attempt) {
         CoarsenSpec red = alg;
         red.theta = alg.theta * std::pow(0.5, attempt);
         int32_t nc2 = 0;
-        CSR P2 = coarsen_and_build_p(A, red, cfg, rng, nc2);
+        CSR P2 = coarsen_and_build_p(A, red, cfg, rng, nc2, hooks);
         if (nc2 < nc) {
             P = std::move(P2);
             nc = nc2;
@@ -82,7 +87,8 @@ bool needs_coloring(Smoother s) { return s == Smoother::ColoredGS || s == Smooth
 
 // Level loop of src/setup.jl:123-166 and the dense coarse operator of
 // src/setup.jl:177-209.
-HostHierarchy amg_setup_host(const CSR& A_in, const Config& cfg, bool allow_partial_resetup, GalerkinFn galerkin) {
+HostHierarchy amg_setup_host(const CSR& A_in, const Config& cfg, bool allow_partial_resetup, const SetupHooks* hooks_in) {
+    const SetupHooks hooks = hooks_in ? *hooks_in : SetupHooks();
     auto t0 = std::chrono::steady_clock::now();
     HostHierarchy h;
     Rng rng(cfg.seed);
@@ -93,14 +99,14 @@ HostHierarchy amg_setup_host(const CSR& A_in, const Config& cfg, bool allow_part
         const CoarsenSpec& alg = lvl <= cfg.initial_coarsening_levels ? cfg.initial_coarsening : cfg.coarsening;
         int32_t nc = 0;
         auto ta = std::chrono::steady_clock::now();
-        CSR P = coarsen_with_fallback(A, alg, cfg, rng, nc);
+        CSR P = coarsen_with_fallback(A, alg, cfg, rng, nc, hooks);
         if (nc >= n || nc == 0) break;
         HostLevel L;
         CSR Ac;
         auto tb = std::chrono::steady_clock::now();
         auto tc = tb;
-        if (galerkin) {
-            galerkin(A, P, nc, Ac, allow_partial_resetup ? &L.R : nullptr, L.Pt);
+        if (hooks.galerkin) {
+            hooks.galerkin(A, P, nc, Ac, allow_partial_resetup ? &L.R : nullptr, L.Pt);
             tc = std::chrono::steady_clock::now();
         } else {
             coarse_sparsity(A, P, nc, Ac, allow_partial_resetup ? &L.R : nullptr);

@@ -11,6 +11,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace draugr {
@@ -78,7 +79,8 @@ struct Config {
     uint64_t seed = 0x5EEDull;              // seed of the setup RNG (PMIS/HMIS measures)
     bool graphs = true;                     // replay the preconditioner cycle from a captured hipGraph
     int kernel = -1;                        // sparse-pass kernel: -1 autotuned, 0 stream, 1 row-sequential, 2 pipelined
-    bool gpu_galerkin = true;               // coarse operators, triple maps and transpose maps built on the GPU
+    bool gpu_setup = true;                  // interpolation, coarse operators and triple maps built on the GPU
+    int gpu_setup_min_rows = 32768;         // levels below this size interpolate on the host (faster there)
 };
 
 // ── hierarchy (host copy; uploaded once to HBM) ──────────────────────────────
@@ -151,6 +153,15 @@ void coarsen_aggressive_cf(const CSR& A, double theta, bool hmis_base, Rng& rng,
 CSR prolongation_from_aggregates(const std::vector<int32_t>& agg, int32_t nc);
 CSR smooth_prolongation(const CSR& A, const CSR& Ptent, double omega);
 CSR filter_prolongation(const CSR& P, double tol);
+// Work arrays of ext_i_row (pmark / fmark: n entries, -1 where unused).
+struct ExtIWork {
+    std::vector<int32_t> pmark, fmark, chat, keep;
+    std::vector<double> pdata;
+    explicit ExtIWork(int32_t n) : pmark(n, -1), fmark(n, -1) {}
+};
+void ext_i_row(const CSR& A, const std::vector<int8_t>& cf, const std::vector<int32_t>& cmap,
+               const std::vector<uint8_t>& s, const InterpSpec& ip, int32_t i, ExtIWork& w,
+               std::vector<std::pair<int32_t, double>>& out);
 CSR interpolation_cf(const CSR& A, const std::vector<int8_t>& cf, int32_t nc,
                      const InterpSpec& ip, double theta);
 
@@ -174,7 +185,17 @@ void ilu0_factorize(const CSR& A, const std::vector<int32_t>& diag_idx, std::vec
 // rmap is not null) and the transpose map of P.  Null = the host's
 // coarse_sparsity + build_transpose_map.
 using GalerkinFn = void (*)(const CSR& A, const CSR& P, int32_t nc, CSR& Ac, RestrictionMap* rmap, TransposeMap& T);
-HostHierarchy amg_setup_host(const CSR& A, const Config& cfg, bool allow_partial_resetup, GalerkinFn galerkin = nullptr);
+// Builder of P from a C/F splitting, followed by filter_prolongation(P,
+// ip.trunc_factor) when truncate is set; levels of fewer than min_rows rows
+// may use the host's routines.  Null = the host's routines.
+using InterpFn = CSR (*)(const CSR& A, const std::vector<int8_t>& cf, int32_t nc, const InterpSpec& ip, double theta,
+                         bool truncate, int32_t min_rows);
+struct SetupHooks {
+    GalerkinFn galerkin = nullptr;
+    InterpFn interp = nullptr;
+};
+HostHierarchy amg_setup_host(const CSR& A, const Config& cfg, bool allow_partial_resetup,
+                             const SetupHooks* hooks = nullptr);
 std::string hierarchy_summary(const HostHierarchy& h, const Config& cfg, int64_t n_finest);
 
 // Dense LU with partial pivoting (column-major), and its explicit inverse.

@@ -3,7 +3,7 @@
 Type tags keep the reference's names, field meaning and defaults; the
 configuration is handed to the C-ABI as the JSON document that
 ``draugr_amg_config_from_json`` accepts (src/cfunction_interface.jl:178-221),
-extended with this backend's keys (``precision``, ``seed``, ``graphs``, ``gpu_galerkin``,
+extended with this backend's keys (``precision``, ``seed``, ``graphs``, ``gpu_setup``,
 ``initial_theta``, ``initial_interpolation``, ``sa_omega``, ``filtering``,
 ``filter_tol``, ``aggressive_base``).
 """
@@ -171,7 +171,8 @@ class AMGConfig:
     seed: int = 0x5EED
     graphs: bool = True
     kernel: str = "auto"  # sparse-pass kernel: "auto" (autotuned), "stream", "rowseq", "pipe"
-    gpu_galerkin: bool = True  # coarse operators / triple maps of the setup built on the GPU
+    gpu_setup: bool = True  # interpolation, coarse operators and triple maps of the setup built on the GPU
+    gpu_setup_min_rows: int = 32768  # smaller levels interpolate on the host
 
     def __post_init__(self):
         if str(self.cycle_type).upper() not in ("V", "W"):
@@ -205,7 +206,8 @@ class AMGConfig:
             "seed": self.seed,
             "graphs": self.graphs,
             "kernel": self.kernel,
-            "gpu_galerkin": self.gpu_galerkin,
+            "gpu_setup": self.gpu_setup,
+            "gpu_setup_min_rows": self.gpu_setup_min_rows,
         }
         if hasattr(c, "interpolation"):
             d["interpolation"] = c.interpolation.to_json()

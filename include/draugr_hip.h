@@ -39,10 +39,12 @@ const char *draugr_amg_last_error(void);
  * "precision" ("fp64" | "fp32" | "bf16": arithmetic of the V/W cycle),
  * "seed" (setup RNG seed), "graphs" (true: the PCG / mixed-precision preconditioner cycle is
  * captured once into a hipGraph and replayed; false: launched eagerly),
- * "kernel" ("auto" | "stream" | "rowseq" | "pipe"), "gpu_galerkin" (default
- * true: when the hierarchy is uploaded, each level's coarse operator, triple
- * map and transpose map are built on the GPU, identical to the host's, and the
- * triple map stays in HBM), "filtering", "filter_tol".  Returns a handle > 0, or -1.
+ * "kernel" ("auto" | "stream" | "rowseq" | "pipe"), "gpu_setup" (default
+ * true: when the hierarchy is uploaded, each level's extended+i interpolation,
+ * coarse operator, triple map and transpose map are built on the GPU,
+ * identical to the host's, and the triple map stays in HBM; levels of fewer
+ * than "gpu_setup_min_rows" rows (default 32768) interpolate on the host),
+ * "filtering", "filter_tol".  Returns a handle > 0, or -1.
  * Replaces c_api/draugr_amg.h:94. */
 int32_t draugr_amg_config_from_json(const char *json_str);
 

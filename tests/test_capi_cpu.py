@@ -35,7 +35,7 @@ def test_config_json_defaults_and_errors(dh):
     assert L.draugr_amg_config_from_json(b"{}") > 0
     # unknown keys ignored, string-encoded numbers accepted, names case-insensitive
     h = L.draugr_amg_config_from_json(b'{"foo": 1, "theta": "0.3", "max_levels": "7", "smoother": "JACOBI",'
-                                      b' "cycle": "W", "coarse_solve_on_cpu": "true", "gpu_galerkin": true,'
+                                      b' "cycle": "W", "coarse_solve_on_cpu": "true", "gpu_setup": true,'
                                       b' "kernel": "pipe", "graphs": false}')
     assert h > 0
     assert L.draugr_amg_config_free(h) == 0

@@ -1,28 +1,58 @@
-"""Setup on the GPU: the coarse operators, triple maps and transpose maps
-built in HBM (config key ``gpu_galerkin``) against the host's.
+"""Setup on the GPU (config key ``gpu_setup``): extended+i interpolation,
+coarse operators, triple maps and transpose maps built in HBM, against the
+host's.
 
-The host's coarse_sparsity / build_transpose_map are pinned on the CPU (the
-oracle's Galerkin product and dense P^T A P, tests/test_setup_host.py); the
-GPU build must reproduce them bit for bit: same pattern, same triples in the
-same order, same values.  Every level of every hierarchy is compared, with
-and without triple maps, then a resetup and a PCG solve on both hierarchies
-must agree to the last bit.
+The host's routines are pinned on the CPU (tests/test_setup_host.py: the
+Python restatement of the setup, the oracle's Galerkin product, dense
+P^T A P); the GPU build must reproduce them bit for bit: same P, same
+pattern, same triples in the same order, same values.  Every level of every
+hierarchy is compared, with and without triple maps, then a resetup and a PCG
+solve on both hierarchies must agree to the last bit.  The "hub" matrix (a
+2-D Poisson grid plus one node weakly coupled to every grid node) gives a
+row whose interpolation set holds hundreds of C points: more than the
+device's per-row list, so that row takes the host routine inside the GPU
+build.
 """
 import dataclasses
 
 import numpy as np
 import pytest
 
-pytestmark = pytest.mark.gpu
-
 SEED = 20261204
+
+
+def hub_matrix(dh, m, w=0.2):
+    import scipy.sparse as sp
+    n = m * m
+    B = sp.lil_matrix((n + 1, n + 1))
+    B[:n, :n] = dh.problems.poisson2d(m).to_scipy() + w * sp.identity(n)
+    B[n, :n] = -w
+    B[:n, n] = -w
+    B[n, n] = w * n + 0.5
+    B = B.tocsr()
+    B.sort_indices()
+    return B
 
 
 def _problems(dh):
     P = dh.problems
     m = 40
     A3 = P.heterogeneous3d(m, P.log_permeability(m, 1.5, SEED))
+    H = hub_matrix(dh, 24)
+    EI = dh.ExtendedIInterpolation
     return [
+        ("hub_pmis_exti", H, dh.AMGConfig(coarsening=dh.PMISCoarsening(0.25, EI(0.0)),
+                                          smoother=dh.JacobiSmootherType())),
+        ("hub_hmis_exti_trunc_max_rescale", H,
+         dh.AMGConfig(coarsening=dh.HMISCoarsening(0.25, EI(0.3, max_elements=4, rescale=True)),
+                      smoother=dh.JacobiSmootherType())),
+        ("hub_aggressive_pmis", H, dh.AMGConfig(coarsening=dh.AggressiveCoarsening(0.25, "pmis", EI(0.2)),
+                                                smoother=dh.JacobiSmootherType())),
+        ("hetero40_exti_max3_rescale", A3,
+         dh.AMGConfig(coarsening=dh.HMISCoarsening(0.5, EI(0.1, max_elements=3, rescale=True)),
+                      smoother=dh.L1JacobiSmootherType())),
+        ("hetero40_exti_norm2_host", A3,
+         dh.AMGConfig(coarsening=dh.HMISCoarsening(0.5, EI(0.3, norm_p=2)), smoother=dh.L1JacobiSmootherType())),
         ("hypre_default_hetero40", A3, dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())),
         ("pmis_direct_poisson2d_96", P.poisson2d(96),
          dh.AMGConfig(coarsening=dh.PMISCoarsening(0.25), smoother=dh.JacobiSmootherType())),
@@ -31,6 +61,15 @@ def _problems(dh):
         ("smoothed_aggregation_hetero40", A3,
          dh.AMGConfig(coarsening=dh.SmoothedAggregationCoarsening(0.08), smoother=dh.L1JacobiSmootherType())),
     ]
+
+
+def _host(cfg):
+    return dataclasses.replace(cfg, gpu_setup=False)
+
+
+def _gpu(cfg):
+    # every level on the device, down to the smallest
+    return dataclasses.replace(cfg, gpu_setup=True, gpu_setup_min_rows=0)
 
 
 def _same(a, b):
@@ -53,23 +92,34 @@ def _compare(h_host, h_gpu, name):
         assert _same(ca[k], cb[k]), (name, "coarsest", k)
 
 
+def test_hub_row_exceeds_device_list(dh):
+    """Host-only: the hub's interpolation set (P row without truncation) holds
+    more C points than the device's per-row list (64)."""
+    cfg = dh.AMGConfig(coarsening=dh.PMISCoarsening(0.25, dh.ExtendedIInterpolation(0.0)),
+                       smoother=dh.JacobiSmootherType())
+    h = dh.amg_setup(hub_matrix(dh, 24), cfg, upload=False)
+    assert int(np.diff(h.level(0, with_rmap=False)["P_rp"]).max()) > 64
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("allow_partial", [True, False])
-def test_gpu_galerkin_matches_host(dh, torch_cuda, allow_partial):
+def test_gpu_setup_matches_host(dh, torch_cuda, allow_partial):
     for name, A, cfg in _problems(dh):
         print("case", name, "allow_partial", allow_partial, flush=True)
-        h_host = dh.amg_setup(A, cfg, allow_partial_resetup=allow_partial)
-        h_gpu = dh.amg_setup(A, dataclasses.replace(cfg, gpu_galerkin=True), allow_partial_resetup=allow_partial)
+        h_host = dh.amg_setup(A, _host(cfg), allow_partial_resetup=allow_partial)
+        h_gpu = dh.amg_setup(A, _gpu(cfg), allow_partial_resetup=allow_partial)
         assert h_host.nlevels >= 3, name
         _compare(h_host, h_gpu, name)
 
 
-def test_gpu_galerkin_resetup_and_solve_bitwise(dh, torch_cuda):
+@pytest.mark.gpu
+def test_gpu_setup_resetup_and_solve_bitwise(dh, torch_cuda):
     torch = torch_cuda
     P = dh.problems
     m = 48
     A = P.heterogeneous3d(m, P.log_permeability(m, 1.5, SEED + 1))
     cfg = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())
-    hs = [dh.amg_setup(A, cfg), dh.amg_setup(A, dataclasses.replace(cfg, gpu_galerkin=True))]
+    hs = [dh.amg_setup(A, _host(cfg)), dh.amg_setup(A, _gpu(cfg))]
     A2 = P.heterogeneous3d(m, P.log_permeability(m, 1.5, SEED + 2))
     b = torch.tensor(P.splitmix_normal(SEED + 3, A.n), device="cuda")
     out = []
@@ -82,11 +132,12 @@ def test_gpu_galerkin_resetup_and_solve_bitwise(dh, torch_cuda):
     _compare(hs[0], hs[1], "after resetup")
 
 
-def test_gpu_galerkin_full_resetup_and_batch(dh, torch_cuda):
+@pytest.mark.gpu
+def test_gpu_setup_full_resetup_and_batch(dh, torch_cuda):
     P = dh.problems
     cfg = dh.AMGConfig(coarsening=dh.HMISCoarsening(0.5, dh.ExtendedIInterpolation(0.3)),
                        smoother=dh.L1JacobiSmootherType())
-    cg = dataclasses.replace(cfg, gpu_galerkin=True)
+    cfg, cg = _host(cfg), _gpu(cfg)
     A = P.poisson2d(80)
     B = P.heterogeneous3d(24, P.log_permeability(24, 1.0, SEED + 4))
     h1, h2 = dh.amg_setup(A, cfg), dh.amg_setup(A, cg)
