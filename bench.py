@@ -73,16 +73,16 @@ def dominant_kernel(torch, h, prec, reps):
     (the densest operator of the hierarchy), in the cycle's precision, timed
     with events on torch's current stream, which is the stream the library
     launches it on.  Returns (seconds per launch, algorithmic bytes, tag)."""
-    li = h.level_info(1)
-    n, nnz = li["n"], li["nnz"]
+    n = h.level_info(1)["n"]
     xa = torch.randn(n, dtype=torch.float32, device="cuda")
     ba = torch.randn(n, dtype=torch.float32, device="cuda")
     oa = torch.empty(n, dtype=torch.float32, device="cuda")
     # back-to-back launches between two events: the per-launch time a cycle sees
     # (an event pair around each single launch adds the events' own overhead)
     t = event_time(torch, lambda: h.level_op(1, 2, prec, xa, ba, oa), reps)
-    sv = 4 if prec == "fp32" else 2
-    nbytes = nnz * (sv + 4) + (n + 1) * 4 + 4 * n * 4  # A_1 once; x, b, dinv read, x_out written
+    # A_1 once (values, column indices in the form the tuned kernel reads:
+    # 16-bit block-relative where chosen, row pointers); x, b, dinv read, x_out written
+    nbytes = int(h.op_bytes(1, 0, prec)) + 4 * n * 4
     return t, nbytes, "level1_jacobi_%s_grid%d" % (prec, h_blocks(h, 1))
 
 

@@ -413,7 +413,8 @@ int32_t draugr_hip_level_info(int32_t handle, int32_t level, int64_t* out, int32
         if (level < 0 || level >= (int32_t)H->host.levels.size()) throw Error("level index out of range");
         const HostLevel& L = H->host.levels[level];
         const int64_t v[] = {L.A.nrow, L.A.nnz(), L.P.ncol, L.P.nnz(), L.R.ntriples(), L.coloring.num_colors,
-                             L.has_rmap ? 1 : 0, H->dev ? H->dev->level_blocks(level) : 0};
+                             L.has_rmap ? 1 : 0, H->dev ? H->dev->level_blocks(level) : 0,
+                             H->dev && H->dev->level_has16(level) ? 1 : 0};
         const int32_t m = std::min<int32_t>(nout, (int32_t)(sizeof v / sizeof v[0]));
         for (int32_t i = 0; i < m; ++i) out[i] = v[i];
         return m;
@@ -516,6 +517,18 @@ int32_t draugr_hip_cycle_bytes(int32_t handle, int32_t config_handle, double* by
         if (!H || !get_cfg(config_handle, cfg)) invalid(handle, config_handle);
         require_device(*H);
         *bytes = H->dev->cycle_bytes(opts_of(cfg));
+        return 0;
+    });
+}
+
+int32_t draugr_hip_op_bytes(int32_t handle, int32_t level, int32_t which, int32_t precision, double* bytes) {
+    return guarded([&]() -> int32_t {
+        auto H = get_hier(handle);
+        if (!H) throw Error("Invalid hierarchy handle: " + std::to_string(handle));
+        require_device(*H);
+        if (which < 0 || which > 2 || precision < 0 || precision > 2 || !bytes)
+            throw Error("ArgumentError: which in 0..2, precision in 0..2, bytes non-null");
+        *bytes = H->dev->op_bytes(level, which, (Precision)precision);
         return 0;
     });
 }

@@ -85,16 +85,17 @@ void launch_reduce(int64_t n, F f, FIN fin, double* partial, hipStream_t st) {
 
 // CSR-stream pass whose per-row epilogue also returns a contribution to a
 // global sum (q = A p fused with p.q in PCG).
-template <int PER, typename TV, typename TX, class XF, class EPI>
+template <int PER, bool C16, typename TV, typename TX, class XF, class EPI>
 __global__ __launch_bounds__(kBlock) void k_stream_reduce(const int* __restrict__ blk, const int* __restrict__ bz,
                                                           const int* __restrict__ rp, const int* __restrict__ cv,
-                                                          const TV* __restrict__ av, XF xf, EPI epi,
-                                                          double* partial) {
+                                                          const uint16_t* __restrict__ c16,
+                                                          const int* __restrict__ cbase, const TV* __restrict__ av,
+                                                          XF xf, EPI epi, double* partial) {
     __shared__ TX prod[kStreamNnz];
     __shared__ double sh[kBlock];
     double dot = 0.0;
-    stream_block<PER, TV, TX>(
-        xcd_block(), blk, bz, rp, cv, av, xf, prod, [](int) { return NoPre{}; },
+    stream_block<PER, C16, TV, TX>(
+        xcd_block(), blk, bz, rp, cv, c16, cbase, av, xf, prod, [](int) { return NoPre{}; },
         [&](int row, const TX* seg, int len, NoPre) {
             TX a = 0;
             for (int k = 0; k < len; ++k) a += seg[k];
@@ -106,14 +107,23 @@ __global__ __launch_bounds__(kBlock) void k_stream_reduce(const int* __restrict_
 
 template <typename TV, typename TX, class XF, class EPI, class FIN>
 void launch_stream_reduce(const DevCSR& M, const TV* av, XF xf, EPI epi, FIN fin, double* partial, hipStream_t st) {
-#define DRAUGR_SR(PER)                                                                                           \
-    hipLaunchKernelGGL((k_stream_reduce<PER, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.blk.p, M.bz.p, \
-                       M.rp.p, M.cv.p, av, xf, epi, partial)
-    switch (M.per) {
-        case 1: DRAUGR_SR(1); break;
-        case 2: DRAUGR_SR(2); break;
-        case 4: DRAUGR_SR(4); break;
-        default: DRAUGR_SR(8); break;
+#define DRAUGR_SR(PER, C)                                                                                       \
+    hipLaunchKernelGGL((k_stream_reduce<PER, C, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.blk.p,     \
+                       M.bz.p, M.rp.p, M.cv.p, M.c16.p, M.cbase.p, av, xf, epi, partial)
+    if (M.has16) {
+        switch (M.per) {
+            case 1: DRAUGR_SR(1, true); break;
+            case 2: DRAUGR_SR(2, true); break;
+            case 4: DRAUGR_SR(4, true); break;
+            default: DRAUGR_SR(8, true); break;
+        }
+    } else {
+        switch (M.per) {
+            case 1: DRAUGR_SR(1, false); break;
+            case 2: DRAUGR_SR(2, false); break;
+            case 4: DRAUGR_SR(4, false); break;
+            default: DRAUGR_SR(8, false); break;
+        }
     }
 #undef DRAUGR_SR
     hipLaunchKernelGGL((k_finalize<FIN>), dim3(1), dim3(kBlock), 0, st, partial, M.nblk, fin);
@@ -363,6 +373,9 @@ template <typename TV, typename TX, class XF, class EPI>
 void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
     if (kind == kOpRowSeq) launch_rowseq<TV, TX>(M.nrow, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
     else if (kind == kOpPipe) launch_pipe<TV, TX>(M.nblk, M.per, M.blk.p, M.bz.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
+    else if (kind == kOpStream16 && M.has16)
+        launch_stream<TV, TX>(M.nblk, M.per, M.blk.p, M.bz.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st, M.c16.p,
+                              M.cbase.p);
     else launch_stream<TV, TX>(M.nblk, M.per, M.blk.p, M.bz.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
 }
 
@@ -388,7 +401,38 @@ void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
     d.per = budget / kBlock;
     d.blk.upload(blk, st);
     d.bz.upload(bz, st);
-    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // blk is pageable and local
+    // 16-bit column offsets from a per-block base for every block that the
+    // stream kernel loads branch-free and that spans fewer than 65536 columns;
+    // the others keep cbase = -1 and are read through the 32-bit columns
+    std::vector<int32_t> cbase(d.nblk + 1, -1);
+    std::vector<uint16_t> c16;
+    int narrow = 0;
+    for (int b = 0; b < d.nblk; ++b) {
+        const int z0 = bz[b], z1 = bz[b + 1];
+        if (z1 - z0 > budget || z1 == z0) continue;
+        const auto mm = std::minmax_element(h.cv.begin() + z0, h.cv.begin() + z1);
+        if ((int64_t)*mm.second - *mm.first > 65535) continue;
+        cbase[b] = *mm.first;
+        ++narrow;
+    }
+    const bool fits = narrow > 0;
+    d.has16 = fits;
+    d.nnz16 = 0;
+    for (int b = 0; b < d.nblk; ++b)
+        if (cbase[b] >= 0) d.nnz16 += bz[b + 1] - bz[b];
+    if (fits) {
+        c16.assign(h.nnz(), 0);
+        for (int b = 0; b < d.nblk; ++b) {
+            if (cbase[b] < 0) continue;
+            for (int k = bz[b]; k < bz[b + 1]; ++k) c16[k] = (uint16_t)(h.cv[k] - cbase[b]);
+        }
+        d.c16.upload(c16, st);
+        d.cbase.upload(cbase, st);
+    } else {
+        d.c16.release();
+        d.cbase.release();
+    }
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // blk, c16 are pageable and local
 }
 
 bool is_l1(Smoother s) { return s == Smoother::L1Jacobi || s == Smoother::L1ColoredGS; }
@@ -808,13 +852,20 @@ void DeviceHierarchy::tune() {
         launch_map(nc, [=] __device__(int64_t i) { xcv[i] = (TX)1; }, st);
         const TX* d = d_of<TX>(D);
         const TX w = (TX)weight();
-        float tA[3], tP[3], tPt[3];
-        for (int k = 0; k < 3; ++k) {
-            tA[k] = timeit([&] { op_kind<TV, TX>(k, D.A, XVec<TX>{xin}, EpiJacobi<TX>{xin, bin, d, s1, w}, st); });
-            tPt[k] = timeit([&] { op_kind<TP, TX>(k, D.Pt, XVec<TX>{xin}, EpiStore<TX>{s2}, st); });
-            tP[k] = timeit([&] { op_kind<TP, TX>(k, D.P, XVec<TX>{xcv}, EpiAdd<TX>{s2}, st); });
+        float tA[kOpKinds], tP[kOpKinds], tPt[kOpKinds];
+        for (int k = 0; k < kOpKinds; ++k) {
+            const float skip = 1e30f;  // the 16-bit form only where the operator has it
+            tA[k] = (k == kOpStream16 && !D.A.has16) ? skip : timeit([&] {
+                op_kind<TV, TX>(k, D.A, XVec<TX>{xin}, EpiJacobi<TX>{xin, bin, d, s1, w}, st);
+            });
+            tPt[k] = (k == kOpStream16 && !D.Pt.has16) ? skip : timeit([&] {
+                op_kind<TP, TX>(k, D.Pt, XVec<TX>{xin}, EpiStore<TX>{s2}, st);
+            });
+            tP[k] = (k == kOpStream16 && !D.P.has16) ? skip : timeit([&] {
+                op_kind<TP, TX>(k, D.P, XVec<TX>{xcv}, EpiAdd<TX>{s2}, st);
+            });
         }
-        auto best = [](const float* t) { return (int)(std::min_element(t, t + 3) - t); };
+        auto best = [](const float* t) { return (int)(std::min_element(t, t + kOpKinds) - t); };
         D.A.kind[pi] = best(tA);
         D.Pt.kind[pi] = best(tPt);
         D.P.kind[pi] = best(tP);
@@ -841,7 +892,7 @@ std::string DeviceHierarchy::tune_report() const {
         if (!tuned_[p]) continue;
         for (int l = 0; l < nlevels(); ++l) {
             const DevLevel& D = L_[l];
-            const char* kn[3] = {"stream", "rowseq", "pipe"};
+            const char* kn[kOpKinds] = {"stream", "rowseq", "pipe", "stream16"};
             std::snprintf(buf, sizeof buf, "%s level %d: A=%s Pt=%s P=%s zero-guess=%s\n", names[p], l,
                           kn[D.A.kind[p]], kn[D.Pt.kind[p]], kn[D.P.kind[p]], D.fuse_zero[p] ? "fused" : "split");
             out += buf;
@@ -1260,15 +1311,31 @@ void DeviceHierarchy::resetup(const double* nzval, hipStream_t st) {
 
 // Algorithmic bytes of one cycle from a zero guess, counting every array a
 // kernel must touch once (value size sv, vector size sx, int32 indices).
+// Column index bytes of one pass: 2 per nonzero of a 16-bit block plus the
+// block bases when the tuned kernel reads the 16-bit form, else 4 per nonzero.
+static double index_bytes(const DevCSR& M, int pi) {
+    if (M.kind[pi] == kOpStream16 && M.has16)
+        return 2.0 * M.nnz16 + 4.0 * (M.nnz - M.nnz16) + 4.0 * M.nblk;
+    return 4.0 * M.nnz;
+}
+
+double DeviceHierarchy::op_bytes(int l, int which, Precision p) const {
+    if (l < 0 || l >= nlevels()) throw Error("level index out of range");
+    const int pi = (int)p;
+    const DevLevel& D = L_[l];
+    const DevCSR& M = which == 0 ? D.A : (which == 1 ? D.P : D.Pt);
+    const double sv = which == 0 ? (p == Precision::FP64 ? 8 : (p == Precision::FP32 ? 4 : 2))
+                                 : (p == Precision::FP64 ? 8 : 4);
+    return M.nnz * sv + index_bytes(M, pi) + (M.nrow + 1) * 4.0;
+}
+
 double DeviceHierarchy::cycle_bytes(const CycleOpts& o) const {
-    const double sv = o.prec == Precision::FP64 ? 8 : (o.prec == Precision::FP32 ? 4 : 2);
-    const double sp = o.prec == Precision::FP64 ? 8 : 4;
     const double sx = o.prec == Precision::FP64 ? 8 : 4;
     double total = 0.0, visits = 1.0;
     for (int l = 0; l < nlevels(); ++l) {
         const DevLevel& D = L_[l];
         const double n = D.A.nrow, nc = D.P.ncol;
-        const double BA = D.A.nnz * (sv + 4) + (n + 1) * 4;
+        const double BA = op_bytes(l, 0, o.prec);
         double lv = 0.0;
         if (is_gs(smoother_)) {
             lv += n * sx;  // zero fill
@@ -1276,7 +1343,8 @@ double DeviceHierarchy::cycle_bytes(const CycleOpts& o) const {
             lv += BA + 3 * n * sx;  // residual
         } else if (smoother_ == Smoother::ILU0) {
             // zero fill, then per step: residual, both triangular sweeps (each reads A's pattern and a factor), update
-            const double BT = D.A.nnz * (sv + 4) + (n + 1) * 4 + n * 4;  // one sweep's factor, pattern, schedule
+            const double sf = o.prec == Precision::FP64 ? 8 : 4;  // factors are FP64 or FP32
+            const double BT = D.A.nnz * (sf + 4) + (n + 1) * 4 + n * 4;  // one sweep's factor, pattern, schedule
             lv += n * sx + (o.pre + o.post) * (BA + 3 * n * sx + 2 * BT + 4 * n * sx + 3 * n * sx);
             lv += BA + 3 * n * sx;  // residual
         } else if (smoother_ == Smoother::Chebyshev) {
@@ -1291,8 +1359,8 @@ double DeviceHierarchy::cycle_bytes(const CycleOpts& o) const {
             lv += o.post * (BA + 4 * n * sx);
             if ((o.pre + o.post) % 2) lv += 2 * n * sx;
         }
-        lv += D.Pt.nnz * (sp + 4) + (nc + 1) * 4 + n * sx + nc * sx;      // restrict
-        lv += D.P.nnz * (sp + 4) + (n + 1) * 4 + nc * sx + 2 * n * sx;    // prolong
+        lv += op_bytes(l, 2, o.prec) + n * sx + nc * sx;      // restrict
+        lv += op_bytes(l, 1, o.prec) + nc * sx + 2 * n * sx;  // prolong
         total += visits * lv;
         if (o.w_cycle) visits *= 2;
     }

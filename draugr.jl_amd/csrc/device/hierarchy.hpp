@@ -29,6 +29,10 @@ struct DevCSR {
     int nblk = 0;
     int per = 8;        // block budget / kBlock: products per thread
     int kind[3] = {0, 0, 0};  // kernel per cycle precision (dev::OpKernel), set by the autotuner
+    bool has16 = false;       // 16-bit column form present (some branch-free block spans < 65536 columns)
+    int64_t nnz16 = 0;        // nonzeros of the blocks read through 16-bit columns
+    DBuf<uint16_t> c16;       // column - cbase[block] per nonzero
+    DBuf<int32_t> cbase;      // per block; -1 = the block reads the 32-bit columns
 };
 
 struct DevLevel {
@@ -99,6 +103,7 @@ public:
     int64_t finest_nnz() const { return L_.empty() ? Ac_.nnz : L_[0].A.nnz; }
     int nlevels() const { return (int)L_.size(); }
     int level_blocks(int l) const { return l < nlevels() ? L_[l].A.nblk : Ac_.nblk; }
+    bool level_has16(int l) const { return l < nlevels() ? L_[l].A.has16 : Ac_.has16; }
     hipStream_t stream() const { return stream_; }
 
     // One V/W cycle on device vectors x (in/out) and b; zero_guess asserts x == 0
@@ -126,6 +131,10 @@ public:
     void export_level_values(int lvl, int which, double* host_dst);  // 0 A vals, 1 dinv, 2 coarse LU, 3 coarsest vals
     int64_t bytes_resident() const;
     double cycle_bytes(const CycleOpts& o) const;  // algorithmic bytes of one V/W cycle (zero guess)
+    // algorithmic bytes of one pass over operator which (0 A, 1 P, 2 P^T) of
+    // level l at precision p: values, column indices in the form the tuned
+    // kernel reads, row pointers (vectors excluded)
+    double op_bytes(int l, int which, Precision p) const;
     void ensure_precision(Precision p);
     // Stream ordering between calls: work a call enqueues on stream st waits
     // for everything the previous call enqueued (possibly on another stream),

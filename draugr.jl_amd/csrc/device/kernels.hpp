@@ -56,6 +56,7 @@ template <> __device__ __forceinline__ float ld<float, bf16_t>(const bf16_t* p, 
     return bf16_to_float(__builtin_nontemporal_load(p + k));
 }
 __device__ __forceinline__ int ldi(const int* p, int64_t k) { return __builtin_nontemporal_load(p + k); }
+__device__ __forceinline__ int ldc16(const uint16_t* p, int64_t k) { return (int)__builtin_nontemporal_load(p + k); }
 
 // XCD-contiguous remap of the block index (bijective for any grid size).
 __device__ __forceinline__ int xcd_block() {
@@ -182,13 +183,20 @@ template <typename TX> struct EpiStoreScaled {
 // the block's end are clamped to its last entry and the result discarded),
 // then its PER gathers together, so each wave keeps 2 PER + PER loads in
 // flight instead of waiting once per product.
-template <int PER, typename TV, typename TX, class XF, class PREF, class SEGF, class TOTF>
+//
+// C16: such blocks read their columns as 16-bit offsets from a per-block base
+// (col = cbase[b] + c16[k]), 2 bytes per nonzero instead of 4, when the block
+// spans fewer than 65536 columns (cbase[b] >= 0); wider and longer blocks read
+// the 32-bit columns.
+template <int PER, bool C16, typename TV, typename TX, class XF, class PREF, class SEGF, class TOTF>
 __device__ __forceinline__ void stream_block(int b, const int* __restrict__ blk, const int* __restrict__ bz,
                                              const int* __restrict__ rp, const int* __restrict__ cv,
+                                             const uint16_t* __restrict__ c16, const int* __restrict__ cbase,
                                              const TV* __restrict__ av, XF xf, TX* prod, PREF pref, SEGF segf,
                                              TOTF totf) {
     const int r0 = blk[b], r1 = blk[b + 1];
     const int z0 = bz[b], z1 = bz[b + 1];
+    const int cb = C16 ? cbase[b] : 0;
     const int nz = z1 - z0;
     const int t = (int)threadIdx.x;
     const int row = r0 + t;
@@ -201,11 +209,20 @@ __device__ __forceinline__ void stream_block(int b, const int* __restrict__ blk,
             if (nz > 0) {
                 int cj[PER];
                 TX v[PER], g[PER];
+                if (C16 && cb >= 0) {  // block-uniform choice, outside the unrolled loads
 #pragma unroll
-                for (int u = 0; u < PER; ++u) {
-                    const int k = min(t + u * kBlock, nz - 1);
-                    cj[u] = ldi(cv, z0 + k);
-                    v[u] = ld<TX, TV>(av, z0 + k);
+                    for (int u = 0; u < PER; ++u) {
+                        const int k = min(t + u * kBlock, nz - 1);
+                        cj[u] = cb + ldc16(c16, z0 + k);
+                        v[u] = ld<TX, TV>(av, z0 + k);
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < PER; ++u) {
+                        const int k = min(t + u * kBlock, nz - 1);
+                        cj[u] = ldi(cv, z0 + k);
+                        v[u] = ld<TX, TV>(av, z0 + k);
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < PER; ++u) g[u] = xf(cj[u]);
@@ -240,14 +257,15 @@ __device__ __forceinline__ void epi_segment(const EPI& epi, int row, const TX* s
     epi.fin(row, a, q);
 }
 
-template <bool XCD, int PER, typename TV, typename TX, class XF, class EPI>
+template <bool XCD, int PER, bool C16, typename TV, typename TX, class XF, class EPI>
 __global__ __launch_bounds__(kBlock) void k_csr_stream(const int* __restrict__ blk, const int* __restrict__ bz,
                                                        const int* __restrict__ rp, const int* __restrict__ cv,
+                                                       const uint16_t* __restrict__ c16, const int* __restrict__ cbase,
                                                        const TV* __restrict__ av, XF xf, EPI epi) {
     __shared__ TX prod[kStreamNnz];
     const int b = XCD ? xcd_block() : (int)blockIdx.x;
-    stream_block<PER, TV, TX>(
-        b, blk, bz, rp, cv, av, xf, prod, [&](int row) { return epi.pre(row); },
+    stream_block<PER, C16, TV, TX>(
+        b, blk, bz, rp, cv, c16, cbase, av, xf, prod, [&](int row) { return epi.pre(row); },
         [&](int row, const TX* seg, int len, const typename EPI::Pre& q) { epi_segment(epi, row, seg, len, q); },
         [&](int row, TX s) { epi.total(row, s); });
 }
@@ -381,30 +399,39 @@ void launch_pipe(int nblk, int per, const int* blk, const int* bz, const int* rp
     }
 }
 
-enum OpKernel : int { kOpStream = 0, kOpRowSeq = 1, kOpPipe = 2 };
+enum OpKernel : int { kOpStream = 0, kOpRowSeq = 1, kOpPipe = 2, kOpStream16 = 3 };
+constexpr int kOpKinds = 4;
 
-template <int PER, typename TV, typename TX, class XF, class EPI>
-void launch_stream_per(int nblk, const int* blk, const int* bz, const int* rp, const int* cv, const TV* av, XF xf,
-                       EPI epi, hipStream_t st) {
+template <int PER, bool C16, typename TV, typename TX, class XF, class EPI>
+void launch_stream_per(int nblk, const int* blk, const int* bz, const int* rp, const int* cv, const uint16_t* c16,
+                       const int* cbase, const TV* av, XF xf, EPI epi, hipStream_t st) {
     if (nblk >= 64)
-        hipLaunchKernelGGL((k_csr_stream<true, PER, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk, bz, rp,
-                           cv, av, xf, epi);
+        hipLaunchKernelGGL((k_csr_stream<true, PER, C16, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk, bz,
+                           rp, cv, c16, cbase, av, xf, epi);
     else
-        hipLaunchKernelGGL((k_csr_stream<false, PER, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk, bz,
-                           rp, cv, av, xf, epi);
+        hipLaunchKernelGGL((k_csr_stream<false, PER, C16, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk,
+                           bz, rp, cv, c16, cbase, av, xf, epi);
 }
 
-// per = the operator's block budget / kBlock (1, 2, 4 or 8)
+template <bool C16, typename TV, typename TX, class XF, class EPI>
+void launch_stream_c(int nblk, int per, const int* blk, const int* bz, const int* rp, const int* cv,
+                     const uint16_t* c16, const int* cbase, const TV* av, XF xf, EPI epi, hipStream_t st) {
+    switch (per) {
+        case 1: launch_stream_per<1, C16, TV, TX>(nblk, blk, bz, rp, cv, c16, cbase, av, xf, epi, st); break;
+        case 2: launch_stream_per<2, C16, TV, TX>(nblk, blk, bz, rp, cv, c16, cbase, av, xf, epi, st); break;
+        case 4: launch_stream_per<4, C16, TV, TX>(nblk, blk, bz, rp, cv, c16, cbase, av, xf, epi, st); break;
+        default: launch_stream_per<8, C16, TV, TX>(nblk, blk, bz, rp, cv, c16, cbase, av, xf, epi, st); break;
+    }
+}
+
+// per = the operator's block budget / kBlock (1, 2, 4 or 8); c16 / cbase
+// non-null = the 16-bit column form
 template <typename TV, typename TX, class XF, class EPI>
 void launch_stream(int nblk, int per, const int* blk, const int* bz, const int* rp, const int* cv, const TV* av,
-                   XF xf, EPI epi, hipStream_t st) {
+                   XF xf, EPI epi, hipStream_t st, const uint16_t* c16 = nullptr, const int* cbase = nullptr) {
     if (nblk <= 0) return;
-    switch (per) {
-        case 1: launch_stream_per<1, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
-        case 2: launch_stream_per<2, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
-        case 4: launch_stream_per<4, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
-        default: launch_stream_per<8, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
-    }
+    if (c16 && cbase) launch_stream_c<true, TV, TX>(nblk, per, blk, bz, rp, cv, c16, cbase, av, xf, epi, st);
+    else launch_stream_c<false, TV, TX>(nblk, per, blk, bz, rp, cv, nullptr, nullptr, av, xf, epi, st);
 }
 
 template <typename TV, typename TX, class XF, class EPI>

@@ -170,7 +170,7 @@ class AMGConfig:
     precision: str = "fp64"
     seed: int = 0x5EED
     graphs: bool = True
-    kernel: str = "auto"  # sparse-pass kernel: "auto" (autotuned), "stream", "rowseq", "pipe"
+    kernel: str = "auto"  # sparse-pass kernel: "auto" (autotuned), "stream", "rowseq", "pipe", "stream16"
     gpu_setup: bool = True  # interpolation, coarse operators and triple maps of the setup built on the GPU
     gpu_setup_min_rows: int = 32768  # smaller levels interpolate on the host
 

@@ -105,9 +105,9 @@ class AMGHierarchy:
         return self.info()["nlevels"]
 
     def level_info(self, l: int) -> dict:
-        out = (ctypes.c_int64 * 8)()
-        m = check(_lib.lib().draugr_hip_level_info(self.handle, l, out, 8), "draugr_hip_level_info")
-        keys = ["n", "nnz", "n_coarse", "nnz_P", "ntriples", "num_colors", "has_rmap", "nblk"]
+        out = (ctypes.c_int64 * 9)()
+        m = check(_lib.lib().draugr_hip_level_info(self.handle, l, out, 9), "draugr_hip_level_info")
+        keys = ["n", "nnz", "n_coarse", "nnz_P", "ntriples", "num_colors", "has_rmap", "nblk", "has16"]
         return dict(zip(keys, [int(out[i]) for i in range(m)]))
 
     def _export(self, l, which, count, dtype):
@@ -160,6 +160,14 @@ class AMGHierarchy:
         c = _Config(config)
         v = ctypes.c_double(0.0)
         check(_lib.lib().draugr_hip_cycle_bytes(self.handle, c.handle, ctypes.byref(v)), "draugr_hip_cycle_bytes")
+        return v.value
+
+    def op_bytes(self, level: int, which: int, precision: str) -> float:
+        """Algorithmic bytes of one pass over operator ``which`` (0 A, 1 P,
+        2 P^T) of ``level`` at ``precision``, index form as tuned."""
+        v = ctypes.c_double(0.0)
+        p = {"fp64": 0, "fp32": 1, "bf16": 2}[precision]
+        check(_lib.lib().draugr_hip_op_bytes(self.handle, level, which, p, ctypes.byref(v)), "draugr_hip_op_bytes")
         return v.value
 
     def tune_report(self) -> str:

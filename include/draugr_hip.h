@@ -39,7 +39,7 @@ const char *draugr_amg_last_error(void);
  * "precision" ("fp64" | "fp32" | "bf16": arithmetic of the V/W cycle),
  * "seed" (setup RNG seed), "graphs" (true: the PCG / mixed-precision preconditioner cycle is
  * captured once into a hipGraph and replayed; false: launched eagerly),
- * "kernel" ("auto" | "stream" | "rowseq" | "pipe"), "gpu_setup" (default
+ * "kernel" ("auto" | "stream" | "rowseq" | "pipe" | "stream16"), "gpu_setup" (default
  * true: when the hierarchy is uploaded, each level's extended+i interpolation,
  * coarse operator, triple map and transpose map are built on the GPU,
  * identical to the host's, and the triple map stays in HBM; levels of fewer
@@ -94,7 +94,8 @@ int32_t draugr_hip_upload(int32_t handle);
 int32_t draugr_hip_hierarchy_info(int32_t handle, int64_t *out, int32_t nout);
 
 /* out[0..]: n, nnz(A), n_coarse, nnz(P), ntriples, num_colors, has_rmap,
- * row blocks of A on the device (0 before upload). */
+ * row blocks of A on the device (0 before upload), A stored with 16-bit
+ * block-relative columns on the device (0/1). */
 int32_t draugr_hip_level_info(int32_t handle, int32_t level, int64_t *out, int32_t nout);
 
 /* Copy a host array of level `level` (0-based) into dst, 0-based indices:
@@ -145,6 +146,12 @@ int32_t draugr_hip_export_device_values(int32_t handle, int32_t level, int32_t w
 /* Algorithmic bytes of one cycle from a zero guess under the given config,
  * and the HBM bytes the hierarchy holds. */
 int32_t draugr_hip_cycle_bytes(int32_t handle, int32_t config_handle, double *bytes);
+/* Algorithmic bytes of one pass over level `level`'s operator `which` (0 A,
+ * 1 P, 2 P^T) at `precision` (0 fp64, 1 fp32, 2 bf16): values, column indices
+ * in the form the autotuned kernel reads (16-bit block-relative where it
+ * chose that form), row pointers; vectors excluded.  No counterpart in the
+ * reference (measurement). */
+int32_t draugr_hip_op_bytes(int32_t handle, int32_t level, int32_t which, int32_t precision, double *bytes);
 int32_t draugr_hip_resident_bytes(int32_t handle, int64_t *bytes);
 
 /* ── Differentiation of x = A(k)^-1 b (device pointers) ─────────────────── */

@@ -38,6 +38,21 @@ def _case(dh, name):
         cfg = dh.AMGConfig(coarsening=dh.HMISCoarsening(0.5, dh.ExtendedIInterpolation(0.3)),
                            smoother=dh.L1ColoredGaussSeidelType())
         sm = "l1_colored_gs"
+    elif name == "wide2d":
+        # 2-D Poisson (73,984 rows) plus a coupling of the first and last rows:
+        # their row blocks span more than 65,535 columns, so the 16-bit column
+        # form reads them through the 32-bit columns
+        import scipy.sparse as sp
+        B = P.poisson2d(272).to_scipy().tolil()
+        n = B.shape[0]
+        B[0, n - 1] = B[n - 1, 0] = -0.5
+        B[0, 0] += 0.5
+        B[n - 1, n - 1] += 0.5
+        B = B.tocsr()
+        B.sort_indices()
+        A = dh.as_csr(B)
+        cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType())
+        sm = "jacobi"
     else:
         raise KeyError(name)
     return A, cfg, sm
@@ -317,17 +332,20 @@ def test_preconditioner_adapter_with_scipy_cg(dh, torch_cuda):
     assert info2 == 0
 
 
-@pytest.mark.parametrize("kernel", ["stream", "rowseq", "pipe"])
-@pytest.mark.parametrize("name", ["hypre3d", "agg2d"])
+@pytest.mark.parametrize("kernel", ["stream", "rowseq", "pipe", "stream16"])
+@pytest.mark.parametrize("name", ["hypre3d", "agg2d", "wide2d"])
 def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
-    """Each of the three sparse-pass kernels the autotuner chooses among,
-    forced on every level, gives the oracle's FP64 cycle bit for bit and FP32
-    cycles equal to each other to rounding."""
+    """Each of the four sparse-pass kernels the autotuner chooses among,
+    forced on every level, gives the oracle's FP64 cycle bit for bit and the
+    FP32-cycle PCG converges.  The 16-bit column form is present on every
+    level of these problems (wide blocks, if any, read the 32-bit columns)."""
     import dataclasses
     torch = torch_cuda
     A, cfg, sm = _case(dh, name)
     c = dataclasses.replace(cfg, kernel=kernel)
     h = dh.amg_setup(A, c)
+    if kernel == "stream16":
+        assert all(h.level_info(l)["has16"] for l in range(h.nlevels))
     oh = oracle_of(h, orc, sm)
     b = dh.problems.splitmix_normal(SEED + 900, A.n)
     x0 = dh.problems.splitmix_normal(SEED + 901, A.n)
