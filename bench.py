@@ -49,6 +49,9 @@ def parse():
                     help="hetero128 (default): the headline line; the others print their own JSON line")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the calibration copy and the dominant-kernel launches (for counter runs)")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "stream", "rowseq", "pipe", "stream16"],
+                    help="sparse-pass kernel (config key kernel); a counter run forces the kernel the "
+                         "autotuner picks without a profiler, whose timings it would perturb")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "stream_jacobi_level1_128cubed_traffic.md"),
                     help="summary of the FETCH_SIZE / WRITE_SIZE passes holding the measured HBM bytes per launch")
     return ap.parse_args()
@@ -83,7 +86,11 @@ def dominant_kernel(torch, h, prec, reps):
     # A_1 once (values, column indices in the form the tuned kernel reads:
     # 16-bit block-relative where chosen, row pointers); x, b, dinv read, x_out written
     nbytes = int(h.op_bytes(1, 0, prec)) + 4 * n * 4
-    return t, nbytes, "level1_jacobi_%s_grid%d" % (prec, h_blocks(h, 1))
+    kind = "stream"
+    for line in h.tune_report().splitlines():  # the kernel the autotuner chose for this pass
+        if line.startswith("%s level 1: A=" % prec):
+            kind = line.split("A=")[1].split()[0]
+    return t, nbytes, "level1_jacobi_%s_%s_grid%d" % (prec, kind, h_blocks(h, 1))
 
 
 def h_blocks(h, l):
@@ -312,8 +319,8 @@ def main():
     A = P.heterogeneous3d(m, logk)
     b_host = P.splitmix_normal(SEED + 2, A.n)
     log("problem %d^3: n=%d nnz=%d (%.1fs)" % (m, A.n, A.nnz, time.perf_counter() - t0))
-    cfg64 = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())
-    cfg32 = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), precision=args.precision)
+    cfg64 = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), kernel=args.kernel)
+    cfg32 = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), precision=args.precision, kernel=args.kernel)
     t0 = time.perf_counter()
     h = dh.amg_setup(A, cfg64)
     t_setup = time.perf_counter() - t0
@@ -322,6 +329,9 @@ def main():
 
     b = torch.tensor(b_host, device="cuda")
     if args.kernel_only:
+        z = torch.zeros_like(b)
+        dh.amg_cycle_(z, b, h, cfg32, zero_guess=True)  # the autotuner picks the pass kernels
+        torch.cuda.synchronize()
         calibration_copy(torch)
         t_k, kbytes, tag = dominant_kernel(torch, h, args.precision, 20)
         print(json.dumps({"kernel": tag, "bytes_per_launch": kbytes, "avg_launch_us": t_k * 1e6,

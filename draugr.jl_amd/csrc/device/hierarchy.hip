@@ -54,7 +54,16 @@ template <class FIN>
 __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partial, int nblocks, FIN fin) {
     __shared__ double sh[kBlock];
     double acc = 0.0;
-    for (int i = threadIdx.x; i < nblocks; i += kBlock) acc += partial[i];
+    // eight partials in flight per thread, added in the same order as one at a time
+    int i = threadIdx.x;
+    for (; i + 7 * kBlock < nblocks; i += 8 * kBlock) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = partial[i + u * kBlock];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; i < nblocks; i += kBlock) acc += partial[i];
     sh[threadIdx.x] = acc;
     __syncthreads();
 #pragma unroll
@@ -826,12 +835,14 @@ void DeviceHierarchy::tune() {
     hipEvent_t e0, e1;
     DRAUGR_HIP_CHECK(hipEventCreate(&e0));
     DRAUGR_HIP_CHECK(hipEventCreate(&e1));
+    // best of 5 repetitions of 4 back-to-back launches: the per-launch time a
+    // cycle sees, with the events' own overhead and the noise spread thin
     auto timeit = [&](auto&& fn) {
         fn();
         float best = 1e30f;
-        for (int rep = 0; rep < 3; ++rep) {
+        for (int rep = 0; rep < 5; ++rep) {
             DRAUGR_HIP_CHECK(hipEventRecord(e0, st));
-            fn();
+            for (int k = 0; k < 4; ++k) fn();
             DRAUGR_HIP_CHECK(hipEventRecord(e1, st));
             DRAUGR_HIP_CHECK(hipEventSynchronize(e1));
             float ms = 0.f;
