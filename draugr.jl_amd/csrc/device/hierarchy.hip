@@ -745,9 +745,22 @@ void DeviceHierarchy::ensure_precision(Precision p) {
 // Jacobi steps ping-pong between x and alt (no copy back: the caller tracks
 // which buffer holds the iterate).  A first step from a zero guess is
 // x1 = w d b, fused with the residual that follows it when fuse_resid.
+// Whether level l's first pre-smoothing step from a zero guess is the plain
+// map x1 = (w dinv) b (a Jacobi-type smoother not fused with the residual):
+// then the parent's restriction can write x1 along with b.
+template <typename TV>
+bool DeviceHierarchy::zero_step_is_map(int l, int pre) const {
+    if (l < 1 || l >= nlevels() || pre < 1) return false;
+    if (!(smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || smoother_ == Smoother::SPAI0))
+        return false;
+    return !(pre == 1 && L_[l].fuse_zero[pidx<TV>()]);
+}
+
+// x1_ready: alt already holds the first step from a zero guess (written by
+// the parent's restriction).
 template <typename TV, typename TX>
 void DeviceHierarchy::smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bool& zero, bool fuse_resid,
-                             bool& resid_done, hipStream_t st) {
+                             bool& resid_done, hipStream_t st, bool x1_ready) {
     DevLevel& D = L_[l];
     const int n = D.A.nrow;
     const TX* d = d_of<TX>(D);
@@ -780,7 +793,7 @@ void DeviceHierarchy::smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bo
             if (fuse_resid && steps == 1 && D.fuse_zero[pidx<TV>()]) {
                 op<TV, TX>(D.A, XZeroJacobi<TX>{d, b, w}, EpiZeroJacobiResid<TX>{b, d, alt, r_of<TX>(D), w}, st);
                 resid_done = true;
-            } else {
+            } else if (!x1_ready) {
                 TX* o = alt;
                 launch_map(n, [=] __device__(int64_t i) { o[i] = (w * d[i]) * b[i]; }, st);
             }
@@ -883,8 +896,12 @@ void DeviceHierarchy::tune() {
         const float fused = timeit([&] {
             op<TV, TX>(D.A, XZeroJacobi<TX>{d, bin, w}, EpiZeroJacobiResid<TX>{bin, d, s1, s2, w}, st);
         });
+        // below the finest level the split form's map rides along with the
+        // parent's restriction, so only its residual pass is counted there
+        const bool map_folded = l >= 1 && (smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi ||
+                                           smoother_ == Smoother::SPAI0);
         const float split = timeit([&] {
-            launch_map(n, [=] __device__(int64_t i) { s1[i] = (w * d[i]) * bin[i]; }, st);
+            if (!map_folded) launch_map(n, [=] __device__(int64_t i) { s1[i] = (w * d[i]) * bin[i]; }, st);
             op<TV, TX>(D.A, XVec<TX>{s1}, EpiResid<TX>{bin, s2}, st);
         });
         D.fuse_zero[pi] = fused < split;
@@ -1004,7 +1021,8 @@ void DeviceHierarchy::chebyshev(int l, TX*& x, TX*& alt, const TX* b, int steps,
 
 // One visit of level l (src/cycle.jl:63-99).
 template <typename TV, typename TX>
-void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const CycleOpts& o, hipStream_t st) {
+void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const CycleOpts& o, hipStream_t st,
+                                  bool x1_ready) {
     typedef typename TransferT<TV>::type TP;
     DevLevel& D = L_[l];
     TX* xcur = x;
@@ -1013,12 +1031,19 @@ void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const Cy
     TX* xc = xc_of<TX>(D);
     TX* bc = bc_of<TX>(D);
     bool resid_done = false;
-    smooth<TV, TX>(l, xcur, xalt, b, o.pre, zero, true, resid_done, st);
+    smooth<TV, TX>(l, xcur, xalt, b, o.pre, zero, true, resid_done, st, x1_ready && zero);
     if (!resid_done) op<TV, TX>(D.A, XVec<TX>{xcur}, EpiResid<TX>{b, r}, st);
-    op<TP, TX>(D.Pt, XVec<TX>{r}, EpiStore<TX>{bc}, st);
+    // the child's first step from zero rides along with the restriction
+    const bool child_x1 = zero_step_is_map<TV>(l + 1, o.pre);
+    if (child_x1) {
+        DevLevel& C = L_[l + 1];
+        op<TP, TX>(D.Pt, XVec<TX>{r}, EpiStoreJac0<TX>{bc, d_of<TX>(C), t_of<TX>(C), (TX)weight()}, st);
+    } else {
+        op<TP, TX>(D.Pt, XVec<TX>{r}, EpiStore<TX>{bc}, st);
+    }
     if (l + 1 < nlevels()) {
         const int reps = o.w_cycle ? 2 : 1;
-        for (int rep = 0; rep < reps; ++rep) cycle_level<TV, TX>(l + 1, xc, bc, rep == 0, o, st);
+        for (int rep = 0; rep < reps; ++rep) cycle_level<TV, TX>(l + 1, xc, bc, rep == 0, o, st, rep == 0 && child_x1);
     } else {
         coarse_solve<TX>(xc, bc, st);
     }
@@ -1077,6 +1102,17 @@ void DeviceHierarchy::precondition(TX* x, const TX* b, const CycleOpts& o, hipSt
 }
 
 void DeviceHierarchy::cycle(double* x, const double* b, bool zero, const CycleOpts& o, hipStream_t st) {
+    if (o.prec == Precision::FP64 && zero && o.graphs && nlevels() > 0) {
+        // x = V(0; b) replayed from the graph on the hierarchy's own vectors (one
+        // graph whatever the caller's pointers); copies do not change values
+        double* r = w_r_.p;
+        double* z = w_z_.p;
+        const int n = n0_;
+        launch_map(n, [=] __device__(int64_t i) { r[i] = b[i]; }, st);
+        precondition<double, double>(z, r, o, st);
+        launch_map(n, [=] __device__(int64_t i) { x[i] = z[i]; }, st);
+        return;
+    }
     if (o.prec == Precision::FP64 || nlevels() == 0) {
         // no AMG levels: the cycle is the direct solve (src/cycle.jl:18-24), kept in FP64
         cycle_top<double, double>(x, b, zero, o, st);

@@ -144,10 +144,12 @@ public:
 
 private:
     template <typename TV, typename TX>
-    void cycle_level(int l, TX* x, const TX* b, bool zero_guess, const CycleOpts& o, hipStream_t st);
+    void cycle_level(int l, TX* x, const TX* b, bool zero_guess, const CycleOpts& o, hipStream_t st,
+                     bool x1_ready = false);
     template <typename TV, typename TX>
     void smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bool& zero, bool fuse_resid, bool& resid_done,
-                hipStream_t st);
+                hipStream_t st, bool x1_ready = false);
+    template <typename TV> bool zero_step_is_map(int l, int pre) const;
     template <typename TX> void coarse_solve(TX* x, const TX* b, hipStream_t st);
     template <typename TV, typename TX> void cycle_top(TX* x, const TX* b, bool zero_guess, const CycleOpts& o,
                                                        hipStream_t st);

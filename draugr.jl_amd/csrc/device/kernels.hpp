@@ -93,6 +93,21 @@ template <typename TX> struct EpiStore {          // y = A x   (src/csr_matrix.j
     __device__ void fin(int i, TX a, const Pre&) const { y[i] = a; }
     __device__ void total(int i, TX s) const { y[i] = s; }
 };
+// Restriction that also forms the coarse level's first Jacobi iterate from a
+// zero guess: bc = P^T r and x1 = (w dinv_c) bc, the value the coarse level's
+// own first step would compute from bc.
+template <typename TX> struct EpiStoreJac0 {
+    TX* y; const TX* d; TX* x1; TX w;
+    struct Pre { TX d; };
+    __device__ Pre pre(int i) const { return {d[i]}; }
+    __device__ TX init(const Pre&) const { return 0; }
+    __device__ TX acc(TX a, TX p) const { return a + p; }
+    __device__ void fin(int i, TX a, const Pre& q) const {
+        y[i] = a;
+        x1[i] = (w * q.d) * a;
+    }
+    __device__ void total(int i, TX s) const { fin(i, s, pre(i)); }
+};
 template <typename TX> struct EpiResid {          // r = b - A x   (src/cycle.jl:32-43)
     const TX* b; TX* r;
     struct Pre { TX b; };
