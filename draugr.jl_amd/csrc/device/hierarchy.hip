@@ -750,7 +750,7 @@ void DeviceHierarchy::ensure_precision(Precision p) {
 // then the parent's restriction can write x1 along with b.
 template <typename TV>
 bool DeviceHierarchy::zero_step_is_map(int l, int pre) const {
-    if (l < 1 || l >= nlevels() || pre < 1) return false;
+    if (l < 0 || l >= nlevels() || pre < 1) return false;
     if (!(smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || smoother_ == Smoother::SPAI0))
         return false;
     return !(pre == 1 && L_[l].fuse_zero[pidx<TV>()]);
@@ -1063,7 +1063,7 @@ void DeviceHierarchy::cycle_top(TX* x, const TX* b, bool zero, const CycleOpts& 
         coarse_solve<TX>(x, b, st);
         return;
     }
-    cycle_level<TV, TX>(0, x, b, zero, o, st);
+    cycle_level<TV, TX>(0, x, b, zero, o, st, zero && o.x1_ready0);
 }
 
 // The graph holds the cycle's launches with their arguments (buffers of this
@@ -1078,7 +1078,8 @@ void DeviceHierarchy::precondition(TX* x, const TX* b, const CycleOpts& o, hipSt
         cycle_top<TV, TX>(x, b, true, o, st);
         return;
     }
-    const auto key = std::make_tuple(pidx<TV>(), (int)o.w_cycle, o.pre, o.post, (const void*)x, (const void*)b);
+    const auto key =
+        std::make_tuple(pidx<TV>(), (int)o.w_cycle, o.pre, o.post, (int)o.x1_ready0, (const void*)x, (const void*)b);
     auto it = graphs_.find(key);
     if (it == graphs_.end()) {
         if (!cap_) DRAUGR_HIP_CHECK(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
@@ -1216,16 +1217,32 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
     if (relres) *relres = rel;
     if (rel < tol) return 0;
     launch_map(n, [=] __device__(int64_t i) { pa[i] = 0.0; }, st);
-    auto apply_prec = [&]() {
+    auto apply_prec = [&](const CycleOpts& oc) {
         if (mixed) {
-            if (o.prec == Precision::FP32) precondition<float, float>(z32, r32, o, st);
-            else precondition<bf16_t, float>(z32, r32, o, st);
+            if (oc.prec == Precision::FP32) precondition<float, float>(z32, r32, oc, st);
+            else precondition<bf16_t, float>(z32, r32, oc, st);
         } else {
-            precondition<double, double>(z, r, o, st);
+            precondition<double, double>(z, r, oc, st);
         }
     };
     if (mixed) launch_map(n, [=] __device__(int64_t i) { r32[i] = (float)r[i]; }, st);
-    apply_prec();
+    apply_prec(o);
+    // From here on, when the finest level's first smoothing step from zero is
+    // the plain map x1 = (w dinv) r, the residual update below writes x1 too
+    // and the preconditioner starts from it (the same values, one pass fewer).
+    CycleOpts oit = o;
+    if (nlevels() > 0) {
+        if (!mixed) oit.x1_ready0 = zero_step_is_map<double>(0, o.pre);
+        else if (o.prec == Precision::FP32) oit.x1_ready0 = zero_step_is_map<float>(0, o.pre);
+        else oit.x1_ready0 = zero_step_is_map<bf16_t>(0, o.pre);
+    }
+    const bool x1_0 = oit.x1_ready0;
+    float* t32 = (mixed && x1_0) ? L_[0].t32.p : nullptr;
+    double* t64 = (!mixed && x1_0) ? L_[0].t64.p : nullptr;
+    const float* d32 = (mixed && x1_0) ? L_[0].d32.p : nullptr;
+    const double* d64 = (!mixed && x1_0) ? L_[0].d64.p : nullptr;
+    const float w32 = (float)weight();
+    const double w64 = weight();
     // rz = r.z ; beta = 0 for the first direction
     if (mixed)
         launch_reduce(n, [=] __device__(int64_t i) { return r[i] * (double)z32[i]; },
@@ -1278,7 +1295,9 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                                   x[i] += a * pb[i];
                                   const double ri = r[i] - a * q[i];
                                   r[i] = ri;
-                                  r32[i] = (float)ri;
+                                  const float rf = (float)ri;
+                                  r32[i] = rf;
+                                  if (x1_0) t32[i] = (w32 * d32[i]) * rf;
                                   return ri * ri;
                               },
                               fin_rr, partial_.p, st);
@@ -1289,12 +1308,13 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                                   x[i] += a * pb[i];
                                   const double ri = r[i] - a * q[i];
                                   r[i] = ri;
+                                  if (x1_0) t64[i] = (w64 * d64[i]) * ri;
                                   return ri * ri;
                               },
                               fin_rr, partial_.p, st);
             DRAUGR_HIP_CHECK(hipEventRecord(ev, st));
             // speculative: next preconditioned residual and beta
-            apply_prec();
+            apply_prec(oit);
             auto fin_rz = [=] __device__(double s) {
                 scal[S_BETA] = s / scal[S_RZ];
                 scal[S_RZ] = s;

@@ -75,6 +75,7 @@ struct CycleOpts {
     int pre = 1, post = 1;
     Precision prec = Precision::FP64;
     bool graphs = true;  // zero-guess cycles on fixed buffers replayed from a hipGraph
+    bool x1_ready0 = false;  // the finest level's first step from zero is already in its work vector
 };
 
 // Kernels shared with the batched path (batch.hip):
@@ -193,7 +194,7 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t last_ev_ = nullptr;
     hipStream_t cap_ = nullptr;  // capture stream for the graphs
-    std::map<std::tuple<int, int, int, int, const void*, const void*>, hipGraphExec_t> graphs_;
+    std::map<std::tuple<int, int, int, int, int, const void*, const void*>, hipGraphExec_t> graphs_;
     hipStream_t last_st_ = nullptr;
     bool last_valid_ = false;
 };
