@@ -13,6 +13,14 @@
 
 namespace draugr {
 
+// Parallel loops here are the order-independent ones: per-row strength,
+// column counts (integer atomics), per-node graph construction (each node's
+// list in the sequential order), and the PMIS sweeps.  A PMIS sweep reads
+// the C/F state of the previous sweep (snapshot): marking C only turns 0 into
+// 1 and the C test asks "not F", marking F only turns 0 into -1 and the F test
+// asks "is C", so the sequential in-place sweep of the reference decides
+// every node exactly as the snapshot sweep does.
+
 namespace {
 
 constexpr double kEps = std::numeric_limits<double>::epsilon();
@@ -38,9 +46,14 @@ void uf_union(std::vector<int32_t>& parent, int32_t a, int32_t b) {
 // src/coarsening.jl:179-192
 std::vector<int32_t> strong_transpose_count(const CSR& A, const std::vector<uint8_t>& s) {
     std::vector<int32_t> cnt(A.nrow, 0);
+    int32_t* c = cnt.data();
+#pragma omp parallel for schedule(static, 4096)
     for (int32_t i = 0; i < A.nrow; ++i)
         for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
-            if (strong_offdiag(s, k, A.cv[k], i)) cnt[A.cv[k]]++;
+            if (strong_offdiag(s, k, A.cv[k], i)) {
+#pragma omp atomic
+                c[A.cv[k]]++;
+            }
     return cnt;
 }
 
@@ -174,12 +187,14 @@ std::vector<int8_t> rs_first_pass(const CSR& A, const std::vector<uint8_t>& s, b
 void pmis_on_undecided(std::vector<int8_t>& cf, const CSR& A, const std::vector<uint8_t>& s,
                        const std::vector<double>& measure) {
     const int32_t n = A.nrow;
+    std::vector<int8_t> nxt(cf);  // snapshot sweeps (see the note at the top)
     for (int64_t iter = 0; iter <= (int64_t)n; ++iter) {
-        bool all_decided = true;
+        int any_undecided = 0;
+#pragma omp parallel for schedule(static, 4096) reduction(| : any_undecided)
         for (int32_t i = 0; i < n; ++i) {
             if (cf[i] != 0) continue;
-            all_decided = false;
-            if (measure[i] < 1.0) { cf[i] = -1; continue; }
+            any_undecided = 1;
+            if (measure[i] < 1.0) { nxt[i] = -1; continue; }
             bool is_max = true;
             for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
                 const int32_t j = A.cv[k];
@@ -188,14 +203,17 @@ void pmis_on_undecided(std::vector<int8_t>& cf, const CSR& A, const std::vector<
                     break;
                 }
             }
-            if (is_max) cf[i] = 1;
+            if (is_max) nxt[i] = 1;
         }
+        cf = nxt;
+#pragma omp parallel for schedule(static, 4096)
         for (int32_t i = 0; i < n; ++i) {
             if (cf[i] != 0) continue;
             for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
-                if (s[k] && cf[A.cv[k]] == 1) { cf[i] = -1; break; }
+                if (s[k] && cf[A.cv[k]] == 1) { nxt[i] = -1; break; }
         }
-        if (all_decided) break;
+        cf = nxt;
+        if (!any_undecided) break;
     }
     for (auto& c : cf)
         if (c == 0) c = -1;
@@ -227,6 +245,7 @@ int32_t count_coarse(const std::vector<int8_t>& cf) {
 // Absolute strength of connection; src/strength.jl:24-61.
 std::vector<uint8_t> strength_absolute(const CSR& A, double theta) {
     std::vector<uint8_t> s(A.nnz(), 0);
+#pragma omp parallel for schedule(static, 4096)
     for (int32_t i = 0; i < A.nrow; ++i) {
         double mx = 0.0;
         for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
@@ -373,11 +392,13 @@ void coarsen_pmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf,
     cf.assign(n, 0);
     for (int32_t i = 0; i < n; ++i)
         if (!has_strong_offdiag(A, s, i) && st[i] == 0) cf[i] = 1;
+    std::vector<int8_t> nxt(cf);  // snapshot sweeps (see the note at the top)
     for (int64_t iter = 0; iter <= (int64_t)n; ++iter) {
-        bool all_decided = true;
+        int any_undecided = 0;
+#pragma omp parallel for schedule(static, 4096) reduction(| : any_undecided)
         for (int32_t i = 0; i < n; ++i) {
             if (cf[i] != 0) continue;
-            all_decided = false;
+            any_undecided = 1;
             bool is_max = true;
             for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
                 const int32_t j = A.cv[k];
@@ -386,14 +407,17 @@ void coarsen_pmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf,
                     break;
                 }
             }
-            if (is_max) cf[i] = 1;
+            if (is_max) nxt[i] = 1;
         }
+        cf = nxt;
+#pragma omp parallel for schedule(static, 4096)
         for (int32_t i = 0; i < n; ++i) {
             if (cf[i] != 0) continue;
             for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
-                if (s[k] && cf[A.cv[k]] == 1) { cf[i] = -1; break; }
+                if (s[k] && cf[A.cv[k]] == 1) { nxt[i] = -1; break; }
         }
-        if (all_decided) break;
+        cf = nxt;
+        if (!any_undecided) break;
     }
     for (auto& c : cf)
         if (c == 0) c = 1;
@@ -481,44 +505,65 @@ void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& r
     }
     // Neighbour set of C point a (local ids): pairs of strong C neighbours of a
     // common row, plus direct strong C-C connections in both directions.
+    // Built per node in parallel (count, then fill), each list in the order
+    // the sequential construction appends it.
     std::vector<int64_t> g_off(m + 1, 0);
     std::vector<int32_t> g;
-    g.reserve((size_t)m * 16);
-    std::vector<int32_t> mark(m, -1);
-    for (int32_t a = 0; a < m; ++a) {
-        auto add = [&](int32_t b) {
-            if (b != a && mark[b] != a) { mark[b] = a; g.push_back(b); }
-        };
-        const int32_t ia = c_idx[a];
-        for (int64_t t = sc_off[ia]; t < sc_off[ia + 1]; ++t) add(sc[t]);
-        for (int64_t t = tr_off[a]; t < tr_off[a + 1]; ++t) {
-            const int32_t i = tr[t];
-            for (int64_t u = sc_off[i]; u < sc_off[i + 1]; ++u) add(sc[u]);
-            if (cf1[i] == 1) add(c_loc[i]);
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            for (int32_t a = 0; a < m; ++a) g_off[a + 1] += g_off[a];
+            g.resize(g_off[m]);
         }
-        g_off[a + 1] = (int64_t)g.size();
+#pragma omp parallel
+        {
+            std::vector<int32_t> mark(m, -1);
+#pragma omp for schedule(dynamic, 1024)
+            for (int32_t a = 0; a < m; ++a) {
+                int64_t cnt = 0;
+                const int64_t base = pass == 1 ? g_off[a] : 0;
+                auto add = [&](int32_t b) {
+                    if (b != a && mark[b] != a) {
+                        mark[b] = a;
+                        if (pass == 1) g[base + cnt] = b;
+                        ++cnt;
+                    }
+                };
+                const int32_t ia = c_idx[a];
+                for (int64_t t = sc_off[ia]; t < sc_off[ia + 1]; ++t) add(sc[t]);
+                for (int64_t t = tr_off[a]; t < tr_off[a + 1]; ++t) {
+                    const int32_t i = tr[t];
+                    for (int64_t u = sc_off[i]; u < sc_off[i + 1]; ++u) add(sc[u]);
+                    if (cf1[i] == 1) add(c_loc[i]);
+                }
+                if (pass == 0) g_off[a + 1] = cnt;
+            }
+        }
     }
     std::vector<double> measure(m);
     for (int32_t a = 0; a < m; ++a) measure[a] = (double)(g_off[a + 1] - g_off[a]) + rng.uniform();
-    std::vector<int8_t> cf2(m, 0);
+    std::vector<int8_t> cf2(m, 0), nxt(m, 0);  // snapshot sweeps (see the note at the top)
     for (int64_t iter = 0; iter <= (int64_t)m; ++iter) {
-        bool all_decided = true;
+        int any_undecided = 0;
+#pragma omp parallel for schedule(static, 4096) reduction(| : any_undecided)
         for (int32_t a = 0; a < m; ++a) {
             if (cf2[a] != 0) continue;
-            all_decided = false;
+            any_undecided = 1;
             bool is_max = true;
             for (int64_t t = g_off[a]; t < g_off[a + 1]; ++t) {
                 const int32_t b = g[t];
                 if (cf2[b] != -1 && measure[b] > measure[a]) { is_max = false; break; }
             }
-            if (is_max) cf2[a] = 1;
+            if (is_max) nxt[a] = 1;
         }
+        cf2 = nxt;
+#pragma omp parallel for schedule(static, 4096)
         for (int32_t a = 0; a < m; ++a) {
             if (cf2[a] != 0) continue;
             for (int64_t t = g_off[a]; t < g_off[a + 1]; ++t)
-                if (cf2[g[t]] == 1) { cf2[a] = -1; break; }
+                if (cf2[g[t]] == 1) { nxt[a] = -1; break; }
         }
-        if (all_decided) break;
+        cf2 = nxt;
+        if (!any_undecided) break;
     }
     for (auto& c : cf2)
         if (c == 0) c = 1;
