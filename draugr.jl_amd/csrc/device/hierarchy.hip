@@ -378,6 +378,46 @@ template <> constexpr int pidx<double>() { return 0; }
 template <> constexpr int pidx<float>() { return 1; }
 template <> constexpr int pidx<bf16_t>() { return 2; }
 
+// ── One-pass sweeps for the small levels ────────────────────────────────────
+// On a small level every pass costs about one launch's latency, so two
+// passes in one launch halve it, at the price of recomputing values a
+// neighbour would otherwise read from memory.  The down sweep is k_down_fused;
+// the up sweep is a sparse pass over A whose gathers form the prolonged
+// iterate on the fly (XProlong / EpiJacobiP).  Both repeat the separate
+// passes' arithmetic exactly (same products, same order), so the results are
+// the same bits; the autotuner decides per level whether they pay.
+//
+// Down: x1 = (w dinv) b for every row (the first Jacobi step from zero) and,
+// for every coarse row I, bc_I = sum_i P_iI r_i over its fine rows with
+// r_i = b_i - sum_k a_ik x1_k formed on the fly (a fine row is formed once
+// per coarse row it feeds).  When x1c is set, the coarse level's own first
+// step x1c = (w dinv_c) bc is written too.
+template <typename TV, typename TP, typename TX>
+__global__ __launch_bounds__(kBlock) void k_down_fused(int n, int nc, const int* __restrict__ arp,
+                                                       const int* __restrict__ acv, const TV* __restrict__ av,
+                                                       const int* __restrict__ trp, const int* __restrict__ tcv,
+                                                       const TP* __restrict__ tv, const TX* __restrict__ b,
+                                                       const TX* __restrict__ d, TX w, TX* __restrict__ x1,
+                                                       TX* __restrict__ bc, const TX* __restrict__ dc,
+                                                       TX* __restrict__ x1c) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < n) x1[t] = (w * d[t]) * b[t];
+    if (t >= nc) return;
+    TX acc = 0;
+    for (int q = trp[t]; q < trp[t + 1]; ++q) {
+        const int i = tcv[q];
+        TX s = 0;
+        for (int k = arp[i]; k < arp[i + 1]; ++k) {
+            const int j = acv[k];
+            s = s + ld<TX, TV>(av, k) * ((w * d[j]) * b[j]);
+        }
+        const TX ri = b[i] - s;
+        acc = acc + ld<TX, TP>(tv, q) * ri;
+    }
+    bc[t] = acc;
+    if (x1c) x1c[t] = (w * dc[t]) * acc;
+}
+
 template <typename TV, typename TX, class XF, class EPI>
 void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
     if (kind == kOpRowSeq) launch_rowseq<TV, TX>(M.nrow, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
@@ -451,7 +491,8 @@ bool is_gs(Smoother s) { return s == Smoother::ColoredGS || s == Smoother::L1Col
 
 // ─────────────────────────────────────────────────────────────────────────────
 DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
-    : smoother_(cfg.smoother), omega_(cfg.jacobi_omega), seed_(cfg.seed), forced_kind_(cfg.kernel) {
+    : smoother_(cfg.smoother), omega_(cfg.jacobi_omega), seed_(cfg.seed), forced_kind_(cfg.kernel),
+      forced_fuse_(cfg.fuse) {
     if (!(smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || is_gs(smoother_) ||
           smoother_ == Smoother::SPAI0 || smoother_ == Smoother::Chebyshev || smoother_ == Smoother::ILU0))
         throw Error("smoother not available in the HIP backend (supported: jacobi, l1_jacobi, colored_gs, "
@@ -753,7 +794,7 @@ bool DeviceHierarchy::zero_step_is_map(int l, int pre) const {
     if (l < 0 || l >= nlevels() || pre < 1) return false;
     if (!(smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || smoother_ == Smoother::SPAI0))
         return false;
-    return !(pre == 1 && L_[l].fuse_zero[pidx<TV>()]);
+    return !(pre == 1 && (L_[l].fuse_zero[pidx<TV>()] || L_[l].fuse_down[pidx<TV>()]));
 }
 
 // x1_ready: alt already holds the first step from a zero guess (written by
@@ -839,6 +880,7 @@ void DeviceHierarchy::tune() {
         for (auto& D : L_) {
             D.A.kind[pi] = D.Pt.kind[pi] = D.P.kind[pi] = forced_kind_;
             D.fuse_zero[pi] = true;
+            D.fuse_down[pi] = D.fuse_up[pi] = forced_fuse_ == 1;
         }
         tuned_[pi] = true;
         return;
@@ -905,6 +947,30 @@ void DeviceHierarchy::tune() {
             op<TV, TX>(D.A, XVec<TX>{s1}, EpiResid<TX>{bin, s2}, st);
         });
         D.fuse_zero[pi] = fused < split;
+        // fused down (zero step + residual + restriction) and up (prolongation +
+        // first post-smoothing step) passes against their separate launches
+        D.fuse_down[pi] = D.fuse_up[pi] = false;
+        const bool jac = smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi ||
+                         smoother_ == Smoother::SPAI0;
+        if (jac) {
+            const float restrict_t = timeit([&] { op<TP, TX>(D.Pt, XVec<TX>{s2}, EpiStore<TX>{s1}, st); });
+            const int nb = (std::max(n, nc) + kBlock - 1) / kBlock;
+            const float down_t = timeit([&] {
+                hipLaunchKernelGGL((k_down_fused<TV, TP, TX>), dim3(nb), dim3(kBlock), 0, st, n, nc, D.A.rp.p,
+                                   D.A.cv.p, vals<TV>(D.A), D.Pt.rp.p, D.Pt.cv.p, vals<TP>(D.Pt), bin, d, w, s1, s2,
+                                   (const TX*)nullptr, (TX*)nullptr);
+            });
+            D.fuse_down[pi] = forced_fuse_ >= 0 ? forced_fuse_ == 1 : down_t < std::min(fused, split) + restrict_t;
+            const float sep_up = timeit([&] {
+                op<TP, TX>(D.P, XVec<TX>{xcv}, EpiAdd<TX>{s2}, st);
+                op<TV, TX>(D.A, XVec<TX>{s2}, EpiJacobi<TX>{s2, bin, d, s1, w}, st);
+            });
+            const XProlong<TP, TX> xpf{s2, D.P.rp.p, D.P.cv.p, vals<TP>(D.P), xcv};
+            const float up_t = timeit([&] {
+                op<TV, TX>(D.A, xpf, EpiJacobiP<TP, TX>{xpf, bin, d, s1, w}, st);
+            });
+            D.fuse_up[pi] = forced_fuse_ >= 0 ? forced_fuse_ == 1 : up_t < sep_up;
+        }
     }
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
     DRAUGR_HIP_CHECK(hipEventDestroy(e0));
@@ -921,8 +987,9 @@ std::string DeviceHierarchy::tune_report() const {
         for (int l = 0; l < nlevels(); ++l) {
             const DevLevel& D = L_[l];
             const char* kn[kOpKinds] = {"stream", "rowseq", "pipe", "stream16"};
-            std::snprintf(buf, sizeof buf, "%s level %d: A=%s Pt=%s P=%s zero-guess=%s\n", names[p], l,
-                          kn[D.A.kind[p]], kn[D.Pt.kind[p]], kn[D.P.kind[p]], D.fuse_zero[p] ? "fused" : "split");
+            std::snprintf(buf, sizeof buf, "%s level %d: A=%s Pt=%s P=%s zero-guess=%s down=%s up=%s\n", names[p], l,
+                          kn[D.A.kind[p]], kn[D.Pt.kind[p]], kn[D.P.kind[p]], D.fuse_zero[p] ? "fused" : "split",
+                          D.fuse_down[p] ? "one-pass" : "separate", D.fuse_up[p] ? "one-pass" : "separate");
             out += buf;
         }
     }
@@ -1024,22 +1091,34 @@ template <typename TV, typename TX>
 void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const CycleOpts& o, hipStream_t st,
                                   bool x1_ready) {
     typedef typename TransferT<TV>::type TP;
+    const int pi = pidx<TV>();
     DevLevel& D = L_[l];
     TX* xcur = x;
     TX* xalt = t_of<TX>(D);
     TX* r = r_of<TX>(D);
     TX* xc = xc_of<TX>(D);
     TX* bc = bc_of<TX>(D);
-    bool resid_done = false;
-    smooth<TV, TX>(l, xcur, xalt, b, o.pre, zero, true, resid_done, st, x1_ready && zero);
-    if (!resid_done) op<TV, TX>(D.A, XVec<TX>{xcur}, EpiResid<TX>{b, r}, st);
+    const bool jac = smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || smoother_ == Smoother::SPAI0;
     // the child's first step from zero rides along with the restriction
     const bool child_x1 = zero_step_is_map<TV>(l + 1, o.pre);
-    if (child_x1) {
-        DevLevel& C = L_[l + 1];
-        op<TP, TX>(D.Pt, XVec<TX>{r}, EpiStoreJac0<TX>{bc, d_of<TX>(C), t_of<TX>(C), (TX)weight()}, st);
+    TX* x1c = child_x1 ? t_of<TX>(L_[l + 1]) : nullptr;
+    const TX* dc = child_x1 ? d_of<TX>(L_[l + 1]) : nullptr;
+    if (zero && jac && o.pre == 1 && D.fuse_down[pi]) {
+        // first step from zero, residual and restriction in one launch
+        const int n = D.A.nrow, nc = D.P.ncol;
+        const int nb = (std::max(n, nc) + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL((k_down_fused<TV, TP, TX>), dim3(nb), dim3(kBlock), 0, st, n, nc, D.A.rp.p, D.A.cv.p,
+                           vals<TV>(D.A), D.Pt.rp.p, D.Pt.cv.p, vals<TP>(D.Pt), b, d_of<TX>(D), (TX)weight(), xalt,
+                           bc, dc, x1c);
+        std::swap(xcur, xalt);
     } else {
-        op<TP, TX>(D.Pt, XVec<TX>{r}, EpiStore<TX>{bc}, st);
+        bool resid_done = false;
+        smooth<TV, TX>(l, xcur, xalt, b, o.pre, zero, true, resid_done, st, x1_ready && zero);
+        if (!resid_done) op<TV, TX>(D.A, XVec<TX>{xcur}, EpiResid<TX>{b, r}, st);
+        if (child_x1)
+            op<TP, TX>(D.Pt, XVec<TX>{r}, EpiStoreJac0<TX>{bc, dc, x1c, (TX)weight()}, st);
+        else
+            op<TP, TX>(D.Pt, XVec<TX>{r}, EpiStore<TX>{bc}, st);
     }
     if (l + 1 < nlevels()) {
         const int reps = o.w_cycle ? 2 : 1;
@@ -1047,9 +1126,17 @@ void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const Cy
     } else {
         coarse_solve<TX>(xc, bc, st);
     }
-    op<TP, TX>(D.P, XVec<TX>{xc}, EpiAdd<TX>{xcur}, st);
     bool z = false, dummy = false;
-    smooth<TV, TX>(l, xcur, xalt, b, o.post, z, false, dummy, st);
+    if (jac && o.post >= 1 && D.fuse_up[pi]) {
+        // prolongation and the first post-smoothing step in one launch
+        const XProlong<TP, TX> xp{xcur, D.P.rp.p, D.P.cv.p, vals<TP>(D.P), xc};
+        op<TV, TX>(D.A, xp, EpiJacobiP<TP, TX>{xp, b, d_of<TX>(D), xalt, (TX)weight()}, st);
+        std::swap(xcur, xalt);
+        smooth<TV, TX>(l, xcur, xalt, b, o.post - 1, z, false, dummy, st);
+    } else {
+        op<TP, TX>(D.P, XVec<TX>{xc}, EpiAdd<TX>{xcur}, st);
+        smooth<TV, TX>(l, xcur, xalt, b, o.post, z, false, dummy, st);
+    }
     if (xcur != x) {
         const TX* s = xcur;
         launch_map(D.A.nrow, [=] __device__(int64_t i) { x[i] = s[i]; }, st);

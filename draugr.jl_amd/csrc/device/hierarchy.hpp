@@ -60,6 +60,8 @@ struct DevLevel {
     std::vector<int32_t> color_off;  // host copy (loop control)
     DBuf<int32_t> color_order;
     bool fuse_zero[3] = {false, false, false};  // zero-guess step fused with its residual (autotuned)
+    bool fuse_down[3] = {false, false, false};  // zero-guess step + residual + restriction in one pass (autotuned)
+    bool fuse_up[3] = {false, false, false};    // prolongation + first post-smoothing step in one pass (autotuned)
     // workspaces: residual, ping-pong smoother buffer, coarse x and b
     DBuf<double> r64, t64, xc64, bc64;
     DBuf<float> r32, t32, xc32, bc32;
@@ -182,6 +184,7 @@ private:
     double omega_;
     uint64_t seed_ = 0;
     int forced_kind_ = -1;  // config "kernel": every sparse pass uses this kernel (no tuning)
+    int forced_fuse_ = -1;  // config "fuse": -1 tuned, 0 never, 1 always (Jacobi-type smoothers)
     bool have32_ = false, have16_ = false;
     bool tuned_[3] = {false, false, false};
     // solver workspaces at the finest size

@@ -108,6 +108,28 @@ template <typename TX> struct EpiStoreJac0 {
     }
     __device__ void total(int i, TX s) const { fin(i, s, pre(i)); }
 };
+// Prolongation folded into the following smoothing pass: the prolonged
+// iterate xp_k = x_k + sum_J P_kJ xc_J (EpiAdd's order) formed on the fly for
+// every column k the pass gathers, and for the row itself in EpiJacobiP.
+template <typename TP, typename TX> struct XProlong {
+    const TX* x; const int* prp; const int* pcv; const TP* pv; const TX* xc;
+    __device__ TX operator()(int k) const {
+        TX a = x[k];
+        const int e = prp[k + 1];
+        for (int q = prp[k]; q < e; ++q) a = a + ld<TX, TP>(pv, q) * xc[ldi(pcv, q)];
+        return a;
+    }
+};
+// Jacobi step from the prolonged iterate: xo = xp + (w d)(b - A xp).
+template <typename TP, typename TX> struct EpiJacobiP {
+    XProlong<TP, TX> xp; const TX* b; const TX* d; TX* xo; TX w;
+    struct Pre { TX x, b, d; };
+    __device__ Pre pre(int i) const { return {xp(i), b[i], d[i]}; }
+    __device__ TX init(const Pre& q) const { return q.b; }
+    __device__ TX acc(TX a, TX p) const { return a - p; }
+    __device__ void fin(int i, TX a, const Pre& q) const { xo[i] = q.x + (w * q.d) * a; }
+    __device__ void total(int i, TX s) const { xo[i] = xp(i) + (w * d[i]) * (b[i] - s); }
+};
 template <typename TX> struct EpiResid {          // r = b - A x   (src/cycle.jl:32-43)
     const TX* b; TX* r;
     struct Pre { TX b; };

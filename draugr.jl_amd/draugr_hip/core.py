@@ -11,6 +11,7 @@ current stream).
 """
 from __future__ import annotations
 
+import collections
 import ctypes
 from typing import Optional, Tuple
 
@@ -61,9 +62,9 @@ def as_csr(A) -> CSR:
 class _Config:
     """A C-ABI config handle made from an AMGConfig (freed with the object)."""
 
-    def __init__(self, config: AMGConfig):
-        self.handle = check(_lib.lib().draugr_amg_config_from_json(config.to_json().encode()),
-                            "draugr_amg_config_from_json")
+    def __init__(self, config: AMGConfig, json_text: Optional[str] = None):
+        text = json_text if json_text is not None else config.to_json()
+        self.handle = check(_lib.lib().draugr_amg_config_from_json(text.encode()), "draugr_amg_config_from_json")
 
     def __del__(self):
         try:
@@ -221,8 +222,26 @@ def upload(h: AMGHierarchy) -> AMGHierarchy:
     return h
 
 
+_HANDLES: "collections.OrderedDict[str, _Config]" = collections.OrderedDict()
+
+
+def _cached_config(config: AMGConfig) -> _Config:
+    """Config handle of the per-call entry points, reused for equal configs
+    (a few dozen kept) so that a cycle or solve call does not parse JSON."""
+    key = config.to_json()
+    c = _HANDLES.get(key)
+    if c is None:
+        c = _Config(config, key)
+        _HANDLES[key] = c
+        if len(_HANDLES) > 32:
+            _HANDLES.popitem(last=False)
+    else:
+        _HANDLES.move_to_end(key)
+    return c
+
+
 def _vec_call(kind, x, b, h: AMGHierarchy, config: AMGConfig, tol=0.0, maxiter=0, zero_guess=False):
-    c = _Config(config)
+    c = _cached_config(config)
     L = _lib.lib()
     rel = ctypes.c_double(0.0)
     if len(x) != h.n or len(b) != h.n:

@@ -356,3 +356,46 @@ def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
     _, it, rel = dh.pcg_solve_(z, torch.tensor(b, device="cuda"), h, dataclasses.replace(c, precision="fp32"),
                               tol=1e-8)
     assert rel < 1e-8
+
+
+@pytest.mark.parametrize("name", ["hypre3d", "agg2d"])
+@pytest.mark.parametrize("steps", [(1, 1), (2, 3), (1, 2)])
+def test_one_pass_sweeps_are_bit_exact(dh, orc, torch_cuda, name, steps):
+    """A level's first smoothing step from zero + residual + restriction in one
+    pass, and its prolongation + first post-smoothing step in one pass
+    (config fuse = "on", every level), give the oracle's FP64 cycle bit for
+    bit and the separate passes' FP32 / BF16 cycles and PCG bit for bit,
+    for V- and W-cycles."""
+    import dataclasses
+    torch = torch_cuda
+    A, cfg, sm = _case(dh, name)
+    cfg = dataclasses.replace(cfg, pre_smoothing_steps=steps[0], post_smoothing_steps=steps[1])
+    b = torch.tensor(dh.problems.splitmix_normal(SEED + 960, A.n), device="cuda")
+    x0 = torch.tensor(dh.problems.splitmix_normal(SEED + 961, A.n), device="cuda")
+
+    def run(c):
+        h = dh.amg_setup(A, c)
+        out = []
+        for prec in ("fp64", "fp32", "bf16"):
+            for cyc in ("V", "W"):
+                cp = dataclasses.replace(c, precision=prec, cycle_type=cyc)
+                x = x0.clone()
+                dh.amg_cycle_(x, b, h, cp)
+                z = torch.zeros_like(b)
+                dh.amg_cycle_(z, b, h, cp, zero_guess=True)
+                out.append((x.cpu().numpy(), z.cpu().numpy()))
+            w = torch.zeros_like(b)
+            _, it, rel = dh.pcg_solve_(w, b, h, dataclasses.replace(c, precision=prec), tol=1e-8)
+            out.append((w.cpu().numpy(), np.array([it, rel])))
+        return h, out
+
+    h_on, on = run(dataclasses.replace(cfg, fuse="on"))
+    assert "down=one-pass up=one-pass" in h_on.tune_report()
+    _, off = run(dataclasses.replace(cfg, fuse="off"))
+    for a, c in zip(on, off):
+        for u, v in zip(a, c):
+            assert u.tobytes() == v.tobytes(), (name, steps)
+    # and the oracle's FP64 V-cycle from a nonzero guess and from zero
+    oh = oracle_of(h_on, orc, sm, pre=steps[0], post=steps[1])
+    assert np.array_equal(on[0][0], oh.cycle(x0.cpu().numpy(), b.cpu().numpy()))
+    assert np.array_equal(on[0][1], oh.cycle(np.zeros(A.n), b.cpu().numpy()))
