@@ -13,9 +13,10 @@
 //     by count ranks |w| descending with ties kept in C-hat order, as a stable
 //     sort does, and the rescale sums dropped weights in C-hat order;
 //   * entries are written in ascending column order.
-// A row whose C-hat would exceed kChatCap is computed by the host's ext_i_row
-// instead (flagged by the counting pass), so no row is ever truncated by the
-// device's capacity.
+// A row whose C-hat would exceed kChatCap, or that is (or has a strong F
+// neighbour that is) longer than kRowCap entries, is computed by the host's
+// ext_i_row instead (flagged by the counting pass): no row is ever truncated
+// by the device's capacity, and a thread's work stays bounded.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -31,6 +32,7 @@ namespace {
 
 constexpr int kIB = 256;
 constexpr int kChatCap = 64;
+constexpr int kRowCap = 256;  // longest row (own or strong F neighbour's) a device thread takes
 
 inline unsigned grid_of(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kIB - 1) / kIB); }
 
@@ -88,6 +90,10 @@ __global__ __launch_bounds__(kIB) void k_ext_i(int32_t n, const int32_t* __restr
     double pdata[kChatCap];
     int nch = 0;
     const int32_t r0 = rp[i], r1 = rp[i + 1];
+    if (r1 - r0 > kRowCap) {
+        if (!WRITE) cnt[i] = -1;
+        return;
+    }
     // phase 1
     for (int32_t k = r0; k < r1; ++k) {
         const int32_t j = cv[k];
@@ -98,6 +104,10 @@ __global__ __launch_bounds__(kIB) void k_ext_i(int32_t n, const int32_t* __restr
                 chat[nch++] = j;
             }
         } else if (cf[j] == -1) {
+            if (rp[j + 1] - rp[j] > kRowCap) {
+                if (!WRITE) cnt[i] = -1;
+                return;
+            }
             for (int32_t k2 = rp[j]; k2 < rp[j + 1]; ++k2) {
                 const int32_t m = cv[k2];
                 if (m != j && s[k2] && cf[m] == 1 && find_pos(chat, nch, m) < 0) {
