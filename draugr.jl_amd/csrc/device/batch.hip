@@ -26,14 +26,33 @@ constexpr int kChunks = 16;  // row groups per block of the PCG reductions
 
 __device__ __forceinline__ size_t kidx(int i, int j, int ks) { return ((size_t)i << ks) + (size_t)j; }
 
+// four consecutive right-hand sides of one row (16- / 32-byte aligned: K is
+// a multiple of 4 and vectors start on 256-byte boundaries)
+__device__ __forceinline__ void ld4(const float* p, float (&o)[4]) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ __forceinline__ void ld4(const double* p, double (&o)[4]) {
+    const double2 a = *reinterpret_cast<const double2*>(p);
+    const double2 c = *reinterpret_cast<const double2*>(p + 2);
+    o[0] = a.x; o[1] = a.y; o[2] = c.x; o[3] = c.y;
+}
+
 // ── K-wide operands and epilogues (same arithmetic as kernels.hpp) ──────────
 template <typename TX> struct KX {
     const TX* x; int ks;
     __device__ TX operator()(int c, int j) const { return x[kidx(c, j, ks)]; }
+    __device__ void get4(int c, int j0, TX (&o)[4]) const { ld4(x + kidx(c, j0, ks), o); }
 };
 template <typename TX> struct KXZero {  // x1_c = w d_c b_c (first Jacobi step from zero)
     const TX* d; const TX* b; TX w; int ks;
     __device__ TX operator()(int c, int j) const { return (w * d[c]) * b[kidx(c, j, ks)]; }
+    __device__ void get4(int c, int j0, TX (&o)[4]) const {
+        const TX t = w * d[c];
+        ld4(b + kidx(c, j0, ks), o);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) o[u] = t * o[u];
+    }
 };
 template <typename TX> struct KStore {
     TX* y; int ks;
@@ -86,9 +105,43 @@ __global__ __launch_bounds__(kBlock) void k_kpass(int n, int ks, const int* __re
     epi.fin(i, j, a);
 }
 
+// K >= 4: one thread per (row, four consecutive right-hand sides), which
+// gathers its four x values with one vector load; each (row, rhs) is still
+// summed on its own in row order.
+template <typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_kpass4(int n, int ks, const int* __restrict__ rp,
+                                                   const int* __restrict__ cv, const TV* __restrict__ av, XF xf,
+                                                   EPI epi) {
+    const int q = ks - 2;  // log2(threads per row)
+    const int b = gridDim.x >= 64 ? xcd_block() : (int)blockIdx.x;
+    const int i = (b << (8 - q)) + ((int)threadIdx.x >> q);
+    const int j0 = ((int)threadIdx.x & ((1 << q) - 1)) << 2;
+    if (i >= n) return;
+    TX a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = epi.init(i, j0 + u);
+    const int e = rp[i + 1];
+#pragma unroll 4
+    for (int k = rp[i]; k < e; ++k) {
+        const TX v = ld<TX, TV>(av, k);
+        TX g[4];
+        xf.get4(ldi(cv, k), j0, g);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = epi.acc(a[u], v * g[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) epi.fin(i, j0 + u, a[u]);
+}
+
 template <typename TV, typename TX, class XF, class EPI>
 void kpass(int n, int ks, const int* rp, const int* cv, const TV* av, XF xf, EPI epi, hipStream_t st) {
     if (n <= 0) return;
+    if (ks >= 2) {
+        const int rows = kBlock >> (ks - 2);
+        const int nb = (n + rows - 1) / rows;
+        hipLaunchKernelGGL((k_kpass4<TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, n, ks, rp, cv, av, xf, epi);
+        return;
+    }
     const int rows = kBlock >> ks;
     const int nb = (n + rows - 1) / rows;
     hipLaunchKernelGGL((k_kpass<TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, n, ks, rp, cv, av, xf, epi);
