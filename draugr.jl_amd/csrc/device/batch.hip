@@ -256,6 +256,73 @@ __global__ __launch_bounds__(kBlock) void k_kpq(int ks, const int* __restrict__ 
     kblock_sum(dot, sh, ks, part, b);
 }
 
+// K >= 4: the same pass with one thread per (row, four consecutive rhs) and
+// vector gathers.  Each (row, rhs) product sum is formed as in k_kpq; the
+// block's partial p.q per rhs is summed in another (fixed) order.
+template <typename TZ>
+__global__ __launch_bounds__(kBlock) void k_kpq4(int ks, const int* __restrict__ br0, const int* __restrict__ br1,
+                                                 const int* __restrict__ bsys, const int* __restrict__ rp,
+                                                 const int* __restrict__ cv, const double* __restrict__ av,
+                                                 const TZ* __restrict__ z, const double* __restrict__ pa,
+                                                 double* __restrict__ pb, double* __restrict__ q,
+                                                 const double* __restrict__ beta, double* __restrict__ part) {
+    __shared__ double sh[kBlock * 4];
+    const int qs = ks - 2;                   // log2(threads per row)
+    const int lanes = kBlock >> qs;          // rows in flight per block
+    const int K = 1 << ks;
+    const int b = blockIdx.x;
+    const int r1 = br1[b], s = bsys[b];
+    const int lr = (int)threadIdx.x >> qs;
+    const int j0 = ((int)threadIdx.x & ((1 << qs) - 1)) << 2;
+    double bt[4], dot[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        bt[u] = beta[kidx(s, j0 + u, ks)];
+        dot[u] = 0.0;
+    }
+    for (int i = br0[b] + lr; i < r1; i += lanes) {
+        const size_t ii = kidx(i, j0, ks);
+        TZ zi[4];
+        double pi4[4], pn[4], a[4];
+        ld4(z + ii, zi);
+        ld4(pa + ii, pi4);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            pn[u] = (double)zi[u] + bt[u] * pi4[u];
+            a[u] = 0.0;
+        }
+        const int e = rp[i + 1];
+        for (int k = rp[i]; k < e; ++k) {
+            const size_t cc = kidx(cv[k], j0, ks);
+            const double v = av[k];
+            TZ zc[4];
+            double pc[4];
+            ld4(z + cc, zc);
+            ld4(pa + cc, pc);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] += v * ((double)zc[u] + bt[u] * pc[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            pb[ii + u] = pn[u];
+            q[ii + u] = a[u];
+            dot[u] += pn[u] * a[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sh[lr * K + j0 + u] = dot[u];
+    __syncthreads();
+    for (int h = lanes >> 1; h > 0; h >>= 1) {
+        if (lr < h)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) sh[lr * K + j0 + u] += sh[(lr + h) * K + j0 + u];
+        __syncthreads();
+    }
+    if (lr == 0)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) part[kidx(b, j0 + u, ks)] = sh[j0 + u];
+}
+
 // x += alpha p; r -= alpha q (and the FP32 copy of r); partial r.r
 template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_kupdate(int ks, const int* __restrict__ br0, const int* __restrict__ br1,
@@ -795,9 +862,15 @@ int BatchedHierarchy::pcg(double* x, const double* b, double tol, int maxiter, c
         rz(F_RZ0, st);
         // the two graphs of an iteration (the first in two forms: p ping-pongs)
         auto step1 = [&](double* pold, double* pnew, hipStream_t s) {
-            if (mixed)
+            if (mixed && ks >= 2)
+                hipLaunchKernelGGL(k_kpq4<float>, dim3(nb), dim3(kBlock), 0, s, ks, br0_.p, br1_.p, bs_.p,
+                                   D0.A.rp.p, D0.A.cv.p, D0.A.v64.p, z32, pold, pnew, q, st_beta_.p, part);
+            else if (mixed)
                 hipLaunchKernelGGL(k_kpq<float>, dim3(nb), dim3(kBlock), 0, s, ks, br0_.p, br1_.p, bs_.p, D0.A.rp.p,
                                    D0.A.cv.p, D0.A.v64.p, z32, pold, pnew, q, st_beta_.p, part);
+            else if (ks >= 2)
+                hipLaunchKernelGGL(k_kpq4<double>, dim3(nb), dim3(kBlock), 0, s, ks, br0_.p, br1_.p, bs_.p,
+                                   D0.A.rp.p, D0.A.cv.p, D0.A.v64.p, z, pold, pnew, q, st_beta_.p, part);
             else
                 hipLaunchKernelGGL(k_kpq<double>, dim3(nb), dim3(kBlock), 0, s, ks, br0_.p, br1_.p, bs_.p,
                                    D0.A.rp.p, D0.A.cv.p, D0.A.v64.p, z, pold, pnew, q, st_beta_.p, part);
