@@ -521,14 +521,15 @@ int32_t draugr_hip_cycle_bytes(int32_t handle, int32_t config_handle, double* by
     });
 }
 
-int32_t draugr_hip_op_bytes(int32_t handle, int32_t level, int32_t which, int32_t precision, double* bytes) {
+int32_t draugr_hip_op_bytes(int32_t handle, int32_t level, int32_t which, int32_t precision, int32_t as_stored,
+                            double* bytes) {
     return guarded([&]() -> int32_t {
         auto H = get_hier(handle);
         if (!H) throw Error("Invalid hierarchy handle: " + std::to_string(handle));
         require_device(*H);
         if (which < 0 || which > 2 || precision < 0 || precision > 2 || !bytes)
             throw Error("ArgumentError: which in 0..2, precision in 0..2, bytes non-null");
-        *bytes = H->dev->op_bytes(level, which, (Precision)precision);
+        *bytes = H->dev->op_bytes(level, which, (Precision)precision, as_stored != 0);
         return 0;
     });
 }

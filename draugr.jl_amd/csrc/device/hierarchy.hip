@@ -1465,22 +1465,24 @@ void DeviceHierarchy::resetup(const double* nzval, hipStream_t st) {
 
 // Algorithmic bytes of one cycle from a zero guess, counting every array a
 // kernel must touch once (value size sv, vector size sx, int32 indices).
-// Column index bytes of one pass: 2 per nonzero of a 16-bit block plus the
-// block bases when the tuned kernel reads the 16-bit form, else 4 per nonzero.
+// Column index bytes of one pass as stored: 2 per nonzero of a 16-bit block
+// plus the block bases when the tuned kernel reads the 16-bit form, else 4 per
+// nonzero.  The algorithmic count of the cycle uses 4 per nonzero whatever
+// the form (int32 indices: the work, not the storage, defines it).
 static double index_bytes(const DevCSR& M, int pi) {
     if (M.kind[pi] == kOpStream16 && M.has16)
         return 2.0 * M.nnz16 + 4.0 * (M.nnz - M.nnz16) + 4.0 * M.nblk;
     return 4.0 * M.nnz;
 }
 
-double DeviceHierarchy::op_bytes(int l, int which, Precision p) const {
+double DeviceHierarchy::op_bytes(int l, int which, Precision p, bool as_stored) const {
     if (l < 0 || l >= nlevels()) throw Error("level index out of range");
     const int pi = (int)p;
     const DevLevel& D = L_[l];
     const DevCSR& M = which == 0 ? D.A : (which == 1 ? D.P : D.Pt);
     const double sv = which == 0 ? (p == Precision::FP64 ? 8 : (p == Precision::FP32 ? 4 : 2))
                                  : (p == Precision::FP64 ? 8 : 4);
-    return M.nnz * sv + index_bytes(M, pi) + (M.nrow + 1) * 4.0;
+    return M.nnz * sv + (as_stored ? index_bytes(M, pi) : 4.0 * M.nnz) + (M.nrow + 1) * 4.0;
 }
 
 double DeviceHierarchy::cycle_bytes(const CycleOpts& o) const {
@@ -1489,7 +1491,7 @@ double DeviceHierarchy::cycle_bytes(const CycleOpts& o) const {
     for (int l = 0; l < nlevels(); ++l) {
         const DevLevel& D = L_[l];
         const double n = D.A.nrow, nc = D.P.ncol;
-        const double BA = op_bytes(l, 0, o.prec);
+        const double BA = op_bytes(l, 0, o.prec, false);
         double lv = 0.0;
         if (is_gs(smoother_)) {
             lv += n * sx;  // zero fill
@@ -1513,8 +1515,8 @@ double DeviceHierarchy::cycle_bytes(const CycleOpts& o) const {
             lv += o.post * (BA + 4 * n * sx);
             if ((o.pre + o.post) % 2) lv += 2 * n * sx;
         }
-        lv += op_bytes(l, 2, o.prec) + n * sx + nc * sx;      // restrict
-        lv += op_bytes(l, 1, o.prec) + nc * sx + 2 * n * sx;  // prolong
+        lv += op_bytes(l, 2, o.prec, false) + n * sx + nc * sx;      // restrict
+        lv += op_bytes(l, 1, o.prec, false) + nc * sx + 2 * n * sx;  // prolong
         total += visits * lv;
         if (o.w_cycle) visits *= 2;
     }

@@ -134,10 +134,11 @@ public:
     void export_level_values(int lvl, int which, double* host_dst);  // 0 A vals, 1 dinv, 2 coarse LU, 3 coarsest vals
     int64_t bytes_resident() const;
     double cycle_bytes(const CycleOpts& o) const;  // algorithmic bytes of one V/W cycle (zero guess)
-    // algorithmic bytes of one pass over operator which (0 A, 1 P, 2 P^T) of
-    // level l at precision p: values, column indices in the form the tuned
-    // kernel reads, row pointers (vectors excluded)
-    double op_bytes(int l, int which, Precision p) const;
+    // bytes of one pass over operator which (0 A, 1 P, 2 P^T) of level l at
+    // precision p: values, column indices, row pointers (vectors excluded);
+    // as_stored: indices in the form the tuned kernel reads (16-bit where it
+    // chose that form), else int32 (the algorithmic count)
+    double op_bytes(int l, int which, Precision p, bool as_stored = true) const;
     void ensure_precision(Precision p);
     // Stream ordering between calls: work a call enqueues on stream st waits
     // for everything the previous call enqueued (possibly on another stream),

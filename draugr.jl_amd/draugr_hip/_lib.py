@@ -43,7 +43,7 @@ SIGNATURES = {
     "draugr_hip_level_op": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     "draugr_hip_export_device_values": (_i32, [_i32, _i32, _i32, _vp]),
     "draugr_hip_cycle_bytes": (_i32, [_i32, _i32, _f64p]),
-    "draugr_hip_op_bytes": (_i32, [_i32, _i32, _i32, _i32, _f64p]),
+    "draugr_hip_op_bytes": (_i32, [_i32, _i32, _i32, _i32, _i32, _f64p]),
     "draugr_hip_resident_bytes": (_i32, [_i32, _i64p]),
     "draugr_hip_tune_report": (_i32, [_i32, ctypes.c_char_p, _i32]),
     "draugr_hip_assemble_7pt": (_i32, [_i32, _i32, _vp, _vp, _vp]),

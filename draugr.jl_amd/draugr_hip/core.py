@@ -163,12 +163,14 @@ class AMGHierarchy:
         check(_lib.lib().draugr_hip_cycle_bytes(self.handle, c.handle, ctypes.byref(v)), "draugr_hip_cycle_bytes")
         return v.value
 
-    def op_bytes(self, level: int, which: int, precision: str) -> float:
-        """Algorithmic bytes of one pass over operator ``which`` (0 A, 1 P,
-        2 P^T) of ``level`` at ``precision``, index form as tuned."""
+    def op_bytes(self, level: int, which: int, precision: str, as_stored: bool = True) -> float:
+        """Bytes of one pass over operator ``which`` (0 A, 1 P, 2 P^T) of
+        ``level`` at ``precision``: indices as the tuned kernel stores them
+        (``as_stored``) or int32 (the algorithmic count)."""
         v = ctypes.c_double(0.0)
         p = {"fp64": 0, "fp32": 1, "bf16": 2}[precision]
-        check(_lib.lib().draugr_hip_op_bytes(self.handle, level, which, p, ctypes.byref(v)), "draugr_hip_op_bytes")
+        check(_lib.lib().draugr_hip_op_bytes(self.handle, level, which, p, int(as_stored), ctypes.byref(v)),
+              "draugr_hip_op_bytes")
         return v.value
 
     def tune_report(self) -> str:

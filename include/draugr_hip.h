@@ -149,12 +149,14 @@ int32_t draugr_hip_export_device_values(int32_t handle, int32_t level, int32_t w
 /* Algorithmic bytes of one cycle from a zero guess under the given config,
  * and the HBM bytes the hierarchy holds. */
 int32_t draugr_hip_cycle_bytes(int32_t handle, int32_t config_handle, double *bytes);
-/* Algorithmic bytes of one pass over level `level`'s operator `which` (0 A,
- * 1 P, 2 P^T) at `precision` (0 fp64, 1 fp32, 2 bf16): values, column indices
- * in the form the autotuned kernel reads (16-bit block-relative where it
- * chose that form), row pointers; vectors excluded.  No counterpart in the
- * reference (measurement). */
-int32_t draugr_hip_op_bytes(int32_t handle, int32_t level, int32_t which, int32_t precision, double *bytes);
+/* Bytes of one pass over level `level`'s operator `which` (0 A, 1 P, 2 P^T)
+ * at `precision` (0 fp64, 1 fp32, 2 bf16): values, column indices, row
+ * pointers; vectors excluded.  as_stored = 1: indices in the form the
+ * autotuned kernel reads (16-bit block-relative where it chose that form);
+ * 0: int32 indices (the algorithmic count, also that of
+ * draugr_hip_cycle_bytes).  No counterpart in the reference (measurement). */
+int32_t draugr_hip_op_bytes(int32_t handle, int32_t level, int32_t which, int32_t precision, int32_t as_stored,
+                            double *bytes);
 int32_t draugr_hip_resident_bytes(int32_t handle, int64_t *bytes);
 
 /* ── Differentiation of x = A(k)^-1 b (device pointers) ─────────────────── */
