@@ -381,7 +381,7 @@ template <> constexpr int pidx<bf16_t>() { return 2; }
 // ── One-pass sweeps for the small levels ────────────────────────────────────
 // On a small level every pass costs about one launch's latency, so two
 // passes in one launch halve it, at the price of recomputing values a
-// neighbour would otherwise read from memory.  The down sweep is k_down_fused;
+// neighbour would otherwise read from memory.  The down sweep is k_down_stream;
 // the up sweep is a sparse pass over A whose gathers form the prolonged
 // iterate on the fly (XProlong / EpiJacobiP).  Both repeat the separate
 // passes' arithmetic exactly (same products, same order), so the results are
@@ -390,32 +390,76 @@ template <> constexpr int pidx<bf16_t>() { return 2; }
 // Down: x1 = (w dinv) b for every row (the first Jacobi step from zero) and,
 // for every coarse row I, bc_I = sum_i P_iI r_i over its fine rows with
 // r_i = b_i - sum_k a_ik x1_k formed on the fly (a fine row is formed once
-// per coarse row it feeds).  When x1c is set, the coarse level's own first
-// step x1c = (w dinv_c) bc is written too.
+// per coarse row it feeds).  A block takes consecutive coarse rows with at
+// most kBlock pairs (I, i) and kDownChunk products in all: the products of
+// every pair's row are formed by all threads into LDS, then one thread per
+// pair sums its row in order, then one thread per coarse row sums its pairs.
+// When x1c is set, the coarse level's own first step x1c = (w dinv_c) bc is
+// written too.
+constexpr int kDownChunk = 4096;
+constexpr int64_t kDownMaxNnz = 4 << 20;  // larger levels keep the separate passes
+
 template <typename TV, typename TP, typename TX>
-__global__ __launch_bounds__(kBlock) void k_down_fused(int n, int nc, const int* __restrict__ arp,
-                                                       const int* __restrict__ acv, const TV* __restrict__ av,
-                                                       const int* __restrict__ trp, const int* __restrict__ tcv,
-                                                       const TP* __restrict__ tv, const TX* __restrict__ b,
-                                                       const TX* __restrict__ d, TX w, TX* __restrict__ x1,
-                                                       TX* __restrict__ bc, const TX* __restrict__ dc,
-                                                       TX* __restrict__ x1c) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t < n) x1[t] = (w * d[t]) * b[t];
-    if (t >= nc) return;
-    TX acc = 0;
-    for (int q = trp[t]; q < trp[t + 1]; ++q) {
-        const int i = tcv[q];
-        TX s = 0;
-        for (int k = arp[i]; k < arp[i + 1]; ++k) {
-            const int j = acv[k];
-            s = s + ld<TX, TV>(av, k) * ((w * d[j]) * b[j]);
-        }
-        const TX ri = b[i] - s;
-        acc = acc + ld<TX, TP>(tv, q) * ri;
+__global__ __launch_bounds__(kBlock) void k_down_stream(int nb, int n, const int* __restrict__ dblk,
+                                                        const int* __restrict__ arp, const int* __restrict__ acv,
+                                                        const TV* __restrict__ av, const int* __restrict__ trp,
+                                                        const int* __restrict__ tcv, const TP* __restrict__ tv,
+                                                        const TX* __restrict__ b, const TX* __restrict__ d, TX w,
+                                                        TX* __restrict__ x1, TX* __restrict__ bc,
+                                                        const TX* __restrict__ dc, TX* __restrict__ x1c) {
+    __shared__ TX prod[kDownChunk];
+    __shared__ int woff[kBlock + 1];
+    __shared__ int qrow[kBlock], qbase[kBlock];
+    __shared__ TX rq[kBlock];
+    const int blk = blockIdx.x;
+    const int t = (int)threadIdx.x;
+    for (int i = blk * kBlock + t; i < n; i += nb * kBlock) x1[i] = (w * d[i]) * b[i];
+    const int I0 = dblk[blk], I1 = dblk[blk + 1];
+    const int q0 = trp[I0], nq = trp[I1] - q0;
+    int len = 0;
+    if (t < nq) {
+        const int i = tcv[q0 + t];
+        const int s = arp[i];
+        qrow[t] = i;
+        qbase[t] = s;
+        len = arp[i + 1] - s;
     }
-    bc[t] = acc;
-    if (x1c) x1c[t] = (w * dc[t]) * acc;
+    woff[t + 1] = len;
+    if (t == 0) woff[0] = 0;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {  // inclusive scan of the row lengths
+        const int v = t + 1 - off >= 1 ? woff[t + 1 - off] : 0;
+        __syncthreads();
+        woff[t + 1] += v;
+        __syncthreads();
+    }
+    const int W = nq > 0 ? woff[nq] : 0;
+    for (int p = t; p < W; p += kBlock) {
+        int lo = 0, hi = nq - 1;  // the pair whose products hold p: last q with woff[q] <= p
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (woff[mid] <= p) lo = mid;
+            else hi = mid - 1;
+        }
+        const int k = qbase[lo] + (p - woff[lo]);
+        const int j = acv[k];
+        prod[p] = ld<TX, TV>(av, k) * ((w * d[j]) * b[j]);
+    }
+    __syncthreads();
+    if (t < nq) {
+        TX s = 0;
+        for (int p = woff[t]; p < woff[t + 1]; ++p) s = s + prod[p];
+        const TX ri = b[qrow[t]] - s;
+        rq[t] = ld<TX, TP>(tv, q0 + t) * ri;
+    }
+    __syncthreads();
+    const int I = I0 + t;
+    if (I < I1) {
+        TX acc = 0;
+        for (int q = trp[I] - q0; q < trp[I + 1] - q0; ++q) acc = acc + rq[q];
+        bc[I] = acc;
+        if (x1c) x1c[I] = (w * dc[I]) * acc;
+    }
 }
 
 template <typename TV, typename TX, class XF, class EPI>
@@ -786,6 +830,56 @@ void DeviceHierarchy::ensure_precision(Precision p) {
 // Jacobi steps ping-pong between x and alt (no copy back: the caller tracks
 // which buffer holds the iterate).  A first step from a zero guess is
 // x1 = w d b, fused with the residual that follows it when fuse_resid.
+// Blocks of the one-pass down sweep of level l (built once, from the row
+// pointers): consecutive coarse rows with at most kBlock rows, kBlock pairs
+// and kDownChunk products.  False when a coarse row alone exceeds a block, or
+// the level is too large for the one-pass form to pay.
+bool DeviceHierarchy::down_blocks(int l) {
+    DevLevel& D = L_[l];
+    if (D.down_state != 0) return D.down_state > 0;
+    D.down_state = -1;
+    if (D.A.nnz > kDownMaxNnz) return false;
+    const int n = D.A.nrow, nc = D.P.ncol;
+    std::vector<int32_t> arp(n + 1), trp(nc + 1), tcv(D.Pt.nnz);
+    DRAUGR_HIP_CHECK(hipMemcpy(arp.data(), D.A.rp.p, arp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    DRAUGR_HIP_CHECK(hipMemcpy(trp.data(), D.Pt.rp.p, trp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (!tcv.empty())
+        DRAUGR_HIP_CHECK(hipMemcpy(tcv.data(), D.Pt.cv.p, tcv.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int32_t> blk{0};
+    int rows = 0, pairs = 0;
+    int64_t work = 0;
+    for (int I = 0; I < nc; ++I) {
+        const int np = trp[I + 1] - trp[I];
+        int64_t w = 0;
+        for (int q = trp[I]; q < trp[I + 1]; ++q) w += arp[tcv[q] + 1] - arp[tcv[q]];
+        if (np > kBlock || w > kDownChunk) return false;
+        if (rows + 1 > kBlock || pairs + np > kBlock || work + w > kDownChunk) {
+            blk.push_back(I);
+            rows = pairs = 0;
+            work = 0;
+        }
+        ++rows;
+        pairs += np;
+        work += w;
+    }
+    if (nc > 0) blk.push_back(nc);
+    D.ndblk = (int)blk.size() - 1;
+    if (D.ndblk < 1) return false;
+    D.dblk.upload(blk, stream_);
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(stream_));  // blk is pageable and local
+    D.down_state = 1;
+    return true;
+}
+
+template <typename TV, typename TX>
+void DeviceHierarchy::launch_down(int l, const TX* b, TX* x1, TX* bc, const TX* dc, TX* x1c, hipStream_t st) {
+    typedef typename TransferT<TV>::type TP;
+    DevLevel& D = L_[l];
+    hipLaunchKernelGGL((k_down_stream<TV, TP, TX>), dim3(D.ndblk), dim3(kBlock), 0, st, D.ndblk, D.A.nrow, D.dblk.p,
+                       D.A.rp.p, D.A.cv.p, vals<TV>(D.A), D.Pt.rp.p, D.Pt.cv.p, vals<TP>(D.Pt), b, d_of<TX>(D),
+                       (TX)weight(), x1, bc, dc, x1c);
+}
+
 // Whether level l's first pre-smoothing step from a zero guess is the plain
 // map x1 = (w dinv) b (a Jacobi-type smoother not fused with the residual):
 // then the parent's restriction can write x1 along with b.
@@ -880,8 +974,9 @@ void DeviceHierarchy::tune() {
         for (auto& D : L_) {
             D.A.kind[pi] = D.Pt.kind[pi] = D.P.kind[pi] = forced_kind_;
             D.fuse_zero[pi] = true;
-            D.fuse_down[pi] = D.fuse_up[pi] = forced_fuse_ == 1;
+            D.fuse_up[pi] = forced_fuse_ == 1;
         }
+        for (int l = 0; l < nlevels(); ++l) L_[l].fuse_down[pi] = forced_fuse_ == 1 && down_blocks(l);
         tuned_[pi] = true;
         return;
     }
@@ -954,13 +1049,13 @@ void DeviceHierarchy::tune() {
                          smoother_ == Smoother::SPAI0;
         if (jac) {
             const float restrict_t = timeit([&] { op<TP, TX>(D.Pt, XVec<TX>{s2}, EpiStore<TX>{s1}, st); });
-            const int nb = (std::max(n, nc) + kBlock - 1) / kBlock;
-            const float down_t = timeit([&] {
-                hipLaunchKernelGGL((k_down_fused<TV, TP, TX>), dim3(nb), dim3(kBlock), 0, st, n, nc, D.A.rp.p,
-                                   D.A.cv.p, vals<TV>(D.A), D.Pt.rp.p, D.Pt.cv.p, vals<TP>(D.Pt), bin, d, w, s1, s2,
-                                   (const TX*)nullptr, (TX*)nullptr);
-            });
-            D.fuse_down[pi] = forced_fuse_ >= 0 ? forced_fuse_ == 1 : down_t < std::min(fused, split) + restrict_t;
+            if (down_blocks(l)) {
+                const float down_t = timeit([&] {
+                    launch_down<TV, TX>(l, bin, s1, s2, (const TX*)nullptr, (TX*)nullptr, st);
+                });
+                D.fuse_down[pi] =
+                    forced_fuse_ >= 0 ? forced_fuse_ == 1 : down_t < std::min(fused, split) + restrict_t;
+            }
             const float sep_up = timeit([&] {
                 op<TP, TX>(D.P, XVec<TX>{xcv}, EpiAdd<TX>{s2}, st);
                 op<TV, TX>(D.A, XVec<TX>{s2}, EpiJacobi<TX>{s2, bin, d, s1, w}, st);
@@ -989,7 +1084,8 @@ std::string DeviceHierarchy::tune_report() const {
             const char* kn[kOpKinds] = {"stream", "rowseq", "pipe", "stream16"};
             std::snprintf(buf, sizeof buf, "%s level %d: A=%s Pt=%s P=%s zero-guess=%s down=%s up=%s\n", names[p], l,
                           kn[D.A.kind[p]], kn[D.Pt.kind[p]], kn[D.P.kind[p]], D.fuse_zero[p] ? "fused" : "split",
-                          D.fuse_down[p] ? "one-pass" : "separate", D.fuse_up[p] ? "one-pass" : "separate");
+                          D.fuse_down[p] ? "one-pass" : (D.down_state < 0 ? "separate(only)" : "separate"),
+                          D.fuse_up[p] ? "one-pass" : "separate");
             out += buf;
         }
     }
@@ -1105,11 +1201,7 @@ void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const Cy
     const TX* dc = child_x1 ? d_of<TX>(L_[l + 1]) : nullptr;
     if (zero && jac && o.pre == 1 && D.fuse_down[pi]) {
         // first step from zero, residual and restriction in one launch
-        const int n = D.A.nrow, nc = D.P.ncol;
-        const int nb = (std::max(n, nc) + kBlock - 1) / kBlock;
-        hipLaunchKernelGGL((k_down_fused<TV, TP, TX>), dim3(nb), dim3(kBlock), 0, st, n, nc, D.A.rp.p, D.A.cv.p,
-                           vals<TV>(D.A), D.Pt.rp.p, D.Pt.cv.p, vals<TP>(D.Pt), b, d_of<TX>(D), (TX)weight(), xalt,
-                           bc, dc, x1c);
+        launch_down<TV, TX>(l, b, xalt, bc, dc, x1c, st);
         std::swap(xcur, xalt);
     } else {
         bool resid_done = false;

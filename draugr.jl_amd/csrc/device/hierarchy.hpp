@@ -61,6 +61,9 @@ struct DevLevel {
     DBuf<int32_t> color_order;
     bool fuse_zero[3] = {false, false, false};  // zero-guess step fused with its residual (autotuned)
     bool fuse_down[3] = {false, false, false};  // zero-guess step + residual + restriction in one pass (autotuned)
+    int down_state = 0;       // blocks of the one-pass down sweep: 0 not built, 1 built, -1 not possible
+    int ndblk = 0;
+    DBuf<int32_t> dblk;       // coarse-row offsets of its blocks
     bool fuse_up[3] = {false, false, false};    // prolongation + first post-smoothing step in one pass (autotuned)
     // workspaces: residual, ping-pong smoother buffer, coarse x and b
     DBuf<double> r64, t64, xc64, bc64;
@@ -154,6 +157,9 @@ private:
     void smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bool& zero, bool fuse_resid, bool& resid_done,
                 hipStream_t st, bool x1_ready = false);
     template <typename TV> bool zero_step_is_map(int l, int pre) const;
+    bool down_blocks(int l);
+    template <typename TV, typename TX>
+    void launch_down(int l, const TX* b, TX* x1, TX* bc, const TX* dc, TX* x1c, hipStream_t st);
     template <typename TX> void coarse_solve(TX* x, const TX* b, hipStream_t st);
     template <typename TV, typename TX> void cycle_top(TX* x, const TX* b, bool zero_guess, const CycleOpts& o,
                                                        hipStream_t st);
