@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--kernel", default="auto", choices=["auto", "stream", "rowseq", "pipe", "stream16"],
                     help="sparse-pass kernel (config key kernel); a counter run forces the kernel the "
                          "autotuner picks without a profiler, whose timings it would perturb")
+    ap.add_argument("--fuse", default="auto", choices=["auto", "off", "on"],
+                    help="zero-guess and one-pass sweep fusions (config key fuse); fixed, like --kernel, "
+                         "in profiled runs")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "stream_jacobi_level1_128cubed_traffic.md"),
                     help="summary of the FETCH_SIZE / WRITE_SIZE passes holding the measured HBM bytes per launch")
     return ap.parse_args()
@@ -323,8 +326,9 @@ def main():
     A = P.heterogeneous3d(m, logk)
     b_host = P.splitmix_normal(SEED + 2, A.n)
     log("problem %d^3: n=%d nnz=%d (%.1fs)" % (m, A.n, A.nnz, time.perf_counter() - t0))
-    cfg64 = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), kernel=args.kernel)
-    cfg32 = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), precision=args.precision, kernel=args.kernel)
+    cfg64 = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), kernel=args.kernel, fuse=args.fuse)
+    cfg32 = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), precision=args.precision, kernel=args.kernel,
+                                    fuse=args.fuse)
     t0 = time.perf_counter()
     h = dh.amg_setup(A, cfg64)
     t_setup = time.perf_counter() - t0

@@ -970,11 +970,11 @@ template <typename TV, typename TX>
 void DeviceHierarchy::tune() {
     typedef typename TransferT<TV>::type TP;
     const int pi = pidx<TV>();
-    if (forced_kind_ >= 0) {
+    // nothing left to time when both the kernel and the fusions are fixed
+    if (forced_kind_ >= 0 && forced_fuse_ >= 0) {
         for (auto& D : L_) {
             D.A.kind[pi] = D.Pt.kind[pi] = D.P.kind[pi] = forced_kind_;
-            D.fuse_zero[pi] = true;
-            D.fuse_up[pi] = forced_fuse_ == 1;
+            D.fuse_zero[pi] = D.fuse_up[pi] = forced_fuse_ == 1;
         }
         for (int l = 0; l < nlevels(); ++l) L_[l].fuse_down[pi] = forced_fuse_ == 1 && down_blocks(l);
         tuned_[pi] = true;
@@ -1015,6 +1015,10 @@ void DeviceHierarchy::tune() {
         const TX w = (TX)weight();
         float tA[kOpKinds], tP[kOpKinds], tPt[kOpKinds];
         for (int k = 0; k < kOpKinds; ++k) {
+            if (forced_kind_ >= 0) {  // fixed kernel: only the fusions are timed
+                tA[k] = tPt[k] = tP[k] = k == forced_kind_ ? 0.f : 1e30f;
+                continue;
+            }
             const float skip = 1e30f;  // the 16-bit form only where the operator has it
             tA[k] = (k == kOpStream16 && !D.A.has16) ? skip : timeit([&] {
                 op_kind<TV, TX>(k, D.A, XVec<TX>{xin}, EpiJacobi<TX>{xin, bin, d, s1, w}, st);
@@ -1041,7 +1045,7 @@ void DeviceHierarchy::tune() {
             if (!map_folded) launch_map(n, [=] __device__(int64_t i) { s1[i] = (w * d[i]) * bin[i]; }, st);
             op<TV, TX>(D.A, XVec<TX>{s1}, EpiResid<TX>{bin, s2}, st);
         });
-        D.fuse_zero[pi] = fused < split;
+        D.fuse_zero[pi] = forced_fuse_ >= 0 ? forced_fuse_ == 1 : fused < split;
         // fused down (zero step + residual + restriction) and up (prolongation +
         // first post-smoothing step) passes against their separate launches
         D.fuse_down[pi] = D.fuse_up[pi] = false;

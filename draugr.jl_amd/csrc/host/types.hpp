@@ -78,7 +78,7 @@ struct Config {
     Precision precision = Precision::FP64;  // arithmetic of the V/W cycle
     uint64_t seed = 0x5EEDull;              // seed of the setup RNG (PMIS/HMIS measures)
     bool graphs = true;                     // replay the preconditioner cycle from a captured hipGraph
-    int fuse = -1;                          // one-pass down / up sweeps of a level: -1 autotuned, 0 never, 1 always
+    int fuse = -1;                          // zero-guess step + residual, one-pass down / up sweeps: -1 autotuned, 0 never, 1 always
     int kernel = -1;                        // sparse-pass kernel: -1 autotuned, 0 stream, 1 row-sequential, 2 pipelined,
                                             // 3 stream with 16-bit columns
     bool gpu_setup = true;                  // interpolation, coarse operators and triple maps built on the GPU

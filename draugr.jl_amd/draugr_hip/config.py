@@ -171,7 +171,7 @@ class AMGConfig:
     seed: int = 0x5EED
     graphs: bool = True
     kernel: str = "auto"  # sparse-pass kernel: "auto" (autotuned), "stream", "rowseq", "pipe", "stream16"
-    fuse: str = "auto"  # one-pass down / up sweeps of a level: "auto" (autotuned), "off", "on"
+    fuse: str = "auto"  # zero-guess step + residual, one-pass down / up sweeps: "auto" (autotuned), "off", "on"
     gpu_setup: bool = True  # interpolation, coarse operators and triple maps of the setup built on the GPU
     gpu_setup_min_rows: int = 32768  # smaller levels interpolate on the host
 

@@ -40,9 +40,10 @@ const char *draugr_amg_last_error(void);
  * "seed" (setup RNG seed), "graphs" (true: the PCG / mixed-precision preconditioner cycle is
  * captured once into a hipGraph and replayed; false: launched eagerly),
  * "kernel" ("auto" | "stream" | "rowseq" | "pipe" | "stream16"), "fuse" ("auto"
- * | "off" | "on": a level's first smoothing step from zero, residual and
- * restriction in one pass, and its prolongation and first post-smoothing
- * step in one pass; Jacobi-type smoothers), "gpu_setup" (default
+ * | "off" | "on": a level's first smoothing step from zero fused with its
+ * residual, that step, residual and restriction in one pass, and its
+ * prolongation and first post-smoothing step in one pass; Jacobi-type
+ * smoothers; "auto" times each on the level's operators), "gpu_setup" (default
  * true: when the hierarchy is uploaded, each level's extended+i interpolation,
  * coarse operator, triple map and transpose map are built on the GPU,
  * identical to the host's, and the triple map stays in HBM; levels of fewer
