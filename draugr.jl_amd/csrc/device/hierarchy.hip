@@ -95,16 +95,15 @@ void launch_reduce(int64_t n, F f, FIN fin, double* partial, hipStream_t st) {
 // CSR-stream pass whose per-row epilogue also returns a contribution to a
 // global sum (q = A p fused with p.q in PCG).
 template <int PER, bool C16, typename TV, typename TX, class XF, class EPI>
-__global__ __launch_bounds__(kBlock) void k_stream_reduce(const int* __restrict__ blk, const int* __restrict__ bz,
-                                                          const int* __restrict__ rp, const int* __restrict__ cv,
-                                                          const uint16_t* __restrict__ c16,
-                                                          const int* __restrict__ cbase, const TV* __restrict__ av,
+__global__ __launch_bounds__(kBlock) void k_stream_reduce(const BlkHdr* __restrict__ hdr, const int* __restrict__ rp,
+                                                          const int* __restrict__ cv,
+                                                          const uint16_t* __restrict__ c16, const TV* __restrict__ av,
                                                           XF xf, EPI epi, double* partial) {
     __shared__ TX prod[kStreamNnz];
     __shared__ double sh[kBlock];
     double dot = 0.0;
     stream_block<PER, C16, TV, TX>(
-        xcd_block(), blk, bz, rp, cv, c16, cbase, av, xf, prod, [](int) { return NoPre{}; },
+        xcd_block(), hdr, rp, cv, c16, av, xf, prod, [](int) { return NoPre{}; },
         [&](int row, const TX* seg, int len, NoPre) {
             TX a = 0;
             for (int k = 0; k < len; ++k) a += seg[k];
@@ -117,8 +116,8 @@ __global__ __launch_bounds__(kBlock) void k_stream_reduce(const int* __restrict_
 template <typename TV, typename TX, class XF, class EPI, class FIN>
 void launch_stream_reduce(const DevCSR& M, const TV* av, XF xf, EPI epi, FIN fin, double* partial, hipStream_t st) {
 #define DRAUGR_SR(PER, C)                                                                                       \
-    hipLaunchKernelGGL((k_stream_reduce<PER, C, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.blk.p,     \
-                       M.bz.p, M.rp.p, M.cv.p, M.c16.p, M.cbase.p, av, xf, epi, partial)
+    hipLaunchKernelGGL((k_stream_reduce<PER, C, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.hdr.p,     \
+                       M.rp.p, M.cv.p, M.c16.p, av, xf, epi, partial)
     if (M.has16) {
         switch (M.per) {
             case 1: DRAUGR_SR(1, true); break;
@@ -467,9 +466,8 @@ void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
     if (kind == kOpRowSeq) launch_rowseq<TV, TX>(M.nrow, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
     else if (kind == kOpPipe) launch_pipe<TV, TX>(M.nblk, M.per, M.blk.p, M.bz.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
     else if (kind == kOpStream16 && M.has16)
-        launch_stream<TV, TX>(M.nblk, M.per, M.blk.p, M.bz.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st, M.c16.p,
-                              M.cbase.p);
-    else launch_stream<TV, TX>(M.nblk, M.per, M.blk.p, M.bz.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
+        launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st, M.c16.p);
+    else launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
 }
 
 // One pass of operator M with the kernel the autotuner picked for this
@@ -520,12 +518,14 @@ void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
             for (int k = bz[b]; k < bz[b + 1]; ++k) c16[k] = (uint16_t)(h.cv[k] - cbase[b]);
         }
         d.c16.upload(c16, st);
-        d.cbase.upload(cbase, st);
     } else {
         d.c16.release();
-        d.cbase.release();
     }
-    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // blk, c16 are pageable and local
+    std::vector<BlkHdr> hdr(d.nblk + 1);
+    for (int b = 0; b <= d.nblk; ++b)
+        hdr[b] = BlkHdr{blk[b], bz[b], fits ? cbase[b] : -1, blk[std::min(b + 1, d.nblk)]};
+    d.hdr.upload(hdr, st);
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // blk, c16, hdr are pageable and local
 }
 
 bool is_l1(Smoother s) { return s == Smoother::L1Jacobi || s == Smoother::L1ColoredGS; }
@@ -1567,7 +1567,7 @@ void DeviceHierarchy::resetup(const double* nzval, hipStream_t st) {
 // the form (int32 indices: the work, not the storage, defines it).
 static double index_bytes(const DevCSR& M, int pi) {
     if (M.kind[pi] == kOpStream16 && M.has16)
-        return 2.0 * M.nnz16 + 4.0 * (M.nnz - M.nnz16) + 4.0 * M.nblk;
+        return 2.0 * M.nnz16 + 4.0 * (M.nnz - M.nnz16) + 16.0 * M.nblk;  // + the block headers
     return 4.0 * M.nnz;
 }
 

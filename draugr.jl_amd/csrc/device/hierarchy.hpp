@@ -31,8 +31,8 @@ struct DevCSR {
     int kind[3] = {0, 0, 0};  // kernel per cycle precision (dev::OpKernel), set by the autotuner
     bool has16 = false;       // 16-bit column form present (some branch-free block spans < 65536 columns)
     int64_t nnz16 = 0;        // nonzeros of the blocks read through 16-bit columns
-    DBuf<uint16_t> c16;       // column - cbase[block] per nonzero
-    DBuf<int32_t> cbase;      // per block; -1 = the block reads the 32-bit columns
+    DBuf<uint16_t> c16;       // column - (the block's base) per nonzero
+    DBuf<dev::BlkHdr> hdr;    // per block: first row, first nonzero, 16-bit column base (-1: 32-bit columns)
 };
 
 struct DevLevel {

@@ -211,10 +211,20 @@ template <typename TX> struct EpiStoreScaled {
     __device__ void total(int i, TX s) const { y[i] = s * dv[i]; }
 };
 
+// Row-block header of the CSR-stream kernels: first row, first nonzero,
+// 16-bit column base (-1: the block reads the 32-bit columns) and end row of
+// block b; entry nblk closes the table (its z0 is the end nonzero of the last
+// block).  One 16-byte record per block, so a block's bounds and base arrive
+// with one scalar load round trip (separate arrays cost a second, dependent
+// one for the base).
+struct alignas(16) BlkHdr {
+    int32_t r0, z0, cb, r1;
+};
+
 // One row block: stage the products, then finish each row on its own thread
 // (segf(row, seg, len, pref(row))), or a long row on thread 0 (totf(row, sum)).
-// The block table holds both row and nonzero offsets (blk, bz), so a block's
-// range is known after one round of loads.  A block of at most PER * kBlock
+// The block header holds both row and nonzero offsets, so a block's range is
+// known after one round of loads.  A block of at most PER * kBlock
 // nonzeros (PER = the operator's budget / kBlock) is loaded without branches:
 // every thread issues its PER column and value loads together (indices past
 // the block's end are clamped to its last entry and the result discarded),
@@ -222,18 +232,18 @@ template <typename TX> struct EpiStoreScaled {
 // flight instead of waiting once per product.
 //
 // C16: such blocks read their columns as 16-bit offsets from a per-block base
-// (col = cbase[b] + c16[k]), 2 bytes per nonzero instead of 4, when the block
-// spans fewer than 65536 columns (cbase[b] >= 0); wider and longer blocks read
-// the 32-bit columns.
+// (col = cb + c16[k]), 2 bytes per nonzero instead of 4, when the block spans
+// fewer than 65536 columns (cb >= 0); wider and longer blocks read the 32-bit
+// columns.
 template <int PER, bool C16, typename TV, typename TX, class XF, class PREF, class SEGF, class TOTF>
-__device__ __forceinline__ void stream_block(int b, const int* __restrict__ blk, const int* __restrict__ bz,
-                                             const int* __restrict__ rp, const int* __restrict__ cv,
-                                             const uint16_t* __restrict__ c16, const int* __restrict__ cbase,
+__device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ hdr, const int* __restrict__ rp,
+                                             const int* __restrict__ cv, const uint16_t* __restrict__ c16,
                                              const TV* __restrict__ av, XF xf, TX* prod, PREF pref, SEGF segf,
                                              TOTF totf) {
-    const int r0 = blk[b], r1 = blk[b + 1];
-    const int z0 = bz[b], z1 = bz[b + 1];
-    const int cb = C16 ? cbase[b] : 0;
+    const BlkHdr h0 = hdr[b];
+    const int r0 = h0.r0, r1 = h0.r1;
+    const int z0 = h0.z0, z1 = hdr[b + 1].z0;
+    const int cb = C16 ? h0.cb : 0;
     const int nz = z1 - z0;
     const int t = (int)threadIdx.x;
     const int row = r0 + t;
@@ -295,14 +305,13 @@ __device__ __forceinline__ void epi_segment(const EPI& epi, int row, const TX* s
 }
 
 template <bool XCD, int PER, bool C16, typename TV, typename TX, class XF, class EPI>
-__global__ __launch_bounds__(kBlock) void k_csr_stream(const int* __restrict__ blk, const int* __restrict__ bz,
-                                                       const int* __restrict__ rp, const int* __restrict__ cv,
-                                                       const uint16_t* __restrict__ c16, const int* __restrict__ cbase,
+__global__ __launch_bounds__(kBlock) void k_csr_stream(const BlkHdr* __restrict__ hdr, const int* __restrict__ rp,
+                                                       const int* __restrict__ cv, const uint16_t* __restrict__ c16,
                                                        const TV* __restrict__ av, XF xf, EPI epi) {
     __shared__ TX prod[kStreamNnz];
     const int b = XCD ? xcd_block() : (int)blockIdx.x;
     stream_block<PER, C16, TV, TX>(
-        b, blk, bz, rp, cv, c16, cbase, av, xf, prod, [&](int row) { return epi.pre(row); },
+        b, hdr, rp, cv, c16, av, xf, prod, [&](int row) { return epi.pre(row); },
         [&](int row, const TX* seg, int len, const typename EPI::Pre& q) { epi_segment(epi, row, seg, len, q); },
         [&](int row, TX s) { epi.total(row, s); });
 }
@@ -440,35 +449,35 @@ enum OpKernel : int { kOpStream = 0, kOpRowSeq = 1, kOpPipe = 2, kOpStream16 = 3
 constexpr int kOpKinds = 4;
 
 template <int PER, bool C16, typename TV, typename TX, class XF, class EPI>
-void launch_stream_per(int nblk, const int* blk, const int* bz, const int* rp, const int* cv, const uint16_t* c16,
-                       const int* cbase, const TV* av, XF xf, EPI epi, hipStream_t st) {
+void launch_stream_per(int nblk, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16, const TV* av,
+                       XF xf, EPI epi, hipStream_t st) {
     if (nblk >= 64)
-        hipLaunchKernelGGL((k_csr_stream<true, PER, C16, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk, bz,
-                           rp, cv, c16, cbase, av, xf, epi);
+        hipLaunchKernelGGL((k_csr_stream<true, PER, C16, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, hdr, rp,
+                           cv, c16, av, xf, epi);
     else
-        hipLaunchKernelGGL((k_csr_stream<false, PER, C16, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, blk,
-                           bz, rp, cv, c16, cbase, av, xf, epi);
+        hipLaunchKernelGGL((k_csr_stream<false, PER, C16, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, hdr,
+                           rp, cv, c16, av, xf, epi);
 }
 
 template <bool C16, typename TV, typename TX, class XF, class EPI>
-void launch_stream_c(int nblk, int per, const int* blk, const int* bz, const int* rp, const int* cv,
-                     const uint16_t* c16, const int* cbase, const TV* av, XF xf, EPI epi, hipStream_t st) {
+void launch_stream_c(int nblk, int per, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16,
+                     const TV* av, XF xf, EPI epi, hipStream_t st) {
     switch (per) {
-        case 1: launch_stream_per<1, C16, TV, TX>(nblk, blk, bz, rp, cv, c16, cbase, av, xf, epi, st); break;
-        case 2: launch_stream_per<2, C16, TV, TX>(nblk, blk, bz, rp, cv, c16, cbase, av, xf, epi, st); break;
-        case 4: launch_stream_per<4, C16, TV, TX>(nblk, blk, bz, rp, cv, c16, cbase, av, xf, epi, st); break;
-        default: launch_stream_per<8, C16, TV, TX>(nblk, blk, bz, rp, cv, c16, cbase, av, xf, epi, st); break;
+        case 1: launch_stream_per<1, C16, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st); break;
+        case 2: launch_stream_per<2, C16, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st); break;
+        case 4: launch_stream_per<4, C16, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st); break;
+        default: launch_stream_per<8, C16, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st); break;
     }
 }
 
-// per = the operator's block budget / kBlock (1, 2, 4 or 8); c16 / cbase
-// non-null = the 16-bit column form
+// per = the operator's block budget / kBlock (1, 2, 4 or 8); c16 non-null =
+// the 16-bit column form (bases in the block headers)
 template <typename TV, typename TX, class XF, class EPI>
-void launch_stream(int nblk, int per, const int* blk, const int* bz, const int* rp, const int* cv, const TV* av,
-                   XF xf, EPI epi, hipStream_t st, const uint16_t* c16 = nullptr, const int* cbase = nullptr) {
+void launch_stream(int nblk, int per, const BlkHdr* hdr, const int* rp, const int* cv, const TV* av, XF xf, EPI epi,
+                   hipStream_t st, const uint16_t* c16 = nullptr) {
     if (nblk <= 0) return;
-    if (c16 && cbase) launch_stream_c<true, TV, TX>(nblk, per, blk, bz, rp, cv, c16, cbase, av, xf, epi, st);
-    else launch_stream_c<false, TV, TX>(nblk, per, blk, bz, rp, cv, nullptr, nullptr, av, xf, epi, st);
+    if (c16) launch_stream_c<true, TV, TX>(nblk, per, hdr, rp, cv, c16, av, xf, epi, st);
+    else launch_stream_c<false, TV, TX>(nblk, per, hdr, rp, cv, nullptr, av, xf, epi, st);
 }
 
 template <typename TV, typename TX, class XF, class EPI>
