@@ -58,20 +58,31 @@ std::vector<int32_t> strong_transpose_count(const CSR& A, const std::vector<uint
 }
 
 // Transpose adjacency of S: for node j, the rows i (ascending) with i -> j strong.
-// src/coarsening.jl:827-860
+// src/coarsening.jl:827-860.  Rows claim slots of their columns' lists in any
+// order, then each list is sorted: a row appears at most once in a column's
+// list, so the sorted list is the one the sequential row-order build makes.
 void strong_transpose_adj(const CSR& A, const std::vector<uint8_t>& s,
                           std::vector<int64_t>& off, std::vector<int32_t>& src) {
     const int32_t n = A.nrow;
+    const std::vector<int32_t> cnt = strong_transpose_count(A, s);
     off.assign(n + 1, 0);
-    for (int32_t i = 0; i < n; ++i)
-        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
-            if (strong_offdiag(s, k, A.cv[k], i)) off[A.cv[k] + 1]++;
-    for (int32_t j = 0; j < n; ++j) off[j + 1] += off[j];
+    for (int32_t j = 0; j < n; ++j) off[j + 1] = off[j] + cnt[j];
     src.resize(off[n]);
     std::vector<int64_t> pos(off.begin(), off.end() - 1);
+    int64_t* ps = pos.data();
+    int32_t* sr = src.data();
+#pragma omp parallel for schedule(static, 4096)
     for (int32_t i = 0; i < n; ++i)
         for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
-            if (strong_offdiag(s, k, A.cv[k], i)) src[pos[A.cv[k]]++] = i;
+            if (strong_offdiag(s, k, A.cv[k], i)) {
+                int64_t p;
+#pragma omp atomic capture
+                p = ps[A.cv[k]]++;
+                sr[p] = i;
+            }
+    const int64_t* o = off.data();
+#pragma omp parallel for schedule(static, 4096)
+    for (int32_t j = 0; j < n; ++j) std::sort(sr + o[j], sr + o[j + 1]);
 }
 
 bool has_strong_offdiag(const CSR& A, const std::vector<uint8_t>& s, int32_t i) {
@@ -80,32 +91,38 @@ bool has_strong_offdiag(const CSR& A, const std::vector<uint8_t>& s, int32_t i) 
     return false;
 }
 
-// Doubly linked buckets keyed by the measure lambda, head insertion.
-// Same selection semantics as the reference's bucket lists
-// (src/coarsening.jl:450-507, 706-776).
+// Doubly linked buckets keyed by the measure lambda, head insertion, with the
+// same selection semantics as the reference's bucket lists
+// (src/coarsening.jl:450-507, 706-776).  A node's links, measure and C/F state
+// share one 16-byte record, so a bucket move touches three records (the node
+// and its two neighbours) instead of a line per array.
+struct Node {
+    int32_t next, prev, lam, cf;  // node ids, -1 = none
+};
 struct Buckets {
-    std::vector<int32_t> head, next, prev;  // node ids, -1 = none
-    std::vector<int32_t>& lam;
-    Buckets(int32_t n, int32_t nb, std::vector<int32_t>& l)
-        : head(std::max(nb, 1), -1), next(n, -1), prev(n, -1), lam(l) {}
+    std::vector<int32_t> head;
+    Node* nd;
+    Buckets(int32_t nb, Node* n) : head(std::max(nb, 1), -1), nd(n) {}
     void push(int32_t i) {
-        int32_t k = lam[i];
+        Node& a = nd[i];
+        const int32_t k = a.lam;
         if (k >= (int32_t)head.size()) head.resize(k + 1, -1);
-        int32_t h = head[k];
+        const int32_t h = head[k];
         head[k] = i;
-        next[i] = h;
-        prev[i] = -1;
-        if (h >= 0) prev[h] = i;
+        a.next = h;
+        a.prev = -1;
+        if (h >= 0) nd[h].prev = i;
     }
     void remove(int32_t i) {
-        int32_t k = lam[i], p = prev[i], nx = next[i];
-        if (p >= 0) next[p] = nx; else head[k] = nx;
-        if (nx >= 0) prev[nx] = p;
-        next[i] = prev[i] = -1;
+        Node& a = nd[i];
+        const int32_t p = a.prev, nx = a.next;
+        if (p >= 0) nd[p].next = nx; else head[a.lam] = nx;
+        if (nx >= 0) nd[nx].prev = p;
+        a.next = a.prev = -1;
     }
     void update(int32_t i, int32_t v) {
         remove(i);
-        lam[i] = v;
+        nd[i].lam = v;
         push(i);
     }
 };
@@ -116,70 +133,76 @@ struct Buckets {
 // (no zero-measure preprocessing, plain decrement): src/coarsening.jl:671-805.
 std::vector<int8_t> rs_first_pass(const CSR& A, const std::vector<uint8_t>& s, bool zpt, bool classical) {
     const int32_t n = A.nrow;
-    const int8_t fpnt = zpt ? -2 : -1;
-    std::vector<int32_t> lam = strong_transpose_count(A, s);
+    const int32_t fpnt = zpt ? -2 : -1;
+    std::vector<Node> nd(n);
+    {
+        const std::vector<int32_t> lam0 = strong_transpose_count(A, s);
+        Node* q = nd.data();
+#pragma omp parallel for schedule(static, 4096)
+        for (int32_t i = 0; i < n; ++i)
+            q[i] = Node{-1, -1, lam0[i], (!has_strong_offdiag(A, s, i) && lam0[i] == 0) ? 1 : 0};
+    }
     std::vector<int64_t> st_off;
     std::vector<int32_t> st_src;
     strong_transpose_adj(A, s, st_off, st_src);
-    std::vector<int8_t> cf(n, 0);
-    for (int32_t i = 0; i < n; ++i)
-        if (!has_strong_offdiag(A, s, i) && lam[i] == 0) cf[i] = 1;
+    const int32_t* rp = A.rp.data();
+    const int32_t* cv = A.cv.data();
+    const uint8_t* sk = s.data();
     if (!classical) {
         for (int32_t i = 0; i < n; ++i) {
-            if (cf[i] != 0 || lam[i] != 0) continue;
-            cf[i] = fpnt;
-            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-                int32_t j = A.cv[k];
-                if (strong_offdiag(s, k, j, i) && cf[j] == 0) lam[j]++;
+            if (nd[i].cf != 0 || nd[i].lam != 0) continue;
+            nd[i].cf = fpnt;
+            for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                const int32_t j = cv[k];
+                if (sk[k] && j != i && nd[j].cf == 0) nd[j].lam++;
             }
         }
     }
     int32_t maxlam = 0;
     for (int32_t i = 0; i < n; ++i)
-        if (cf[i] == 0) maxlam = std::max(maxlam, lam[i]);
-    Buckets B(n, classical ? std::max(maxlam + 1, n) : maxlam + 1, lam);
+        if (nd[i].cf == 0) maxlam = std::max(maxlam, nd[i].lam);
+    Buckets B(classical ? std::max(maxlam + 1, n) : maxlam + 1, nd.data());
     for (int32_t i = 0; i < n; ++i)
-        if (cf[i] == 0) B.push(i);
+        if (nd[i].cf == 0) B.push(i);
+    // lambda + 1 for the undecided strong neighbours of a new F point j
+    auto raise_neighbours = [&](int32_t j, int32_t& top) {
+        for (int32_t k2 = rp[j]; k2 < rp[j + 1]; ++k2) {
+            const int32_t k = cv[k2];
+            if (sk[k2] && k != j && nd[k].cf == 0) {
+                const int32_t v = nd[k].lam + 1;
+                B.update(k, v);
+                if (v > top) top = v;
+            }
+        }
+    };
     int32_t top = maxlam;
     for (;;) {
         while (top >= 0 && B.head[top] < 0) --top;
         if (top < 0) break;
         const int32_t best = B.head[top];
         B.remove(best);
-        cf[best] = 1;
+        nd[best].cf = 1;
         for (int64_t t = st_off[best]; t < st_off[best + 1]; ++t) {
             const int32_t j = st_src[t];
-            if (cf[j] != 0) continue;
+            if (nd[j].cf != 0) continue;
             B.remove(j);
-            cf[j] = -1;
-            for (int32_t k2 = A.rp[j]; k2 < A.rp[j + 1]; ++k2) {
-                const int32_t k = A.cv[k2];
-                if (strong_offdiag(s, k2, k, j) && cf[k] == 0) {
-                    const int32_t v = lam[k] + 1;
-                    B.update(k, v);
-                    if (v > top) top = v;
-                }
-            }
+            nd[j].cf = -1;
+            raise_neighbours(j, top);
         }
-        for (int32_t kk = A.rp[best]; kk < A.rp[best + 1]; ++kk) {
-            const int32_t j = A.cv[kk];
-            if (!(strong_offdiag(s, kk, j, best) && cf[j] == 0)) continue;
-            const int32_t v = std::max(0, lam[j] - 1);
+        for (int32_t kk = rp[best]; kk < rp[best + 1]; ++kk) {
+            const int32_t j = cv[kk];
+            if (!(sk[kk] && j != best && nd[j].cf == 0)) continue;
+            const int32_t v = std::max(0, nd[j].lam - 1);
             B.update(j, v);
             if (!classical && v == 0) {
                 B.remove(j);
-                cf[j] = fpnt;
-                for (int32_t k2 = A.rp[j]; k2 < A.rp[j + 1]; ++k2) {
-                    const int32_t k = A.cv[k2];
-                    if (strong_offdiag(s, k2, k, j) && cf[k] == 0) {
-                        const int32_t v2 = lam[k] + 1;
-                        B.update(k, v2);
-                        if (v2 > top) top = v2;
-                    }
-                }
+                nd[j].cf = fpnt;
+                raise_neighbours(j, top);
             }
         }
     }
+    std::vector<int8_t> cf(n);
+    for (int32_t i = 0; i < n; ++i) cf[i] = (int8_t)nd[i].cf;
     return cf;
 }
 
@@ -482,26 +505,54 @@ void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& r
     const int32_t m = (int32_t)c_idx.size();
 
     // Strong C neighbours of every row (CSR), and its transpose restricted to C.
+    // Per-row lists in parallel (count, scan, fill); the transpose claims slots
+    // in any order and sorts each list, which gives the ascending row order of
+    // the sequential build (a row appears at most once in a list).
     std::vector<int64_t> sc_off(n + 1, 0);
-    for (int32_t i = 0; i < n; ++i)
+    int64_t* so = sc_off.data();
+#pragma omp parallel for schedule(static, 4096)
+    for (int32_t i = 0; i < n; ++i) {
+        int64_t c = 0;
         for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
-            if (strong_offdiag(s, k, A.cv[k], i) && cf1[A.cv[k]] == 1) sc_off[i + 1]++;
+            if (strong_offdiag(s, k, A.cv[k], i) && cf1[A.cv[k]] == 1) ++c;
+        so[i + 1] = c;
+    }
     for (int32_t i = 0; i < n; ++i) sc_off[i + 1] += sc_off[i];
     std::vector<int32_t> sc(sc_off[n]);
-    {
-        int64_t p = 0;
-        for (int32_t i = 0; i < n; ++i)
-            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
-                if (strong_offdiag(s, k, A.cv[k], i) && cf1[A.cv[k]] == 1) sc[p++] = c_loc[A.cv[k]];
+    int32_t* scp = sc.data();
+#pragma omp parallel for schedule(static, 4096)
+    for (int32_t i = 0; i < n; ++i) {
+        int64_t p = so[i];
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (strong_offdiag(s, k, A.cv[k], i) && cf1[A.cv[k]] == 1) scp[p++] = c_loc[A.cv[k]];
     }
     std::vector<int64_t> tr_off(m + 1, 0);
-    for (int64_t t = 0; t < sc_off[n]; ++t) tr_off[sc[t] + 1]++;
-    for (int32_t a = 0; a < m; ++a) tr_off[a + 1] += tr_off[a];
+    {
+        std::vector<int32_t> tcnt(m, 0);
+        int32_t* tc = tcnt.data();
+#pragma omp parallel for schedule(static, 65536)
+        for (int64_t t = 0; t < sc_off[n]; ++t) {
+#pragma omp atomic
+            tc[scp[t]]++;
+        }
+        for (int32_t a = 0; a < m; ++a) tr_off[a + 1] = tr_off[a] + tcnt[a];
+    }
     std::vector<int32_t> tr(tr_off[m]);
     {
         std::vector<int64_t> pos(tr_off.begin(), tr_off.end() - 1);
+        int64_t* ps = pos.data();
+        int32_t* trp = tr.data();
+#pragma omp parallel for schedule(static, 4096)
         for (int32_t i = 0; i < n; ++i)
-            for (int64_t t = sc_off[i]; t < sc_off[i + 1]; ++t) tr[pos[sc[t]]++] = i;
+            for (int64_t t = so[i]; t < so[i + 1]; ++t) {
+                int64_t p;
+#pragma omp atomic capture
+                p = ps[scp[t]]++;
+                trp[p] = i;
+            }
+        const int64_t* to = tr_off.data();
+#pragma omp parallel for schedule(static, 4096)
+        for (int32_t a = 0; a < m; ++a) std::sort(trp + to[a], trp + to[a + 1]);
     }
     // Neighbour set of C point a (local ids): pairs of strong C neighbours of a
     // common row, plus direct strong C-C connections in both directions.
