@@ -16,6 +16,7 @@
 
 #include "../../../include/draugr_hip.h"
 #include "batch.hpp"
+#include "coarsen.hpp"
 #include "galerkin.hpp"
 #include "interp.hpp"
 #include "hierarchy.hpp"
@@ -133,7 +134,7 @@ void check_n(Hier& H, int32_t n) {
 // Setup phases run on the GPU (config gpu_setup): interpolation and the
 // Galerkin product; null = everything on the host.
 const SetupHooks* gpu_hooks(const Config& cfg) {
-    static const SetupHooks hooks{galerkin_device, interpolation_device};
+    static const SetupHooks hooks{galerkin_device, interpolation_device, pmis_sweeps_device};
     return cfg.gpu_setup ? &hooks : nullptr;
 }
 

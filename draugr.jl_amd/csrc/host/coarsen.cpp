@@ -206,22 +206,23 @@ std::vector<int8_t> rs_first_pass(const CSR& A, const std::vector<uint8_t>& s, b
     return cf;
 }
 
-// PMIS sweeps on the undecided nodes (cf == 0); src/coarsening.jl:575-624.
-void pmis_on_undecided(std::vector<int8_t>& cf, const CSR& A, const std::vector<uint8_t>& s,
-                       const std::vector<double>& measure) {
-    const int32_t n = A.nrow;
-    std::vector<int8_t> nxt(cf);  // snapshot sweeps (see the note at the top)
+// Snapshot PMIS sweeps (see the note at the top); src/coarsening.jl:227-283,
+// 575-624.
+template <typename OFF>
+void sweeps_host(std::vector<int8_t>& cf, int32_t n, const OFF* off, const int32_t* adj, const uint8_t* s,
+                 const std::vector<double>& measure, bool low_measure_fine) {
+    std::vector<int8_t> nxt(cf);
     for (int64_t iter = 0; iter <= (int64_t)n; ++iter) {
         int any_undecided = 0;
 #pragma omp parallel for schedule(static, 4096) reduction(| : any_undecided)
         for (int32_t i = 0; i < n; ++i) {
             if (cf[i] != 0) continue;
             any_undecided = 1;
-            if (measure[i] < 1.0) { nxt[i] = -1; continue; }
+            if (low_measure_fine && measure[i] < 1.0) { nxt[i] = -1; continue; }
             bool is_max = true;
-            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-                const int32_t j = A.cv[k];
-                if (strong_offdiag(s, k, j, i) && cf[j] != -1 && measure[j] > measure[i]) {
+            for (OFF k = off[i]; k < off[i + 1]; ++k) {
+                const int32_t j = adj[k];
+                if (j != i && (!s || s[k]) && cf[j] != -1 && measure[j] > measure[i]) {
                     is_max = false;
                     break;
                 }
@@ -232,14 +233,23 @@ void pmis_on_undecided(std::vector<int8_t>& cf, const CSR& A, const std::vector<
 #pragma omp parallel for schedule(static, 4096)
         for (int32_t i = 0; i < n; ++i) {
             if (cf[i] != 0) continue;
-            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
-                if (s[k] && cf[A.cv[k]] == 1) { nxt[i] = -1; break; }
+            for (OFF k = off[i]; k < off[i + 1]; ++k)
+                if ((!s || s[k]) && cf[adj[k]] == 1) { nxt[i] = -1; break; }
         }
         cf = nxt;
         if (!any_undecided) break;
     }
-    for (auto& c : cf)
-        if (c == 0) c = -1;
+}
+
+// The sweeps over the strong graph of A, on the device when the hook asks.
+void pmis_on_matrix(std::vector<int8_t>& cf, const CSR& A, const std::vector<uint8_t>& s,
+                    const std::vector<double>& measure, bool low_measure_fine, const PmisHook& ph) {
+    if (ph.fn && A.nrow >= ph.min_rows) {
+        const std::vector<int64_t> off(A.rp.begin(), A.rp.end());
+        ph.fn(cf, A.nrow, off.data(), A.cv.data(), s.data(), measure, low_measure_fine);
+    } else {
+        sweeps_host<int32_t>(cf, A.nrow, A.rp.data(), A.cv.data(), s.data(), measure, low_measure_fine);
+    }
 }
 
 // Promote F points without a strong C neighbour; src/coarsening.jl:291-313.
@@ -264,6 +274,11 @@ int32_t count_coarse(const std::vector<int8_t>& cf) {
 }
 
 }  // namespace
+
+void pmis_sweeps(std::vector<int8_t>& cf, int32_t n, const int64_t* off, const int32_t* adj, const uint8_t* s,
+                 const std::vector<double>& measure, bool low_measure_fine) {
+    sweeps_host<int64_t>(cf, n, off, adj, s, measure, low_measure_fine);
+}
 
 // Absolute strength of connection; src/strength.jl:24-61.
 std::vector<uint8_t> strength_absolute(const CSR& A, double theta) {
@@ -401,7 +416,7 @@ void coarsen_aggregation(const CSR& A, double theta, std::vector<int32_t>& agg, 
 }
 
 // PMIS with the column-count measure plus a uniform random; src/coarsening.jl:202-283.
-void coarsen_pmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf, int32_t& nc) {
+void coarsen_pmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf, int32_t& nc, const PmisHook& ph) {
     const int32_t n = A.nrow;
     if (n <= 1) {
         cf.assign(n, 1);
@@ -415,40 +430,14 @@ void coarsen_pmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf,
     cf.assign(n, 0);
     for (int32_t i = 0; i < n; ++i)
         if (!has_strong_offdiag(A, s, i) && st[i] == 0) cf[i] = 1;
-    std::vector<int8_t> nxt(cf);  // snapshot sweeps (see the note at the top)
-    for (int64_t iter = 0; iter <= (int64_t)n; ++iter) {
-        int any_undecided = 0;
-#pragma omp parallel for schedule(static, 4096) reduction(| : any_undecided)
-        for (int32_t i = 0; i < n; ++i) {
-            if (cf[i] != 0) continue;
-            any_undecided = 1;
-            bool is_max = true;
-            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-                const int32_t j = A.cv[k];
-                if (strong_offdiag(s, k, j, i) && cf[j] != -1 && measure[j] > measure[i]) {
-                    is_max = false;
-                    break;
-                }
-            }
-            if (is_max) nxt[i] = 1;
-        }
-        cf = nxt;
-#pragma omp parallel for schedule(static, 4096)
-        for (int32_t i = 0; i < n; ++i) {
-            if (cf[i] != 0) continue;
-            for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
-                if (s[k] && cf[A.cv[k]] == 1) { nxt[i] = -1; break; }
-        }
-        cf = nxt;
-        if (!any_undecided) break;
-    }
+    pmis_on_matrix(cf, A, s, measure, false, ph);
     for (auto& c : cf)
         if (c == 0) c = 1;
     nc = count_coarse(cf);
 }
 
 // HMIS = RS first pass (Z_PT variant) + PMIS on the rest; src/coarsening.jl:330-394.
-void coarsen_hmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf, int32_t& nc) {
+void coarsen_hmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf, int32_t& nc, const PmisHook& ph) {
     const int32_t n = A.nrow;
     if (n <= 1) {
         cf.assign(n, 1);
@@ -464,7 +453,9 @@ void coarsen_hmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf,
         if (cf[i] != -2) continue;
         cf[i] = (measure[i] >= 1.0 || has_strong_offdiag(A, s, i)) ? 0 : -1;
     }
-    pmis_on_undecided(cf, A, s, measure);
+    pmis_on_matrix(cf, A, s, measure, true, ph);
+    for (auto& c : cf)
+        if (c == 0) c = -1;
     nc = count_coarse(cf);
 }
 
@@ -485,7 +476,7 @@ void coarsen_rs(const CSR& A, double theta, std::vector<int8_t>& cf, int32_t& nc
 // Two-pass aggressive C/F splitting: base splitting, then PMIS on the
 // distance-2 C graph; src/coarsening.jl:990-1116.
 void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& rng,
-                           std::vector<int8_t>& cf, int32_t& nc) {
+                           std::vector<int8_t>& cf, int32_t& nc, const PmisHook& ph) {
     const int32_t n = A_in.nrow;
     if (n <= 1) {
         cf.assign(n, 1);
@@ -494,8 +485,8 @@ void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& r
     }
     std::vector<int8_t> cf1;
     int32_t nc1 = 0;
-    if (hmis_base) coarsen_hmis(A_in, theta, rng, cf1, nc1);
-    else coarsen_pmis(A_in, theta, rng, cf1, nc1);
+    if (hmis_base) coarsen_hmis(A_in, theta, rng, cf1, nc1, ph);
+    else coarsen_pmis(A_in, theta, rng, cf1, nc1, ph);
     const CSR& A = A_in;
     const std::vector<uint8_t> s = strength_absolute(A, theta);
 
@@ -592,30 +583,9 @@ void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& r
     }
     std::vector<double> measure(m);
     for (int32_t a = 0; a < m; ++a) measure[a] = (double)(g_off[a + 1] - g_off[a]) + rng.uniform();
-    std::vector<int8_t> cf2(m, 0), nxt(m, 0);  // snapshot sweeps (see the note at the top)
-    for (int64_t iter = 0; iter <= (int64_t)m; ++iter) {
-        int any_undecided = 0;
-#pragma omp parallel for schedule(static, 4096) reduction(| : any_undecided)
-        for (int32_t a = 0; a < m; ++a) {
-            if (cf2[a] != 0) continue;
-            any_undecided = 1;
-            bool is_max = true;
-            for (int64_t t = g_off[a]; t < g_off[a + 1]; ++t) {
-                const int32_t b = g[t];
-                if (cf2[b] != -1 && measure[b] > measure[a]) { is_max = false; break; }
-            }
-            if (is_max) nxt[a] = 1;
-        }
-        cf2 = nxt;
-#pragma omp parallel for schedule(static, 4096)
-        for (int32_t a = 0; a < m; ++a) {
-            if (cf2[a] != 0) continue;
-            for (int64_t t = g_off[a]; t < g_off[a + 1]; ++t)
-                if (cf2[g[t]] == 1) { nxt[a] = -1; break; }
-        }
-        cf2 = nxt;
-        if (!any_undecided) break;
-    }
+    std::vector<int8_t> cf2(m, 0);  // every edge of the C graph is strong, no node lists itself
+    if (ph.fn && m >= ph.min_rows) ph.fn(cf2, m, g_off.data(), g.data(), nullptr, measure, false);
+    else sweeps_host<int64_t>(cf2, m, g_off.data(), g.data(), nullptr, measure, false);
     for (auto& c : cf2)
         if (c == 0) c = 1;
     cf = cf1;
@@ -627,11 +597,12 @@ void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& r
 
 // Aggregation-style aggressive coarsening built on PMIS (AggressiveCoarsening
 // with a base other than :pmis/:hmis); src/coarsening.jl:869-957.
-void coarsen_aggressive_agg(const CSR& A, double theta, Rng& rng, std::vector<int32_t>& agg, int32_t& nc) {
+void coarsen_aggressive_agg(const CSR& A, double theta, Rng& rng, std::vector<int32_t>& agg, int32_t& nc,
+                            const PmisHook& ph) {
     const int32_t n = A.nrow;
     std::vector<int8_t> cf;
     int32_t nc1 = 0;
-    coarsen_pmis(A, theta, rng, cf, nc1);
+    coarsen_pmis(A, theta, rng, cf, nc1, ph);
     const std::vector<uint8_t> s = strength_absolute(A, theta);
     agg.assign(n, -1);
     int32_t cnt = 0;

@@ -23,15 +23,16 @@ CSR coarsen_and_build_p(const CSR& A, const CoarsenSpec& alg, const Config& cfg,
     CSR Aw;
     if (weaken) Aw = apply_max_row_sum(A, cfg.max_row_sum);
     const CSR& As = weaken ? Aw : A;  // matrix the splitting / aggregation sees
+    const PmisHook ph{hooks.pmis, cfg.gpu_setup_min_rows};
     if (uses_cf(alg.type)) {
         std::vector<int8_t> cf;
         const auto t0 = std::chrono::steady_clock::now();
         switch (alg.type) {
-            case Coarsen::PMIS: coarsen_pmis(As, alg.theta, rng, cf, nc); break;
-            case Coarsen::HMIS: coarsen_hmis(As, alg.theta, rng, cf, nc); break;
+            case Coarsen::PMIS: coarsen_pmis(As, alg.theta, rng, cf, nc, ph); break;
+            case Coarsen::HMIS: coarsen_hmis(As, alg.theta, rng, cf, nc, ph); break;
             case Coarsen::RS: coarsen_rs(As, alg.theta, cf, nc); break;
-            case Coarsen::AggressivePMIS: coarsen_aggressive_cf(As, alg.theta, false, rng, cf, nc); break;
-            case Coarsen::AggressiveHMIS: coarsen_aggressive_cf(As, alg.theta, true, rng, cf, nc); break;
+            case Coarsen::AggressivePMIS: coarsen_aggressive_cf(As, alg.theta, false, rng, cf, nc, ph); break;
+            case Coarsen::AggressiveHMIS: coarsen_aggressive_cf(As, alg.theta, true, rng, cf, nc, ph); break;
             default: break;
         }
         const auto t1 = std::chrono::steady_clock::now();
@@ -50,7 +51,7 @@ CSR coarsen_and_build_p(const CSR& A, const CoarsenSpec& alg, const Config& cfg,
     }
     std::vector<int32_t> agg;
     if (alg.type == Coarsen::AggressiveAggregation) {
-        coarsen_aggressive_agg(As, alg.theta, rng, agg, nc);
+        coarsen_aggressive_agg(As, alg.theta, rng, agg, nc, ph);
         return prolongation_from_aggregates(agg, nc);
     }
     coarsen_aggregation(As, alg.theta, agg, nc);

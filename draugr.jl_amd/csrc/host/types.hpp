@@ -144,13 +144,32 @@ struct Rng {
 std::vector<uint8_t> strength_absolute(const CSR& A, double theta);
 CSR apply_max_row_sum(const CSR& A, double threshold);
 
+// PMIS sweeps (src/coarsening.jl:202-283, 575-624) over a graph: the nodes
+// with cf == 0 are decided by snapshot sweeps (select C, then mark F) over row
+// i's entries off[i]..off[i+1] of adj that are strong (s[k] != 0; every entry
+// when s is null), a row's own index never counting; with low_measure_fine an
+// undecided node of measure < 1 becomes F first (HMIS's second phase).  Nodes
+// still undecided after n sweeps keep 0.  pmis_sweeps is the host's.
+using PmisFn = void (*)(std::vector<int8_t>& cf, int32_t n, const int64_t* off, const int32_t* adj, const uint8_t* s,
+                        const std::vector<double>& measure, bool low_measure_fine);
+void pmis_sweeps(std::vector<int8_t>& cf, int32_t n, const int64_t* off, const int32_t* adj, const uint8_t* s,
+                 const std::vector<double>& measure, bool low_measure_fine);
+// Sweeps of graphs of at least min_rows nodes by fn (null: the host's).
+struct PmisHook {
+    PmisFn fn = nullptr;
+    int32_t min_rows = 0;
+};
+
 void coarsen_aggregation(const CSR& A, double theta, std::vector<int32_t>& agg, int32_t& nc);
-void coarsen_aggressive_agg(const CSR& A, double theta, Rng& rng, std::vector<int32_t>& agg, int32_t& nc);
-void coarsen_pmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf, int32_t& nc);
-void coarsen_hmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf, int32_t& nc);
+void coarsen_aggressive_agg(const CSR& A, double theta, Rng& rng, std::vector<int32_t>& agg, int32_t& nc,
+                            const PmisHook& ph = PmisHook());
+void coarsen_pmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf, int32_t& nc,
+                  const PmisHook& ph = PmisHook());
+void coarsen_hmis(const CSR& A, double theta, Rng& rng, std::vector<int8_t>& cf, int32_t& nc,
+                  const PmisHook& ph = PmisHook());
 void coarsen_rs(const CSR& A, double theta, std::vector<int8_t>& cf, int32_t& nc);
 void coarsen_aggressive_cf(const CSR& A, double theta, bool hmis_base, Rng& rng,
-                           std::vector<int8_t>& cf, int32_t& nc);
+                           std::vector<int8_t>& cf, int32_t& nc, const PmisHook& ph = PmisHook());
 
 CSR prolongation_from_aggregates(const std::vector<int32_t>& agg, int32_t nc);
 CSR smooth_prolongation(const CSR& A, const CSR& Ptent, double omega);
@@ -195,6 +214,7 @@ using InterpFn = CSR (*)(const CSR& A, const std::vector<int8_t>& cf, int32_t nc
 struct SetupHooks {
     GalerkinFn galerkin = nullptr;
     InterpFn interp = nullptr;
+    PmisFn pmis = nullptr;  // PMIS sweeps of C/F splittings of >= cfg.gpu_setup_min_rows rows
 };
 HostHierarchy amg_setup_host(const CSR& A, const Config& cfg, bool allow_partial_resetup,
                              const SetupHooks* hooks = nullptr);

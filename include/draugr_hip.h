@@ -45,9 +45,10 @@ const char *draugr_amg_last_error(void);
  * prolongation and first post-smoothing step in one pass; Jacobi-type
  * smoothers; "auto" times each on the level's operators), "gpu_setup" (default
  * true: when the hierarchy is uploaded, each level's extended+i interpolation,
- * coarse operator, triple map and transpose map are built on the GPU,
- * identical to the host's, and the triple map stays in HBM; levels of fewer
- * than "gpu_setup_min_rows" rows (default 32768) interpolate on the host),
+ * coarse operator, triple map, transpose map and the PMIS sweeps of its C/F
+ * splitting run on the GPU, identical to the host's, and the triple map stays
+ * in HBM; levels of fewer than "gpu_setup_min_rows" rows (default 32768)
+ * interpolate and sweep on the host),
  * "filtering", "filter_tol".  Returns a handle > 0, or -1.
  * Replaces c_api/draugr_amg.h:94. */
 int32_t draugr_amg_config_from_json(const char *json_str);

@@ -1,6 +1,7 @@
 """Setup on the GPU (config key ``gpu_setup``): extended+i interpolation,
-coarse operators, triple maps and transpose maps built in HBM, against the
-host's.
+coarse operators, triple maps and transpose maps built in HBM, and the PMIS
+sweeps of the C/F splittings (PMIS, HMIS's second phase, aggressive
+coarsening's second stage), against the host's.
 
 The host's routines are pinned on the CPU (tests/test_setup_host.py: the
 Python restatement of the setup, the oracle's Galerkin product, dense
