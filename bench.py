@@ -49,7 +49,7 @@ def parse():
                     help="hetero128 (default): the headline line; the others print their own JSON line")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the calibration copy and the dominant-kernel launches (for counter runs)")
-    ap.add_argument("--kernel", default="auto", choices=["auto", "stream", "rowseq", "pipe", "stream16"],
+    ap.add_argument("--kernel", default="auto", choices=["auto", "stream", "rowseq", "pipe", "stream16", "sorted"],
                     help="sparse-pass kernel (config key kernel); a counter run forces the kernel the "
                          "autotuner picks without a profiler, whose timings it would perturb")
     ap.add_argument("--fuse", default="auto", choices=["auto", "off", "on"],

@@ -351,6 +351,11 @@ template <typename TV> const TV* vals(const DevCSR& m);
 template <> const double* vals<double>(const DevCSR& m) { return m.v64.p; }
 template <> const float* vals<float>(const DevCSR& m) { return m.v32.p; }
 template <> const bf16_t* vals<bf16_t>(const DevCSR& m) { return m.v16.p; }
+// the values of the column-sorted form
+template <typename TV> const TV* svals(const DevCSR& m);
+template <> const double* svals<double>(const DevCSR& m) { return m.sv64.p; }
+template <> const float* svals<float>(const DevCSR& m) { return m.sv32.p; }
+template <> const bf16_t* svals<bf16_t>(const DevCSR& m) { return m.sv16.p; }
 
 // Transfer operators use FP32 values when the cycle stores A in BF16.
 template <typename TV> struct TransferT { typedef TV type; };
@@ -467,6 +472,8 @@ void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
     else if (kind == kOpPipe) launch_pipe<TV, TX>(M.nblk, M.per, M.blk.p, M.bz.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
     else if (kind == kOpStream16 && M.has16)
         launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st, M.c16.p);
+    else if (kind == kOpSorted && M.hasS && svals<TV>(M))
+        launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, svals<TV>(M), xf, epi, st, M.scol.p, M.slot.p);
     else launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
 }
 
@@ -475,6 +482,61 @@ void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
 template <typename TV, typename TX, class XF, class EPI>
 void op(const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
     op_kind<TV, TX>(M.kind[pidx<TV>()], M, xf, epi, st);
+}
+
+// Column-sorted form of the 16-bit blocks: a bitonic sort in LDS of the keys
+// (column offset << 12 | position in the block), unique per entry; blocks
+// without a base keep row order (slot = position).  The positions of a 16-bit
+// block are below its budget (<= kStreamNnz entries), so 12 bits hold them.
+__global__ __launch_bounds__(kBlock) void k_sort_blocks(const BlkHdr* __restrict__ hdr, const uint16_t* __restrict__ c16,
+                                                        const double* __restrict__ v, uint16_t* __restrict__ scol,
+                                                        uint16_t* __restrict__ slot, double* __restrict__ sv) {
+    __shared__ uint32_t key[kStreamNnz];
+    const int b = blockIdx.x;
+    const int z0 = hdr[b].z0, nz = hdr[b + 1].z0 - z0, cb = hdr[b].cb;
+    const int t = (int)threadIdx.x;
+    if (cb < 0 || nz > kStreamNnz) {
+        for (int k = t; k < nz; k += kBlock) {
+            scol[z0 + k] = 0;
+            slot[z0 + k] = (uint16_t)k;
+            sv[z0 + k] = v[z0 + k];
+        }
+        return;
+    }
+    int m = 1;
+    while (m < nz) m <<= 1;
+    for (int k = t; k < m; k += kBlock) key[k] = k < nz ? ((uint32_t)c16[z0 + k] << 12) | (uint32_t)k : 0xFFFFFFFFu;
+    __syncthreads();
+    for (int size = 2; size <= m; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = t; i < m; i += kBlock) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const uint32_t a = key[i], c = key[j];
+                    if ((a > c) == ((i & size) == 0)) {
+                        key[i] = c;
+                        key[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int k = t; k < nz; k += kBlock) {
+        const uint32_t kk = key[k];
+        const int pos = (int)(kk & 0xFFFu);
+        scol[z0 + k] = (uint16_t)(kk >> 12);
+        slot[z0 + k] = (uint16_t)pos;
+        sv[z0 + k] = v[z0 + pos];
+    }
+}
+
+// sv = the values in the sorted form's order (after new values in v)
+__global__ __launch_bounds__(kBlock) void k_permute_blocks(const BlkHdr* __restrict__ hdr,
+                                                           const uint16_t* __restrict__ slot,
+                                                           const double* __restrict__ v, double* __restrict__ sv) {
+    const int b = blockIdx.x;
+    const int z0 = hdr[b].z0, nz = hdr[b + 1].z0 - z0;
+    for (int k = (int)threadIdx.x; k < nz; k += kBlock) sv[z0 + k] = v[z0 + slot[z0 + k]];
 }
 
 void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
@@ -525,6 +587,18 @@ void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
     for (int b = 0; b <= d.nblk; ++b)
         hdr[b] = BlkHdr{blk[b], bz[b], fits ? cbase[b] : -1, blk[std::min(b + 1, d.nblk)]};
     d.hdr.upload(hdr, st);
+    d.hasS = fits;
+    if (fits) {
+        d.scol.alloc(d.nnz);
+        d.slot.alloc(d.nnz);
+        d.sv64.alloc(d.nnz);
+        hipLaunchKernelGGL(k_sort_blocks, dim3(d.nblk), dim3(kBlock), 0, st, d.hdr.p, d.c16.p, d.v64.p, d.scol.p,
+                           d.slot.p, d.sv64.p);
+    } else {
+        d.scol.release();
+        d.slot.release();
+        d.sv64.release();
+    }
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // blk, c16, hdr are pageable and local
 }
 
@@ -656,6 +730,8 @@ int64_t DeviceHierarchy::bytes_resident() const {
     int64_t b = 0;
     auto csr = [&](const DevCSR& m) {
         b += m.rp.bytes() + m.cv.bytes() + m.v64.bytes() + m.v32.bytes() + m.v16.bytes() + m.blk.bytes();
+        b += m.c16.bytes() + m.hdr.bytes() + m.scol.bytes() + m.slot.bytes() + m.sv64.bytes() + m.sv32.bytes() +
+             m.sv16.bytes();
     };
     for (auto& l : L_) {
         csr(l.A); csr(l.P); csr(l.Pt);
@@ -762,18 +838,32 @@ void DeviceHierarchy::coarse_factor(hipStream_t st) {
     hipLaunchKernelGGL(k_dense_lu, dim3(1), dim3(kBlock), 0, st, n, clu_.p, flag_.p + 1, flag_.p);
 }
 
-void DeviceHierarchy::refresh_level_mirrors(int l, hipStream_t st) {
-    DevLevel& D = L_[l];
-    const double* s = D.A.v64.p;
-    if (D.A.v32.p) {
-        float* d = D.A.v32.p;
-        launch_map(D.A.nnz, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
+// the reduced-precision and column-sorted copies of a matrix's values
+void refresh_mirrors(DevCSR& M, hipStream_t st) {
+    const double* s = M.v64.p;
+    if (M.v32.p) {
+        float* d = M.v32.p;
+        launch_map(M.nnz, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
     }
-    if (D.A.v16.p) {
-        bf16_t* d = D.A.v16.p;
-        launch_map(D.A.nnz, [=] __device__(int64_t i) { d[i] = float_to_bf16((float)s[i]); }, st);
+    if (M.v16.p) {
+        bf16_t* d = M.v16.p;
+        launch_map(M.nnz, [=] __device__(int64_t i) { d[i] = float_to_bf16((float)s[i]); }, st);
+    }
+    if (M.hasS && M.nblk > 0) {
+        hipLaunchKernelGGL(k_permute_blocks, dim3(M.nblk), dim3(kBlock), 0, st, M.hdr.p, M.slot.p, M.v64.p, M.sv64.p);
+        const double* ss = M.sv64.p;
+        if (M.sv32.p) {
+            float* d = M.sv32.p;
+            launch_map(M.nnz, [=] __device__(int64_t i) { d[i] = (float)ss[i]; }, st);
+        }
+        if (M.sv16.p) {
+            bf16_t* d = M.sv16.p;
+            launch_map(M.nnz, [=] __device__(int64_t i) { d[i] = float_to_bf16((float)ss[i]); }, st);
+        }
     }
 }
+
+void DeviceHierarchy::refresh_level_mirrors(int l, hipStream_t st) { refresh_mirrors(L_[l].A, st); }
 
 void DeviceHierarchy::ensure_precision(Precision p) {
     if (p == Precision::FP64) return;
@@ -805,6 +895,12 @@ void DeviceHierarchy::ensure_precision(Precision p) {
                 const double* s = m->v64.p;
                 float* d = m->v32.p;
                 launch_map(m->nnz, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
+                if (m->hasS) {
+                    m->sv32.alloc(m->nnz);
+                    const double* ss = m->sv64.p;
+                    float* ds = m->sv32.p;
+                    launch_map(m->nnz, [=] __device__(int64_t i) { ds[i] = (float)ss[i]; }, st);
+                }
             }
             const double* s = D.d64.p;
             float* d = D.d32.p;
@@ -820,6 +916,12 @@ void DeviceHierarchy::ensure_precision(Precision p) {
             const double* s = D.A.v64.p;
             bf16_t* d = D.A.v16.p;
             launch_map(D.A.nnz, [=] __device__(int64_t i) { d[i] = float_to_bf16((float)s[i]); }, st);
+            if (D.A.hasS) {
+                D.A.sv16.alloc(D.A.nnz);
+                const double* ss = D.A.sv64.p;
+                bf16_t* ds = D.A.sv16.p;
+                launch_map(D.A.nnz, [=] __device__(int64_t i) { ds[i] = float_to_bf16((float)ss[i]); }, st);
+            }
         }
         have16_ = true;
     }
@@ -1019,14 +1121,17 @@ void DeviceHierarchy::tune() {
                 tA[k] = tPt[k] = tP[k] = k == forced_kind_ ? 0.f : 1e30f;
                 continue;
             }
-            const float skip = 1e30f;  // the 16-bit form only where the operator has it
-            tA[k] = (k == kOpStream16 && !D.A.has16) ? skip : timeit([&] {
+            const float skip = 1e30f;  // the 16-bit and sorted forms only where the operator has them
+            auto absent = [&](const DevCSR& M) {
+                return (k == kOpStream16 && !M.has16) || (k == kOpSorted && !M.hasS);
+            };
+            tA[k] = absent(D.A) ? skip : timeit([&] {
                 op_kind<TV, TX>(k, D.A, XVec<TX>{xin}, EpiJacobi<TX>{xin, bin, d, s1, w}, st);
             });
-            tPt[k] = (k == kOpStream16 && !D.Pt.has16) ? skip : timeit([&] {
+            tPt[k] = absent(D.Pt) ? skip : timeit([&] {
                 op_kind<TP, TX>(k, D.Pt, XVec<TX>{xin}, EpiStore<TX>{s2}, st);
             });
-            tP[k] = (k == kOpStream16 && !D.P.has16) ? skip : timeit([&] {
+            tP[k] = absent(D.P) ? skip : timeit([&] {
                 op_kind<TP, TX>(k, D.P, XVec<TX>{xcv}, EpiAdd<TX>{s2}, st);
             });
         }
@@ -1085,7 +1190,7 @@ std::string DeviceHierarchy::tune_report() const {
         if (!tuned_[p]) continue;
         for (int l = 0; l < nlevels(); ++l) {
             const DevLevel& D = L_[l];
-            const char* kn[kOpKinds] = {"stream", "rowseq", "pipe", "stream16"};
+            const char* kn[kOpKinds] = {"stream", "rowseq", "pipe", "stream16", "sorted"};
             std::snprintf(buf, sizeof buf, "%s level %d: A=%s Pt=%s P=%s zero-guess=%s down=%s up=%s\n", names[p], l,
                           kn[D.A.kind[p]], kn[D.Pt.kind[p]], kn[D.P.kind[p]], D.fuse_zero[p] ? "fused" : "split",
                           D.fuse_down[p] ? "one-pass" : (D.down_state < 0 ? "separate(only)" : "separate"),
@@ -1568,6 +1673,8 @@ void DeviceHierarchy::resetup(const double* nzval, hipStream_t st) {
 static double index_bytes(const DevCSR& M, int pi) {
     if (M.kind[pi] == kOpStream16 && M.has16)
         return 2.0 * M.nnz16 + 4.0 * (M.nnz - M.nnz16) + 16.0 * M.nblk;  // + the block headers
+    if (M.kind[pi] == kOpSorted && M.hasS)  // + the slot of every entry of a 16-bit block
+        return 4.0 * M.nnz16 + 4.0 * (M.nnz - M.nnz16) + 16.0 * M.nblk;
     return 4.0 * M.nnz;
 }
 

@@ -235,11 +235,19 @@ struct alignas(16) BlkHdr {
 // (col = cb + c16[k]), 2 bytes per nonzero instead of 4, when the block spans
 // fewer than 65536 columns (cb >= 0); wider and longer blocks read the 32-bit
 // columns.
-template <int PER, bool C16, typename TV, typename TX, class XF, class PREF, class SEGF, class TOTF>
+//
+// SRT (with C16): the entries of each such block are stored sorted by column
+// (c16 and av in that order) with slot[k] = the entry's position in the block's
+// row order, where its product is staged.  A wave's gathers then touch a few
+// adjacent cache lines instead of the lines of every column cluster of its
+// rows; the staged products, and so the row sums, are the same.  Other blocks
+// keep row order (av in row order, columns from cv).
+template <int PER, bool C16, typename TV, typename TX, class XF, class PREF, class SEGF, class TOTF,
+          bool SRT = false>
 __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ hdr, const int* __restrict__ rp,
                                              const int* __restrict__ cv, const uint16_t* __restrict__ c16,
                                              const TV* __restrict__ av, XF xf, TX* prod, PREF pref, SEGF segf,
-                                             TOTF totf) {
+                                             TOTF totf, const uint16_t* __restrict__ slot = nullptr) {
     const BlkHdr h0 = hdr[b];
     const int r0 = h0.r0, r1 = h0.r1;
     const int z0 = h0.z0, z1 = hdr[b + 1].z0;
@@ -254,7 +262,7 @@ __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ h
         if (own) q = pref(row);
         if (nz <= PER * kBlock) {
             if (nz > 0) {
-                int cj[PER];
+                int cj[PER], sl[PER];
                 TX v[PER], g[PER];
                 if (C16 && cb >= 0) {  // block-uniform choice, outside the unrolled loads
 #pragma unroll
@@ -262,6 +270,7 @@ __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ h
                         const int k = min(t + u * kBlock, nz - 1);
                         cj[u] = cb + ldc16(c16, z0 + k);
                         v[u] = ld<TX, TV>(av, z0 + k);
+                        sl[u] = SRT ? ldc16(slot, z0 + k) : t + u * kBlock;
                     }
                 } else {
 #pragma unroll
@@ -269,13 +278,14 @@ __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ h
                         const int k = min(t + u * kBlock, nz - 1);
                         cj[u] = ldi(cv, z0 + k);
                         v[u] = ld<TX, TV>(av, z0 + k);
+                        sl[u] = t + u * kBlock;
                     }
                 }
 #pragma unroll
                 for (int u = 0; u < PER; ++u) g[u] = xf(cj[u]);
 #pragma unroll
                 for (int u = 0; u < PER; ++u)
-                    if (t + u * kBlock < nz) prod[t + u * kBlock] = v[u] * g[u];
+                    if (t + u * kBlock < nz) prod[sl[u]] = v[u] * g[u];
             }
         } else {  // a single row longer than the budget (still summed in row order)
 #pragma unroll 4
@@ -304,16 +314,18 @@ __device__ __forceinline__ void epi_segment(const EPI& epi, int row, const TX* s
     epi.fin(row, a, q);
 }
 
-template <bool XCD, int PER, bool C16, typename TV, typename TX, class XF, class EPI>
+template <bool XCD, int PER, bool C16, typename TV, typename TX, class XF, class EPI, bool SRT = false>
 __global__ __launch_bounds__(kBlock) void k_csr_stream(const BlkHdr* __restrict__ hdr, const int* __restrict__ rp,
                                                        const int* __restrict__ cv, const uint16_t* __restrict__ c16,
-                                                       const TV* __restrict__ av, XF xf, EPI epi) {
+                                                       const TV* __restrict__ av, XF xf, EPI epi,
+                                                       const uint16_t* __restrict__ slot) {
     __shared__ TX prod[kStreamNnz];
     const int b = XCD ? xcd_block() : (int)blockIdx.x;
-    stream_block<PER, C16, TV, TX>(
-        b, hdr, rp, cv, c16, av, xf, prod, [&](int row) { return epi.pre(row); },
-        [&](int row, const TX* seg, int len, const typename EPI::Pre& q) { epi_segment(epi, row, seg, len, q); },
-        [&](int row, TX s) { epi.total(row, s); });
+    auto pref = [&](int row) { return epi.pre(row); };
+    auto segf = [&](int row, const TX* seg, int len, const typename EPI::Pre& q) { epi_segment(epi, row, seg, len, q); };
+    auto totf = [&](int row, TX s) { epi.total(row, s); };
+    stream_block<PER, C16, TV, TX, XF, decltype(pref), decltype(segf), decltype(totf), SRT>(
+        b, hdr, rp, cv, c16, av, xf, prod, pref, segf, totf, slot);
 }
 
 // Thread-per-row variant without LDS: fewer dependent round trips, for the
@@ -445,39 +457,41 @@ void launch_pipe(int nblk, int per, const int* blk, const int* bz, const int* rp
     }
 }
 
-enum OpKernel : int { kOpStream = 0, kOpRowSeq = 1, kOpPipe = 2, kOpStream16 = 3 };
-constexpr int kOpKinds = 4;
+enum OpKernel : int { kOpStream = 0, kOpRowSeq = 1, kOpPipe = 2, kOpStream16 = 3, kOpSorted = 4 };
+constexpr int kOpKinds = 5;
 
-template <int PER, bool C16, typename TV, typename TX, class XF, class EPI>
+template <int PER, bool C16, bool SRT, typename TV, typename TX, class XF, class EPI>
 void launch_stream_per(int nblk, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16, const TV* av,
-                       XF xf, EPI epi, hipStream_t st) {
+                       XF xf, EPI epi, hipStream_t st, const uint16_t* slot) {
     if (nblk >= 64)
-        hipLaunchKernelGGL((k_csr_stream<true, PER, C16, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, hdr, rp,
-                           cv, c16, av, xf, epi);
+        hipLaunchKernelGGL((k_csr_stream<true, PER, C16, TV, TX, XF, EPI, SRT>), dim3(nblk), dim3(kBlock), 0, st, hdr,
+                           rp, cv, c16, av, xf, epi, slot);
     else
-        hipLaunchKernelGGL((k_csr_stream<false, PER, C16, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, hdr,
-                           rp, cv, c16, av, xf, epi);
+        hipLaunchKernelGGL((k_csr_stream<false, PER, C16, TV, TX, XF, EPI, SRT>), dim3(nblk), dim3(kBlock), 0, st,
+                           hdr, rp, cv, c16, av, xf, epi, slot);
 }
 
-template <bool C16, typename TV, typename TX, class XF, class EPI>
+template <bool C16, bool SRT, typename TV, typename TX, class XF, class EPI>
 void launch_stream_c(int nblk, int per, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16,
-                     const TV* av, XF xf, EPI epi, hipStream_t st) {
+                     const TV* av, XF xf, EPI epi, hipStream_t st, const uint16_t* slot) {
     switch (per) {
-        case 1: launch_stream_per<1, C16, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st); break;
-        case 2: launch_stream_per<2, C16, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st); break;
-        case 4: launch_stream_per<4, C16, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st); break;
-        default: launch_stream_per<8, C16, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st); break;
+        case 1: launch_stream_per<1, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+        case 2: launch_stream_per<2, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+        case 4: launch_stream_per<4, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+        default: launch_stream_per<8, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
     }
 }
 
 // per = the operator's block budget / kBlock (1, 2, 4 or 8); c16 non-null =
-// the 16-bit column form (bases in the block headers)
+// the 16-bit column form (bases in the block headers); slot non-null = its
+// column-sorted form (c16 and av sorted within each 16-bit block)
 template <typename TV, typename TX, class XF, class EPI>
 void launch_stream(int nblk, int per, const BlkHdr* hdr, const int* rp, const int* cv, const TV* av, XF xf, EPI epi,
-                   hipStream_t st, const uint16_t* c16 = nullptr) {
+                   hipStream_t st, const uint16_t* c16 = nullptr, const uint16_t* slot = nullptr) {
     if (nblk <= 0) return;
-    if (c16) launch_stream_c<true, TV, TX>(nblk, per, hdr, rp, cv, c16, av, xf, epi, st);
-    else launch_stream_c<false, TV, TX>(nblk, per, hdr, rp, cv, nullptr, av, xf, epi, st);
+    if (c16 && slot) launch_stream_c<true, true, TV, TX>(nblk, per, hdr, rp, cv, c16, av, xf, epi, st, slot);
+    else if (c16) launch_stream_c<true, false, TV, TX>(nblk, per, hdr, rp, cv, c16, av, xf, epi, st, nullptr);
+    else launch_stream_c<false, false, TV, TX>(nblk, per, hdr, rp, cv, nullptr, av, xf, epi, st, nullptr);
 }
 
 template <typename TV, typename TX, class XF, class EPI>

@@ -273,7 +273,7 @@ Config config_from_json(const char* json) {
     if (const JVal* v = get("gpu_setup_min_rows")) cfg.gpu_setup_min_rows = to_int(*v);
     if (const JVal* v = get("kernel")) {
         static const std::map<std::string, int> kKernel = {{"auto", -1}, {"stream", 0}, {"rowseq", 1}, {"pipe", 2},
-                                                                 {"stream16", 3}};
+                                                                 {"stream16", 3}, {"sorted", 4}};
         cfg.kernel = lookup(kKernel, to_string(*v), "kernel");
     }
     if (const JVal* v = get("filtering")) {

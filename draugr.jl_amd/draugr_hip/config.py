@@ -170,7 +170,7 @@ class AMGConfig:
     precision: str = "fp64"
     seed: int = 0x5EED
     graphs: bool = True
-    kernel: str = "auto"  # sparse-pass kernel: "auto" (autotuned), "stream", "rowseq", "pipe", "stream16"
+    kernel: str = "auto"  # sparse-pass kernel: "auto" (autotuned), "stream", "rowseq", "pipe", "stream16", "sorted"
     fuse: str = "auto"  # zero-guess step + residual, one-pass down / up sweeps: "auto" (autotuned), "off", "on"
     gpu_setup: bool = True  # interpolation, coarse operators, triple maps and PMIS sweeps of the setup on the GPU
     gpu_setup_min_rows: int = 32768  # smaller levels interpolate on the host

@@ -39,7 +39,7 @@ const char *draugr_amg_last_error(void);
  * "precision" ("fp64" | "fp32" | "bf16": arithmetic of the V/W cycle),
  * "seed" (setup RNG seed), "graphs" (true: the PCG / mixed-precision preconditioner cycle is
  * captured once into a hipGraph and replayed; false: launched eagerly),
- * "kernel" ("auto" | "stream" | "rowseq" | "pipe" | "stream16"), "fuse" ("auto"
+ * "kernel" ("auto" | "stream" | "rowseq" | "pipe" | "stream16" | "sorted"), "fuse" ("auto"
  * | "off" | "on": a level's first smoothing step from zero fused with its
  * residual, that step, residual and restriction in one pass, and its
  * prolongation and first post-smoothing step in one pass; Jacobi-type

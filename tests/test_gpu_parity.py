@@ -332,13 +332,15 @@ def test_preconditioner_adapter_with_scipy_cg(dh, torch_cuda):
     assert info2 == 0
 
 
-@pytest.mark.parametrize("kernel", ["stream", "rowseq", "pipe", "stream16"])
+@pytest.mark.parametrize("kernel", ["stream", "rowseq", "pipe", "stream16", "sorted"])
 @pytest.mark.parametrize("name", ["hypre3d", "agg2d", "wide2d"])
 def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
-    """Each of the four sparse-pass kernels the autotuner chooses among,
-    forced on every level, gives the oracle's FP64 cycle bit for bit and the
-    FP32-cycle PCG converges.  The 16-bit column form is present on every
-    level of these problems (wide blocks, if any, read the 32-bit columns)."""
+    """Each of the five sparse-pass kernels the autotuner chooses among,
+    forced on every level, gives the oracle's FP64 cycle bit for bit, also
+    after a resetup (new values: the reduced-precision and column-sorted
+    copies follow), and the FP32-cycle PCG converges.  The 16-bit column form
+    and its column-sorted copy are present on every level of these problems
+    (wide blocks, if any, read the 32-bit columns in row order)."""
     import dataclasses
     torch = torch_cuda
     A, cfg, sm = _case(dh, name)
@@ -353,9 +355,27 @@ def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
     dh.amg_cycle_(x, torch.tensor(b, device="cuda"), h, c)
     assert np.array_equal(x.cpu().numpy(), oh.cycle(x0, b))
     z = torch.zeros(A.n, dtype=torch.float64, device="cuda")
-    _, it, rel = dh.pcg_solve_(z, torch.tensor(b, device="cuda"), h, dataclasses.replace(c, precision="fp32"),
-                              tol=1e-8)
+    c32 = dataclasses.replace(c, precision="fp32")
+    _, it, rel = dh.pcg_solve_(z, torch.tensor(b, device="cuda"), h, c32, tol=1e-8)
     assert rel < 1e-8
+    # new values, same pattern
+    s = np.exp(0.3 * np.random.default_rng(SEED + 902).standard_normal(A.n))
+    rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+    new = A.nzval * s[rows] * s[A.colval]
+    dh.amg_resetup_(h, dh.problems.CSR(A.rowptr, A.colval, new), c)
+    oh.resetup(new)
+    x = torch.tensor(x0, device="cuda")
+    dh.amg_cycle_(x, torch.tensor(b, device="cuda"), h, c)
+    assert np.array_equal(x.cpu().numpy(), oh.cycle(x0, b))
+    # the FP32 cycle after the resetup equals, bit for bit, the row-order
+    # 32-bit-column kernel's on a second hierarchy with the same values
+    cs = dataclasses.replace(cfg, kernel="stream")
+    h2 = dh.amg_setup(A, cs)
+    dh.amg_resetup_(h2, dh.problems.CSR(A.rowptr, A.colval, new), cs)
+    y = [torch.tensor(x0, device="cuda") for _ in range(2)]
+    dh.amg_cycle_(y[0], torch.tensor(b, device="cuda"), h, c32)
+    dh.amg_cycle_(y[1], torch.tensor(b, device="cuda"), h2, dataclasses.replace(cs, precision="fp32"))
+    assert np.array_equal(y[0].cpu().numpy(), y[1].cpu().numpy())
 
 
 @pytest.mark.parametrize("name", ["hypre3d", "agg2d"])
