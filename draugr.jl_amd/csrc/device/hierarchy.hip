@@ -94,43 +94,55 @@ void launch_reduce(int64_t n, F f, FIN fin, double* partial, hipStream_t st) {
 
 // CSR-stream pass whose per-row epilogue also returns a contribution to a
 // global sum (q = A p fused with p.q in PCG).
-template <int PER, bool C16, typename TV, typename TX, class XF, class EPI>
+template <int PER, bool C16, bool SRT, typename TV, typename TX, class XF, class EPI>
 __global__ __launch_bounds__(kBlock) void k_stream_reduce(const BlkHdr* __restrict__ hdr, const int* __restrict__ rp,
                                                           const int* __restrict__ cv,
                                                           const uint16_t* __restrict__ c16, const TV* __restrict__ av,
-                                                          XF xf, EPI epi, double* partial) {
+                                                          XF xf, EPI epi, double* partial,
+                                                          const uint16_t* __restrict__ slot) {
     __shared__ TX prod[kStreamNnz];
     __shared__ double sh[kBlock];
     double dot = 0.0;
-    stream_block<PER, C16, TV, TX>(
-        xcd_block(), hdr, rp, cv, c16, av, xf, prod, [](int) { return NoPre{}; },
-        [&](int row, const TX* seg, int len, NoPre) {
-            TX a = 0;
-            for (int k = 0; k < len; ++k) a += seg[k];
-            dot = epi(row, a);
-        },
-        [&](int row, TX s) { dot = epi(row, s); });
+    auto pref = [](int) { return NoPre{}; };
+    auto segf = [&](int row, const TX* seg, int len, NoPre) {
+        TX a = 0;
+        for (int k = 0; k < len; ++k) a += seg[k];
+        dot = epi(row, a);
+    };
+    auto totf = [&](int row, TX s) { dot = epi(row, s); };
+    stream_block<PER, C16, TV, TX, XF, decltype(pref), decltype(segf), decltype(totf), SRT>(
+        xcd_block(), hdr, rp, cv, c16, av, xf, prod, pref, segf, totf, slot);
     block_sum_store(dot, sh, partial, blockIdx.x);
 }
 
+// sorted: the column-sorted form (values from M.sv64; TV = double)
 template <typename TV, typename TX, class XF, class EPI, class FIN>
-void launch_stream_reduce(const DevCSR& M, const TV* av, XF xf, EPI epi, FIN fin, double* partial, hipStream_t st) {
-#define DRAUGR_SR(PER, C)                                                                                       \
-    hipLaunchKernelGGL((k_stream_reduce<PER, C, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.hdr.p,     \
-                       M.rp.p, M.cv.p, M.c16.p, av, xf, epi, partial)
-    if (M.has16) {
+void launch_stream_reduce(const DevCSR& M, const TV* av, XF xf, EPI epi, FIN fin, double* partial, hipStream_t st,
+                          bool sorted = false) {
+#define DRAUGR_SR(PER, C, S, COLS, VALS, SLOT)                                                                   \
+    hipLaunchKernelGGL((k_stream_reduce<PER, C, S, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.hdr.p,  \
+                       M.rp.p, M.cv.p, COLS, VALS, xf, epi, partial, SLOT)
+    if (sorted && M.hasS) {
+        const TV* sv = reinterpret_cast<const TV*>(M.sv64.p);
         switch (M.per) {
-            case 1: DRAUGR_SR(1, true); break;
-            case 2: DRAUGR_SR(2, true); break;
-            case 4: DRAUGR_SR(4, true); break;
-            default: DRAUGR_SR(8, true); break;
+            case 1: DRAUGR_SR(1, true, true, M.scol.p, sv, M.slot.p); break;
+            case 2: DRAUGR_SR(2, true, true, M.scol.p, sv, M.slot.p); break;
+            case 4: DRAUGR_SR(4, true, true, M.scol.p, sv, M.slot.p); break;
+            default: DRAUGR_SR(8, true, true, M.scol.p, sv, M.slot.p); break;
+        }
+    } else if (M.has16) {
+        switch (M.per) {
+            case 1: DRAUGR_SR(1, true, false, M.c16.p, av, nullptr); break;
+            case 2: DRAUGR_SR(2, true, false, M.c16.p, av, nullptr); break;
+            case 4: DRAUGR_SR(4, true, false, M.c16.p, av, nullptr); break;
+            default: DRAUGR_SR(8, true, false, M.c16.p, av, nullptr); break;
         }
     } else {
         switch (M.per) {
-            case 1: DRAUGR_SR(1, false); break;
-            case 2: DRAUGR_SR(2, false); break;
-            case 4: DRAUGR_SR(4, false); break;
-            default: DRAUGR_SR(8, false); break;
+            case 1: DRAUGR_SR(1, false, false, M.c16.p, av, nullptr); break;
+            case 2: DRAUGR_SR(2, false, false, M.c16.p, av, nullptr); break;
+            case 4: DRAUGR_SR(4, false, false, M.c16.p, av, nullptr); break;
+            default: DRAUGR_SR(8, false, false, M.c16.p, av, nullptr); break;
         }
     }
 #undef DRAUGR_SR
@@ -839,7 +851,7 @@ void DeviceHierarchy::coarse_factor(hipStream_t st) {
 }
 
 // the reduced-precision and column-sorted copies of a matrix's values
-void refresh_mirrors(DevCSR& M, hipStream_t st) {
+static void refresh_mirrors(DevCSR& M, hipStream_t st) {
     const double* s = M.v64.p;
     if (M.v32.p) {
         float* d = M.v32.p;
@@ -1469,6 +1481,47 @@ int DeviceHierarchy::solve(double* x, const double* b, double tol, int maxiter, 
     return maxiter;
 }
 
+// Whether the PCG's q = A p pass reads the column-sorted form: timed once
+// against the 16-bit form (4 back-to-back launches, best of 3, on scratch
+// vectors with the pass's two gathers per nonzero); a forced kernel kind
+// decides instead.  Both forms give the same values.
+bool DeviceHierarchy::pcg_sorted(const DevCSR& A, hipStream_t st) {
+    if (pcg_sorted_ >= 0) return pcg_sorted_ == 1;
+    pcg_sorted_ = 0;
+    if (!A.hasS) return false;
+    if (forced_kind_ >= 0) {
+        pcg_sorted_ = forced_kind_ == kOpSorted;
+        return pcg_sorted_ == 1;
+    }
+    const double* u = w_z_.p;
+    const double* v = w_p0_.p;
+    double* q = w_q_.p;
+    auto xf = [=] __device__(int j) -> double { return u[j] + 0.5 * v[j]; };
+    auto epi = [=] __device__(int i, double s) -> double {
+        q[i] = s;
+        return s;
+    };
+    auto fin = [=] __device__(double) {};
+    hipEvent_t e0, e1;
+    DRAUGR_HIP_CHECK(hipEventCreate(&e0));
+    DRAUGR_HIP_CHECK(hipEventCreate(&e1));
+    float best[2] = {1e30f, 1e30f};
+    for (int rep = 0; rep < 3; ++rep)
+        for (int s = 0; s < 2; ++s) {
+            DRAUGR_HIP_CHECK(hipEventRecord(e0, st));
+            for (int k = 0; k < 4; ++k) launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin, partial_.p, st, s);
+            DRAUGR_HIP_CHECK(hipEventRecord(e1, st));
+            DRAUGR_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            DRAUGR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            best[s] = std::min(best[s], ms);
+        }
+    DRAUGR_HIP_CHECK(hipEventDestroy(e0));
+    DRAUGR_HIP_CHECK(hipEventDestroy(e1));
+    pcg_sorted_ = best[1] < best[0];
+    return pcg_sorted_ == 1;
+}
+
 // FP64 preconditioned CG (Hestenes-Stiefel) with z = V(0; r).  Stopping test
 // on the recurrence residual ||r||/||b|| < tol.  Kernel fusion: p = z + beta p
 // is formed inside the q = A p pass (ping-pong p), the pass also reduces p.q
@@ -1498,6 +1551,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
         return 0;
     }
     const DevCSR& A = nlevels() > 0 ? L_[0].A : Ac_;
+    const bool srt = pcg_sorted(A, st);
     op<double, double>(A, XVec<double>{x}, EpiResid<double>{b, r}, st);  // r = b - A x
     launch_reduce(n, [=] __device__(int64_t i) { return r[i] * r[i]; },
                   [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, st);
@@ -1554,7 +1608,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                     q[i] = s;
                     return pn * s;
                 };
-                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, partial_.p, st);
+                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, partial_.p, st, srt);
             } else {
                 auto xf = [=] __device__(int j) -> double { return z[j] + *betap * pa[j]; };
                 auto epi = [=] __device__(int i, double s) -> double {
@@ -1563,7 +1617,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                     q[i] = s;
                     return pn * s;
                 };
-                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, partial_.p, st);
+                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, partial_.p, st, srt);
             }
             // x += alpha p; r -= alpha q; r.r  (and the FP32 copy of r).  The
             // finalize also writes r.r and the iteration number straight into
@@ -1657,6 +1711,7 @@ void DeviceHierarchy::resetup(const double* nzval, hipStream_t st) {
             }
         }
     }
+    refresh_mirrors(Ac_, st);  // the PCG's operator when there are no AMG levels
     coarse_factor(st);
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
     int fl = 0;

@@ -200,6 +200,8 @@ private:
     uint64_t seed_ = 0;
     int forced_kind_ = -1;  // config "kernel": every sparse pass uses this kernel (no tuning)
     int forced_fuse_ = -1;  // config "fuse": -1 tuned, 0 never, 1 always (Jacobi-type smoothers)
+    int pcg_sorted_ = -1;   // PCG's q = A p pass on the column-sorted form: -1 not timed yet, 0 no, 1 yes
+    bool pcg_sorted(const DevCSR& A, hipStream_t st);
     bool have32_ = false, have16_ = false;
     bool tuned_[3] = {false, false, false};
     // solver workspaces at the finest size
