@@ -376,6 +376,13 @@ def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
     dh.amg_cycle_(y[0], torch.tensor(b, device="cuda"), h, c32)
     dh.amg_cycle_(y[1], torch.tensor(b, device="cuda"), h2, dataclasses.replace(cs, precision="fp32"))
     assert np.array_equal(y[0].cpu().numpy(), y[1].cpu().numpy())
+    # and so is the FP64 PCG (its q = A p pass reads the column-sorted form when that kernel is forced)
+    out = []
+    for hh, cc in ((h, c), (h2, cs)):
+        zz = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+        _, it2, rel2 = dh.pcg_solve_(zz, torch.tensor(b, device="cuda"), hh, cc, tol=1e-8, maxiter=400)
+        out.append((it2, rel2, zz.cpu().numpy()))
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1] and np.array_equal(out[0][2], out[1][2])
 
 
 @pytest.mark.parametrize("name", ["hypre3d", "agg2d"])
