@@ -24,6 +24,9 @@ namespace draugr {
 namespace {
 
 constexpr double kEps = std::numeric_limits<double>::epsilon();
+// Loops over fewer nodes run on one thread: waking the team costs more than
+// the loop on the small levels (every parallel loop here is order-independent).
+constexpr int32_t kOmpMin = 32768;
 
 inline bool strong_offdiag(const std::vector<uint8_t>& s, int64_t nz, int32_t j, int32_t i) {
     return j != i && s[nz];
@@ -47,7 +50,7 @@ void uf_union(std::vector<int32_t>& parent, int32_t a, int32_t b) {
 std::vector<int32_t> strong_transpose_count(const CSR& A, const std::vector<uint8_t>& s) {
     std::vector<int32_t> cnt(A.nrow, 0);
     int32_t* c = cnt.data();
-#pragma omp parallel for schedule(static, 4096)
+#pragma omp parallel for schedule(static, 4096) if(A.nrow >= kOmpMin)
     for (int32_t i = 0; i < A.nrow; ++i)
         for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
             if (strong_offdiag(s, k, A.cv[k], i)) {
@@ -71,7 +74,7 @@ void strong_transpose_adj(const CSR& A, const std::vector<uint8_t>& s,
     std::vector<int64_t> pos(off.begin(), off.end() - 1);
     int64_t* ps = pos.data();
     int32_t* sr = src.data();
-#pragma omp parallel for schedule(static, 4096)
+#pragma omp parallel for schedule(static, 4096) if(n >= kOmpMin)
     for (int32_t i = 0; i < n; ++i)
         for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
             if (strong_offdiag(s, k, A.cv[k], i)) {
@@ -81,7 +84,7 @@ void strong_transpose_adj(const CSR& A, const std::vector<uint8_t>& s,
                 sr[p] = i;
             }
     const int64_t* o = off.data();
-#pragma omp parallel for schedule(static, 4096)
+#pragma omp parallel for schedule(static, 4096) if(n >= kOmpMin)
     for (int32_t j = 0; j < n; ++j) std::sort(sr + o[j], sr + o[j + 1]);
 }
 
@@ -138,7 +141,7 @@ std::vector<int8_t> rs_first_pass(const CSR& A, const std::vector<uint8_t>& s, b
     {
         const std::vector<int32_t> lam0 = strong_transpose_count(A, s);
         Node* q = nd.data();
-#pragma omp parallel for schedule(static, 4096)
+#pragma omp parallel for schedule(static, 4096) if(n >= kOmpMin)
         for (int32_t i = 0; i < n; ++i)
             q[i] = Node{-1, -1, lam0[i], (!has_strong_offdiag(A, s, i) && lam0[i] == 0) ? 1 : 0};
     }
@@ -214,7 +217,7 @@ void sweeps_host(std::vector<int8_t>& cf, int32_t n, const OFF* off, const int32
     std::vector<int8_t> nxt(cf);
     for (int64_t iter = 0; iter <= (int64_t)n; ++iter) {
         int any_undecided = 0;
-#pragma omp parallel for schedule(static, 4096) reduction(| : any_undecided)
+#pragma omp parallel for schedule(static, 4096) reduction(| : any_undecided) if(n >= kOmpMin)
         for (int32_t i = 0; i < n; ++i) {
             if (cf[i] != 0) continue;
             any_undecided = 1;
@@ -230,7 +233,7 @@ void sweeps_host(std::vector<int8_t>& cf, int32_t n, const OFF* off, const int32
             if (is_max) nxt[i] = 1;
         }
         cf = nxt;
-#pragma omp parallel for schedule(static, 4096)
+#pragma omp parallel for schedule(static, 4096) if(n >= kOmpMin)
         for (int32_t i = 0; i < n; ++i) {
             if (cf[i] != 0) continue;
             for (OFF k = off[i]; k < off[i + 1]; ++k)
@@ -283,7 +286,7 @@ void pmis_sweeps(std::vector<int8_t>& cf, int32_t n, const int64_t* off, const i
 // Absolute strength of connection; src/strength.jl:24-61.
 std::vector<uint8_t> strength_absolute(const CSR& A, double theta) {
     std::vector<uint8_t> s(A.nnz(), 0);
-#pragma omp parallel for schedule(static, 4096)
+#pragma omp parallel for schedule(static, 4096) if(A.nrow >= kOmpMin)
     for (int32_t i = 0; i < A.nrow; ++i) {
         double mx = 0.0;
         for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
@@ -501,7 +504,7 @@ void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& r
     // the sequential build (a row appears at most once in a list).
     std::vector<int64_t> sc_off(n + 1, 0);
     int64_t* so = sc_off.data();
-#pragma omp parallel for schedule(static, 4096)
+#pragma omp parallel for schedule(static, 4096) if(n >= kOmpMin)
     for (int32_t i = 0; i < n; ++i) {
         int64_t c = 0;
         for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
@@ -511,7 +514,7 @@ void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& r
     for (int32_t i = 0; i < n; ++i) sc_off[i + 1] += sc_off[i];
     std::vector<int32_t> sc(sc_off[n]);
     int32_t* scp = sc.data();
-#pragma omp parallel for schedule(static, 4096)
+#pragma omp parallel for schedule(static, 4096) if(n >= kOmpMin)
     for (int32_t i = 0; i < n; ++i) {
         int64_t p = so[i];
         for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
@@ -521,7 +524,7 @@ void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& r
     {
         std::vector<int32_t> tcnt(m, 0);
         int32_t* tc = tcnt.data();
-#pragma omp parallel for schedule(static, 65536)
+#pragma omp parallel for schedule(static, 65536) if(n >= kOmpMin)
         for (int64_t t = 0; t < sc_off[n]; ++t) {
 #pragma omp atomic
             tc[scp[t]]++;
@@ -533,7 +536,7 @@ void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& r
         std::vector<int64_t> pos(tr_off.begin(), tr_off.end() - 1);
         int64_t* ps = pos.data();
         int32_t* trp = tr.data();
-#pragma omp parallel for schedule(static, 4096)
+#pragma omp parallel for schedule(static, 4096) if(n >= kOmpMin)
         for (int32_t i = 0; i < n; ++i)
             for (int64_t t = so[i]; t < so[i + 1]; ++t) {
                 int64_t p;
@@ -542,7 +545,7 @@ void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& r
                 trp[p] = i;
             }
         const int64_t* to = tr_off.data();
-#pragma omp parallel for schedule(static, 4096)
+#pragma omp parallel for schedule(static, 4096) if(m >= kOmpMin)
         for (int32_t a = 0; a < m; ++a) std::sort(trp + to[a], trp + to[a + 1]);
     }
     // Neighbour set of C point a (local ids): pairs of strong C neighbours of a
@@ -556,7 +559,7 @@ void coarsen_aggressive_cf(const CSR& A_in, double theta, bool hmis_base, Rng& r
             for (int32_t a = 0; a < m; ++a) g_off[a + 1] += g_off[a];
             g.resize(g_off[m]);
         }
-#pragma omp parallel
+#pragma omp parallel if(m >= kOmpMin)
         {
             std::vector<int32_t> mark(m, -1);
 #pragma omp for schedule(dynamic, 1024)
