@@ -1210,6 +1210,10 @@ std::string DeviceHierarchy::tune_report() const {
             out += buf;
         }
     }
+    if (pcg_sorted_ >= 0) {  // the form of the PCG's q = A p pass, once a PCG has run
+        const DevCSR& A0 = nlevels() > 0 ? L_[0].A : Ac_;
+        out += pcg_sorted_ ? "pcg q = A p: sorted\n" : (A0.has16 ? "pcg q = A p: stream16\n" : "pcg q = A p: stream\n");
+    }
     return out;
 }
 
