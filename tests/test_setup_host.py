@@ -169,3 +169,33 @@ def test_setup_is_deterministic_for_a_seed(dh):
         assert np.array_equal(a[k], b[k])
     c = dh.amg_setup(A, dh.hypre_default_config(seed=12345), upload=False).level(0)
     assert c["n_coarse"] > 0
+
+
+def test_setup_generator_is_the_problems_splitmix(dh):
+    """The oracle's stand-in for the reference's rand(rng) is the same
+    SplitMix64 stream as the product's own counter-based generator."""
+    for seed in (0x5EED, SEED, 1):
+        assert np.array_equal(ref.splitmix_uniforms(seed, 257), dh.problems.splitmix_uniform(seed, 257))
+
+
+@pytest.mark.parametrize("name", ["poisson1d", "poisson2d", "aniso", "hetero3d", "random"])
+def test_pmis_matches_reference_sequential_sweeps(dh, name):
+    """The host's PMIS (snapshot sweeps, run in parallel) decides every node as
+    the reference's sequential in-place sweeps do (src/coarsening.jl:202-283),
+    on the setup's seeded draws: with direct interpolation each C point's row
+    of P is its coarse index with weight 1, so every C point of the
+    restatement must have exactly that row, and the coarse counts agree."""
+    A = _mats(dh)[name]
+    rp, cv, nz = (np.asarray(a) for a in (A.rowptr, A.colval, A.nzval))
+    theta = 0.25
+    cfg = dh.AMGConfig(coarsening=dh.PMISCoarsening(theta), smoother=dh.JacobiSmootherType(), max_coarse_size=4)
+    h = dh.amg_setup(A, cfg, upload=False)
+    cf = ref.coarsen_pmis(rp, cv, nz, theta, cfg.seed)
+    n = len(rp) - 1
+    L = h.level(0, with_rmap=False)
+    assert L["n_coarse"] < 0.8 * n  # no theta-halving retry (which would draw again)
+    assert L["n_coarse"] == int((cf == 1).sum())
+    coarse = np.cumsum(cf == 1) - 1
+    Prp, Pcv, Pnz = L["P_rp"], L["P_cv"], L["P_nz"]
+    for i in np.flatnonzero(cf == 1):
+        assert Prp[i + 1] - Prp[i] == 1 and Pcv[Prp[i]] == coarse[i] and Pnz[Prp[i]] == 1.0, (name, i)
