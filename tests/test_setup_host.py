@@ -178,22 +178,28 @@ def test_setup_generator_is_the_problems_splitmix(dh):
         assert np.array_equal(ref.splitmix_uniforms(seed, 257), dh.problems.splitmix_uniform(seed, 257))
 
 
-@pytest.mark.parametrize("coarsening", ["pmis", "hmis"])
+@pytest.mark.parametrize("coarsening", ["pmis", "hmis", "aggressive_hmis", "aggressive_pmis"])
 @pytest.mark.parametrize("name", ["poisson1d", "poisson2d", "aniso", "hetero3d", "random"])
 def test_pmis_matches_reference_sequential_sweeps(dh, name, coarsening):
-    """The host's PMIS and HMIS (snapshot sweeps, run in parallel; HMIS's RS
-    first pass over bucket lists) decide every node as the reference's
-    sequential code does (src/coarsening.jl:202-283, 330-624), on the setup's
-    seeded draws: with direct interpolation each C point's row of P is its
+    """The host's PMIS, HMIS and aggressive CF coarsening (snapshot sweeps, run
+    in parallel; HMIS's RS first pass over bucket lists; the second stage over
+    the C points' distance-2 graph) decide every node as the reference's
+    sequential code does (src/coarsening.jl:202-283, 291-313, 330-624,
+    990-1116), on the setup's seeded draws: with direct interpolation each C point's row of P is its
     coarse index with weight 1, so every C point of the restatement must have
     exactly that row, and the coarse counts agree."""
     A = _mats(dh)[name]
     rp, cv, nz = (np.asarray(a) for a in (A.rowptr, A.colval, A.nzval))
     theta = 0.25
-    alg = dh.PMISCoarsening(theta) if coarsening == "pmis" else dh.HMISCoarsening(theta)
+    alg = {"pmis": dh.PMISCoarsening(theta), "hmis": dh.HMISCoarsening(theta),
+           "aggressive_hmis": dh.AggressiveCoarsening(theta, "hmis", dh.DirectInterpolation()),
+           "aggressive_pmis": dh.AggressiveCoarsening(theta, "pmis", dh.DirectInterpolation())}[coarsening]
     cfg = dh.AMGConfig(coarsening=alg, smoother=dh.JacobiSmootherType(), max_coarse_size=4)
     h = dh.amg_setup(A, cfg, upload=False)
-    cf = (ref.coarsen_pmis if coarsening == "pmis" else ref.coarsen_hmis)(rp, cv, nz, theta, cfg.seed)
+    if coarsening.startswith("aggressive"):
+        cf = ref.coarsen_aggressive_cf(rp, cv, nz, theta, coarsening.endswith("hmis"), cfg.seed)
+    else:
+        cf = (ref.coarsen_pmis if coarsening == "pmis" else ref.coarsen_hmis)(rp, cv, nz, theta, cfg.seed)
     n = len(rp) - 1
     L = h.level(0, with_rmap=False)
     assert L["n_coarse"] < 0.8 * n  # no theta-halving retry (which would draw again)
