@@ -208,3 +208,31 @@ def test_pmis_matches_reference_sequential_sweeps(dh, name, coarsening):
     Prp, Pcv, Pnz = L["P_rp"], L["P_cv"], L["P_nz"]
     for i in np.flatnonzero(cf == 1):
         assert Prp[i + 1] - Prp[i] == 1 and Pcv[Prp[i]] == coarse[i] and Pnz[Prp[i]] == 1.0, (name, i)
+
+
+@pytest.mark.parametrize("interp", ["plain", "trunc", "trunc_max_rescale", "norm2"])
+@pytest.mark.parametrize("name", ["poisson2d", "aniso", "hetero3d", "random"])
+def test_ext_i_matches_reference(dh, name, interp):
+    """The host's extended+i interpolation with its truncations
+    (src/prolongation.jl:467-690, then src/setup.jl:263-266) gives the
+    reference's P bit for bit on the HMIS splitting pinned above: same rows,
+    columns and values (the GPU build is pinned to the host's by
+    tests/test_gpu_setup.py)."""
+    A = _mats(dh)[name]
+    rp, cv, nz = (np.asarray(a) for a in (A.rowptr, A.colval, A.nzval))
+    theta = 0.25
+    spec = {"plain": dict(trunc_factor=0.0), "trunc": dict(trunc_factor=0.3),
+            "trunc_max_rescale": dict(trunc_factor=0.1, max_elements=3, rescale=True),
+            "norm2": dict(trunc_factor=0.3, norm_p=2)}[interp]
+    cfg = dh.AMGConfig(coarsening=dh.HMISCoarsening(theta, dh.ExtendedIInterpolation(**spec)),
+                       smoother=dh.JacobiSmootherType(), max_coarse_size=4)
+    h = dh.amg_setup(A, cfg, upload=False)
+    L = h.level(0, with_rmap=False)
+    assert L["n_coarse"] < 0.8 * (len(rp) - 1)
+    cf = ref.coarsen_hmis(rp, cv, nz, theta, cfg.seed)
+    P = ref.interp_ext_i(rp, cv, nz, cf, theta, spec.get("trunc_factor", 0.0), spec.get("max_elements", 0),
+                         spec.get("norm_p", 1), spec.get("rescale", False))
+    if spec["trunc_factor"] > 0:
+        P = ref.filter_prolongation(*P, spec["trunc_factor"])
+    assert np.array_equal(L["P_rp"], P[0]) and np.array_equal(L["P_cv"], P[1])
+    assert L["P_nz"].tobytes() == P[2].tobytes()
