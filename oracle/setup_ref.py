@@ -20,6 +20,7 @@ consumes as its contract, for small matrices (n up to a few thousand):
   truncation, max_elements, norm_p and rescale) and ``filter_prolongation``
   :134-182, the second truncation of src/setup.jl:263-266
 * ``interp_direct``, ``interp_standard`` src/prolongation.jl:219-335, 349-456
+* ``coarsen_rs``            src/coarsening.jl:671-818 (classical first pass)
 
 They pin the product's C++ setup bit for bit on the same input.  Indices are
 0-based.  Matrices are (rp, cv, nz) numpy triples.
@@ -314,10 +315,12 @@ def _strong_transpose_adj(rp, cv, s):
     return lists
 
 
-def rs_first_pass(rp, cv, s, use_zpt):
+def rs_first_pass(rp, cv, s, use_zpt, classical=False):
     """src/coarsening.jl:406-567: greedy Ruge-Stueben first pass over bucket
     lists keyed by lambda (head insertion; the head of the highest non-empty
-    bucket is taken).  States: 1 C, -1 F, -2 tentative F (use_zpt)."""
+    bucket is taken).  States: 1 C, -1 F, -2 tentative F (use_zpt).
+    classical: coarsen_rs's own first pass (src/coarsening.jl:671-805): no
+    zero-measure step and no marking when a decrement reaches zero."""
     n = len(rp) - 1
     f_pnt = -2 if use_zpt else -1
     lam = [int(c) for c in strong_transpose_count(rp, cv, s)]
@@ -328,14 +331,14 @@ def rs_first_pass(rp, cv, s, use_zpt):
         if not strong_rows[i] and lam[i] == 0:
             cf[i] = 1
     for i in range(n):
-        if cf[i] != 0:
+        if classical or cf[i] != 0:
             continue
         if lam[i] == 0:
             cf[i] = f_pnt
             for j in strong_rows[i]:
                 if cf[j] == 0:
                     lam[j] += 1
-    max_lam = max([lam[i] for i in range(n) if cf[i] == 0], default=0)
+    max_lam = max([lam[i] for i in range(n) if cf[i] == 0 or classical], default=0)
     head = [-1] * max(max_lam + 1, 1)
     nxt = [-1] * n
     prv = [-1] * n
@@ -401,7 +404,7 @@ def rs_first_pass(rp, cv, s, use_zpt):
                 continue
             v = max(0, lam[j] - 1)
             update(j, v)
-            if v == 0:
+            if v == 0 and not classical:
                 remove(j)
                 cf[j] = f_pnt
                 raise_around(j)
@@ -778,3 +781,14 @@ def interp_standard(rp, cv, nz, cf, theta):
             continue
         rows.append(sorted((cm, (-v / d) if abs(d) > _EPS else 0.0) for cm, v in contrib.items()))
     return _rows_to_csr(rows)
+
+
+def coarsen_rs(rp, cv, nz, theta):
+    """src/coarsening.jl:671-818: classical first pass, then every F point
+    without a strong C neighbour promoted (:291-313)."""
+    n = len(rp) - 1
+    if n <= 1:
+        return np.ones(n, dtype=np.int64)
+    s = strength_absolute(rp, cv, nz, theta)
+    cf = rs_first_pass(rp, cv, s, False, classical=True)
+    return ensure_fine_have_coarse_neighbor(cf, rp, cv, s)

@@ -238,23 +238,28 @@ def test_ext_i_matches_reference(dh, name, interp):
     assert L["P_nz"].tobytes() == P[2].tobytes()
 
 
-@pytest.mark.parametrize("interp", ["direct_pmis", "direct_hmis", "standard_pmis", "standard_hmis"])
+@pytest.mark.parametrize("interp", ["direct_pmis", "direct_hmis", "standard_pmis", "standard_hmis", "direct_rs",
+                                    "standard_rs"])
 @pytest.mark.parametrize("name", ["poisson2d", "aniso", "hetero3d", "random"])
 def test_direct_standard_interpolation_match_reference(dh, name, interp):
     """The host's direct and standard interpolations (src/prolongation.jl:219-335,
     349-456) give the reference's P bit for bit on the pinned PMIS / HMIS
-    splittings."""
+    splittings and on classical RS (src/coarsening.jl:671-818), which this
+    pins too."""
     A = _mats(dh)[name]
     rp, cv, nz = (np.asarray(a) for a in (A.rowptr, A.colval, A.nzval))
     theta = 0.25
     kind, base = interp.split("_")
     ip = dh.DirectInterpolation() if kind == "direct" else dh.StandardInterpolation()
-    alg = dh.PMISCoarsening(theta, ip) if base == "pmis" else dh.HMISCoarsening(theta, ip)
+    alg = {"pmis": dh.PMISCoarsening, "hmis": dh.HMISCoarsening, "rs": dh.RSCoarsening}[base](theta, ip)
     cfg = dh.AMGConfig(coarsening=alg, smoother=dh.JacobiSmootherType(), max_coarse_size=4)
     h = dh.amg_setup(A, cfg, upload=False)
     L = h.level(0, with_rmap=False)
     assert L["n_coarse"] < 0.8 * (len(rp) - 1)
-    cf = (ref.coarsen_pmis if base == "pmis" else ref.coarsen_hmis)(rp, cv, nz, theta, cfg.seed)
+    if base == "rs":
+        cf = ref.coarsen_rs(rp, cv, nz, theta)
+    else:
+        cf = (ref.coarsen_pmis if base == "pmis" else ref.coarsen_hmis)(rp, cv, nz, theta, cfg.seed)
     P = (ref.interp_direct if kind == "direct" else ref.interp_standard)(rp, cv, nz, cf, theta)
     assert np.array_equal(L["P_rp"], P[0]) and np.array_equal(L["P_cv"], P[1])
     assert L["P_nz"].tobytes() == P[2].tobytes()
