@@ -21,6 +21,7 @@ consumes as its contract, for small matrices (n up to a few thousand):
   :134-182, the second truncation of src/setup.jl:263-266
 * ``interp_direct``, ``interp_standard`` src/prolongation.jl:219-335, 349-456
 * ``coarsen_rs``            src/coarsening.jl:671-818 (classical first pass)
+* ``smooth_prolongation``   src/prolongation.jl:38-120 (smoothed aggregation)
 
 They pin the product's C++ setup bit for bit on the same input.  Indices are
 0-based.  Matrices are (rp, cv, nz) numpy triples.
@@ -792,3 +793,33 @@ def coarsen_rs(rp, cv, nz, theta):
     s = strength_absolute(rp, cv, nz, theta)
     cf = rs_first_pass(rp, cv, s, False, classical=True)
     return ensure_fine_have_coarse_neighbor(cf, rp, cv, s)
+
+
+def smooth_prolongation(rp, cv, nz, prp, pcv, pnz, omega):
+    """src/prolongation.jl:38-120: P = (I - omega D^-1 A) P_tent, entries with
+    |p| <= eps dropped (the row's first tentative entry kept if all go)."""
+    n = len(rp) - 1
+    inv = np.zeros(n)
+    for i in range(n):
+        d = 0.0
+        for k in range(rp[i], rp[i + 1]):
+            if cv[k] == i:
+                d = nz[k]
+                break
+        inv[i] = 1.0 / d if abs(d) > _EPS * max(1.0, abs(d)) else 0.0
+    rows = []
+    for i in range(n):
+        ent = {}
+        for q in range(prp[i], prp[i + 1]):
+            ent[pcv[q]] = ent.get(pcv[q], 0.0) + pnz[q]
+        factor = -omega * inv[i]
+        for k in range(rp[i], rp[i + 1]):
+            j = cv[k]
+            w = factor * nz[k]
+            for q in range(prp[j], prp[j + 1]):
+                ent[pcv[q]] = ent.get(pcv[q], 0.0) + w * pnz[q]
+        row = sorted((int(J), v) for J, v in ent.items() if abs(v) > _EPS)
+        if not row and prp[i] < prp[i + 1]:
+            row = [(int(pcv[prp[i]]), pnz[prp[i]])]
+        rows.append(row)
+    return _rows_to_csr(rows)
