@@ -22,6 +22,8 @@ consumes as its contract, for small matrices (n up to a few thousand):
 * ``interp_direct``, ``interp_standard`` src/prolongation.jl:219-335, 349-456
 * ``coarsen_rs``            src/coarsening.jl:671-818 (classical first pass)
 * ``smooth_prolongation``   src/prolongation.jl:38-120 (smoothed aggregation)
+* ``coarsen_aggressive_agg`` src/coarsening.jl:869-957 (aggressive coarsening
+  on an aggregation base: PMIS, aggregates merged through F points)
 
 They pin the product's C++ setup bit for bit on the same input.  Indices are
 0-based.  Matrices are (rp, cv, nz) numpy triples.
@@ -823,3 +825,72 @@ def smooth_prolongation(rp, cv, nz, prp, pcv, pnz, omega):
             row = [(int(pcv[prp[i]]), pnz[prp[i]])]
         rows.append(row)
     return _rows_to_csr(rows)
+
+
+def coarsen_aggressive_agg(rp, cv, nz, theta, seed):
+    """src/coarsening.jl:869-957, 1-based aggregate numbers inside (0 = none);
+    returns 0-based aggregates and their count."""
+    n = len(rp) - 1
+    cf = coarsen_pmis(rp, cv, nz, theta, seed)
+    s = strength_absolute(rp, cv, nz, theta)
+    agg = [0] * n
+    cnt = 0
+    for i in range(n):
+        if cf[i] == 1:
+            cnt += 1
+            agg[i] = cnt
+    for i in range(n):
+        if cf[i] == 1:
+            continue
+        best, bv = 0, 0.0
+        for k in range(rp[i], rp[i + 1]):
+            j = cv[k]
+            if j != i and cf[j] == 1 and abs(nz[k]) > bv:
+                bv, best = abs(nz[k]), agg[j]
+        if best != 0:
+            agg[i] = best
+    parent = list(range(cnt + 1))  # 1-based, entry 0 unused
+
+    def find(i):
+        while parent[i] != i:
+            parent[i] = parent[parent[i]]
+            i = parent[i]
+        return i
+
+    def union(a, b):
+        ra, rb = find(a), find(b)
+        if ra != rb:
+            parent[rb] = ra
+
+    for i in range(n):
+        if cf[i] != -1:
+            continue
+        roots = [find(agg[cv[k]]) for k in range(rp[i], rp[i + 1]) if s[k] and cf[cv[k]] == 1]
+        for r in roots[1:]:
+            union(roots[0], r)
+    for i in range(1, cnt + 1):
+        find(i)
+    new_id = [0] * (cnt + 1)
+    nc = 0
+    for i in range(1, cnt + 1):
+        r = find(i)
+        if new_id[r] == 0:
+            nc += 1
+            new_id[r] = nc
+        new_id[i] = new_id[r]
+    for i in range(n):
+        if agg[i] > 0:
+            agg[i] = new_id[find(agg[i])]
+    for i in range(n):
+        if agg[i] == 0:
+            best, bv = 0, 0.0
+            for k in range(rp[i], rp[i + 1]):
+                j = cv[k]
+                if j != i and agg[j] != 0 and abs(nz[k]) > bv:
+                    bv, best = abs(nz[k]), agg[j]
+            if best != 0:
+                agg[i] = best
+            else:
+                nc += 1
+                agg[i] = nc
+    return np.array(agg, dtype=np.int64) - 1, nc

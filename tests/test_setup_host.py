@@ -282,3 +282,21 @@ def test_smoothed_aggregation_matches_reference(dh, name):
     P = ref.smooth_prolongation(rp, cv, nz, prp, pcv, pnz, alg.ω)
     assert np.array_equal(L["P_rp"], P[0]) and np.array_equal(L["P_cv"], P[1])
     assert L["P_nz"].tobytes() == P[2].tobytes()
+
+
+@pytest.mark.parametrize("name", ["poisson1d", "poisson2d", "aniso", "random", "hetero3d"])
+def test_aggressive_aggregation_matches_reference(dh, name):
+    """The host's aggressive coarsening on an aggregation base
+    (src/coarsening.jl:869-957: PMIS, F points joined to their strongest C
+    neighbour's aggregate, aggregates merged through F points) gives the
+    reference's aggregates on the pinned PMIS splitting."""
+    A = _mats(dh)[name]
+    rp, cv, nz = (np.asarray(a) for a in (A.rowptr, A.colval, A.nzval))
+    theta = 0.25
+    cfg = dh.AMGConfig(coarsening=dh.AggressiveCoarsening(theta, "aggregation"), smoother=dh.JacobiSmootherType(),
+                       max_coarse_size=4)
+    h = dh.amg_setup(A, cfg, upload=False)
+    L = h.level(0, with_rmap=False)
+    agg, nc = ref.coarsen_aggressive_agg(rp, cv, nz, theta, cfg.seed)
+    assert L["n_coarse"] == nc
+    assert np.array_equal(L["P_cv"], agg) and np.array_equal(L["P_nz"], np.ones(len(rp) - 1))
