@@ -265,14 +265,17 @@ def test_direct_standard_interpolation_match_reference(dh, name, interp):
     assert L["P_nz"].tobytes() == P[2].tobytes()
 
 
+@pytest.mark.parametrize("filtering", [False, True])
 @pytest.mark.parametrize("name", ["poisson1d", "poisson2d", "aniso", "random", "hetero3d"])
-def test_smoothed_aggregation_matches_reference(dh, name):
+def test_smoothed_aggregation_matches_reference(dh, name, filtering):
     """The host's smoothed-aggregation P = (I - w D^-1 A) P_tent
-    (src/prolongation.jl:38-120) bit for bit on the pinned aggregation."""
+    (src/prolongation.jl:38-120), and with filtering the entries below
+    0.1 max |p| of each row dropped (:134-182, src/setup.jl:286-290), bit for
+    bit on the pinned aggregation."""
     A = _mats(dh)[name]
     rp, cv, nz = (np.asarray(a) for a in (A.rowptr, A.colval, A.nzval))
     theta = 0.08
-    alg = dh.SmoothedAggregationCoarsening(theta)
+    alg = dh.SmoothedAggregationCoarsening(theta, filtering=filtering, filter_tol=0.1)
     cfg = dh.AMGConfig(coarsening=alg, smoother=dh.JacobiSmootherType(), max_coarse_size=4)
     h = dh.amg_setup(A, cfg, upload=False)
     L = h.level(0, with_rmap=False)
@@ -280,6 +283,8 @@ def test_smoothed_aggregation_matches_reference(dh, name):
     assert L["n_coarse"] == nc
     (prp, pcv, pnz), _ = ref.prolongation_from_aggregates(agg, nc)
     P = ref.smooth_prolongation(rp, cv, nz, prp, pcv, pnz, alg.ω)
+    if filtering:
+        P = ref.filter_prolongation(*P, 0.1)
     assert np.array_equal(L["P_rp"], P[0]) and np.array_equal(L["P_cv"], P[1])
     assert L["P_nz"].tobytes() == P[2].tobytes()
 
