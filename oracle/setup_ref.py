@@ -24,6 +24,8 @@ consumes as its contract, for small matrices (n up to a few thousand):
 * ``smooth_prolongation``   src/prolongation.jl:38-120 (smoothed aggregation)
 * ``coarsen_aggressive_agg`` src/coarsening.jl:869-957 (aggressive coarsening
   on an aggregation base: PMIS, aggregates merged through F points)
+* ``apply_max_row_sum``     src/strength.jl:183-213 (the matrix the C/F
+  splitting sees when max_row_sum < 1)
 
 They pin the product's C++ setup bit for bit on the same input.  Indices are
 0-based.  Matrices are (rp, cv, nz) numpy triples.
@@ -894,3 +896,23 @@ def coarsen_aggressive_agg(rp, cv, nz, theta, seed):
                 nc += 1
                 agg[i] = nc
     return np.array(agg, dtype=np.int64) - 1, nc
+
+
+def apply_max_row_sum(rp, cv, nz, threshold):
+    """src/strength.jl:183-213: rows with |row sum| > |a_ii| * threshold
+    (threshold < 1) lose their off-diagonal values (set to 0, pattern kept)."""
+    n = len(rp) - 1
+    out = np.array(nz, dtype=np.float64, copy=True)
+    for i in range(n):
+        d, rs = 0.0, 0.0
+        for k in range(rp[i], rp[i + 1]):
+            rs += nz[k]
+            if cv[k] == i:
+                d = nz[k]
+        if abs(d) < _EPS:
+            continue
+        if abs(rs) > abs(d) * threshold and threshold < 1.0:
+            for k in range(rp[i], rp[i + 1]):
+                if cv[k] != i:
+                    out[k] = 0.0
+    return out

@@ -305,3 +305,29 @@ def test_aggressive_aggregation_matches_reference(dh, name):
     agg, nc = ref.coarsen_aggressive_agg(rp, cv, nz, theta, cfg.seed)
     assert L["n_coarse"] == nc
     assert np.array_equal(L["P_cv"], agg) and np.array_equal(L["P_nz"], np.ones(len(rp) - 1))
+
+
+@pytest.mark.parametrize("coarsening", ["hmis", "aggressive_hmis"])
+@pytest.mark.parametrize("name", ["poisson2d", "random", "hetero3d"])  # (aniso's rows sum to ~1e-3: none weakened)
+def test_max_row_sum_weakening_matches_reference(dh, name, coarsening):
+    """With max_row_sum < 1 the C/F splitting sees the weakened matrix
+    (src/strength.jl:183-213; src/coarsening.jl:999, 1120-1185) and the
+    interpolation the original one: the host's P equals the restatement's."""
+    A = _mats(dh)[name]
+    rp, cv, nz = (np.asarray(a) for a in (A.rowptr, A.colval, A.nzval))
+    theta, mrs = 0.25, 0.2  # boundary rows (|row sum| > 0.2 |a_ii|) are weakened
+    ip = dh.DirectInterpolation()
+    alg = dh.HMISCoarsening(theta, ip) if coarsening == "hmis" else dh.AggressiveCoarsening(theta, "hmis", ip)
+    cfg = dh.AMGConfig(coarsening=alg, smoother=dh.JacobiSmootherType(), max_coarse_size=4, max_row_sum=mrs)
+    h = dh.amg_setup(A, cfg, upload=False)
+    L = h.level(0, with_rmap=False)
+    assert L["n_coarse"] < 0.8 * (len(rp) - 1)
+    weak = ref.apply_max_row_sum(rp, cv, nz, mrs)
+    assert not np.array_equal(weak, nz)  # the weakening acts on these matrices
+    if coarsening == "hmis":
+        cf = ref.coarsen_hmis(rp, cv, weak, theta, cfg.seed)
+    else:
+        cf = ref.coarsen_aggressive_cf(rp, cv, weak, theta, True, cfg.seed)
+    P = ref.interp_direct(rp, cv, nz, cf, theta)
+    assert np.array_equal(L["P_rp"], P[0]) and np.array_equal(L["P_cv"], P[1])
+    assert L["P_nz"].tobytes() == P[2].tobytes()
