@@ -613,11 +613,10 @@ int32_t draugr_hip_batch_setup(int32_t nsys, const int32_t* n, const int32_t* nn
         if (nsys < 1 || !n || !nnz || !rowptr || !colval || !nzval)
             throw Error("ArgumentError: a batch needs nsys >= 1 and non-null per-system arrays");
         const auto t0 = std::chrono::steady_clock::now();
-        std::vector<HostHierarchy> host(nsys);
-        for (int32_t s = 0; s < nsys; ++s) {
-            CSR A = csr_from_raw(n[s], nnz[s], rowptr[s], colval[s], nzval[s], index_base);
-            host[s] = amg_setup_host(A, cfg, false, gpu_hooks(cfg));
-        }
+        std::vector<CSR> As(nsys);
+        for (int32_t s = 0; s < nsys; ++s) As[s] = csr_from_raw(n[s], nnz[s], rowptr[s], colval[s], nzval[s], index_base);
+        // independent systems: built concurrently, each hierarchy as a lone setup would build it
+        std::vector<HostHierarchy> host = amg_setup_host_many(As, cfg, false, gpu_hooks(cfg));
         std::vector<const HostHierarchy*> hp(nsys);
         for (int32_t s = 0; s < nsys; ++s) hp[s] = &host[s];
         auto B = std::make_shared<Batch>();

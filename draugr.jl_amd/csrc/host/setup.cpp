@@ -2,10 +2,16 @@
 // uploaded to HBM and stays resident).
 #include "types.hpp"
 
+#include <omp.h>
+
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <exception>
 #include <sstream>
+#include <thread>
 
 namespace draugr {
 
@@ -134,6 +140,34 @@ HostHierarchy amg_setup_host(const CSR& A_in, const Config& cfg, bool allow_part
     h.A_coarsest = std::move(A);
     h.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return h;
+}
+
+std::vector<HostHierarchy> amg_setup_host_many(const std::vector<CSR>& As, const Config& cfg,
+                                               bool allow_partial_resetup, const SetupHooks* hooks) {
+    const int nsys = (int)As.size();
+    std::vector<HostHierarchy> out(nsys);
+    if (nsys == 0) return out;
+    const int procs = std::max(1, omp_get_num_procs());
+    const int workers = std::min(nsys, std::max(1, std::min(8, procs / 2)));
+    const int per_worker = std::max(1, procs / workers);
+    std::atomic<int> next{0};
+    std::vector<std::exception_ptr> errs(nsys);
+    auto work = [&]() {
+        omp_set_num_threads(per_worker);  // this thread's parallel regions
+        for (int s = next++; s < nsys; s = next++) {
+            try {
+                out[s] = amg_setup_host(As[s], cfg, allow_partial_resetup, hooks);
+            } catch (...) {
+                errs[s] = std::current_exception();
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 0; w < workers; ++w) pool.emplace_back(work);
+    for (auto& t : pool) t.join();
+    for (auto& e : errs)
+        if (e) std::rethrow_exception(e);
+    return out;
 }
 
 // Hierarchy table in the shape of src/setup.jl:405-473.

@@ -218,6 +218,12 @@ struct SetupHooks {
 };
 HostHierarchy amg_setup_host(const CSR& A, const Config& cfg, bool allow_partial_resetup,
                              const SetupHooks* hooks = nullptr);
+// Hierarchies of independent systems, built concurrently by a few host
+// threads (each with its share of the OpenMP threads); the same hierarchies
+// as one amg_setup_host call per system.  The first failure (by system
+// index) is rethrown after every worker has finished.
+std::vector<HostHierarchy> amg_setup_host_many(const std::vector<CSR>& As, const Config& cfg,
+                                               bool allow_partial_resetup, const SetupHooks* hooks = nullptr);
 std::string hierarchy_summary(const HostHierarchy& h, const Config& cfg, int64_t n_finest);
 
 // Dense LU with partial pivoting (column-major), and its explicit inverse.
