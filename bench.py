@@ -6,9 +6,11 @@ trunc 0.3) with l1-Jacobi V(1,1), FP64 outer PCG.
 
 One step = one PCG solve from x = 0 with the hierarchy and b resident in HBM.
 The headline value is seconds per solve with the FP32 cycle; the FP64-cycle
-solve time, the speedup, iteration counts, V-cycle time and bandwidth, the
-roofline of the dominant kernel and a CPU baseline (the oracle's OpenMP
-restatement of the same solve on a bounded sample) ride along in the line.
+and BF16-value-cycle solve times and speedups, iteration counts, V-cycle time
+and bandwidth per precision, the roofline of the dominant kernel, each GPU
+solution's error against the CPU oracle's FP64 solution, and a CPU baseline
+(the oracle's OpenMP restatement of the same solve: one warm-up, then the
+median of full solves) ride along in the line.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--precision fp32|bf16]
        python bench.py --kernel-only   (setup, a 256 MiB calibration copy and the
@@ -41,15 +43,16 @@ def parse():
     ap.add_argument("--sigma", type=float, default=1.5)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
-    ap.add_argument("--cpu-sample-iters", type=int, default=3,
-                    help="PCG iterations of the CPU baseline sample (its time is scaled to a full solve)")
+    ap.add_argument("--cpu-solves", type=int, default=5,
+                    help="timed full CPU oracle PCG solves (after one warm-up); the baseline is their median")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--workload", default="hetero128",
                     choices=["hetero128", "poisson256", "resetup192", "batched48", "adjoint96"],
                     help="hetero128 (default): the headline line; the others print their own JSON line")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the calibration copy and the dominant-kernel launches (for counter runs)")
-    ap.add_argument("--kernel", default="auto", choices=["auto", "stream", "rowseq", "pipe", "stream16", "sorted"],
+    ap.add_argument("--kernel", default="auto", choices=["auto", "stream", "rowseq", "pipe", "stream16", "sorted", "stream16pipe",
+                                                  "sortedpipe"],
                     help="sparse-pass kernel (config key kernel); a counter run forces the kernel the "
                          "autotuner picks without a profiler, whose timings it would perturb")
     ap.add_argument("--fuse", default="auto", choices=["auto", "off", "on"],
@@ -112,6 +115,29 @@ def calibration_copy(torch):
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
+
+
+def cpu_model():
+    """CPU model name (lscpu, else /proc/cpuinfo) and the logical CPUs visible."""
+    name = None
+    try:
+        import subprocess
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout.splitlines():
+            if line.startswith("Model name:"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    if name is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        name = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            name = "unknown"
+    return "%s (%d logical CPUs visible)" % (name, os.cpu_count() or 0)
 
 
 def log(*a):
@@ -349,19 +375,32 @@ def main():
     log("FP64 cycle: %.4f s/solve, iterations %s, relres %.2e" % (t64, its64, rel64))
     t32, its32, rel32, x32 = time_solves(dh, torch, h, cfg32, b, args.steps, args.warmup, args.tol)
     log("%s cycle: %.4f s/solve, iterations %s, relres %.2e" % (args.precision, t32, its32, rel32))
+    other = "bf16" if args.precision == "fp32" else "fp32"
+    cfg_o = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType(), precision=other, kernel=args.kernel,
+                                    fuse=args.fuse)
+    t_o, its_o, rel_o, x_o = time_solves(dh, torch, h, cfg_o, b, args.steps, args.warmup, args.tol)
+    log("%s cycle: %.4f s/solve, iterations %s, relres %.2e" % (other, t_o, its_o, rel_o))
     log(h.tune_report())
     xr = x64.cpu().numpy()
     x3 = x32.cpu().numpy()
-    true32 = float(np.linalg.norm(b_host - A.matvec(x3)) / np.linalg.norm(b_host))
-    err32 = float(np.linalg.norm(x3 - xr) / np.linalg.norm(xr))
+    xo3 = x_o.cpu().numpy()
+    true = lambda v: float(np.linalg.norm(b_host - A.matvec(v)) / np.linalg.norm(b_host))  # noqa: E731
+    relerr = lambda u, v: float(np.linalg.norm(u - v) / np.linalg.norm(v))  # noqa: E731
+    true32 = true(x3)
+    err32 = relerr(x3, xr)
 
     # V-cycle time and bandwidth (zero-guess preconditioner application)
     z = torch.zeros_like(b)
     # best of three runs of 20 back-to-back cycles (the first run after host
     # work can see the GPU still clocking up)
-    t_cyc32 = min(event_time(torch, lambda: dh.amg_cycle_(z, b, h, cfg32, zero_guess=True), 20) for _ in range(3))
-    t_cyc64 = min(event_time(torch, lambda: dh.amg_cycle_(z, b, h, cfg64, zero_guess=True), 20) for _ in range(3))
-    bytes32, bytes64 = h.cycle_bytes(cfg32), h.cycle_bytes(cfg64)
+    cyc = {}
+    for name, c in (("fp64", cfg64), (args.precision, cfg32), (other, cfg_o)):
+        t_c = min(event_time(torch, lambda: dh.amg_cycle_(z, b, h, c, zero_guess=True), 20) for _ in range(3))
+        nb = h.cycle_bytes(c)
+        cyc[name] = {"ms": t_c * 1e3, "bytes": nb, "GBps": nb / t_c / 1e9,
+                     "frac_of_hbm_peak": nb / t_c / 1e9 / HBM_PEAK_GBS}
+    t_cyc32, t_cyc64 = cyc[args.precision]["ms"] / 1e3, cyc["fp64"]["ms"] / 1e3
+    bytes32, bytes64 = cyc[args.precision]["bytes"], cyc["fp64"]["bytes"]
 
     # dominant kernel, timed live
     t_k, kbytes, kstored, tag = dominant_kernel(torch, h, args.precision, 50)
@@ -389,13 +428,19 @@ def main():
         },
         "fp64_cycle_s_per_solve": t64,
         "speedup_fp32_cycle_vs_fp64": t64 / t32,
+        "%s_cycle_s_per_solve" % other: t_o,
+        "speedup_%s_cycle_vs_fp64" % other: t64 / t_o,
         "iters_fp64": its64[-1],
         "iters_mixed": its32[-1],
+        "iters_%s" % other: its_o[-1],
         "mixed_true_relres": true32,
         "mixed_rel_err_vs_fp64_solution": err32,
+        "%s_true_relres" % other: true(xo3),
+        "fp64_true_relres": true(xr),
         "vcycle_ms": {"mixed": t_cyc32 * 1e3, "fp64": t_cyc64 * 1e3},
         "vcycle_GBps_algorithmic": {"mixed": bytes32 / t_cyc32 / 1e9, "fp64": bytes64 / t_cyc64 / 1e9},
         "vcycle_bytes": {"mixed": bytes32, "fp64": bytes64},
+        "vcycle_by_precision": cyc,
         "setup_s": t_setup,
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
@@ -405,19 +450,30 @@ def main():
     }
 
     if not args.skip_cpu:
+        # The oracle (OpenMP C restatement of Draugr's CPU cycle and a textbook
+        # FP64 PCG) on the same hierarchy and right-hand side, after the GPU
+        # timing: its solution anchors the GPU solutions, and one warm-up plus
+        # the median of full solves is the CPU baseline.
         from oracle import oracle as orc
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         levels = [h.level(l, with_rmap=False) for l in range(h.nlevels)]
         c = h.coarsest()
         oh = orc.OracleHierarchy(levels, c["rp"], c["cv"], c["nz"], smoother="l1_jacobi")
-        k = args.cpu_sample_iters
-        t0 = time.perf_counter()
-        _, itc, _ = oh.pcg(np.zeros(A.n), b_host, 0.0, k)  # tol 0: exactly k iterations
-        tc = time.perf_counter() - t0
-        per_solve = tc / k * its64[-1]
-        out["cpu_baseline"] = {"value": per_solve, "unit": "s", "cores": threads, "kind": "port",
-                               "sample": "%d FP64 PCG iterations (oracle OpenMP restatement, same hierarchy), "
-                                         "scaled to %d iterations" % (k, its64[-1])}
+        ts = []
+        for rep in range(1 + args.cpu_solves):
+            t0 = time.perf_counter()
+            xo, itc, _ = oh.pcg(np.zeros(A.n), b_host, args.tol, 500)
+            if rep:
+                ts.append(time.perf_counter() - t0)
+        log("CPU oracle: %d iterations, %s s" % (itc, ["%.3f" % t for t in ts]))
+        out["cpu_oracle_iterations"] = itc
+        out["rel_err_vs_cpu_oracle"] = {"fp64": relerr(xr, xo), args.precision: relerr(x3, xo), other: relerr(xo3, xo)}
+        out["cpu_baseline"] = {"value": float(np.median(ts)), "unit": "s", "cores": threads, "kind": "port",
+                               "cpu": cpu_model(),
+                               "sample": "1 warm-up + median of %d full FP64 PCG solves to %.0e (%d iterations; "
+                                         "oracle OpenMP restatement of the CPU cycle, same hierarchy and b); "
+                                         "min %.3f s, max %.3f s" % (args.cpu_solves, args.tol, itc, min(ts),
+                                                                     max(ts))}
     print(json.dumps(out))
 
 

@@ -118,6 +118,7 @@ CycleOpts opts_of(const Config& c) {
     o.post = c.post_smoothing_steps;
     o.prec = c.precision;
     o.graphs = c.graphs;
+    o.gs_sym = c.gs_symmetric;
     return o;
 }
 
@@ -180,6 +181,21 @@ hipStream_t stream_of(Hier&, void* s) { return (hipStream_t)s; }
 using namespace draugr;
 
 extern "C" {
+
+void init_julia(int, char**) {}
+
+void shutdown_julia(int) {
+    std::map<int32_t, std::shared_ptr<Hier>> hier;
+    std::map<int32_t, std::shared_ptr<Batch>> batch;
+    {
+        std::lock_guard<std::mutex> g(g_lock);
+        hier.swap(g_hier);
+        batch.swap(g_batch);
+        g_cfg.clear();
+    }
+    for (auto& kv : hier) std::lock_guard<std::mutex> g(kv.second->mu);  // wait for calls in flight
+    for (auto& kv : batch) std::lock_guard<std::mutex> g(kv.second->mu);
+}
 
 const char* draugr_amg_last_error(void) { return g_last_error.c_str(); }
 

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <string>
 
 namespace draugr {
 
@@ -19,6 +20,7 @@ namespace {
 
 constexpr int kRedBlocks = 4096;   // grid of the vector reductions
 constexpr int kCoarseLdsMax = 60 * 1024;
+constexpr int kLuOneGroupMax = 256;  // larger coarsest operators are factored column by column over the chip
 
 template <class F>
 __global__ __launch_bounds__(kBlock) void k_map(int64_t n, F f) {
@@ -104,14 +106,11 @@ __global__ __launch_bounds__(kBlock) void k_stream_reduce(const BlkHdr* __restri
     __shared__ double sh[kBlock];
     double dot = 0.0;
     auto pref = [](int) { return NoPre{}; };
-    auto segf = [&](int row, const TX* seg, int len, NoPre) {
-        TX a = 0;
-        for (int k = 0; k < len; ++k) a += seg[k];
-        dot = epi(row, a);
-    };
-    auto totf = [&](int row, TX s) { dot = epi(row, s); };
-    stream_block<PER, C16, TV, TX, XF, decltype(pref), decltype(segf), decltype(totf), SRT>(
-        xcd_block(), hdr, rp, cv, c16, av, xf, prod, pref, segf, totf, slot);
+    auto initf = [](NoPre) { return (TX)0; };
+    auto accf = [](TX a, TX p) { return a + p; };
+    auto finf = [&](int row, TX a, NoPre) { dot = epi(row, a); };
+    stream_block<PER, C16, TV, TX, XF, decltype(pref), decltype(initf), decltype(accf), decltype(finf), SRT>(
+        xcd_block(), hdr, rp, cv, c16, av, xf, prod, pref, initf, accf, finf, slot);
     block_sum_store(dot, sh, partial, blockIdx.x);
 }
 
@@ -280,17 +279,89 @@ __global__ __launch_bounds__(kBlock) void k_dense_lu(int n, double* __restrict__
     }
 }
 
+// The same factorisation for a large coarsest operator, one launch pair per
+// column: k_lu_pivot (one workgroup: pivot search, row swap, column scale)
+// then k_lu_update (the trailing update spread over the chip).  Every entry
+// sees the operations of k_dense_lu in the same order, so the factors are the
+// same bits; a zero pivot sets the flag and the remaining launches return.
+__global__ __launch_bounds__(kBlock) void k_lu_pivot(int n, int k, double* __restrict__ a, int* __restrict__ piv,
+                                                     int* __restrict__ flag) {
+    __shared__ double sv[kBlock];
+    __shared__ int si[kBlock];
+    if (*flag) return;
+    auto at = [&](int r, int c) -> double& { return a[(size_t)c * n + r]; };
+    double best = -1.0;
+    int bi = n;
+    for (int r = k + threadIdx.x; r < n; r += kBlock) {
+        const double v = fabs(at(r, k));
+        if (v > best) { best = v; bi = r; }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double ov = sv[threadIdx.x + s];
+            const int oi = si[threadIdx.x + s];
+            if (ov > sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+                sv[threadIdx.x] = ov;
+                si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const int p = si[0];
+    if (threadIdx.x == 0) piv[k] = p;
+    if (at(p, k) == 0.0) {
+        if (threadIdx.x == 0) *flag = 1;
+        return;
+    }
+    if (p != k)
+        for (int c = threadIdx.x; c < n; c += kBlock) {
+            const double tmp = at(k, c);
+            at(k, c) = at(p, c);
+            at(p, c) = tmp;
+        }
+    __syncthreads();
+    const double dkk = at(k, k);
+    for (int r = k + 1 + threadIdx.x; r < n; r += kBlock) at(r, k) /= dkk;
+}
+
+__global__ __launch_bounds__(kBlock) void k_lu_update(int n, int k, double* __restrict__ a,
+                                                      const int* __restrict__ flag) {
+    if (*flag) return;
+    const int64_t m = n - k - 1;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < m * m; e += stride) {
+        const int r = k + 1 + (int)(e % m), c = k + 1 + (int)(e / m);
+        a[(size_t)c * n + r] -= a[(size_t)k * n + r] * a[(size_t)c * n + k];
+    }
+}
+
+// the row interchanges as one permutation: (P b)[i] = b[perm[i]]
+__global__ void k_lu_perm(int n, const int* __restrict__ piv, int* __restrict__ perm, const int* __restrict__ flag) {
+    if (*flag || threadIdx.x != 0) return;
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) {
+        const int t = perm[k];
+        perm[k] = perm[piv[k]];
+        perm[piv[k]] = t;
+    }
+}
+
 // x = A_c^-1 b from the LU factors in the order of the reference LAPACK
 // getrs for one right-hand side: row interchanges, then column-oriented unit
 // lower and upper triangular solves (the loops of reference BLAS trsm).  FP64
 // arithmetic; b and x in the cycle's vector type.  Factors staged in LDS when
 // they fit.
+// xg non-null: the vector is kept in that global workspace (n doubles)
+// instead of LDS, for a coarsest level too large for LDS; same operations.
 template <typename TX>
 __global__ __launch_bounds__(kBlock) void k_coarse_lu_solve(int n, const double* __restrict__ lu,
                                                             const int* __restrict__ piv, const TX* __restrict__ b,
-                                                            TX* __restrict__ x, int stage) {
+                                                            TX* __restrict__ x, int stage, double* xg) {
     extern __shared__ double sm[];
-    double* xs = sm;
+    double* xs = xg ? xg : sm;
     const double* L = lu;
     if (stage) {
         double* ls = sm + n;
@@ -486,6 +557,11 @@ void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
         launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st, M.c16.p);
     else if (kind == kOpSorted && M.hasS && svals<TV>(M))
         launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, svals<TV>(M), xf, epi, st, M.scol.p, M.slot.p);
+    else if (kind == kOpStream16Pipe && M.has16)
+        launch_stream_pipe<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, M.c16.p, vals<TV>(M), xf, epi, st, nullptr);
+    else if (kind == kOpSortedPipe && M.hasS && svals<TV>(M))
+        launch_stream_pipe<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, M.scol.p, svals<TV>(M), xf, epi, st,
+                                   M.slot.p);
     else launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
 }
 
@@ -627,8 +703,18 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
           smoother_ == Smoother::SPAI0 || smoother_ == Smoother::Chebyshev || smoother_ == Smoother::ILU0))
         throw Error("smoother not available in the HIP backend (supported: jacobi, l1_jacobi, colored_gs, "
                     "l1_colored_gs, spai0, chebyshev, ilu0)");
+    try {
+        build(h);
+    } catch (...) {
+        release_handles();  // the destructor does not run for a constructor that throws
+        throw;
+    }
+}
+
+void DeviceHierarchy::build(const HostHierarchy& h) {
     DRAUGR_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     DRAUGR_HIP_CHECK(hipEventCreateWithFlags(&last_ev_, hipEventDisableTiming));
+    DRAUGR_HIP_CHECK(hipEventCreateWithFlags(&pcg_ev_, hipEventDisableTiming));
     hipStream_t st = stream_;
     n0_ = h.levels.empty() ? h.A_coarsest.nrow : h.levels[0].A.nrow;
     L_.resize(h.levels.size());
@@ -695,6 +781,7 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
     max_blocks = std::max(max_blocks, Ac_.nblk);
     nc_ = h.n_coarse;
     clu_.alloc((size_t)nc_ * nc_);
+    if ((size_t)nc_ * sizeof(double) > (size_t)kCoarseLdsMax) cx_.alloc(nc_);  // coarse solve vector
     flag_.alloc(1 + 2 * (size_t)nc_);  // [0] singular flag, [1..n] pivots, [n+1..2n] permutation
     const int64_t n0 = n0_;
     w_r_.alloc(n0);
@@ -728,14 +815,22 @@ void DeviceHierarchy::end(hipStream_t st) {
     last_valid_ = true;
 }
 
-DeviceHierarchy::~DeviceHierarchy() {
-    if (last_valid_) (void)hipEventSynchronize(last_ev_);
+DeviceHierarchy::~DeviceHierarchy() { release_handles(); }
+
+void DeviceHierarchy::release_handles() {
+    if (last_valid_ && last_ev_) (void)hipEventSynchronize(last_ev_);
     for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    graphs_.clear();
     if (cap_) (void)hipStreamDestroy(cap_);
     if (last_ev_) (void)hipEventDestroy(last_ev_);
+    if (pcg_ev_) (void)hipEventDestroy(pcg_ev_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (pinned_) (void)hipHostFree(pinned_);
     if (stream_) (void)hipStreamDestroy(stream_);
+    cap_ = stream_ = nullptr;
+    last_ev_ = pcg_ev_ = nullptr;
+    pinned_ = nullptr;
+    last_valid_ = false;
 }
 
 int64_t DeviceHierarchy::bytes_resident() const {
@@ -821,9 +916,21 @@ void DeviceHierarchy::galerkin(int l, hipStream_t st) {
 
 // Coarsest operator: dense from its CSR values, then LU (src/setup.jl:177-209,
 // src/resetup.jl:123-158 and :44), all on the device.
-void dense_lu_device(int n, double* a, int* piv, int* flag, hipStream_t st) {
+void dense_lu_device(int n, double* a, int* piv, int* flag, hipStream_t st, bool stepwise) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_dense_lu, dim3(1), dim3(kBlock), 0, st, n, a, piv, flag);
+    if (!stepwise) {
+        hipLaunchKernelGGL(k_dense_lu, dim3(1), dim3(kBlock), 0, st, n, a, piv, flag);
+        return;
+    }
+    for (int k = 0; k < n; ++k) {
+        hipLaunchKernelGGL(k_lu_pivot, dim3(1), dim3(kBlock), 0, st, n, k, a, piv, flag);
+        const int64_t m = n - k - 1;
+        if (m > 0) {
+            const int64_t nb = std::min<int64_t>((m * m + kBlock - 1) / kBlock, 4096);
+            hipLaunchKernelGGL(k_lu_update, dim3((unsigned)nb), dim3(kBlock), 0, st, n, k, a, flag);
+        }
+    }
+    hipLaunchKernelGGL(k_lu_perm, dim3(1), dim3(1), 0, st, n, piv, piv + n, flag);
 }
 
 void invdiag_device(int n, const int* rp, const int* cv, const double* av, double* d, int mode, hipStream_t st) {
@@ -834,7 +941,7 @@ void invdiag_device(int n, const int* rp, const int* cv, const double* av, doubl
     else hipLaunchKernelGGL(k_invdiag<0>, dim3(nb), dim3(kBlock), 0, st, n, rp, cv, av, d);
 }
 
-void DeviceHierarchy::coarse_factor(hipStream_t st) {
+void DeviceHierarchy::coarse_factor(hipStream_t st, bool stepwise) {
     const int n = nc_;
     if (n == 0) return;
     double* dn = clu_.p;
@@ -847,7 +954,7 @@ void DeviceHierarchy::coarse_factor(hipStream_t st) {
     }, st);
     int* fl = flag_.p;
     launch_map(1, [=] __device__(int64_t) { fl[0] = 0; }, st);
-    hipLaunchKernelGGL(k_dense_lu, dim3(1), dim3(kBlock), 0, st, n, clu_.p, flag_.p + 1, flag_.p);
+    dense_lu_device(n, clu_.p, flag_.p + 1, flag_.p, st, stepwise || n > kLuOneGroupMax);
 }
 
 // the reduced-precision and column-sorted copies of a matrix's values
@@ -1009,7 +1116,7 @@ bool DeviceHierarchy::zero_step_is_map(int l, int pre) const {
 // the parent's restriction).
 template <typename TV, typename TX>
 void DeviceHierarchy::smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bool& zero, bool fuse_resid,
-                             bool& resid_done, hipStream_t st, bool x1_ready) {
+                             bool& resid_done, hipStream_t st, bool x1_ready, bool reverse) {
     DevLevel& D = L_[l];
     const int n = D.A.nrow;
     const TX* d = d_of<TX>(D);
@@ -1029,7 +1136,8 @@ void DeviceHierarchy::smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bo
             zero = false;
         }
         for (int s = 0; s < steps; ++s)
-            for (int c = 0; c < D.ncolors; ++c) {
+            for (int cc = 0; cc < D.ncolors; ++cc) {
+                const int c = reverse ? D.ncolors - 1 - cc : cc;  // reverse: the transposed sweep
                 const int off = D.color_off[c], cnt = D.color_off[c + 1] - off;
                 if (cnt == 0) continue;
                 hipLaunchKernelGGL((k_gs_color<TV, TX>), dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cnt,
@@ -1060,18 +1168,25 @@ void DeviceHierarchy::smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bo
 }
 
 template <typename TX>
-void DeviceHierarchy::coarse_solve(TX* x, const TX* b, hipStream_t st) {
+void DeviceHierarchy::coarse_solve(TX* x, const TX* b, hipStream_t st, bool global_x) {
     const int n = nc_;
     if (n == 0) return;
-    if (n <= 64) {
+    if (n <= 64 && !global_x) {
         hipLaunchKernelGGL(k_coarse_lu_solve_wave<TX>, dim3(1), dim3(64), 0, st, n, clu_.p, flag_.p + 1 + n, b, x);
         return;
     }
     const size_t full = (size_t)(n + (size_t)n * n) * sizeof(double);
     const int stage = full <= (size_t)kCoarseLdsMax ? 1 : 0;
-    const size_t lds = stage ? full : (size_t)n * sizeof(double);
-    if (lds > (size_t)kCoarseLdsMax) throw Error("coarsest level too large for the device direct solve");
-    hipLaunchKernelGGL(k_coarse_lu_solve<TX>, dim3(1), dim3(kBlock), lds, st, n, clu_.p, flag_.p + 1, b, x, stage);
+    if (stage || (!global_x && (size_t)n * sizeof(double) <= (size_t)kCoarseLdsMax)) {
+        const size_t lds = stage ? full : (size_t)n * sizeof(double);
+        hipLaunchKernelGGL(k_coarse_lu_solve<TX>, dim3(1), dim3(kBlock), lds, st, n, clu_.p, flag_.p + 1, b, x,
+                           stage, (double*)nullptr);
+        return;
+    }
+    // the vector does not fit in LDS (or the test hook asks for this form): a
+    // global workspace, allocated with the hierarchy when the level needs it
+    if (!cx_.p) cx_.alloc(n);
+    hipLaunchKernelGGL(k_coarse_lu_solve<TX>, dim3(1), dim3(kBlock), 0, st, n, clu_.p, flag_.p + 1, b, x, 0, cx_.p);
 }
 
 // Autotuner: for every level, time the candidate kernels of each operator
@@ -1135,7 +1250,8 @@ void DeviceHierarchy::tune() {
             }
             const float skip = 1e30f;  // the 16-bit and sorted forms only where the operator has them
             auto absent = [&](const DevCSR& M) {
-                return (k == kOpStream16 && !M.has16) || (k == kOpSorted && !M.hasS);
+                return ((k == kOpStream16 || k == kOpStream16Pipe) && !M.has16) ||
+                       ((k == kOpSorted || k == kOpSortedPipe) && !M.hasS);
             };
             tA[k] = absent(D.A) ? skip : timeit([&] {
                 op_kind<TV, TX>(k, D.A, XVec<TX>{xin}, EpiJacobi<TX>{xin, bin, d, s1, w}, st);
@@ -1202,7 +1318,7 @@ std::string DeviceHierarchy::tune_report() const {
         if (!tuned_[p]) continue;
         for (int l = 0; l < nlevels(); ++l) {
             const DevLevel& D = L_[l];
-            const char* kn[kOpKinds] = {"stream", "rowseq", "pipe", "stream16", "sorted"};
+            const char* kn[kOpKinds] = {"stream", "rowseq", "pipe", "stream16", "sorted", "stream16pipe", "sortedpipe"};
             std::snprintf(buf, sizeof buf, "%s level %d: A=%s Pt=%s P=%s zero-guess=%s down=%s up=%s\n", names[p], l,
                           kn[D.A.kind[p]], kn[D.Pt.kind[p]], kn[D.P.kind[p]], D.fuse_zero[p] ? "fused" : "split",
                           D.fuse_down[p] ? "one-pass" : (D.down_state < 0 ? "separate(only)" : "separate"),
@@ -1349,10 +1465,10 @@ void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const Cy
         const XProlong<TP, TX> xp{xcur, D.P.rp.p, D.P.cv.p, vals<TP>(D.P), xc};
         op<TV, TX>(D.A, xp, EpiJacobiP<TP, TX>{xp, b, d_of<TX>(D), xalt, (TX)weight()}, st);
         std::swap(xcur, xalt);
-        smooth<TV, TX>(l, xcur, xalt, b, o.post - 1, z, false, dummy, st);
+        smooth<TV, TX>(l, xcur, xalt, b, o.post - 1, z, false, dummy, st, false, o.gs_sym);
     } else {
         op<TP, TX>(D.P, XVec<TX>{xc}, EpiAdd<TX>{xcur}, st);
-        smooth<TV, TX>(l, xcur, xalt, b, o.post, z, false, dummy, st);
+        smooth<TV, TX>(l, xcur, xalt, b, o.post, z, false, dummy, st, false, o.gs_sym);
     }
     if (xcur != x) {
         const TX* s = xcur;
@@ -1382,8 +1498,8 @@ void DeviceHierarchy::precondition(TX* x, const TX* b, const CycleOpts& o, hipSt
         cycle_top<TV, TX>(x, b, true, o, st);
         return;
     }
-    const auto key =
-        std::make_tuple(pidx<TV>(), (int)o.w_cycle, o.pre, o.post, (int)o.x1_ready0, (const void*)x, (const void*)b);
+    const auto key = std::make_tuple(pidx<TV>(), (int)o.w_cycle, o.pre, o.post, (int)o.x1_ready0, (int)o.gs_sym,
+                                     (const void*)x, (const void*)b);
     auto it = graphs_.find(key);
     if (it == graphs_.end()) {
         if (!cap_) DRAUGR_HIP_CHECK(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
@@ -1441,8 +1557,10 @@ void DeviceHierarchy::cycle(double* x, const double* b, bool zero, const CycleOp
     else launch_map(n, [=] __device__(int64_t i) { x[i] += (double)z32[i]; }, st);
 }
 
-double DeviceHierarchy::reduce_to_host(hipStream_t st, int slot) {
-    DRAUGR_HIP_CHECK(hipMemcpyAsync(pinned_ + slot, scal_.p + slot, sizeof(double), hipMemcpyDeviceToHost, st));
+double DeviceHierarchy::reduce_to_host(hipStream_t st, int slot) { return reduce_to_host(st, scal_.p, slot); }
+
+double DeviceHierarchy::reduce_to_host(hipStream_t st, const double* scal, int slot) {
+    DRAUGR_HIP_CHECK(hipMemcpyAsync(pinned_ + slot, scal + slot, sizeof(double), hipMemcpyDeviceToHost, st));
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
     return pinned_[slot];
 }
@@ -1479,8 +1597,9 @@ int DeviceHierarchy::solve(double* x, const double* b, double tol, int maxiter, 
         rel = std::sqrt(reduce_to_host(st, S_RR)) / bnorm;
         if (relres) *relres = rel;
         if (verbose >= 2) std::printf("  AMG iter %4d: relative residual = %.6e\n", it, rel);
+        // a non-finite residual never passes the test: the iteration runs on to
+        // maxiter and returns it, as src/cycle.jl:135-159 does
         if (rel < tol) return it;
-        if (!std::isfinite(rel)) throw Error("AMG iteration diverged (non-finite residual)");
     }
     return maxiter;
 }
@@ -1494,7 +1613,7 @@ bool DeviceHierarchy::pcg_sorted(const DevCSR& A, hipStream_t st) {
     pcg_sorted_ = 0;
     if (!A.hasS) return false;
     if (forced_kind_ >= 0) {
-        pcg_sorted_ = forced_kind_ == kOpSorted;
+        pcg_sorted_ = forced_kind_ == kOpSorted || forced_kind_ == kOpSortedPipe;
         return pcg_sorted_ == 1;
     }
     const double* u = w_z_.p;
@@ -1536,30 +1655,46 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                          hipStream_t st, double* relres, int verbose) {
     const int n = n0_;
     double* scal = scal_.p;
+    double* part = partial_.p;
     double* r = w_r_.p;
     double* q = w_q_.p;
     double* z = w_z_.p;
     double* pa = w_p0_.p;
     double* pb = w_p1_.p;
     const bool mixed = o.prec != Precision::FP64 && nlevels() > 0;
+    const DevCSR& A = nlevels() > 0 ? L_[0].A : Ac_;
+    if (nlevels() > 0) {
+        // CG needs a symmetric preconditioner: the cycle is symmetric with as
+        // many post- as pre-smoothing steps and a smoother that is its own
+        // transpose (Jacobi-type, Chebyshev) or is paired with its transpose
+        // (coloured GS with the post-sweep in reverse colour order)
+        if (o.pre != o.post)
+            throw Error("PCG needs a symmetric preconditioner: pre_smoothing_steps (" + std::to_string(o.pre) +
+                        ") must equal post_smoothing_steps (" + std::to_string(o.post) + ")");
+        if (smoother_ == Smoother::ILU0)
+            throw Error("PCG needs a symmetric preconditioner: the ilu0 smoother is not (use amg_solve or "
+                        "another smoother)");
+        if (is_gs(smoother_) && !o.gs_sym)
+            throw Error("PCG needs a symmetric preconditioner: coloured Gauss-Seidel sweeps the colours in the "
+                        "same order before and after (set gs_symmetric for a reverse-order post-sweep)");
+    }
     if (mixed) ensure_precision(o.prec);
     float* r32 = w_r32_.p;
     float* z32 = w_z32_.p;
     launch_reduce(n, [=] __device__(int64_t i) { return b[i] * b[i]; },
-                  [=] __device__(double s) { scal[S_BB] = s; }, partial_.p, st);
-    const double bb = reduce_to_host(st, S_BB);
+                  [=] __device__(double s) { scal[S_BB] = s; }, part, st);
+    const double bb = reduce_to_host(st, scal, S_BB);
     if (relres) *relres = 0.0;
     if (bb == 0.0) {
         launch_map(n, [=] __device__(int64_t i) { x[i] = 0.0; }, st);
         DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
         return 0;
     }
-    const DevCSR& A = nlevels() > 0 ? L_[0].A : Ac_;
     const bool srt = pcg_sorted(A, st);
     op<double, double>(A, XVec<double>{x}, EpiResid<double>{b, r}, st);  // r = b - A x
     launch_reduce(n, [=] __device__(int64_t i) { return r[i] * r[i]; },
-                  [=] __device__(double s) { scal[S_RR] = s; }, partial_.p, st);
-    double rel = std::sqrt(reduce_to_host(st, S_RR) / bb);
+                  [=] __device__(double s) { scal[S_RR] = s; }, part, st);
+    double rel = std::sqrt(reduce_to_host(st, scal, S_RR) / bb);
     if (relres) *relres = rel;
     if (rel < tol) return 0;
     launch_map(n, [=] __device__(int64_t i) { pa[i] = 0.0; }, st);
@@ -1592,12 +1727,11 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
     // rz = r.z ; beta = 0 for the first direction
     if (mixed)
         launch_reduce(n, [=] __device__(int64_t i) { return r[i] * (double)z32[i]; },
-                      [=] __device__(double s) { scal[S_RZ] = s; scal[S_BETA] = 0.0; }, partial_.p, st);
+                      [=] __device__(double s) { scal[S_RZ] = s; scal[S_BETA] = 0.0; }, part, st);
     else
         launch_reduce(n, [=] __device__(int64_t i) { return r[i] * z[i]; },
-                      [=] __device__(double s) { scal[S_RZ] = s; scal[S_BETA] = 0.0; }, partial_.p, st);
-    hipEvent_t ev;
-    DRAUGR_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                      [=] __device__(double s) { scal[S_RZ] = s; scal[S_BETA] = 0.0; }, part, st);
+    hipEvent_t ev = pcg_ev_;
     int result = maxiter;
     try {
         for (int it = 1; it <= maxiter; ++it) {
@@ -1612,7 +1746,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                     q[i] = s;
                     return pn * s;
                 };
-                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, partial_.p, st, srt);
+                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, part, st, srt);
             } else {
                 auto xf = [=] __device__(int j) -> double { return z[j] + *betap * pa[j]; };
                 auto epi = [=] __device__(int i, double s) -> double {
@@ -1621,7 +1755,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                     q[i] = s;
                     return pn * s;
                 };
-                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, partial_.p, st, srt);
+                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, part, st, srt);
             }
             // x += alpha p; r -= alpha q; r.r  (and the FP32 copy of r).  The
             // finalize also writes r.r and the iteration number straight into
@@ -1646,7 +1780,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                                   if (x1_0) t32[i] = (w32 * d32[i]) * rf;
                                   return ri * ri;
                               },
-                              fin_rr, partial_.p, st);
+                              fin_rr, part, st);
             else
                 launch_reduce(n,
                               [=] __device__(int64_t i) {
@@ -1657,7 +1791,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                                   if (x1_0) t64[i] = (w64 * d64[i]) * ri;
                                   return ri * ri;
                               },
-                              fin_rr, partial_.p, st);
+                              fin_rr, part, st);
             DRAUGR_HIP_CHECK(hipEventRecord(ev, st));
             // speculative: next preconditioned residual and beta
             apply_prec(oit);
@@ -1666,9 +1800,9 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                 scal[S_RZ] = s;
             };
             if (mixed)
-                launch_reduce(n, [=] __device__(int64_t i) { return r[i] * (double)z32[i]; }, fin_rz, partial_.p, st);
+                launch_reduce(n, [=] __device__(int64_t i) { return r[i] * (double)z32[i]; }, fin_rz, part, st);
             else
-                launch_reduce(n, [=] __device__(int64_t i) { return r[i] * z[i]; }, fin_rz, partial_.p, st);
+                launch_reduce(n, [=] __device__(int64_t i) { return r[i] * z[i]; }, fin_rz, part, st);
             std::swap(pa, pb);
             DRAUGR_HIP_CHECK(hipEventSynchronize(ev));
             double rr = pinned_[kPinRR];
@@ -1682,11 +1816,9 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
         }
     } catch (...) {
         (void)hipStreamSynchronize(st);
-        (void)hipEventDestroy(ev);
         throw;
     }
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
-    DRAUGR_HIP_CHECK(hipEventDestroy(ev));
     return result;
 }
 
@@ -1730,9 +1862,9 @@ void DeviceHierarchy::resetup(const double* nzval, hipStream_t st) {
 // nonzero.  The algorithmic count of the cycle uses 4 per nonzero whatever
 // the form (int32 indices: the work, not the storage, defines it).
 static double index_bytes(const DevCSR& M, int pi) {
-    if (M.kind[pi] == kOpStream16 && M.has16)
+    if ((M.kind[pi] == kOpStream16 || M.kind[pi] == kOpStream16Pipe) && M.has16)
         return 2.0 * M.nnz16 + 4.0 * (M.nnz - M.nnz16) + 16.0 * M.nblk;  // + the block headers
-    if (M.kind[pi] == kOpSorted && M.hasS)  // + the slot of every entry of a 16-bit block
+    if ((M.kind[pi] == kOpSorted || M.kind[pi] == kOpSortedPipe) && M.hasS)  // + the slot of each 16-bit entry
         return 4.0 * M.nnz16 + 4.0 * (M.nnz - M.nnz16) + 16.0 * M.nblk;
     return 4.0 * M.nnz;
 }
@@ -1789,9 +1921,13 @@ double DeviceHierarchy::cycle_bytes(const CycleOpts& o) const {
 // ── per-kernel hooks for parity tests and measurement ───────────────────────
 void DeviceHierarchy::level_op(int lvl, int op_code, int prec, const void* in1, const void* in2, void* out,
                                void* out2, hipStream_t st) {
-    if (op_code == 7) {
-        if (prec == 0) coarse_solve<double>((double*)out, (const double*)in1, st);
-        else coarse_solve<float>((float*)out, (const float*)in1, st);
+    if (op_code == 7 || op_code == 12) {  // 12: the form with the vector in a global workspace
+        if (prec == 0) coarse_solve<double>((double*)out, (const double*)in1, st, op_code == 12);
+        else coarse_solve<float>((float*)out, (const float*)in1, st, op_code == 12);
+        return;
+    }
+    if (op_code == 13) {  // refactor the coarsest operator column by column (the large-operator form)
+        coarse_factor(st, true);
         return;
     }
     if (lvl < 0 || lvl >= nlevels()) throw Error("level index out of range");

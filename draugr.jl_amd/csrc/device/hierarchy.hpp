@@ -89,13 +89,16 @@ struct CycleOpts {
     Precision prec = Precision::FP64;
     bool graphs = true;  // zero-guess cycles on fixed buffers replayed from a hipGraph
     bool x1_ready0 = false;  // the finest level's first step from zero is already in its work vector
+    bool gs_sym = false;     // coloured GS post-smoothing sweeps the colours in reverse order
 };
 
 // Kernels shared with the batched path (batch.hip):
 // dense LU with partial pivoting in place (column-major n x n; piv[0..n) the
 // LAPACK pivots, piv[n..2n) the row permutation; *flag = 1 if singular), and
 // the plain / l1 inverse diagonal of a CSR operator.
-void dense_lu_device(int n, double* a, int* piv, int* flag, hipStream_t st);
+// stepwise: one launch pair per column over the whole chip (large n); the
+// factors are the same bits either way.
+void dense_lu_device(int n, double* a, int* piv, int* flag, hipStream_t st, bool stepwise = false);
 // mode 0: 1 / a_ii (src/smoothers.jl:47-63), 1: 1 / sum |a_ij| (:816-827),
 // 2: SPAI(0) a_ii / sum a_ij^2 (:502-532).
 void invdiag_device(int n, const int* rp, const int* cv, const double* av, double* d, int mode, hipStream_t st);
@@ -161,19 +164,25 @@ private:
     template <typename TV, typename TX>
     void cycle_level(int l, TX* x, const TX* b, bool zero_guess, const CycleOpts& o, hipStream_t st,
                      bool x1_ready = false);
+    // reverse: coloured GS sweeps the colours in reverse order (the
+    // post-smoothing of the symmetric option)
     template <typename TV, typename TX>
     void smooth(int l, TX*& x, TX*& alt, const TX* b, int steps, bool& zero, bool fuse_resid, bool& resid_done,
-                hipStream_t st, bool x1_ready = false);
+                hipStream_t st, bool x1_ready = false, bool reverse = false);
     template <typename TV> bool zero_step_is_map(int l, int pre) const;
     bool down_blocks(int l);
     template <typename TV, typename TX>
     void launch_down(int l, const TX* b, TX* x1, TX* bc, const TX* dc, TX* x1c, hipStream_t st);
-    template <typename TX> void coarse_solve(TX* x, const TX* b, hipStream_t st);
+    // global_x: keep the vector in a global workspace (the form for a coarsest
+    // level whose vector does not fit in LDS)
+    template <typename TX> void coarse_solve(TX* x, const TX* b, hipStream_t st, bool global_x = false);
     template <typename TV, typename TX> void cycle_top(TX* x, const TX* b, bool zero_guess, const CycleOpts& o,
                                                        hipStream_t st);
     // zero-guess cycle x = V(0; b) on buffers that stay fixed for the
     // hierarchy's life: captured once into a hipGraph, then replayed
     template <typename TV, typename TX> void precondition(TX* x, const TX* b, const CycleOpts& o, hipStream_t st);
+    void build(const HostHierarchy& h);
+    void release_handles();
     void refresh_level_mirrors(int l, hipStream_t st);
     template <typename TV, typename TX> void tune();
     void compute_dinv(int l, hipStream_t st);
@@ -186,8 +195,9 @@ private:
     void chebyshev(int l, TX*& x, TX*& alt, const TX* b, int steps, bool& zero, hipStream_t st);
     double weight() const { return smoother_ == Smoother::SPAI0 ? 1.0 : omega_; }
     void galerkin(int l, hipStream_t st);
-    void coarse_factor(hipStream_t st);
+    void coarse_factor(hipStream_t st, bool stepwise = false);
     double reduce_to_host(hipStream_t st, int slot);
+    double reduce_to_host(hipStream_t st, const double* scal, int slot);
 
     std::vector<DevLevel> L_;
     int32_t n0_ = 0;
@@ -195,6 +205,7 @@ private:
     DevCSR Ac_;                        // coarsest operator (sparse; values by Galerkin)
     DBuf<double> clu_;                 // dense LU factors of the coarsest operator (column-major)
     DBuf<int32_t> flag_;               // [0] singular flag, [1..n] pivots, [n+1..2n] permutation
+    DBuf<double> cx_;                  // coarse solve vector when it does not fit in LDS
     Smoother smoother_;
     double omega_;
     uint64_t seed_ = 0;
@@ -213,8 +224,9 @@ private:
     static constexpr int kPinRR = 8, kPinIter = 9;
     hipStream_t stream_ = nullptr;
     hipEvent_t last_ev_ = nullptr;
+    hipEvent_t pcg_ev_ = nullptr;  // marks each PCG iteration's r.r (the host waits on it)
     hipStream_t cap_ = nullptr;  // capture stream for the graphs
-    std::map<std::tuple<int, int, int, int, int, const void*, const void*>, hipGraphExec_t> graphs_;
+    std::map<std::tuple<int, int, int, int, int, int, const void*, const void*>, hipGraphExec_t> graphs_;
     hipStream_t last_st_ = nullptr;
     bool last_valid_ = false;
 };

@@ -81,8 +81,7 @@ template <typename TX> struct XZeroJacobi {
 //    q = pre(i) (the row's own operands, loaded before the products are
 //    staged so that their latency overlaps the gathers); a = init(q);
 //    a = acc(a, p_k) for each k; fin(i, a, q) — the reference's own
-//    accumulation for that operation.  total(i, s) finishes a long row whose
-//    products were summed by the whole block (tree order). ────────────────────
+//    accumulation for that operation. ─────────────────────────────────────────
 struct NoPre {};
 template <typename TX> struct EpiStore {          // y = A x   (src/csr_matrix.jl:150-159)
     TX* y;
@@ -91,7 +90,6 @@ template <typename TX> struct EpiStore {          // y = A x   (src/csr_matrix.j
     __device__ TX init(const Pre&) const { return 0; }
     __device__ TX acc(TX a, TX p) const { return a + p; }
     __device__ void fin(int i, TX a, const Pre&) const { y[i] = a; }
-    __device__ void total(int i, TX s) const { y[i] = s; }
 };
 // Restriction that also forms the coarse level's first Jacobi iterate from a
 // zero guess: bc = P^T r and x1 = (w dinv_c) bc, the value the coarse level's
@@ -106,7 +104,6 @@ template <typename TX> struct EpiStoreJac0 {
         y[i] = a;
         x1[i] = (w * q.d) * a;
     }
-    __device__ void total(int i, TX s) const { fin(i, s, pre(i)); }
 };
 // Prolongation folded into the following smoothing pass: the prolonged
 // iterate xp_k = x_k + sum_J P_kJ xc_J (EpiAdd's order) formed on the fly for
@@ -128,7 +125,6 @@ template <typename TP, typename TX> struct EpiJacobiP {
     __device__ TX init(const Pre& q) const { return q.b; }
     __device__ TX acc(TX a, TX p) const { return a - p; }
     __device__ void fin(int i, TX a, const Pre& q) const { xo[i] = q.x + (w * q.d) * a; }
-    __device__ void total(int i, TX s) const { xo[i] = xp(i) + (w * d[i]) * (b[i] - s); }
 };
 template <typename TX> struct EpiResid {          // r = b - A x   (src/cycle.jl:32-43)
     const TX* b; TX* r;
@@ -137,7 +133,6 @@ template <typename TX> struct EpiResid {          // r = b - A x   (src/cycle.jl
     __device__ TX init(const Pre&) const { return 0; }
     __device__ TX acc(TX a, TX p) const { return a + p; }
     __device__ void fin(int i, TX a, const Pre& q) const { r[i] = q.b - a; }
-    __device__ void total(int i, TX s) const { r[i] = b[i] - s; }
 };
 template <typename TX> struct EpiJacobi {         // xo = x + w d (b - A x)   (src/smoothers.jl:78-94)
     const TX* x; const TX* b; const TX* d; TX* xo; TX w;
@@ -146,7 +141,6 @@ template <typename TX> struct EpiJacobi {         // xo = x + w d (b - A x)   (s
     __device__ TX init(const Pre& q) const { return q.b; }  // residual by successive subtraction
     __device__ TX acc(TX a, TX p) const { return a - p; }
     __device__ void fin(int i, TX a, const Pre& q) const { xo[i] = q.x + (w * q.d) * a; }
-    __device__ void total(int i, TX s) const { xo[i] = x[i] + (w * d[i]) * (b[i] - s); }
 };
 template <typename TX> struct EpiZeroJacobiResid {  // x1 = w d b, r = b - A x1
     const TX* b; const TX* d; TX* x1; TX* r; TX w;
@@ -158,7 +152,6 @@ template <typename TX> struct EpiZeroJacobiResid {  // x1 = w d b, r = b - A x1
         x1[i] = (w * q.d) * q.b;
         r[i] = q.b - a;
     }
-    __device__ void total(int i, TX s) const { fin(i, s, pre(i)); }
 };
 template <typename TX> struct EpiAdd {            // x += P xc, one product at a time (src/prolongation.jl:750-759)
     TX* x;
@@ -167,7 +160,6 @@ template <typename TX> struct EpiAdd {            // x += P xc, one product at a
     __device__ TX init(const Pre& q) const { return q.x; }
     __device__ TX acc(TX a, TX p) const { return a + p; }
     __device__ void fin(int i, TX a, const Pre&) const { x[i] = a; }
-    __device__ void total(int i, TX s) const { x[i] += s; }
 };
 
 // Chebyshev smoother passes (src/smoothers.jl:925-990): with r = b - A x
@@ -185,7 +177,6 @@ template <typename TX> struct EpiCheb0 {
         d[i] = dn;
         xo[i] = q.x + dn;
     }
-    __device__ void total(int i, TX s) const { fin(i, s, pre(i)); }
 };
 template <typename TX> struct EpiChebK {
     const TX* x; const TX* b; const TX* dv; TX* d; TX* xo; TX cr, cd;
@@ -198,7 +189,6 @@ template <typename TX> struct EpiChebK {
         d[i] = dn;
         xo[i] = q.x + dn;
     }
-    __device__ void total(int i, TX s) const { fin(i, s, pre(i)); }
 };
 // w = (A v) .* dinv: one power-iteration step of src/smoothers.jl:866-885
 template <typename TX> struct EpiStoreScaled {
@@ -208,7 +198,6 @@ template <typename TX> struct EpiStoreScaled {
     __device__ TX init(const Pre&) const { return 0; }
     __device__ TX acc(TX a, TX p) const { return a + p; }
     __device__ void fin(int i, TX a, const Pre& q) const { y[i] = a * q.dv; }
-    __device__ void total(int i, TX s) const { y[i] = s * dv[i]; }
 };
 
 // Row-block header of the CSR-stream kernels: first row, first nonzero,
@@ -222,14 +211,17 @@ struct alignas(16) BlkHdr {
 };
 
 // One row block: stage the products, then finish each row on its own thread
-// (segf(row, seg, len, pref(row))), or a long row on thread 0 (totf(row, sum)).
-// The block header holds both row and nonzero offsets, so a block's range is
-// known after one round of loads.  A block of at most PER * kBlock
-// nonzeros (PER = the operator's budget / kBlock) is loaded without branches:
-// every thread issues its PER column and value loads together (indices past
-// the block's end are clamped to its last entry and the result discarded),
-// then its PER gathers together, so each wave keeps 2 PER + PER loads in
-// flight instead of waiting once per product.
+// with the epilogue's accumulation a = initf(q); a = accf(a, p_k) in row
+// order; finf(row, a, q), q = pref(row).  The block header holds both row and
+// nonzero offsets, so a block's range is known after one round of loads.  A
+// block of at most PER * kBlock nonzeros (PER = the operator's budget /
+// kBlock) is loaded without branches: every thread issues its PER column and
+// value loads together (indices past the block's end are clamped to its last
+// entry and the result discarded), then its PER gathers together, so each
+// wave keeps 2 PER + PER loads in flight instead of waiting once per product.
+// A row longer than the LDS stage (alone in its block) is staged kStreamNnz
+// products at a time and summed by one thread in row order, so every row sum,
+// long or short, is the reference's sequential one.
 //
 // C16: such blocks read their columns as 16-bit offsets from a per-block base
 // (col = cb + c16[k]), 2 bytes per nonzero instead of 4, when the block spans
@@ -242,12 +234,12 @@ struct alignas(16) BlkHdr {
 // adjacent cache lines instead of the lines of every column cluster of its
 // rows; the staged products, and so the row sums, are the same.  Other blocks
 // keep row order (av in row order, columns from cv).
-template <int PER, bool C16, typename TV, typename TX, class XF, class PREF, class SEGF, class TOTF,
+template <int PER, bool C16, typename TV, typename TX, class XF, class PREF, class INITF, class ACCF, class FINF,
           bool SRT = false>
 __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ hdr, const int* __restrict__ rp,
                                              const int* __restrict__ cv, const uint16_t* __restrict__ c16,
-                                             const TV* __restrict__ av, XF xf, TX* prod, PREF pref, SEGF segf,
-                                             TOTF totf, const uint16_t* __restrict__ slot = nullptr) {
+                                             const TV* __restrict__ av, XF xf, TX* prod, PREF pref, INITF initf,
+                                             ACCF accf, FINF finf, const uint16_t* __restrict__ slot = nullptr) {
     const BlkHdr h0 = hdr[b];
     const int r0 = h0.r0, r1 = h0.r1;
     const int z0 = h0.z0, z1 = hdr[b + 1].z0;
@@ -292,17 +284,28 @@ __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ h
             for (int k = t; k < nz; k += kBlock) prod[k] = ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
         }
         __syncthreads();
-        if (own) segf(row, prod + (rs - z0), re - rs, q);
-    } else {
-        TX acc = 0;
-        for (int k = t; k < nz; k += kBlock) acc += ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
-        prod[t] = acc;
-        __syncthreads();
-        for (int s = kBlock / 2; s > 0; s >>= 1) {
-            if (t < s) prod[t] += prod[t + s];
+        if (own) {
+            const TX* seg = prod + (rs - z0);
+            auto a = initf(q);
+            for (int k = 0; k < re - rs; ++k) a = accf(a, seg[k]);
+            finf(row, a, q);
+        }
+    } else {  // one row longer than the LDS stage: chunk by chunk, summed in row order by thread 0
+        decltype(pref(0)) q{};
+        decltype(initf(q)) a{};
+        if (t == 0) {
+            q = pref(r0);
+            a = initf(q);
+        }
+        for (int c0 = 0; c0 < nz; c0 += kStreamNnz) {
+            const int cn = min(kStreamNnz, nz - c0);
+            for (int k = t; k < cn; k += kBlock) prod[k] = ld<TX, TV>(av, z0 + c0 + k) * xf(ldi(cv, z0 + c0 + k));
+            __syncthreads();
+            if (t == 0)
+                for (int k = 0; k < cn; ++k) a = accf(a, prod[k]);
             __syncthreads();
         }
-        if (t == 0) totf(r0, prod[0]);
+        if (t == 0) finf(r0, a, q);
     }
 }
 
@@ -322,10 +325,11 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(const BlkHdr* __restrict_
     __shared__ TX prod[kStreamNnz];
     const int b = XCD ? xcd_block() : (int)blockIdx.x;
     auto pref = [&](int row) { return epi.pre(row); };
-    auto segf = [&](int row, const TX* seg, int len, const typename EPI::Pre& q) { epi_segment(epi, row, seg, len, q); };
-    auto totf = [&](int row, TX s) { epi.total(row, s); };
-    stream_block<PER, C16, TV, TX, XF, decltype(pref), decltype(segf), decltype(totf), SRT>(
-        b, hdr, rp, cv, c16, av, xf, prod, pref, segf, totf, slot);
+    auto initf = [&](const typename EPI::Pre& q) { return epi.init(q); };
+    auto accf = [&](TX a, TX p) { return epi.acc(a, p); };
+    auto finf = [&](int row, TX a, const typename EPI::Pre& q) { epi.fin(row, a, q); };
+    stream_block<PER, C16, TV, TX, XF, decltype(pref), decltype(initf), decltype(accf), decltype(finf), SRT>(
+        b, hdr, rp, cv, c16, av, xf, prod, pref, initf, accf, finf, slot);
 }
 
 // Thread-per-row variant without LDS: fewer dependent round trips, for the
@@ -410,16 +414,22 @@ __global__ __launch_bounds__(kBlock) void k_csr_pipe(int nblk, const int* __rest
             __syncthreads();
             if (own) epi_segment(epi, row, prod + (rs - z0), re - rs, q);
             if (nnz > 0 && nnz <= PER * kBlock) load(nz0, nnz);
-        } else {  // a row longer than the LDS stage: tree sum
-            TX acc = 0;
-            for (int k = t; k < nz; k += kBlock) acc += ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
-            prod[t] = acc;
-            __syncthreads();
-            for (int s = kBlock / 2; s > 0; s >>= 1) {
-                if (t < s) prod[t] += prod[t + s];
+        } else {  // a row longer than the LDS stage: chunk by chunk, row order, on thread 0
+            typename EPI::Pre q0{};
+            TX a = 0;
+            if (t == 0) {
+                q0 = epi.pre(r0);
+                a = epi.init(q0);
+            }
+            for (int c0 = 0; c0 < nz; c0 += kStreamNnz) {
+                const int cn = min(kStreamNnz, nz - c0);
+                for (int k = t; k < cn; k += kBlock) prod[k] = ld<TX, TV>(av, z0 + c0 + k) * xf(ldi(cv, z0 + c0 + k));
+                __syncthreads();
+                if (t == 0)
+                    for (int k = 0; k < cn; ++k) a = epi.acc(a, prod[k]);
                 __syncthreads();
             }
-            if (t == 0) epi.total(r0, prod[0]);
+            if (t == 0) epi.fin(r0, a, q0);
             if (nnz > 0 && nnz <= PER * kBlock) load(nz0, nnz);
         }
         __syncthreads();  // the stage is reused by the next block
@@ -430,8 +440,147 @@ __global__ __launch_bounds__(kBlock) void k_csr_pipe(int nblk, const int* __rest
     }
 }
 
-// Resident workgroups of the pipelined kernel: about 8 per CU on 256 CUs.
+// Software-pipelined CSR-stream over the block-header table, for the 16-bit
+// column form (C16) and its column-sorted copy (SRT): workgroup w walks a
+// contiguous run of row blocks (the runs of one XCD are adjacent); block b+1's
+// column, value and slot loads are issued right after block b's gathers, so
+// they are in flight while block b's gathers return, its products are staged
+// and its rows are summed.  Block headers are read two blocks ahead.  Same
+// products, same LDS slots and the same sequential row sums as k_csr_stream,
+// so the results are the same bits.  Blocks longer than PER * kBlock nonzeros
+// (a single long row) are staged kStreamNnz products at a time and summed in
+// row order by one thread.
+template <bool XCD, int PER, bool SRT, typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_stream_pipe(int nblk, const BlkHdr* __restrict__ hdr,
+                                                        const int* __restrict__ rp, const int* __restrict__ cv,
+                                                        const uint16_t* __restrict__ c16, const TV* __restrict__ av,
+                                                        XF xf, EPI epi, const uint16_t* __restrict__ slot) {
+    __shared__ TX prod[kStreamNnz];
+    const int w = XCD ? xcd_block() : (int)blockIdx.x;
+    const int per_wg = (nblk + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int b0 = w * per_wg, b1 = min(nblk, b0 + per_wg);
+    if (b0 >= b1) return;
+    const int t = (int)threadIdx.x;
+    int cj[PER], sl[PER];
+    TX v[PER];
+    // loads of a block of 0 < nz <= PER * kBlock nonzeros (indices past its end
+    // clamped to its last entry; their products are discarded)
+    auto load = [&](const BlkHdr& h, int nz) {
+        if (h.cb >= 0) {
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int k = min(t + u * kBlock, nz - 1);
+                cj[u] = h.cb + ldc16(c16, h.z0 + k);
+                v[u] = ld<TX, TV>(av, h.z0 + k);
+                sl[u] = SRT ? ldc16(slot, h.z0 + k) : t + u * kBlock;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int k = min(t + u * kBlock, nz - 1);
+                cj[u] = ldi(cv, h.z0 + k);
+                v[u] = ld<TX, TV>(av, h.z0 + k);
+                sl[u] = t + u * kBlock;
+            }
+        }
+    };
+    BlkHdr hc = hdr[b0], hn = hdr[b0 + 1];
+    int nz = hn.z0 - hc.z0;
+    if (nz > 0 && nz <= PER * kBlock) load(hc, nz);
+    for (int b = b0; b < b1; ++b) {
+        const bool more = b + 1 < b1;
+        BlkHdr hn2{};
+        if (more) hn2 = hdr[b + 2];  // entry nblk closes the table, so b + 2 <= nblk
+        const int nz_next = more ? hn2.z0 - hn.z0 : 0;
+        const bool pipe_next = nz_next > 0 && nz_next <= PER * kBlock;
+        const int row = hc.r0 + t;
+        const bool own = row < hc.r1;
+        const int rs = own ? rp[row] : 0, re = own ? rp[row + 1] : 0;
+        typename EPI::Pre q{};
+        if (own) q = epi.pre(row);
+        if (nz <= PER * kBlock) {
+            if (nz > 0) {
+                TX g[PER], vv[PER];
+                int ss[PER];
+#pragma unroll
+                for (int u = 0; u < PER; ++u) g[u] = xf(cj[u]);
+#pragma unroll
+                for (int u = 0; u < PER; ++u) {
+                    vv[u] = v[u];
+                    ss[u] = sl[u];
+                }
+                if (pipe_next) load(hn, nz_next);
+#pragma unroll
+                for (int u = 0; u < PER; ++u)
+                    if (t + u * kBlock < nz) prod[ss[u]] = vv[u] * g[u];
+            } else if (pipe_next) {
+                load(hn, nz_next);
+            }
+            __syncthreads();
+            if (own) epi_segment(epi, row, prod + (rs - hc.z0), re - rs, q);
+        } else {  // one long row: chunk by chunk, row order, on thread 0
+            typename EPI::Pre q0{};
+            TX a = 0;
+            if (t == 0) {
+                q0 = epi.pre(hc.r0);
+                a = epi.init(q0);
+            }
+            for (int c0 = 0; c0 < nz; c0 += kStreamNnz) {
+                const int cn = min(kStreamNnz, nz - c0);
+                for (int k = t; k < cn; k += kBlock)
+                    prod[k] = ld<TX, TV>(av, hc.z0 + c0 + k) * xf(ldi(cv, hc.z0 + c0 + k));
+                __syncthreads();
+                if (t == 0)
+                    for (int k = 0; k < cn; ++k) a = epi.acc(a, prod[k]);
+                __syncthreads();
+            }
+            if (t == 0) epi.fin(hc.r0, a, q0);
+            if (pipe_next) load(hn, nz_next);
+        }
+        __syncthreads();  // the stage is reused by the next block
+        hc = hn;
+        hn = hn2;
+        nz = nz_next;
+    }
+}
+
+// Resident workgroups of the pipelined kernels: about 8 per CU on 256 CUs.
 constexpr int kPipeGrid = 2048;
+
+template <int PER, bool SRT, typename TV, typename TX, class XF, class EPI>
+void launch_stream_pipe_per(int nblk, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16,
+                            const TV* av, XF xf, EPI epi, hipStream_t st, const uint16_t* slot) {
+    const int grid = std::min(nblk, kPipeGrid);
+    if (grid >= 64)
+        hipLaunchKernelGGL((k_stream_pipe<true, PER, SRT, TV, TX, XF, EPI>), dim3(grid), dim3(kBlock), 0, st, nblk,
+                           hdr, rp, cv, c16, av, xf, epi, slot);
+    else
+        hipLaunchKernelGGL((k_stream_pipe<false, PER, SRT, TV, TX, XF, EPI>), dim3(grid), dim3(kBlock), 0, st,
+                           nblk, hdr, rp, cv, c16, av, xf, epi, slot);
+}
+
+// per = the operator's block budget / kBlock; slot non-null = the
+// column-sorted form (c16 and av sorted within each 16-bit block)
+template <typename TV, typename TX, class XF, class EPI>
+void launch_stream_pipe(int nblk, int per, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16,
+                        const TV* av, XF xf, EPI epi, hipStream_t st, const uint16_t* slot) {
+    if (nblk <= 0) return;
+    if (slot) {
+        switch (per) {
+            case 1: launch_stream_pipe_per<1, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+            case 2: launch_stream_pipe_per<2, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+            case 4: launch_stream_pipe_per<4, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+            default: launch_stream_pipe_per<8, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+        }
+    } else {
+        switch (per) {
+            case 1: launch_stream_pipe_per<1, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+            case 2: launch_stream_pipe_per<2, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+            case 4: launch_stream_pipe_per<4, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+            default: launch_stream_pipe_per<8, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+        }
+    }
+}
 
 template <int PER, typename TV, typename TX, class XF, class EPI>
 void launch_pipe_per(int nblk, const int* blk, const int* bz, const int* rp, const int* cv, const TV* av, XF xf,
@@ -457,8 +606,10 @@ void launch_pipe(int nblk, int per, const int* blk, const int* bz, const int* rp
     }
 }
 
-enum OpKernel : int { kOpStream = 0, kOpRowSeq = 1, kOpPipe = 2, kOpStream16 = 3, kOpSorted = 4 };
-constexpr int kOpKinds = 5;
+enum OpKernel : int {
+    kOpStream = 0, kOpRowSeq = 1, kOpPipe = 2, kOpStream16 = 3, kOpSorted = 4, kOpStream16Pipe = 5, kOpSortedPipe = 6
+};
+constexpr int kOpKinds = 7;
 
 template <int PER, bool C16, bool SRT, typename TV, typename TX, class XF, class EPI>
 void launch_stream_per(int nblk, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16, const TV* av,

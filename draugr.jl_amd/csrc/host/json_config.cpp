@@ -265,6 +265,7 @@ Config config_from_json(const char* json) {
     if (const JVal* v = get("precision")) cfg.precision = lookup(kPrecision, to_string(*v), "precision");
     if (const JVal* v = get("seed")) cfg.seed = (uint64_t)std::llround(to_float(*v));
     if (const JVal* v = get("graphs")) cfg.graphs = to_bool(*v);
+    if (const JVal* v = get("gs_symmetric")) cfg.gs_symmetric = to_bool(*v);
     if (const JVal* v = get("gpu_setup")) cfg.gpu_setup = to_bool(*v);
     if (const JVal* v = get("fuse")) {
         static const std::map<std::string, int> kFuse = {{"auto", -1}, {"off", 0}, {"on", 1}};
@@ -273,7 +274,8 @@ Config config_from_json(const char* json) {
     if (const JVal* v = get("gpu_setup_min_rows")) cfg.gpu_setup_min_rows = to_int(*v);
     if (const JVal* v = get("kernel")) {
         static const std::map<std::string, int> kKernel = {{"auto", -1}, {"stream", 0}, {"rowseq", 1}, {"pipe", 2},
-                                                                 {"stream16", 3}, {"sorted", 4}};
+                                                                 {"stream16", 3}, {"sorted", 4},
+                                                                 {"stream16pipe", 5}, {"sortedpipe", 6}};
         cfg.kernel = lookup(kKernel, to_string(*v), "kernel");
     }
     if (const JVal* v = get("filtering")) {

@@ -83,6 +83,7 @@ struct Config {
                                             // 3 stream with 16-bit columns
     bool gpu_setup = true;                  // interpolation, coarse operators and triple maps built on the GPU
     int gpu_setup_min_rows = 32768;         // levels below this size interpolate on the host (faster there)
+    bool gs_symmetric = false;              // coloured GS post-smoothing sweeps the colours in reverse order
 };
 
 // ── hierarchy (host copy; uploaded once to HBM) ──────────────────────────────

@@ -20,6 +20,8 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _f64p = ctypes.POINTER(ctypes.c_double)
 
 SIGNATURES = {
+    "init_julia": (None, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
+    "shutdown_julia": (None, [ctypes.c_int]),
     "draugr_amg_last_error": (ctypes.c_char_p, []),
     "draugr_amg_config_from_json": (_i32, [ctypes.c_char_p]),
     "draugr_amg_setup": (_i32, [_i32, _i32, _vp, _vp, _vp, _i32, _i32, _i32]),

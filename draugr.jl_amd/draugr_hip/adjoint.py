@@ -24,22 +24,12 @@ condition number.
 """
 from __future__ import annotations
 
-import ctypes
 from typing import Optional
 
 from . import _lib
 from ._lib import DraugrError, check
 from .config import AMGConfig
-from .core import AMGHierarchy, _is_torch, _torch_stream, amg_resetup_, pcg_solve_
-
-
-def _dptr(t, n, what):
-    import torch
-    if not _is_torch(t) or not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
-        raise TypeError("%s must be a contiguous float64 CUDA tensor" % what)
-    if t.numel() != n:
-        raise DraugrError("DimensionMismatch: %s has %d entries, expected %d" % (what, t.numel(), n))
-    return ctypes.c_void_p(t.data_ptr())
+from .core import AMGHierarchy, _dptr, _torch_stream, amg_resetup_, pcg_solve_
 
 
 def finest_nnz(h: AMGHierarchy) -> int:

@@ -170,10 +170,13 @@ class AMGConfig:
     precision: str = "fp64"
     seed: int = 0x5EED
     graphs: bool = True
-    kernel: str = "auto"  # sparse-pass kernel: "auto" (autotuned), "stream", "rowseq", "pipe", "stream16", "sorted"
+    # sparse-pass kernel: "auto" (autotuned), "stream", "rowseq", "pipe", "stream16", "sorted", "stream16pipe",
+    # "sortedpipe"
+    kernel: str = "auto"
     fuse: str = "auto"  # zero-guess step + residual, one-pass down / up sweeps: "auto" (autotuned), "off", "on"
     gpu_setup: bool = True  # interpolation, coarse operators, triple maps and PMIS sweeps of the setup on the GPU
     gpu_setup_min_rows: int = 32768  # smaller levels interpolate on the host
+    gs_symmetric: bool = False  # coloured GS post-sweeps in reverse colour order (a symmetric cycle, for PCG)
 
     def __post_init__(self):
         if str(self.cycle_type).upper() not in ("V", "W"):
@@ -210,6 +213,7 @@ class AMGConfig:
             "gpu_setup": self.gpu_setup,
             "fuse": self.fuse,
             "gpu_setup_min_rows": self.gpu_setup_min_rows,
+            "gs_symmetric": self.gs_symmetric,
         }
         if hasattr(c, "interpolation"):
             d["interpolation"] = c.interpolation.to_json()

@@ -298,6 +298,17 @@ def pcg_solve_(x, b, h: AMGHierarchy, config: Optional[AMGConfig] = None, tol: f
     return x, it, rel
 
 
+def _dptr(t, n, what):
+    """Device address of a contiguous float64 CUDA tensor of n entries; a CPU,
+    non-float64 or strided tensor is refused before any device call."""
+    import torch
+    if not _is_torch(t) or not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
+        raise TypeError("%s must be a contiguous float64 CUDA tensor" % what)
+    if t.numel() != n:
+        raise DraugrError("DimensionMismatch: %s has %d entries, expected %d" % (what, t.numel(), n))
+    return ctypes.c_void_p(t.data_ptr())
+
+
 def amg_resetup_(h: AMGHierarchy, A_new, config: Optional[AMGConfig] = None, partial: bool = True,
                  allow_partial_resetup: bool = True) -> AMGHierarchy:
     """Numeric (partial) or full resetup (src/resetup.jl:20-70).  ``A_new`` may
@@ -307,7 +318,7 @@ def amg_resetup_(h: AMGHierarchy, A_new, config: Optional[AMGConfig] = None, par
     if _is_torch(A_new):
         if not partial:
             raise DraugrError("a full resetup needs the matrix, not only its values")
-        check(L.draugr_hip_resetup_device(h.handle, A_new.numel(), ctypes.c_void_p(A_new.data_ptr()),
+        check(L.draugr_hip_resetup_device(h.handle, A_new.numel(), _dptr(A_new, A_new.numel(), "nzval"),
                                           _torch_stream()), "draugr_hip_resetup_device")
         return h
     A = as_csr(A_new)

@@ -5,7 +5,8 @@
  * argument meaning, return codes and error convention as
  * c_api/draugr_amg.h (implemented there by src/cfunction_interface.jl).
  * A host that links libdraugr.so can link libdraugr_hip.so instead; no Julia
- * runtime is involved, so init_julia/shutdown_julia have no counterpart.
+ * runtime is involved, so init_julia/shutdown_julia keep only their place in
+ * the lifecycle (see their declarations).
  *
  * Part 2 adds entry points the reference does not have: device-pointer
  * (HBM-resident) vectors with an explicit HIP stream, the FP64 PCG driver,
@@ -31,6 +32,16 @@ extern "C" {
 
 /* ── Part 1: reference C API ─────────────────────────────────────────────── */
 
+/* Runtime lifecycle.  Replace c_api/draugr_amg.h:26-36.  There is no runtime
+ * to start: init_julia ignores its arguments and does nothing, and the other
+ * entry points do not need it (a host may still call it first, as the
+ * reference requires).  shutdown_julia releases every hierarchy, batch and
+ * config handle still open (their device memory included), as the end of the
+ * reference's runtime would; retcode is ignored and later calls with those
+ * handles fail with "Invalid handle". */
+void init_julia(int argc, char **argv);
+void shutdown_julia(int retcode);
+
 /* Last error message ("" if none).  Replaces c_api/draugr_amg.h:46. */
 const char *draugr_amg_last_error(void);
 
@@ -39,7 +50,8 @@ const char *draugr_amg_last_error(void);
  * "precision" ("fp64" | "fp32" | "bf16": arithmetic of the V/W cycle),
  * "seed" (setup RNG seed), "graphs" (true: the PCG / mixed-precision preconditioner cycle is
  * captured once into a hipGraph and replayed; false: launched eagerly),
- * "kernel" ("auto" | "stream" | "rowseq" | "pipe" | "stream16" | "sorted"), "fuse" ("auto"
+ * "kernel" ("auto" | "stream" | "rowseq" | "pipe" | "stream16" | "sorted" | "stream16pipe" |
+ * "sortedpipe"), "fuse" ("auto"
  * | "off" | "on": a level's first smoothing step from zero fused with its
  * residual, that step, residual and restriction in one pass, and its
  * prolongation and first post-smoothing step in one pass; Jacobi-type
@@ -137,7 +149,11 @@ int32_t draugr_hip_resetup_device(int32_t handle, int64_t nnz, const double *nzv
  *  op 7 out = A_coarse^-1 in1        op 8 Galerkin values of level+1
  *  op 9 out = inverse diagonal of the smoother (f64; SPAI(0): a_ii / sum a_ij^2)
  *  op 10 out: one Chebyshev smoothing step in place (out holds x), rhs in2
- *  op 11 out: one ILU(0) smoothing step in place (out holds x), rhs in2 */
+ *  op 11 out: one ILU(0) smoothing step in place (out holds x), rhs in2
+ *  op 12 as op 7 with the vector in a global workspace (the form of a coarsest
+ *        level whose vector does not fit in LDS)
+ *  op 13 refactor the coarsest operator column by column over the chip (the
+ *        form used above 256 rows; same factors) */
 int32_t draugr_hip_level_op(int32_t handle, int32_t level, int32_t op, int32_t prec, const void *in1,
                             const void *in2, void *out, void *out2, void *stream);
 

@@ -52,6 +52,7 @@ typedef struct {
     int32_t smoother;              /* 0 jacobi, 1 colored gs, 3 spai0, 5 l1 jacobi, 6 l1 colored gs, 8 chebyshev, 9 ilu0 */
     double omega;
     int32_t w_cycle, pre, post;
+    int32_t gs_sym;                /* coloured GS post-sweeps in reverse colour order (no reference counterpart) */
 } OHier;
 
 /* y = A x, one row at a time (src/csr_matrix.jl:150-159). */
@@ -109,11 +110,13 @@ void o_jacobi_step(int32_t n, const int32_t *rp, const int32_t *cv, const double
 }
 
 /* One coloured Gauss-Seidel sweep in place, colours ascending
- * (src/smoothers.jl:203-241).  Rows of one colour are independent. */
-void o_gs_sweep(int32_t n, const int32_t *rp, const int32_t *cv, const double *nz, double *x, const double *b,
-                const double *d, int32_t ncolors, const int32_t *coff, const int32_t *order) {
-    (void)n;
-    for (int32_t c = 0; c < ncolors; ++c) {
+ * (src/smoothers.jl:203-241), or descending (reverse != 0: the transposed
+ * sweep of the symmetric option, which the reference does not have).  Rows of
+ * one colour are independent. */
+static void o_gs_sweep_dir(const int32_t *rp, const int32_t *cv, const double *nz, double *x, const double *b,
+                           const double *d, int32_t ncolors, const int32_t *coff, const int32_t *order, int reverse) {
+    for (int32_t cc = 0; cc < ncolors; ++cc) {
+        const int32_t c = reverse ? ncolors - 1 - cc : cc;
 #pragma omp parallel for schedule(static)
         for (int32_t t = coff[c]; t < coff[c + 1]; ++t) {
             const int32_t i = order[t];
@@ -122,6 +125,12 @@ void o_gs_sweep(int32_t n, const int32_t *rp, const int32_t *cv, const double *n
             x[i] += d[i] * ri;
         }
     }
+}
+
+void o_gs_sweep(int32_t n, const int32_t *rp, const int32_t *cv, const double *nz, double *x, const double *b,
+                const double *d, int32_t ncolors, const int32_t *coff, const int32_t *order) {
+    (void)n;
+    o_gs_sweep_dir(rp, cv, nz, x, b, d, ncolors, coff, order, 0);
 }
 
 /* bc_J = sum over the transpose map of P[p_nz_idx] * r[fine_row]
@@ -287,7 +296,7 @@ void o_ilu0_smooth(int32_t n, const int32_t *rp, const int32_t *cv, const double
     }
 }
 
-static void o_smooth(const OHier *h, OLevel *L, double *x, const double *b, int32_t steps) {
+static void o_smooth(const OHier *h, OLevel *L, double *x, const double *b, int32_t steps, int post) {
     if (h->smoother == 9) {
         o_ilu0_smooth(L->n, L->a_rp, L->a_cv, L->a_nz, L->ilu_diag, L->ilu_l, L->ilu_u, x, b, L->tmp, steps);
         return;
@@ -300,7 +309,8 @@ static void o_smooth(const OHier *h, OLevel *L, double *x, const double *b, int3
     const int gs = (h->smoother == 1 || h->smoother == 6);
     if (gs) {
         for (int32_t s = 0; s < steps; ++s)
-            o_gs_sweep(L->n, L->a_rp, L->a_cv, L->a_nz, x, b, L->dinv, L->ncolors, L->c_off, L->c_order);
+            o_gs_sweep_dir(L->a_rp, L->a_cv, L->a_nz, x, b, L->dinv, L->ncolors, L->c_off, L->c_order,
+                           post && h->gs_sym);
         return;
     }
     /* ping-pong with a copy back on odd step counts (src/smoothers.jl:103-124) */
@@ -318,7 +328,7 @@ static void o_smooth(const OHier *h, OLevel *L, double *x, const double *b, int3
 /* _vcycle_descend! (src/cycle.jl:63-99). */
 static void o_descend(const OHier *h, int32_t l, double *x, const double *b) {
     OLevel *L = &h->lv[l];
-    o_smooth(h, L, x, b, h->pre);
+    o_smooth(h, L, x, b, h->pre, 0);
     o_residual(L->n, L->a_rp, L->a_cv, L->a_nz, x, b, L->r);
     o_restrict(L->nc, L->t_off, L->t_rows, L->t_pnz, L->p_nz, L->r, L->bc);
     memset(L->xc, 0, (size_t)L->nc * sizeof(double));
@@ -331,7 +341,7 @@ static void o_descend(const OHier *h, int32_t l, double *x, const double *b) {
         memcpy(L->xc, h->coarse_x, (size_t)L->nc * sizeof(double));
     }
     o_prolong(L->n, L->p_rp, L->p_cv, L->p_nz, L->xc, x);
-    o_smooth(h, L, x, b, h->post);
+    o_smooth(h, L, x, b, h->post, 1);
 }
 
 /* amg_cycle! (src/cycle.jl:12-28). */

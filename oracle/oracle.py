@@ -56,6 +56,7 @@ class _OHier(ctypes.Structure):
         ("ac_rp", _i32p), ("ac_cv", _i32p), ("ac_nz", _f64p),
         ("smoother", ctypes.c_int32), ("omega", ctypes.c_double),
         ("w_cycle", ctypes.c_int32), ("pre", ctypes.c_int32), ("post", ctypes.c_int32),
+        ("gs_sym", ctypes.c_int32),
     ]
 
 
@@ -241,7 +242,7 @@ class OracleHierarchy:
     """A fixed hierarchy on which the oracle runs cycles, solves and resetups."""
 
     def __init__(self, levels, coarse_rp, coarse_cv, coarse_nz, smoother="l1_jacobi", omega=2.0 / 3.0,
-                 cycle="v", pre=1, post=1, cheb=None):
+                 cycle="v", pre=1, post=1, cheb=None, gs_symmetric=False):
         self._keep = []
         keep = self._keep.append
         L = len(levels)
@@ -298,6 +299,7 @@ class OracleHierarchy:
         h.omega = omega
         h.w_cycle = 1 if str(cycle).lower() == "w" else 0
         h.pre, h.post = pre, post
+        h.gs_sym = 1 if gs_symmetric else 0
         self._h = h
         self.n = (len(self.levels[0]["A_rp"]) - 1) if L else nco
         lib().o_prepare(ctypes.byref(h))

@@ -40,10 +40,10 @@ def hierarchy_levels(h):
     return [h.level(l) for l in range(h.nlevels)], h.coarsest()
 
 
-def oracle_of(h, orc, smoother, cycle="v", pre=1, post=1, omega=2.0 / 3.0):
+def oracle_of(h, orc, smoother, cycle="v", pre=1, post=1, omega=2.0 / 3.0, gs_symmetric=False):
     levels, c = hierarchy_levels(h)
     return orc.OracleHierarchy(levels, c["rp"], c["cv"], c["nz"], smoother=smoother, omega=omega, cycle=cycle,
-                               pre=pre, post=post)
+                               pre=pre, post=post, gs_symmetric=gs_symmetric)
 
 
 def rel_err(a, b):

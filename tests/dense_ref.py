@@ -60,7 +60,8 @@ def ilu0(A):
 
 
 class DenseAMG:
-    def __init__(self, levels, coarse, smoother, omega=2.0 / 3.0, cycle="v", pre=1, post=1, cheb=None):
+    def __init__(self, levels, coarse, smoother, omega=2.0 / 3.0, cycle="v", pre=1, post=1, cheb=None,
+                 gs_symmetric=False):
         self.A = [dense(l["A_rp"], l["A_cv"], l["A_nz"]) for l in levels]
         self.P = [dense(l["P_rp"], l["P_cv"], l["P_nz"], len(l["Pt_off"]) - 1) for l in levels]
         self.colors = [(l.get("c_off"), l.get("c_order")) for l in levels]
@@ -71,8 +72,9 @@ class DenseAMG:
         self.LU = [ilu0(A) for A in self.A] if smoother == "ilu0" else None
         self.cheb = cheb
         self.omega, self.cycle_type, self.pre, self.post = omega, cycle, pre, post
+        self.gs_symmetric = gs_symmetric
 
-    def smooth(self, l, x, b, steps):
+    def smooth(self, l, x, b, steps, post=False):
         A, D = self.A[l], self.D[l]
         if self.smoother == "chebyshev":
             lo, hi = self.cheb[l]
@@ -99,8 +101,11 @@ class DenseAMG:
             return x
         if "gs" in self.smoother:
             off, order = self.colors[l]
+            colours = list(range(len(off) - 1))
+            if post and self.gs_symmetric:
+                colours.reverse()
             for _ in range(steps):
-                for c in range(len(off) - 1):
+                for c in colours:
                     rows = order[off[c]:off[c + 1]]
                     r = b[rows] - A[rows] @ x
                     x = x.copy()
@@ -121,7 +126,7 @@ class DenseAMG:
         else:
             xc = np.linalg.solve(self.Ac, bc)
         x = x + self.P[l] @ xc
-        return self.smooth(l, x, b, self.post)
+        return self.smooth(l, x, b, self.post, post=True)
 
     def cycle(self, x, b):
         if not self.A:

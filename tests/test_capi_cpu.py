@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "draugr_hip.h")
 def declared_symbols():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(draugr_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(draugr_\w+|init_julia|shutdown_julia)\s*\(", text)))
 
 
 def test_header_symbols_exported(dh):
@@ -104,3 +104,38 @@ def test_invalid_handles_and_index_base(dh):
 
 def test_device_count_without_gpu_is_clean(dh):
     assert dh.device_count() >= 0
+
+
+def test_runtime_lifecycle_entry_points(dh):
+    """init_julia / shutdown_julia (c_api/draugr_amg.h:26-36) are exported:
+    init_julia is a no-op; shutdown_julia releases every open handle, after
+    which the old handles are invalid.  Run in a child process so that the
+    shutdown does not touch this process's handles."""
+    import subprocess
+    import sys
+    L = ctypes.CDLL(dh.LIB_PATH)
+    assert hasattr(L, "init_julia") and hasattr(L, "shutdown_julia")
+    code = r'''
+import ctypes, sys
+sys.path[:0] = [%r, %r]
+import numpy as np
+import draugr_hip as dh
+L = dh.lib()
+L.init_julia(0, None)
+c = L.draugr_amg_config_from_json(b'{"coarsening": "aggregation", "smoother": "jacobi"}')
+A = dh.problems.poisson2d(10)
+h = L.draugr_hip_setup_host(A.n, A.nnz, A.rowptr.ctypes.data_as(ctypes.c_void_p),
+                            A.colval.ctypes.data_as(ctypes.c_void_p), A.nzval.ctypes.data_as(ctypes.c_void_p), c, 0, 1)
+assert c > 0 and h > 0
+out = (ctypes.c_int64 * 8)()
+assert L.draugr_hip_hierarchy_info(h, out, 8) > 0
+L.shutdown_julia(0)
+assert L.draugr_hip_hierarchy_info(h, out, 8) == -1 and "Invalid" in dh.last_error()
+assert L.draugr_amg_config_free(c) == -1 and "Invalid config handle" in dh.last_error()
+assert L.draugr_amg_free(h) == -1
+c2 = L.draugr_amg_config_from_json(b"{}")
+assert c2 > h  # handles are never reused
+print("lifecycle ok")
+''' % (ROOT, os.path.join(ROOT, "draugr.jl_amd"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "lifecycle ok" in r.stdout, r.stdout + r.stderr

@@ -174,12 +174,12 @@ def test_c5_adjoint_96(dh, orc, adj, torch_cuda):
             xm = dadj.solve_7pt(lm, bt.detach(), h, m, tol=tol_fd, maxiter=2000)
             fd.append(float(torch.dot(xp - xm, xp + xm - 2 * xst)) / 2 / (2 * H_FD))
             ad.append(float(torch.dot(gk, (lp - lm) / (2 * H_FD))))
-            if t < 5:
-                bp, bm = bt.detach() + H_FD * dt, bt.detach() - H_FD * dt
-                xp = dadj.solve_7pt(lk.detach(), bp, h, m, tol=tol_fd, maxiter=2000)
-                xm = dadj.solve_7pt(lk.detach(), bm, h, m, tol=tol_fd, maxiter=2000)
-                fdb.append(float(torch.dot(xp - xm, xp + xm - 2 * xst)) / 2 / (2 * H_FD))
-                adb.append(float(torch.dot(gb, (bp - bm) / (2 * H_FD))))
+            # dL/db along the same 20 directions
+            bp, bm = bt.detach() + H_FD * dt, bt.detach() - H_FD * dt
+            xp = dadj.solve_7pt(lk.detach(), bp, h, m, tol=tol_fd, maxiter=2000)
+            xm = dadj.solve_7pt(lk.detach(), bm, h, m, tol=tol_fd, maxiter=2000)
+            fdb.append(float(torch.dot(xp - xm, xp + xm - 2 * xst)) / 2 / (2 * H_FD))
+            adb.append(float(torch.dot(gb, (bp - bm) / (2 * H_FD))))
     e_k, e_b = rel_err(fd, ad), rel_err(fdb, adb)
     print("C5 96^3: FD vs adjoint rel (log k) %.3e, (b) %.3e" % (e_k, e_b), flush=True)
     assert e_k < 1e-6 and e_b < 1e-6

@@ -193,3 +193,85 @@ def test_batch_errors(dh, torch_cuda):
     with pytest.raises(dh.DraugrError, match="Invalid batch handle"):
         batch.handle = 99999
         batch.info()
+
+
+# The C4 workload's shape at sizes the oracle finishes in seconds: 8 uneven
+# systems (edges 9 to 30), 16 and 8 right-hand sides.  K = 16 runs the
+# smoothing passes with four threads per row and four right-hand sides per
+# thread; K = 8 with two threads per row.
+C4_EDGES = [24, 9, 30, 13, 26, 16, 11, 28]
+
+
+@pytest.mark.parametrize("K", [16, 8])
+def test_c4_shape_cycle_equals_single_cycles(dh, torch_cuda, K):
+    torch = torch_cuda
+    mats, cfg, batch, singles = _setup(dh, C4_EDGES, K, max_coarse_size=50)
+    assert len({h.nlevels for h in singles}) > 1
+    for prec in ("fp64", "fp32"):
+        c = dataclasses.replace(cfg, precision=prec)
+        bs = _rhs(dh, batch, SEED + 600 + K)
+        b = batch.pack(bs)
+        x = torch.zeros_like(b)
+        batch.cycle_(x, b, c)
+        torch.cuda.synchronize()
+        got = batch.unpack(x)
+        print("C4-shape batched cycle", prec, "K", K, flush=True)
+        for s, h in enumerate(singles):
+            for j in range(K):
+                bj = torch.tensor(np.ascontiguousarray(bs[s][:, j]), device="cuda")
+                xj = torch.zeros_like(bj)
+                dh.amg_cycle_(xj, bj, h, c, zero_guess=True)
+                torch.cuda.synchronize()
+                assert np.array_equal(got[s][:, j], xj.cpu().numpy()), (prec, K, s, j)
+
+
+@pytest.mark.parametrize("K", [16, 8])
+def test_c4_shape_pcg_fp64_matches_oracle(dh, orc, torch_cuda, K):
+    torch = torch_cuda
+    mats, cfg, batch, singles = _setup(dh, C4_EDGES, K, max_coarse_size=50)
+    bs = _rhs(dh, batch, SEED + 620 + K)
+    b = batch.pack(bs)
+    x = torch.zeros_like(b)
+    _, its, rel = batch.pcg_solve_(x, b, dataclasses.replace(cfg, precision="fp64"), tol=1e-10, maxiter=300)
+    got = batch.unpack(x)
+    for s, h in enumerate(singles):
+        oh = oracle_of(h, orc, "l1_jacobi")
+        for j in range(K):
+            xo, ito, _ = oh.pcg(np.zeros(batch.sizes[s]), bs[s][:, j], tol=1e-10, maxiter=300)
+            assert abs(int(its[s, j]) - ito) <= 1, (s, j, its[s, j], ito)
+            assert rel_err(got[s][:, j], xo) < 1e-10, (s, j)
+            assert rel[s, j] < 1e-10
+    # the FP32-cycle batched PCG: every pair's true residual at the tolerance
+    x32 = torch.zeros_like(b)
+    _, its32, rel32 = batch.pcg_solve_(x32, b, dataclasses.replace(cfg, precision="fp32"), tol=1e-8)
+    g32 = batch.unpack(x32)
+    for s, A in enumerate(mats):
+        for j in range(K):
+            r = bs[s][:, j] - A.matvec(g32[s][:, j])
+            assert np.linalg.norm(r) / np.linalg.norm(bs[s][:, j]) < 1.5e-8, (s, j)
+
+
+def test_batch_setup_with_gpu_setup_hooks_concurrent(dh, torch_cuda):
+    """gpu_setup_min_rows = 0: every level's interpolation, PMIS sweeps and
+    Galerkin product run through the GPU setup hooks, from the batch setup's
+    concurrent host threads; each system's hierarchy is still the one a lone
+    setup builds (batched cycle = single cycles, bit for bit)."""
+    torch = torch_cuda
+    mats, cfg, batch, singles = _setup(dh, C4_EDGES, 4, max_coarse_size=50, gpu_setup_min_rows=0)
+    c = dataclasses.replace(cfg, precision="fp64")
+    bs = _rhs(dh, batch, SEED + 640)
+    b = batch.pack(bs)
+    x = torch.zeros_like(b)
+    batch.cycle_(x, b, c)
+    torch.cuda.synchronize()
+    got = batch.unpack(x)
+    # the lone setups of the host-only configuration give the same hierarchies
+    host_cfg = dataclasses.replace(cfg, gpu_setup=False)
+    for s, A in enumerate(mats):
+        h = dh.amg_setup(A, host_cfg)
+        for j in range(4):
+            bj = torch.tensor(np.ascontiguousarray(bs[s][:, j]), device="cuda")
+            xj = torch.zeros_like(bj)
+            dh.amg_cycle_(xj, bj, h, dataclasses.replace(host_cfg, precision="fp64"), zero_guess=True)
+            torch.cuda.synchronize()
+            assert np.array_equal(got[s][:, j], xj.cpu().numpy()), (s, j)

@@ -1,0 +1,150 @@
+"""GPU parity of the branches that the configuration problems do not reach:
+
+* the coarsest direct solve (src/cycle.jl:88-93, src/setup.jl:177-209) for
+  coarsest sizes above one wavefront: factors staged in LDS (n_c <= 87), the
+  vector alone in LDS (n_c of a few hundred), the vector in a global
+  workspace (any size), and the column-by-column factorisation used above 256
+  rows -- each bit for bit against the oracle's LAPACK-order LU solve and,
+  through a whole V-cycle, the oracle's cycle;
+* rows longer than a row block's LDS stage (hub rows), summed chunk by chunk
+  in row order: level kernels bit for bit against the oracle;
+* the stationary solve's non-convergence semantics (src/cycle.jl:135-159):
+  a non-finite residual never passes the test, the iteration count returned
+  is maxiter and no error is raised.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_of, rel_err
+from test_gpu_parity import hub_matrix, same
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261202
+
+
+def _two_level(dh, m):
+    """2-D Poisson m x m, aggregation, one AMG level: the coarsest operator is
+    the aggregates' Galerkin product (69 rows at m = 20, 85 at 22, 288 at 40,
+    655 at 60)."""
+    A = dh.problems.poisson2d(m)
+    cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType(), max_levels=2)
+    return A, cfg
+
+
+@pytest.mark.parametrize("m,nc", [(20, 69), (22, 85), (40, 288), (60, 655)])
+def test_coarse_solve_above_one_wavefront(dh, orc, torch_cuda, m, nc):
+    torch = torch_cuda
+    A, cfg = _two_level(dh, m)
+    h = dh.amg_setup(A, cfg)
+    c = h.coarsest()
+    assert c["n"] == nc and h.nlevels == 1
+    print("coarse solve n_c =", nc, flush=True)
+    rng = np.random.default_rng(SEED + nc)
+    bc = rng.standard_normal(nc)
+    ref = orc.lu_solve_dense(c["dense"], bc)
+    assert rel_err(ref, np.linalg.solve(c["dense"], bc)) < 1e-12
+    outs = {}
+    for op in (7, 12):  # 12: the vector in a global workspace
+        out = torch.zeros(nc, dtype=torch.float64, device="cuda")
+        h.level_op(0, op, "fp64", torch.tensor(bc, device="cuda"), None, out)
+        outs[op] = out.cpu().numpy()
+        same(outs[op], ref)
+        o32 = torch.zeros(nc, dtype=torch.float32, device="cuda")
+        h.level_op(0, op, "fp32", torch.tensor(bc, dtype=torch.float32, device="cuda"), None, o32)
+        assert rel_err(o32.cpu().numpy(), ref) < 1e-4
+    # the column-by-column factorisation gives the same factors, bit for bit
+    f1 = h.device_values(0, 2, nc * nc)
+    h.level_op(0, 13, "fp64")
+    f2 = h.device_values(0, 2, nc * nc)
+    assert np.array_equal(f1, f2)
+    out = torch.zeros(nc, dtype=torch.float64, device="cuda")
+    h.level_op(0, 7, "fp64", torch.tensor(bc, device="cuda"), None, out)
+    same(out.cpu().numpy(), ref)
+    # a whole V-cycle through this coarsest level, and a converged solve
+    oh = oracle_of(h, orc, "jacobi")
+    b = dh.problems.splitmix_normal(SEED + 970, A.n)
+    x0 = dh.problems.splitmix_normal(SEED + 971, A.n)
+    x = torch.tensor(x0, device="cuda")
+    dh.amg_cycle_(x, torch.tensor(b, device="cuda"), h, cfg)
+    same(x.cpu().numpy(), oh.cycle(x0, b))
+    z = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+    _, it, rel = dh.pcg_solve_(z, torch.tensor(b, device="cuda"), h, cfg, tol=1e-10, maxiter=200)
+    xo, ito, _ = oh.pcg(np.zeros(A.n), b, 1e-10, 200)
+    assert abs(it - ito) <= 1 and rel_err(z.cpu().numpy(), xo) < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_long_row_level_kernels(dh, orc, torch_cuda, prec):
+    """A 3,000-nonzero hub row on the finest level (longer than the 2,048
+    products a row block stages): SpMV, residual, Jacobi and the fused
+    zero-guess step are the oracle's bit for bit in FP64 (1e-5 in FP32),
+    whichever sparse-pass kernel is forced."""
+    import dataclasses
+    torch = torch_cuda
+    B = hub_matrix(dh.problems, 64, 3000)
+    A = dh.as_csr(B)
+    assert np.diff(A.rowptr).max() == 3000
+    base = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType())
+    tdt = torch.float64 if prec == "fp64" else torch.float32
+    rng = np.random.default_rng(SEED + 975)
+    x, b = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    rp, cv, nz = A.rowptr, A.colval, A.nzval
+    for kernel in ("stream", "stream16", "sorted", "pipe", "rowseq", "stream16pipe", "sortedpipe"):
+        cfg = dataclasses.replace(base, kernel=kernel)
+        h = dh.amg_setup(A, cfg)
+        oh = oracle_of(h, orc, "jacobi")
+        d = oh.dinv(0)
+        dev = lambda a: torch.tensor(a, dtype=tdt, device="cuda")  # noqa: E731
+        out, out2 = torch.zeros(A.n, dtype=tdt, device="cuda"), torch.zeros(A.n, dtype=tdt, device="cuda")
+        print("long row", kernel, prec, flush=True)
+        h.level_op(0, 0, prec, dev(x), None, out)
+        same(out.cpu().numpy(), orc.spmv(rp, cv, nz, x), prec)
+        h.level_op(0, 1, prec, dev(x), dev(b), out)
+        same(out.cpu().numpy(), orc.residual(rp, cv, nz, x, b), prec)
+        h.level_op(0, 2, prec, dev(x), dev(b), out)
+        same(out.cpu().numpy(), orc.jacobi_step(rp, cv, nz, x, b, d, cfg.jacobi_omega), prec)
+        h.level_op(0, 3, prec, None, dev(b), out, out2)
+        x1 = orc.jacobi_step(rp, cv, nz, np.zeros(A.n), b, d, cfg.jacobi_omega)
+        same(out.cpu().numpy(), x1, prec)
+        same(out2.cpu().numpy(), orc.residual(rp, cv, nz, x1, b), prec)
+
+
+def test_stationary_solve_nonfinite_returns_maxiter(dh, torch_cuda):
+    """A right-hand side with a NaN: ||b|| is NaN, no residual passes
+    rel < tol, and amg_solve! returns (x, maxiter) without an error."""
+    torch = torch_cuda
+    P = dh.problems
+    A = P.heterogeneous3d(10, P.log_permeability(10, 1.0, SEED + 980))
+    cfg = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())
+    h = dh.amg_setup(A, cfg)
+    b = P.splitmix_normal(SEED + 981, A.n)
+    b[17] = np.nan
+    x = np.zeros(A.n)
+    _, it = dh.amg_solve_(x, b, h, cfg, tol=1e-8, maxiter=4)
+    assert it == 4
+    xd = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+    _, it = dh.amg_solve_(xd, torch.tensor(b, device="cuda"), h, cfg, tol=1e-8, maxiter=3)
+    assert it == 3
+    # the same hierarchy still solves a finite system afterwards
+    b[17] = 0.0
+    x = np.zeros(A.n)
+    _, it = dh.amg_solve_(x, b, h, cfg, tol=1e-8, maxiter=100)
+    assert it < 100 and np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b) < 1e-8
+
+
+def test_resetup_rejects_wrong_device_values(dh, torch_cuda):
+    """amg_resetup_ with a tensor of values checks dtype, device and layout in
+    Python before any device call."""
+    torch = torch_cuda
+    A = dh.problems.poisson2d(12)
+    cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType())
+    h = dh.amg_setup(A, cfg)
+    nz = torch.tensor(A.nzval, device="cuda")
+    with pytest.raises(TypeError):
+        dh.amg_resetup_(h, nz.float(), cfg)
+    with pytest.raises(TypeError):
+        dh.amg_resetup_(h, nz.cpu(), cfg)
+    with pytest.raises(TypeError):
+        dh.amg_resetup_(h, torch.stack([nz, nz], 1)[:, 0], cfg)  # strided view
+    dh.amg_resetup_(h, nz, cfg)  # the valid form still works

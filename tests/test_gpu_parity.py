@@ -53,9 +53,31 @@ def _case(dh, name):
         A = dh.as_csr(B)
         cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType())
         sm = "jacobi"
+    elif name == "hub2d":
+        # 2-D Poisson (4,096 rows) plus one hub row coupled to 2,999 others: the
+        # hub row (3,000 nonzeros) is longer than a row block's LDS stage, so
+        # the stream kernels sum it chunk by chunk
+        A = dh.as_csr(hub_matrix(P, 64, 3000))
+        cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType())
+        sm = "jacobi"
     else:
         raise KeyError(name)
     return A, cfg, sm
+
+
+def hub_matrix(P, m, hub):
+    """2-D Poisson m x m with row/column 0 coupled to rows 1..hub-1 by -0.01
+    (diagonals raised to keep the matrix SPD and diagonally dominant)."""
+    import scipy.sparse as sp
+    B = P.poisson2d(m).to_scipy().tolil()
+    for j in range(1, hub):
+        B[0, j] = B[0, j] - 0.01
+        B[j, 0] = B[j, 0] - 0.01
+        B[j, j] = B[j, j] + 0.01
+    B[0, 0] = B[0, 0] + 0.01 * (hub - 1)
+    B = B.tocsr()
+    B.sort_indices()
+    return B
 
 
 CASES = ["agg2d", "hypre3d", "gs2d", "l1gs3d"]
@@ -201,11 +223,29 @@ def test_stationary_solve(dh, orc, name):
     assert np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b) < 1e-8 or it == 300
 
 
-@pytest.mark.parametrize("name", ["agg2d", "hypre3d", "l1gs3d"])
+@pytest.mark.parametrize("name", ["agg2d", "hypre3d", "l1gs3d", "gs2d"])
 def test_pcg_fp64(dh, orc, torch_cuda, name):
+    """FP64 PCG vs the oracle's.  Coloured GS cycles are symmetric only with
+    the reverse-colour post-sweep (gs_symmetric): without it PCG refuses the
+    preconditioner; with it the cycle is the oracle's bit for bit and PCG
+    matches the oracle's PCG."""
+    import dataclasses
     torch = torch_cuda
     A, cfg, sm, h, oh = _built(dh, orc, name)
     b = dh.problems.splitmix_normal(SEED + 5, A.n)
+    if "gs" in sm:
+        xd = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+        with pytest.raises(dh.DraugrError, match="symmetric preconditioner"):
+            dh.pcg_solve_(xd, torch.tensor(b, device="cuda"), h, cfg, tol=1e-8, maxiter=500)
+        cfg = dataclasses.replace(cfg, gs_symmetric=True)
+        oh = oracle_of(h, orc, sm, gs_symmetric=True)
+        z = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+        dh.amg_cycle_(z, torch.tensor(b, device="cuda"), h, cfg, zero_guess=True)
+        same(z.cpu().numpy(), oh.cycle(np.zeros(A.n), b))
+        for c in (cfg, dataclasses.replace(cfg, pre_smoothing_steps=1, post_smoothing_steps=2)):
+            if c.pre_smoothing_steps != c.post_smoothing_steps:
+                with pytest.raises(dh.DraugrError, match="must equal"):
+                    dh.pcg_solve_(xd, torch.tensor(b, device="cuda"), h, c, tol=1e-8, maxiter=500)
     xd = torch.zeros(A.n, dtype=torch.float64, device="cuda")
     _, it, rel = dh.pcg_solve_(xd, torch.tensor(b, device="cuda"), h, cfg, tol=1e-8, maxiter=500)
     torch.cuda.synchronize()
@@ -332,8 +372,8 @@ def test_preconditioner_adapter_with_scipy_cg(dh, torch_cuda):
     assert info2 == 0
 
 
-@pytest.mark.parametrize("kernel", ["stream", "rowseq", "pipe", "stream16", "sorted"])
-@pytest.mark.parametrize("name", ["hypre3d", "agg2d", "wide2d"])
+@pytest.mark.parametrize("kernel", ["stream", "rowseq", "pipe", "stream16", "sorted", "stream16pipe", "sortedpipe"])
+@pytest.mark.parametrize("name", ["hypre3d", "agg2d", "wide2d", "hub2d"])
 def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
     """Each of the five sparse-pass kernels the autotuner chooses among,
     forced on every level, gives the oracle's FP64 cycle bit for bit, also
@@ -346,7 +386,7 @@ def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
     A, cfg, sm = _case(dh, name)
     c = dataclasses.replace(cfg, kernel=kernel)
     h = dh.amg_setup(A, c)
-    if kernel == "stream16":
+    if kernel in ("stream16", "stream16pipe") and name != "hub2d":
         assert all(h.level_info(l)["has16"] for l in range(h.nlevels))
     oh = oracle_of(h, orc, sm)
     b = dh.problems.splitmix_normal(SEED + 900, A.n)
@@ -391,8 +431,8 @@ def test_one_pass_sweeps_are_bit_exact(dh, orc, torch_cuda, name, steps):
     """A level's first smoothing step from zero + residual + restriction in one
     pass, and its prolongation + first post-smoothing step in one pass
     (config fuse = "on", every level), give the oracle's FP64 cycle bit for
-    bit and the separate passes' FP32 / BF16 cycles and PCG bit for bit,
-    for V- and W-cycles."""
+    bit and the separate passes' FP32 / BF16 cycles (and PCG, where the step
+    counts are equal) bit for bit, for V- and W-cycles."""
     import dataclasses
     torch = torch_cuda
     A, cfg, sm = _case(dh, name)
@@ -411,9 +451,10 @@ def test_one_pass_sweeps_are_bit_exact(dh, orc, torch_cuda, name, steps):
                 z = torch.zeros_like(b)
                 dh.amg_cycle_(z, b, h, cp, zero_guess=True)
                 out.append((x.cpu().numpy(), z.cpu().numpy()))
-            w = torch.zeros_like(b)
-            _, it, rel = dh.pcg_solve_(w, b, h, dataclasses.replace(c, precision=prec), tol=1e-8)
-            out.append((w.cpu().numpy(), np.array([it, rel])))
+            if steps[0] == steps[1]:  # PCG needs the symmetric cycle of equal step counts
+                w = torch.zeros_like(b)
+                _, it, rel = dh.pcg_solve_(w, b, h, dataclasses.replace(c, precision=prec), tol=1e-8)
+                out.append((w.cpu().numpy(), np.array([it, rel])))
         return h, out
 
     h_on, on = run(dataclasses.replace(cfg, fuse="on"))

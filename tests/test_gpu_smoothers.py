@@ -181,14 +181,14 @@ def test_ilu0_cycle_pcg_and_resetup(dh, orc, torch_cuda):
     z = torch.zeros(A.n, dtype=torch.float64, device="cuda")
     dh.amg_cycle_(z, torch.tensor(b, device="cuda"), h, cfg, zero_guess=True)
     assert np.array_equal(z.cpu().numpy(), oh.cycle(np.zeros(A.n), b))
+    # the ILU(0) cycle is not symmetric: PCG refuses it; the stationary solve
+    # (the reference's own iteration) matches the oracle
     x = torch.zeros_like(z)
-    _, it, rel = dh.pcg_solve_(x, torch.tensor(b, device="cuda"), h, cfg, tol=1e-10, maxiter=200)
-    xo, ito, _ = oh.pcg(np.zeros(A.n), b, 1e-10, 200)
+    with pytest.raises(dh.DraugrError, match="symmetric preconditioner"):
+        dh.pcg_solve_(x, torch.tensor(b, device="cuda"), h, cfg, tol=1e-10, maxiter=200)
+    _, it = dh.amg_solve_(x, torch.tensor(b, device="cuda"), h, cfg, tol=1e-10, maxiter=200)
+    xo, ito, _ = oh.solve(np.zeros(A.n), b, 1e-10, 200)
     assert abs(it - ito) <= 1 and rel_err(x.cpu().numpy(), xo) < 1e-10
-    x32 = torch.zeros_like(z)
-    _, it32, rel32 = dh.pcg_solve_(x32, torch.tensor(b, device="cuda"), h, dataclasses.replace(cfg, precision="fp32"),
-                                  tol=1e-8, maxiter=200)
-    assert rel32 < 1e-8
     # resetup refactorizes every level (on the host, as the reference)
     A2 = P.heterogeneous3d(12, P.log_permeability(12, 1.0, SEED + 700) + 0.4 * P.splitmix_normal(SEED + 731, A.n))
     dh.amg_resetup_(h, A2, cfg)

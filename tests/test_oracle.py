@@ -180,6 +180,32 @@ def test_vcycle_jacobi_is_symmetric(dh, orc, agg_case):
     assert abs(u @ Mv - Mu @ v) < 1e-12 * abs(u @ Mv)
 
 
+@pytest.mark.parametrize("smoother", ["colored_gs", "l1_colored_gs"])
+def test_symmetric_gs_cycle(dh, orc, smoother):
+    """The reverse-colour post-sweep (gs_symmetric; no reference counterpart)
+    pinned twice: against the dense restatement, and by the symmetry it is
+    for -- <u, M v> = <M u, v> with it, but not with the reference's
+    same-order post-sweep (the planted asymmetry is detected)."""
+    P = dh.problems
+    A = P.heterogeneous3d(7, P.log_permeability(7, 1.0, SEED + 4))
+    sm = {"colored_gs": dh.ColoredGaussSeidelType(), "l1_colored_gs": dh.L1ColoredGaussSeidelType()}[smoother]
+    cfg = dh.AMGConfig(coarsening=dh.HMISCoarsening(0.25, dh.ExtendedIInterpolation(0.3)), smoother=sm,
+                       max_coarse_size=20)
+    h = _hier(dh, A, cfg)
+    levels = [h.level(l) for l in range(h.nlevels)]
+    rng = np.random.default_rng(SEED + 5)
+    u, v = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    for cycle in ("v", "w"):
+        sym = oracle_of(h, orc, smoother, cycle=cycle, gs_symmetric=True)
+        ref = DenseAMG(levels, h.coarsest(), smoother, cycle=cycle, gs_symmetric=True)
+        assert rel_err(sym.cycle(np.zeros(A.n), u), ref.cycle(np.zeros(A.n), u)) < 1e-12
+        Mu, Mv = sym.cycle(np.zeros(A.n), u), sym.cycle(np.zeros(A.n), v)
+        assert abs(u @ Mv - Mu @ v) < 1e-12 * abs(u @ Mv)
+        fwd = oracle_of(h, orc, smoother, cycle=cycle)
+        Mu, Mv = fwd.cycle(np.zeros(A.n), u), fwd.cycle(np.zeros(A.n), v)
+        assert abs(u @ Mv - Mu @ v) > 1e-6 * abs(u @ Mv)
+
+
 # ── solvers: reference invariants and scipy ──────────────────────────────────
 def test_stationary_solve_reference_thresholds(dh, orc):
     """Relative residual < 1e-8 in < 200 cycles on 2-D Poisson 10x10

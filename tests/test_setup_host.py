@@ -153,8 +153,7 @@ def test_cf_interpolation_invariants(dh, config):
     arow = np.zeros(A.n)
     np.add.at(arow, Arows, A.nzval)
     interior = np.abs(arow) < 1e-12
-    if config != "pmis_direct":
-        assert np.allclose(rowsum[interior], 1.0, atol=1e-12)
+    assert np.allclose(rowsum[interior], 1.0, atol=1e-12)
     # unit rows: a row with a single entry 1.0 is a C point; there are n_coarse of them at least
     unit = (lens == 1) & np.isclose(lv["P_nz"][lv["P_rp"][:-1]], 1.0)
     assert unit.sum() >= lv["n_coarse"]
