@@ -351,7 +351,8 @@ static int32_t device_vector_call(int32_t handle, int32_t n, double* x, const do
         H->dev->begin(st);
         if (kind == 0) H->dev->cycle(x, b, zero_guess != 0, o, st);
         else if (kind == 1) ret = H->dev->solve(x, b, tol, maxiter, o, st, &rr, cfg.verbose);
-        else ret = H->dev->pcg(x, b, tol, maxiter, o, st, &rr, cfg.verbose);
+        else if (kind == 2) ret = H->dev->pcg(x, b, tol, maxiter, o, st, &rr, cfg.verbose);
+        else ret = H->dev->pcg_no_alloc(x, b, tol, maxiter, o, st, &rr, cfg.verbose);
         H->dev->end(st);
         if (relres) *relres = rr;
         return ret;
@@ -371,6 +372,11 @@ int32_t draugr_hip_solve_device(int32_t handle, int32_t n, double* x, const doub
 int32_t draugr_hip_pcg_device(int32_t handle, int32_t n, double* x, const double* b, int32_t config_handle,
                               double tol, int32_t maxiter, void* stream, double* relres) {
     return device_vector_call(handle, n, x, b, config_handle, 2, 0, tol, maxiter, stream, relres);
+}
+
+int32_t draugr_hip_pcg_device_noalloc(int32_t handle, int32_t n, double* x, const double* b, int32_t config_handle,
+                                      double tol, int32_t maxiter, void* stream, double* relres) {
+    return device_vector_call(handle, n, x, b, config_handle, 3, 0, tol, maxiter, stream, relres);
 }
 
 int32_t draugr_amg_free(int32_t handle) {

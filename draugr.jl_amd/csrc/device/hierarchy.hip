@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 
 namespace draugr {
@@ -703,6 +704,7 @@ DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
           smoother_ == Smoother::SPAI0 || smoother_ == Smoother::Chebyshev || smoother_ == Smoother::ILU0))
         throw Error("smoother not available in the HIP backend (supported: jacobi, l1_jacobi, colored_gs, "
                     "l1_colored_gs, spai0, chebyshev, ilu0)");
+    DRAUGR_HIP_CHECK(hipGetDevice(&device_));
     try {
         build(h);
     } catch (...) {
@@ -1492,7 +1494,7 @@ void DeviceHierarchy::cycle_top(TX* x, const TX* b, bool zero, const CycleOpts& 
 // before capture (both synchronize).  The cycle itself contains no memset
 // nodes (buffers are cleared by fill kernels).
 template <typename TV, typename TX>
-void DeviceHierarchy::precondition(TX* x, const TX* b, const CycleOpts& o, hipStream_t st) {
+void DeviceHierarchy::precondition(TX* x, const TX* b, const CycleOpts& o, hipStream_t st, bool capture) {
     if (!tuned_[pidx<TV>()]) tune<TV, TX>();
     if (!o.graphs || nlevels() == 0) {
         cycle_top<TV, TX>(x, b, true, o, st);
@@ -1501,6 +1503,10 @@ void DeviceHierarchy::precondition(TX* x, const TX* b, const CycleOpts& o, hipSt
     const auto key = std::make_tuple(pidx<TV>(), (int)o.w_cycle, o.pre, o.post, (int)o.x1_ready0, (int)o.gs_sym,
                                      (const void*)x, (const void*)b);
     auto it = graphs_.find(key);
+    if (it == graphs_.end() && !capture) {  // no graph yet and none may be made: the same launches, eagerly
+        cycle_top<TV, TX>(x, b, true, o, st);
+        return;
+    }
     if (it == graphs_.end()) {
         if (!cap_) DRAUGR_HIP_CHECK(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
         hipGraph_t g = nullptr;
@@ -1651,11 +1657,19 @@ bool DeviceHierarchy::pcg_sorted(const DevCSR& A, hipStream_t st) {
 // and sets alpha on the device; the x/r update reduces r.r.  The cycle for the
 // next z is enqueued before the host reads r.r, so the GPU never idles on the
 // per-iteration convergence check (the last cycle is speculative).
+//
+// ws (allocation-free entry point): the device scalars and the reduction
+// partials live in caller-provided memory instead of the hierarchy's, and the
+// call allocates nothing: the precision mirrors, the autotuner's choices and
+// the form of the q = A p pass must already exist (a previous call of the
+// allocating PCG with the same precision made them), cycles without a graph
+// captured for them are launched eagerly (same kernels, same values), and the
+// scalars are cleared by a fill kernel.
 int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, const CycleOpts& o,
-                         hipStream_t st, double* relres, int verbose) {
+                         hipStream_t st, double* relres, int verbose, const PcgScratch* ws) {
     const int n = n0_;
-    double* scal = scal_.p;
-    double* part = partial_.p;
+    double* scal = ws ? ws->scal : scal_.p;
+    double* part = ws ? ws->partial : partial_.p;
     double* r = w_r_.p;
     double* q = w_q_.p;
     double* z = w_z_.p;
@@ -1678,6 +1692,18 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
             throw Error("PCG needs a symmetric preconditioner: coloured Gauss-Seidel sweeps the colours in the "
                         "same order before and after (set gs_symmetric for a reverse-order post-sweep)");
     }
+    if (ws) {
+        const int pi = mixed ? (o.prec == Precision::FP32 ? 1 : 2) : 0;
+        const bool mirrors = !mixed || (have32_ && (o.prec != Precision::BF16 || have16_));
+        if (!mirrors || (nlevels() > 0 && !tuned_[pi]) || pcg_sorted_ < 0)
+            throw Error("allocation-free PCG: the hierarchy is not prepared for this precision (call the "
+                        "allocating PCG once with the same config first)");
+        const int64_t need = std::max<int64_t>(kRedBlocks, A.nblk);
+        if (need > ws->partial_cap)
+            throw Error("allocation-free PCG: the finest operator needs " + std::to_string(need) +
+                        " reduction partials; the static scratch holds " + std::to_string(ws->partial_cap));
+        launch_map(S_COUNT_, [=] __device__(int64_t i) { scal[i] = 0.0; }, st);
+    }
     if (mixed) ensure_precision(o.prec);
     float* r32 = w_r32_.p;
     float* z32 = w_z32_.p;
@@ -1698,12 +1724,13 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
     if (relres) *relres = rel;
     if (rel < tol) return 0;
     launch_map(n, [=] __device__(int64_t i) { pa[i] = 0.0; }, st);
+    const bool capture = ws == nullptr;
     auto apply_prec = [&](const CycleOpts& oc) {
         if (mixed) {
-            if (oc.prec == Precision::FP32) precondition<float, float>(z32, r32, oc, st);
-            else precondition<bf16_t, float>(z32, r32, oc, st);
+            if (oc.prec == Precision::FP32) precondition<float, float>(z32, r32, oc, st, capture);
+            else precondition<bf16_t, float>(z32, r32, oc, st, capture);
         } else {
-            precondition<double, double>(z, r, oc, st);
+            precondition<double, double>(z, r, oc, st, capture);
         }
     };
     if (mixed) launch_map(n, [=] __device__(int64_t i) { r32[i] = (float)r[i]; }, st);
@@ -1820,6 +1847,55 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
     }
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
     return result;
+}
+
+// ── Allocation-free PCG: its scalars and reduction partials in a static
+//    device array ──────────────────────────────────────────────────────────
+// scratch[0, kScratchScal) holds the device scalars (S_COUNT_ used; the size
+// keeps the partials 256-byte aligned), scratch[kScratchScal, +kScratchPartials)
+// the per-block partial sums: 513 KiB per GPU, held from the library's load
+// until the process ends.  The finest operator may have at most
+// kScratchPartials row blocks (about 16 M rows).  Host code uses the array only
+// through the device address hipGetSymbolAddress gives for it on each device.
+namespace {
+constexpr int kScratchScal = 64;
+constexpr int64_t kScratchPartials = 65536;
+constexpr int kMaxDevices = 64;
+static_assert(kScratchScal >= S_COUNT_, "scalar slots");
+static __device__ __align__(256) double scratch[kScratchScal + kScratchPartials];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call per device (with that device current) and kept.
+double* static_scratch_current_device() {
+    int dev = 0;
+    DRAUGR_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error("allocation-free PCG: device index out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw Error(std::string("allocation-free PCG: hipGetSymbolAddress(scratch) failed: ") +
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+int DeviceHierarchy::pcg_no_alloc(double* x, const double* b, double tol, int maxiter, const CycleOpts& o,
+                                  hipStream_t st, double* relres, int verbose) {
+    int dev = 0;
+    DRAUGR_HIP_CHECK(hipGetDevice(&dev));
+    if (dev != device_)
+        throw Error("allocation-free PCG: the current HIP device (" + std::to_string(dev) +
+                    ") is not the hierarchy's (" + std::to_string(device_) + ")");
+    double* base = static_scratch_current_device();
+    const PcgScratch ws{base, base + kScratchScal, kScratchPartials};
+    return pcg(x, b, tol, maxiter, o, st, relres, verbose, &ws);
 }
 
 // Partial resetup (src/resetup.jl:27-45): new fine values -> smoother ->

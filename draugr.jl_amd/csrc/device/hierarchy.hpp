@@ -83,6 +83,15 @@ enum Scalar : int {
     S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_RHO, S_COUNT_ = 16
 };
 
+// Device memory for the PCG's scalars (S_COUNT_ doubles) and reduction
+// partials (partial_cap doubles) supplied by the caller instead of the
+// hierarchy's own buffers (the allocation-free entry point).
+struct PcgScratch {
+    double* scal;
+    double* partial;
+    int64_t partial_cap;
+};
+
 struct CycleOpts {
     bool w_cycle = false;
     int pre = 1, post = 1;
@@ -131,7 +140,13 @@ public:
               double* relres, int verbose);
     // FP64 PCG preconditioned by one cycle; returns the iteration count.
     int pcg(double* x, const double* b, double tol, int maxiter, const CycleOpts& o, hipStream_t st,
-            double* relres, int verbose);
+            double* relres, int verbose, const PcgScratch* ws = nullptr);
+    // The same PCG with its scalars and partials in the library's static
+    // device scratch: allocates nothing (see pcg); calls that overlap on one
+    // GPU race on that scratch.
+    int pcg_no_alloc(double* x, const double* b, double tol, int maxiter, const CycleOpts& o, hipStream_t st,
+                     double* relres, int verbose);
+    int device() const { return device_; }
     // Partial numeric resetup from new fine-level values (device pointer, FP64).
     void resetup(const double* nzval_dev, hipStream_t st);
 
@@ -180,7 +195,9 @@ private:
                                                        hipStream_t st);
     // zero-guess cycle x = V(0; b) on buffers that stay fixed for the
     // hierarchy's life: captured once into a hipGraph, then replayed
-    template <typename TV, typename TX> void precondition(TX* x, const TX* b, const CycleOpts& o, hipStream_t st);
+    // capture = false: a cycle with no graph yet is launched eagerly instead
+    template <typename TV, typename TX>
+    void precondition(TX* x, const TX* b, const CycleOpts& o, hipStream_t st, bool capture = true);
     void build(const HostHierarchy& h);
     void release_handles();
     void refresh_level_mirrors(int l, hipStream_t st);
@@ -225,6 +242,7 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t last_ev_ = nullptr;
     hipEvent_t pcg_ev_ = nullptr;  // marks each PCG iteration's r.r (the host waits on it)
+    int device_ = 0;               // the HIP device the hierarchy lives on
     hipStream_t cap_ = nullptr;  // capture stream for the graphs
     std::map<std::tuple<int, int, int, int, int, int, const void*, const void*>, hipGraphExec_t> graphs_;
     hipStream_t last_st_ = nullptr;

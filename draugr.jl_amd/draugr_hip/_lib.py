@@ -41,6 +41,7 @@ SIGNATURES = {
     "draugr_hip_solve_device": (_i32, [_i32, _i32, _vp, _vp, _i32, _f64, _i32, _vp, _f64p]),
     "draugr_hip_pcg_device": (_i32, [_i32, _i32, _vp, _vp, _i32, _f64, _i32, _vp, _f64p]),
     "draugr_hip_pcg": (_i32, [_i32, _i32, _vp, _vp, _i32, _f64, _i32, _f64p]),
+    "draugr_hip_pcg_device_noalloc": (_i32, [_i32, _i32, _vp, _vp, _i32, _f64, _i32, _vp, _f64p]),
     "draugr_hip_resetup_device": (_i32, [_i32, _i64, _vp, _vp]),
     "draugr_hip_level_op": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     "draugr_hip_export_device_values": (_i32, [_i32, _i32, _i32, _vp]),

@@ -136,6 +136,23 @@ int32_t draugr_hip_pcg_device(int32_t handle, int32_t n, double *x, const double
 int32_t draugr_hip_pcg(int32_t handle, int32_t n, double *x, const double *b, int32_t config_handle,
                        double tol, int32_t maxiter, double *relres);
 
+/* draugr_hip_pcg_device without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory during a solve
+ * (tight serving loops, code that must not call hipMalloc).  Same arguments,
+ * kernels and results (bit for bit) as draugr_hip_pcg_device.  The PCG's
+ * device scalars and reduction partials live in a static device array of the
+ * library (513 KiB per GPU, kept until the process ends), so the finest
+ * operator may have at most 65536 row blocks (about 16 M rows); larger
+ * hierarchies get -1 with a message.  The hierarchy must already be prepared
+ * for the config's precision by one call of draugr_hip_pcg_device (which
+ * makes the precision mirrors, the kernel choices and the preconditioner
+ * graph); otherwise -1.  The current HIP device must be the hierarchy's.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (different hierarchies or streams) race and must be serialised by the
+ * caller. */
+int32_t draugr_hip_pcg_device_noalloc(int32_t handle, int32_t n, double *x, const double *b, int32_t config_handle,
+                                      double tol, int32_t maxiter, void *stream, double *relres);
+
 /* Partial resetup from new fine-level values already in HBM (same pattern). */
 int32_t draugr_hip_resetup_device(int32_t handle, int64_t nnz, const double *nzval, void *stream);
 
