@@ -300,6 +300,21 @@ def workload_batched48(args, torch, dh):
                     worst = max(worst, float(np.linalg.norm(r) / np.linalg.norm(bs[s_][:, j])))
             out[prec]["true_relres_worst_of_sample"] = worst
     out["speedup_fp32"] = out["fp64"]["s_per_batched_solve"] / out["fp32"]["s_per_batched_solve"]
+    log(batch.tune_report())
+    # dominant pass: the K-wide Jacobi step on packed level 0 (all 48 systems),
+    # FP32, timed back to back with events on the stream it runs on
+    li = batch.level_info(0)
+    n0, nnz0 = li["n_smooth"], li["nnz"]
+    xa = torch.randn(li["n_vec"] * K, dtype=torch.float32, device="cuda")
+    ba = torch.randn(li["n_vec"] * K, dtype=torch.float32, device="cuda")
+    oa = torch.empty_like(xa)
+    t_k = min(event_time(torch, lambda: batch.jacobi_pass(0, "fp32", xa, ba, oa), 20) for _ in range(3))
+    kbytes = nnz0 * (4 + 4) + (n0 + 1) * 4 + n0 * 4 + 3 * K * n0 * 4  # A, d, then x, b, out (K wide)
+    out["roofline"] = {"bound": "hbm", "achieved": kbytes / t_k / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": kbytes / t_k / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                       "kernel": "K-wide l1-Jacobi step on packed level 0 (%d rows x %d rhs, %d nnz), FP32"
+                                 % (n0, K, nnz0),
+                       "bytes_per_launch": kbytes, "avg_launch_us": t_k * 1e6}
     print(json.dumps(out))
 
 

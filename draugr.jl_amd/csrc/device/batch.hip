@@ -9,6 +9,7 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cmath>
 #include <numeric>
 #include <string>
@@ -133,18 +134,66 @@ __global__ __launch_bounds__(kBlock) void k_kpass4(int n, int ks, const int* __r
     for (int u = 0; u < 4; ++u) epi.fin(i, j0 + u, a[u]);
 }
 
-template <typename TV, typename TX, class XF, class EPI>
-void kpass(int n, int ks, const int* rp, const int* cv, const TV* av, XF xf, EPI epi, hipStream_t st) {
-    if (n <= 0) return;
-    if (ks >= 2) {
-        const int rows = kBlock >> (ks - 2);
-        const int nb = (n + rows - 1) / rows;
-        hipLaunchKernelGGL((k_kpass4<TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, n, ks, rp, cv, av, xf, epi);
-        return;
+// K >= 8: one thread per (row, LR = 8 or 16 consecutive right-hand sides):
+// fewer threads share (and re-load) a row's column and value, each gathering
+// LR / 4 vector loads per product; every (row, rhs) still summed on its own
+// in row order, so all mappings give the same values.
+template <int LR, typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_kpassR(int n, int ks, const int* __restrict__ rp,
+                                                   const int* __restrict__ cv, const TV* __restrict__ av, XF xf,
+                                                   EPI epi) {
+    constexpr int lg = LR == 8 ? 3 : 4;
+    const int q = ks - lg;  // log2(threads per row)
+    const int b = gridDim.x >= 64 ? xcd_block() : (int)blockIdx.x;
+    const int i = (b << (8 - q)) + ((int)threadIdx.x >> q);
+    const int j0 = ((int)threadIdx.x & ((1 << q) - 1)) << lg;
+    if (i >= n) return;
+    TX a[LR];
+#pragma unroll
+    for (int u = 0; u < LR; ++u) a[u] = epi.init(i, j0 + u);
+    const int e = rp[i + 1];
+#pragma unroll 2
+    for (int k = rp[i]; k < e; ++k) {
+        const TX v = ld<TX, TV>(av, k);
+        const int c = ldi(cv, k);
+#pragma unroll
+        for (int h = 0; h < LR / 4; ++h) {
+            TX g[4];
+            xf.get4(c, j0 + 4 * h, g);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[4 * h + u] = epi.acc(a[4 * h + u], v * g[u]);
+        }
     }
-    const int rows = kBlock >> ks;
+#pragma unroll
+    for (int u = 0; u < LR; ++u) epi.fin(i, j0 + u, a[u]);
+}
+
+// Right-hand sides per thread of a K-wide pass: rpt in {1, 4, 8, 16}, at
+// most K (the per-level choice of the batch's autotuner; every choice gives
+// the same values).
+inline int rpt_fit(int rpt, int K) {
+    int r = std::min(rpt, K);
+    if (r >= 16) return 16;
+    if (r >= 8) return 8;
+    if (r >= 4) return 4;
+    return 1;
+}
+
+template <typename TV, typename TX, class XF, class EPI>
+void kpass(int n, int ks, const int* rp, const int* cv, const TV* av, XF xf, EPI epi, hipStream_t st, int rpt = 4) {
+    if (n <= 0) return;
+    const int r = rpt_fit(rpt, 1 << ks);
+    const int lr = r == 16 ? 4 : (r == 8 ? 3 : (r == 4 ? 2 : 0));
+    const int rows = kBlock >> (ks - lr);
     const int nb = (n + rows - 1) / rows;
-    hipLaunchKernelGGL((k_kpass<TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, n, ks, rp, cv, av, xf, epi);
+    if (r == 16)
+        hipLaunchKernelGGL((k_kpassR<16, TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, n, ks, rp, cv, av, xf, epi);
+    else if (r == 8)
+        hipLaunchKernelGGL((k_kpassR<8, TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, n, ks, rp, cv, av, xf, epi);
+    else if (r == 4)
+        hipLaunchKernelGGL((k_kpass4<TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, n, ks, rp, cv, av, xf, epi);
+    else
+        hipLaunchKernelGGL((k_kpass<TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, n, ks, rp, cv, av, xf, epi);
 }
 
 __device__ __forceinline__ double bcast(double v, int lane) {
@@ -493,6 +542,7 @@ BatchedHierarchy::BatchedHierarchy(const std::vector<const HostHierarchy*>& hs, 
     if (!(cfg.smoother == Smoother::Jacobi || cfg.smoother == Smoother::L1Jacobi))
         throw Error("the batched path supports the jacobi and l1_jacobi smoothers");
     K_ = nrhs;
+    forced_rpt_ = cfg.batch_rpt;
     while ((1 << ks_) < K_) ++ks_;
     l1_ = cfg.smoother == Smoother::L1Jacobi;
     omega_ = cfg.jacobi_omega;
@@ -699,6 +749,7 @@ void BatchedHierarchy::ensure_vectors() {
 template <typename TX>
 void BatchedHierarchy::cycle_t(TX* f0, const TX* b0, hipStream_t st) {
     const int ks = ks_;
+    const int pi = sizeof(TX) == 8 ? 0 : 1;
     const TX w = (TX)omega_;
     auto xb = [](PLevel& D) -> TX* { if constexpr (sizeof(TX) == 8) return D.x64.p; else return D.x32.p; };
     auto rb = [](PLevel& D) -> TX* { if constexpr (sizeof(TX) == 8) return D.r64.p; else return D.r32.p; };
@@ -721,10 +772,11 @@ void BatchedHierarchy::cycle_t(TX* f0, const TX* b0, hipStream_t st) {
         PLevel& C = L_[l + 1];
         const TX* b = l == 0 ? b0 : bb(D);
         const TX* d = db(D);
+        const int* rs = D.rpt[pi];
         kpass<TX, TX>(D.nA, ks, D.A.rp.p, D.A.cv.p, vals<TX>(D.A.v64, D.A.v32), KXZero<TX>{d, b, w, ks},
-                      KZeroJacobiResid<TX>{b, d, xb(D), rb(D), w, ks}, st);
+                      KZeroJacobiResid<TX>{b, d, xb(D), rb(D), w, ks}, st, rs[0]);
         kpass<TX, TX>(C.nV, ks, D.Pt.rp.p, D.Pt.cv.p, vals<TX>(D.Pt.v64, D.Pt.v32), KX<TX>{rb(D), ks},
-                      KStore<TX>{bb(C), ks}, st);
+                      KStore<TX>{bb(C), ks}, st, rs[1]);
         coarse(C, bb(C), fb(C));
     }
     for (int l = lmax_ - 1; l >= 0; --l) {
@@ -732,16 +784,163 @@ void BatchedHierarchy::cycle_t(TX* f0, const TX* b0, hipStream_t st) {
         PLevel& C = L_[l + 1];
         const TX* b = l == 0 ? b0 : bb(D);
         TX* f = l == 0 ? f0 : fb(D);
+        const int* rs = D.rpt[pi];
         kpass<TX, TX>(D.nA, ks, D.P.rp.p, D.P.cv.p, vals<TX>(D.P.v64, D.P.v32), KX<TX>{fb(C), ks},
-                      KAdd<TX>{xb(D), ks}, st);
+                      KAdd<TX>{xb(D), ks}, st, rs[2]);
         kpass<TX, TX>(D.nA, ks, D.A.rp.p, D.A.cv.p, vals<TX>(D.A.v64, D.A.v32), KX<TX>{xb(D), ks},
-                      KJacobi<TX>{xb(D), b, db(D), f, w, ks}, st);
+                      KJacobi<TX>{xb(D), b, db(D), f, w, ks}, st, rs[3]);
     }
+}
+
+// Autotuner of the K-wide passes: for every level and each of its four passes
+// (zero-guess step + residual, restriction, prolongation, Jacobi step), the
+// right-hand sides per thread (1, 4, 8 or 16, at most K) that time fastest
+// (best of 3 x 3 back-to-back launches on the level's real operators and
+// workspaces, before any graph is captured).  Every choice gives the same
+// values.  A forced choice (config batch_rhs_per_thread) skips the timing.
+template <typename TX>
+void BatchedHierarchy::tune() {
+    const int pi = sizeof(TX) == 8 ? 0 : 1;
+    if (tuned_[pi]) return;
+    if (forced_rpt_ > 0) {
+        for (auto& D : L_)
+            for (int k = 0; k < 4; ++k) D.rpt[pi][k] = rpt_fit(forced_rpt_, K_);
+        tuned_[pi] = true;
+        return;
+    }
+    DRAUGR_HIP_CHECK(hipDeviceSynchronize());
+    hipStream_t st = nullptr;
+    DRAUGR_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    DRAUGR_HIP_CHECK(hipEventCreate(&e0));
+    DRAUGR_HIP_CHECK(hipEventCreate(&e1));
+    auto timeit = [&](auto&& fn) {
+        fn();
+        float best = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {
+            DRAUGR_HIP_CHECK(hipEventRecord(e0, st));
+            for (int k = 0; k < 3; ++k) fn();
+            DRAUGR_HIP_CHECK(hipEventRecord(e1, st));
+            DRAUGR_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            DRAUGR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, ms);
+        }
+        return best;
+    };
+    const int ks = ks_;
+    const TX w = (TX)omega_;
+    auto xb = [](PLevel& D) -> TX* { if constexpr (sizeof(TX) == 8) return D.x64.p; else return D.x32.p; };
+    auto rb = [](PLevel& D) -> TX* { if constexpr (sizeof(TX) == 8) return D.r64.p; else return D.r32.p; };
+    auto bb = [](PLevel& D) -> TX* { if constexpr (sizeof(TX) == 8) return D.b64.p; else return D.b32.p; };
+    auto fb = [](PLevel& D) -> TX* { if constexpr (sizeof(TX) == 8) return D.f64.p; else return D.f32.p; };
+    auto db = [](PLevel& D) -> const TX* { if constexpr (sizeof(TX) == 8) return D.d64.p; else return D.d32.p; };
+    const int cands[4] = {1, 4, 8, 16};
+    for (int l = 0; l < lmax_; ++l) {
+        PLevel& D = L_[l];
+        PLevel& C = L_[l + 1];
+        // finite inputs in the workspaces the passes read (level 0's right-hand
+        // side and f are the level's own r and x here)
+        TX* x = xb(D);
+        TX* r = rb(D);
+        TX* bc = bb(C);
+        TX* fc = fb(C);
+        const int64_t m = (int64_t)D.nV * K_, mc = (int64_t)C.nV * K_;
+        kmap(m, [=] __device__(int64_t i) { x[i] = (TX)1; r[i] = (TX)1; }, st);
+        kmap(mc, [=] __device__(int64_t i) { bc[i] = (TX)0; fc[i] = (TX)1; }, st);
+        const TX* d = db(D);
+        float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+        for (int c : cands) {
+            if (c > K_) continue;
+            const float t[4] = {
+                timeit([&] {
+                    kpass<TX, TX>(D.nA, ks, D.A.rp.p, D.A.cv.p, vals<TX>(D.A.v64, D.A.v32), KXZero<TX>{d, r, w, ks},
+                                  KZeroJacobiResid<TX>{r, d, x, x, w, ks}, st, c);
+                }),
+                timeit([&] {
+                    kpass<TX, TX>(C.nV, ks, D.Pt.rp.p, D.Pt.cv.p, vals<TX>(D.Pt.v64, D.Pt.v32), KX<TX>{r, ks},
+                                  KStore<TX>{bc, ks}, st, c);
+                }),
+                timeit([&] {
+                    kpass<TX, TX>(D.nA, ks, D.P.rp.p, D.P.cv.p, vals<TX>(D.P.v64, D.P.v32), KX<TX>{fc, ks},
+                                  KAdd<TX>{x, ks}, st, c);
+                }),
+                timeit([&] {
+                    kpass<TX, TX>(D.nA, ks, D.A.rp.p, D.A.cv.p, vals<TX>(D.A.v64, D.A.v32), KX<TX>{r, ks},
+                                  KJacobi<TX>{r, r, d, x, w, ks}, st, c);
+                })};
+            for (int k = 0; k < 4; ++k)
+                if (t[k] < best[k]) {
+                    best[k] = t[k];
+                    D.rpt[pi][k] = c;
+                    D.rpt_us[pi][k] = t[k] * 1e3f / 3.0f;
+                }
+        }
+    }
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+    DRAUGR_HIP_CHECK(hipEventDestroy(e0));
+    DRAUGR_HIP_CHECK(hipEventDestroy(e1));
+    DRAUGR_HIP_CHECK(hipStreamDestroy(st));
+    tuned_[pi] = true;
+}
+
+// One K-wide Jacobi / l1-Jacobi step on packed level l with the tuned mapping
+// (a measurement and test hook): out = x + w d (b - A x) for the nA rows of
+// the systems that smooth on level l, vectors n x K row-major.
+void BatchedHierarchy::jacobi_pass(int l, Precision p, const void* x, const void* b, void* out, hipStream_t st) {
+    if (l < 0 || l >= lmax_) throw Error("batched level index out of range");
+    if (p == Precision::BF16) throw Error("the batched path supports precision fp64 and fp32");
+    PLevel& D = L_[l];
+    const int ks = ks_;
+    if (p == Precision::FP64) {
+        ensure_vectors<double>();
+        tune<double>();
+        kpass<double, double>(D.nA, ks, D.A.rp.p, D.A.cv.p, D.A.v64.p, KX<double>{(const double*)x, ks},
+                              KJacobi<double>{(const double*)x, (const double*)b, D.d64.p, (double*)out,
+                                              (double)omega_, ks},
+                              st, D.rpt[0][3]);
+    } else {
+        ensure_vectors<float>();
+        tune<float>();
+        kpass<float, float>(D.nA, ks, D.A.rp.p, D.A.cv.p, D.A.v32.p, KX<float>{(const float*)x, ks},
+                            KJacobi<float>{(const float*)x, (const float*)b, D.d32.p, (float*)out, (float)omega_, ks},
+                            st, D.rpt[1][3]);
+    }
+}
+
+void BatchedHierarchy::level_counts(int l, int64_t* out) const {
+    if (l < 0 || l >= lmax_) throw Error("batched level index out of range");
+    const PLevel& D = L_[l];
+    out[0] = D.nA;
+    out[1] = D.nV;
+    out[2] = D.A.nnz;
+    out[3] = D.P.nnz;
+    out[4] = D.Pt.nrow;
+}
+
+std::string BatchedHierarchy::tune_report() const {
+    std::string out;
+    char buf[200];
+    const char* pn[2] = {"fp64", "fp32"};
+    for (int p = 0; p < 2; ++p) {
+        if (!tuned_[p]) continue;
+        for (int l = 0; l < lmax_; ++l) {
+            const PLevel& D = L_[l];
+            std::snprintf(buf, sizeof buf,
+                          "%s level %d (%d rows x %d rhs): rhs/thread zero-step %d, restrict %d, prolong %d, "
+                          "jacobi %d (%.1f / %.1f / %.1f / %.1f us)\n",
+                          pn[p], l, D.nA, K_, D.rpt[p][0], D.rpt[p][1], D.rpt[p][2], D.rpt[p][3], D.rpt_us[p][0],
+                          D.rpt_us[p][1], D.rpt_us[p][2], D.rpt_us[p][3]);
+            out += buf;
+        }
+    }
+    return out;
 }
 
 template <typename TX>
 void BatchedHierarchy::precondition(TX* f0, const TX* b0, hipStream_t st, bool graphs) {
     ensure_vectors<TX>();
+    tune<TX>();
     if (!graphs) {
         cycle_t<TX>(f0, b0, st);
         return;
@@ -856,8 +1055,13 @@ int BatchedHierarchy::pcg(double* x, const double* b, double tol, int maxiter, c
                 hipLaunchKernelGGL(k_kdot<double>, dim3(nb), dim3(kBlock), 0, s, ks, br0_.p, br1_.p, r, z, part);
             fin(mode, s);
         };
-        if (mixed) ensure_vectors<float>();
-        else ensure_vectors<double>();
+        if (mixed) {  // the autotuner synchronizes: it runs before any capture
+            ensure_vectors<float>();
+            tune<float>();
+        } else {
+            ensure_vectors<double>();
+            tune<double>();
+        }
         apply(st);
         rz(F_RZ0, st);
         // the two graphs of an iteration (the first in two forms: p ping-pongs)

@@ -17,6 +17,7 @@
 
 #include <cstdint>
 #include <map>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -47,6 +48,12 @@ public:
     int sys_levels(int s) const { return sys_levels_[s]; }
     int sys_coarse(int s) const { return sys_coarse_[s]; }
     int64_t bytes_resident() const;
+    // the autotuner's right-hand sides per thread per level and pass, as text
+    std::string tune_report() const;
+    // one K-wide Jacobi step on packed level l, out = x + w d (b - A x) (hook)
+    void jacobi_pass(int l, Precision p, const void* x, const void* b, void* out, hipStream_t st);
+    // out[0..5): smoothing rows nA, vector rows nV, nnz(A), nnz(P), rows of P^T of level l
+    void level_counts(int l, int64_t* out) const;
 
     // x = V(0; b) for every system and right-hand side (packed n x K, FP64).
     void cycle(double* x, const double* b, const CycleOpts& o, hipStream_t st);
@@ -73,13 +80,21 @@ private:
         DBuf<CoarseDesc> cw, cs;
         DBuf<double> x64, r64, b64, f64;  // K-wide cycle vectors (level 0: only x, r)
         DBuf<float> x32, r32, b32, f32;
+        // right-hand sides per thread of the zero-guess step, restriction,
+        // prolongation and Jacobi passes, per precision (the autotuner's
+        // choice) and the time it measured for each
+        int rpt[2][4] = {{4, 4, 4, 4}, {4, 4, 4, 4}};
+        float rpt_us[2][4] = {};
     };
     template <typename TX> void ensure_vectors();
+    template <typename TX> void tune();
     template <typename TX> void cycle_t(TX* f0, const TX* b0, hipStream_t st);
     template <typename TX> void precondition(TX* f0, const TX* b0, hipStream_t st, bool graphs);
     void check_opts(const CycleOpts& o) const;
 
     int K_ = 1, ks_ = 0;
+    int forced_rpt_ = 0;                         // config batch_rhs_per_thread (0: autotuned)
+    bool tuned_[2] = {false, false};
     int lmax_ = 0;
     int64_t N0_ = 0;
     double omega_ = 2.0 / 3.0;

@@ -711,6 +711,46 @@ int32_t draugr_hip_batch_pcg_device(int32_t batch, double* x, const double* b, i
     });
 }
 
+int32_t draugr_hip_batch_tune_report(int32_t batch, char* buf, int32_t buflen) {
+    return guarded([&]() -> int32_t {
+        auto B = get_batch(batch);
+        std::lock_guard<std::mutex> g(B->mu);
+        const std::string s = B->dev->tune_report();
+        if (buf && buflen > 0) {
+            const size_t m = std::min<size_t>(s.size(), (size_t)buflen - 1);
+            std::memcpy(buf, s.data(), m);
+            buf[m] = 0;
+        }
+        return (int32_t)s.size();
+    });
+}
+
+int32_t draugr_hip_batch_level_info(int32_t batch, int32_t level, int64_t* out, int32_t nout) {
+    return guarded([&]() -> int32_t {
+        auto B = get_batch(batch);
+        int64_t v[5];
+        B->dev->level_counts(level, v);
+        const int32_t m = std::min<int32_t>(nout, 5);
+        for (int32_t i = 0; i < m; ++i) out[i] = v[i];
+        return m;
+    });
+}
+
+int32_t draugr_hip_batch_jacobi_pass(int32_t batch, int32_t level, int32_t prec, const void* x, const void* b,
+                                     void* out, void* stream) {
+    return guarded([&]() -> int32_t {
+        auto B = get_batch(batch);
+        if (!x || !b || !out) throw Error("ArgumentError: null vector");
+        if (prec < 0 || prec > 1) throw Error("prec must be 0 (fp64) or 1 (fp32)");
+        std::lock_guard<std::mutex> g(B->mu);
+        hipStream_t st = (hipStream_t)stream;
+        batch_begin(*B, st);
+        B->dev->jacobi_pass(level, prec == 0 ? Precision::FP64 : Precision::FP32, x, b, out, st);
+        batch_end(*B, st);
+        return 0;
+    });
+}
+
 int32_t draugr_hip_batch_free(int32_t batch) {
     return guarded([&]() -> int32_t {
         std::shared_ptr<Batch> B;

@@ -12,6 +12,7 @@
 #include <limits>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 namespace draugr {
 
@@ -442,8 +443,7 @@ template <> const float* svals<float>(const DevCSR& m) { return m.sv32.p; }
 template <> const bf16_t* svals<bf16_t>(const DevCSR& m) { return m.sv16.p; }
 
 // Transfer operators use FP32 values when the cycle stores A in BF16.
-template <typename TV> struct TransferT { typedef TV type; };
-template <> struct TransferT<bf16_t> { typedef float type; };
+template <typename TV> using TransferT = TransferOf<TV>;
 
 template <typename TX> TX* r_of(DevLevel& l);
 template <> double* r_of<double>(DevLevel& l) { return l.r64.p; }
@@ -698,8 +698,8 @@ bool is_gs(Smoother s) { return s == Smoother::ColoredGS || s == Smoother::L1Col
 
 // ─────────────────────────────────────────────────────────────────────────────
 DeviceHierarchy::DeviceHierarchy(const HostHierarchy& h, const Config& cfg)
-    : smoother_(cfg.smoother), omega_(cfg.jacobi_omega), seed_(cfg.seed), forced_kind_(cfg.kernel),
-      forced_fuse_(cfg.fuse) {
+    : forced_tail_(cfg.tail), smoother_(cfg.smoother), omega_(cfg.jacobi_omega), seed_(cfg.seed),
+      forced_kind_(cfg.kernel), forced_fuse_(cfg.fuse) {
     if (!(smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || is_gs(smoother_) ||
           smoother_ == Smoother::SPAI0 || smoother_ == Smoother::Chebyshev || smoother_ == Smoother::ILU0))
         throw Error("smoother not available in the HIP backend (supported: jacobi, l1_jacobi, colored_gs, "
@@ -1109,6 +1109,7 @@ void DeviceHierarchy::launch_down(int l, const TX* b, TX* x1, TX* bc, const TX* 
 template <typename TV>
 bool DeviceHierarchy::zero_step_is_map(int l, int pre) const {
     if (l < 0 || l >= nlevels() || pre < 1) return false;
+    if (l == tail_lt_[pidx<TV>()]) return false;  // the tail forms its own first step
     if (!(smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || smoother_ == Smoother::SPAI0))
         return false;
     return !(pre == 1 && (L_[l].fuse_zero[pidx<TV>()] || L_[l].fuse_down[pidx<TV>()]));
@@ -1191,6 +1192,130 @@ void DeviceHierarchy::coarse_solve(TX* x, const TX* b, hipStream_t st, bool glob
     hipLaunchKernelGGL(k_coarse_lu_solve<TX>, dim3(1), dim3(kBlock), 0, st, n, clu_.p, flag_.p + 1, b, x, 0, cx_.p);
 }
 
+// ── the LDS-resident last level ─────────────────────────────────────────────
+// It qualifies with a Jacobi-type smoother, at least two AMG levels (its
+// parent launches it), a coarsest level of at most 64 rows and its image plus
+// vectors within the workgroup's LDS.
+template <typename TV, typename TX>
+bool DeviceHierarchy::tail_fits(TailLayout* lay) const {
+    typedef typename TransferT<TV>::type TP;
+    const int L = nlevels();
+    if (L < 2 || nc_ < 1 || nc_ > 64) return false;
+    if (!(smoother_ == Smoother::Jacobi || smoother_ == Smoother::L1Jacobi || smoother_ == Smoother::SPAI0))
+        return false;
+    const DevLevel& D = L_[L - 1];
+    if (D.A.nrow >= 65536 || D.A.nnz > (1 << 20) || D.P.nnz > (1 << 20)) return false;
+    const TailLayout y = tail_layout(D.A.nrow, nc_, (int)D.A.nnz, (int)D.P.nnz, (int)sizeof(TV), (int)sizeof(TP),
+                                     (int)sizeof(TX));
+    if (y.lds_bytes > kTailLds) return false;
+    if (lay) *lay = y;
+    return true;
+}
+
+// Pack the last level's image for precision TV: A, P, P^T (16-bit columns),
+// the smoother's inverse diagonal, the coarsest LU factors and permutation,
+// copied on the device from the hierarchy's own arrays (again after every
+// resetup, since values, diagonal and factors change).
+template <typename TV, typename TX>
+void DeviceHierarchy::pack_tail() {
+    typedef typename TransferT<TV>::type TP;
+    const int pi = pidx<TV>();
+    TailLayout y;
+    if (!tail_fits<TV, TX>(&y)) return;
+    const DevLevel& D = L_[nlevels() - 1];
+    hipStream_t st = stream_;
+    tail_img_[pi].alloc(y.img_bytes);
+    tail_lay_[pi] = y;
+    uint8_t* im = tail_img_[pi].p;
+    auto copy_i32 = [&](int off, const int* s, int64_t cnt) {
+        int* o = reinterpret_cast<int*>(im + off);
+        launch_map(cnt, [=] __device__(int64_t i) { o[i] = s[i]; }, st);
+    };
+    auto copy_c16 = [&](int off, const int* s, int64_t cnt) {
+        uint16_t* o = reinterpret_cast<uint16_t*>(im + off);
+        launch_map(cnt, [=] __device__(int64_t i) { o[i] = (uint16_t)s[i]; }, st);
+    };
+    auto copy_v = [&](int off, auto* s, int64_t cnt) {
+        typedef typename std::remove_const<typename std::remove_pointer<decltype(s)>::type>::type T;
+        T* o = reinterpret_cast<T*>(im + off);
+        launch_map(cnt, [=] __device__(int64_t i) { o[i] = s[i]; }, st);
+    };
+    copy_i32(y.a_rp, D.A.rp.p, y.n + 1);
+    copy_c16(y.a_cv, D.A.cv.p, y.nnz_a);
+    copy_v(y.a_v, vals<TV>(D.A), y.nnz_a);
+    copy_i32(y.p_rp, D.P.rp.p, y.n + 1);
+    copy_c16(y.p_cv, D.P.cv.p, y.nnz_p);
+    copy_v(y.p_v, vals<TP>(D.P), y.nnz_p);
+    copy_i32(y.t_rp, D.Pt.rp.p, y.nc + 1);
+    copy_c16(y.t_cv, D.Pt.cv.p, y.nnz_p);
+    copy_v(y.t_v, vals<TP>(D.Pt), y.nnz_p);
+    copy_v(y.d, d_of<TX>(D), y.n);
+    copy_v(y.lu, (const double*)clu_.p, (int64_t)y.nc * y.nc);
+    copy_i32(y.perm, flag_.p + 1 + nc_, y.nc);
+}
+
+void DeviceHierarchy::refresh_tails(hipStream_t) {
+    if (tail_img_[0].p) pack_tail<double, double>();
+    if (tail_img_[1].p) pack_tail<float, float>();
+    if (tail_img_[2].p) pack_tail<bf16_t, float>();
+}
+
+template <typename TV, typename TX>
+void DeviceHierarchy::launch_tail(const CycleOpts& o, hipStream_t st) {
+    typedef typename TransferT<TV>::type TP;
+    const int pi = pidx<TV>();
+    DevLevel& parent = L_[nlevels() - 2];
+    TailRun<TX> run{tail_img_[pi].p, bc_of<TX>(parent), xc_of<TX>(parent), (TX)weight(), o.pre, o.post,
+                    tail_lay_[pi]};
+    hipLaunchKernelGGL((k_tail_lds<TV, TP, TX>), dim3(1), dim3(kTailThreads), 0, st, run);
+}
+
+// Whether the LDS-resident last level pays: the zero-guess cycle (the
+// preconditioner's form) with and without it, best of 5 x 4 back-to-back
+// cycles; either way gives the same values.
+template <typename TV, typename TX>
+void DeviceHierarchy::tune_tail() {
+    const int pi = pidx<TV>();
+    tail_lt_[pi] = -1;
+    if (forced_tail_ == 0 || !tail_fits<TV, TX>(nullptr)) return;
+    pack_tail<TV, TX>();
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(stream_));
+    if (forced_tail_ == 1) {
+        tail_lt_[pi] = nlevels() - 1;
+        return;
+    }
+    hipStream_t st = stream_;
+    hipEvent_t e0, e1;
+    DRAUGR_HIP_CHECK(hipEventCreate(&e0));
+    DRAUGR_HIP_CHECK(hipEventCreate(&e1));
+    TX* x = reinterpret_cast<TX*>(w_q_.p);
+    TX* b = reinterpret_cast<TX*>(w_z_.p);
+    launch_map(n0_, [=] __device__(int64_t i) { b[i] = (TX)1; }, st);
+    CycleOpts o;
+    o.prec = pi == 0 ? Precision::FP64 : (pi == 1 ? Precision::FP32 : Precision::BF16);
+    auto time_cycle = [&]() {
+        cycle_top<TV, TX>(x, b, true, o, st);
+        float best = 1e30f;
+        for (int rep = 0; rep < 5; ++rep) {
+            DRAUGR_HIP_CHECK(hipEventRecord(e0, st));
+            for (int k = 0; k < 4; ++k) cycle_top<TV, TX>(x, b, true, o, st);
+            DRAUGR_HIP_CHECK(hipEventRecord(e1, st));
+            DRAUGR_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            DRAUGR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, ms);
+        }
+        return best * 1e3f / 4.0f;
+    };
+    notail_us_[pi] = time_cycle();
+    tail_lt_[pi] = nlevels() - 1;
+    tail_us_[pi] = time_cycle();
+    if (!(tail_us_[pi] < notail_us_[pi])) tail_lt_[pi] = -1;
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
+    DRAUGR_HIP_CHECK(hipEventDestroy(e0));
+    DRAUGR_HIP_CHECK(hipEventDestroy(e1));
+}
+
 // Autotuner: for every level, time the candidate kernels of each operator
 // pass (CSR-stream with LDS-staged row blocks vs thread-per-row) and the two
 // forms of the zero-guess smoothing step (fused with the residual vs a map
@@ -1209,6 +1334,7 @@ void DeviceHierarchy::tune() {
         }
         for (int l = 0; l < nlevels(); ++l) L_[l].fuse_down[pi] = forced_fuse_ == 1 && down_blocks(l);
         tuned_[pi] = true;
+        tune_tail<TV, TX>();
         return;
     }
     DRAUGR_HIP_CHECK(hipDeviceSynchronize());
@@ -1266,6 +1392,7 @@ void DeviceHierarchy::tune() {
             });
         }
         auto best = [](const float* t) { return (int)(std::min_element(t, t + kOpKinds) - t); };
+        for (int k = 0; k < kOpKinds; ++k) D.tA_us[pi][k] = tA[k] < 1e29f ? tA[k] * 1e3f / 4.0f : -1.0f;
         D.A.kind[pi] = best(tA);
         D.Pt.kind[pi] = best(tPt);
         D.P.kind[pi] = best(tP);
@@ -1310,6 +1437,7 @@ void DeviceHierarchy::tune() {
     DRAUGR_HIP_CHECK(hipEventDestroy(e0));
     DRAUGR_HIP_CHECK(hipEventDestroy(e1));
     tuned_[pi] = true;
+    tune_tail<TV, TX>();
 }
 
 std::string DeviceHierarchy::tune_report() const {
@@ -1326,7 +1454,30 @@ std::string DeviceHierarchy::tune_report() const {
                           D.fuse_down[p] ? "one-pass" : (D.down_state < 0 ? "separate(only)" : "separate"),
                           D.fuse_up[p] ? "one-pass" : "separate");
             out += buf;
+            if (D.A.nnz >= (1 << 20) && forced_kind_ < 0) {  // the big levels: every kind's time for the Jacobi pass
+                std::string tl = std::string(names[p]) + " level " + std::to_string(l) + " Jacobi pass us:";
+                for (int k = 0; k < kOpKinds; ++k) {
+                    if (D.tA_us[p][k] < 0.f) continue;
+                    std::snprintf(buf, sizeof buf, " %s %.1f", kn[k], D.tA_us[p][k]);
+                    tl += buf;
+                }
+                out += tl + "\n";
+            }
         }
+    }
+    for (int p = 0; p < 3; ++p) {
+        if (!tuned_[p]) continue;
+        if (tail_lt_[p] >= 0)
+            std::snprintf(buf, sizeof buf, "%s tail: level %d and the coarse solve in one launch from LDS "
+                          "(zero-guess cycle %.1f us, %.1f us without)\n", names[p], tail_lt_[p], tail_us_[p],
+                          notail_us_[p]);
+        else if (notail_us_[p] > 0.f)
+            std::snprintf(buf, sizeof buf, "%s tail: separate launches (zero-guess cycle %.1f us, %.1f us with "
+                          "the LDS-resident last level)\n", names[p], notail_us_[p], tail_us_[p]);
+        else
+            std::snprintf(buf, sizeof buf, "%s tail: separate launches (the last level does not qualify or the "
+                          "choice is fixed)\n", names[p]);
+        out += buf;
     }
     if (pcg_sorted_ >= 0) {  // the form of the PCG's q = A p pass, once a PCG has run
         const DevCSR& A0 = nlevels() > 0 ? L_[0].A : Ac_;
@@ -1455,7 +1606,9 @@ void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const Cy
         else
             op<TP, TX>(D.Pt, XVec<TX>{r}, EpiStore<TX>{bc}, st);
     }
-    if (l + 1 < nlevels()) {
+    if (l + 1 < nlevels() && l + 1 == tail_lt_[pi] && !o.w_cycle && jac) {
+        launch_tail<TV, TX>(o, st);  // the last level and the coarse solve in one launch
+    } else if (l + 1 < nlevels()) {
         const int reps = o.w_cycle ? 2 : 1;
         for (int rep = 0; rep < reps; ++rep) cycle_level<TV, TX>(l + 1, xc, bc, rep == 0, o, st, rep == 0 && child_x1);
     } else {
@@ -1925,6 +2078,7 @@ void DeviceHierarchy::resetup(const double* nzval, hipStream_t st) {
     }
     refresh_mirrors(Ac_, st);  // the PCG's operator when there are no AMG levels
     coarse_factor(st);
+    refresh_tails(st);
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
     int fl = 0;
     DRAUGR_HIP_CHECK(hipMemcpy(&fl, flag_.p, sizeof(int), hipMemcpyDeviceToHost));

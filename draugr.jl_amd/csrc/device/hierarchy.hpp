@@ -14,8 +14,14 @@
 #include "dbuf.hpp"
 #include "galerkin.hpp"
 #include "kernels.hpp"
+#include "tail.hpp"
 
 namespace draugr {
+
+// Value type of the transfer operators for a value type of A: FP32 when A is
+// stored in BF16, else the same.
+template <typename TV> struct TransferOf { typedef TV type; };
+template <> struct TransferOf<dev::bf16_t> { typedef float type; };
 
 struct DevCSR {
     int32_t nrow = 0, ncol = 0;
@@ -73,6 +79,9 @@ struct DevLevel {
     int ndblk = 0;
     DBuf<int32_t> dblk;       // coarse-row offsets of its blocks
     bool fuse_up[3] = {false, false, false};    // prolongation + first post-smoothing step in one pass (autotuned)
+    // row chunks of A, P and P^T for the one-launch tail (tail_ok: every row
+    // of the three fits one LDS stage and the level is small enough)
+    float tA_us[3][dev::kOpKinds] = {};  // the autotuner's time per kernel kind of the Jacobi pass over A
     // workspaces: residual, ping-pong smoother buffer, coarse x and b
     DBuf<double> r64, t64, xc64, bc64;
     DBuf<float> r32, t32, xc32, bc32;
@@ -202,6 +211,19 @@ private:
     void release_handles();
     void refresh_level_mirrors(int l, hipStream_t st);
     template <typename TV, typename TX> void tune();
+    // the LDS-resident last level (with the coarse solve) in one launch: the
+    // level it covers per cycle precision (-1: none), its packed image and
+    // layout per precision, and the tuner's measurements
+    int tail_lt_[3] = {-1, -1, -1};
+    int forced_tail_ = -1;  // config "tail": -1 tuned, 0 never, 1 whenever the last level qualifies
+    float tail_us_[3] = {0.f, 0.f, 0.f}, notail_us_[3] = {0.f, 0.f, 0.f};
+    DBuf<uint8_t> tail_img_[3];
+    dev::TailLayout tail_lay_[3] = {};
+    template <typename TV, typename TX> bool tail_fits(dev::TailLayout* lay) const;
+    template <typename TV, typename TX> void pack_tail();
+    void refresh_tails(hipStream_t st);
+    template <typename TV, typename TX> void tune_tail();
+    template <typename TV, typename TX> void launch_tail(const CycleOpts& o, hipStream_t st);
     void compute_dinv(int l, hipStream_t st);
     double estimate_rho(int l, hipStream_t st);  // power iteration on D^-1 A (FP64)
     void chebyshev_bounds(hipStream_t st);

@@ -266,6 +266,16 @@ Config config_from_json(const char* json) {
     if (const JVal* v = get("seed")) cfg.seed = (uint64_t)std::llround(to_float(*v));
     if (const JVal* v = get("graphs")) cfg.graphs = to_bool(*v);
     if (const JVal* v = get("gs_symmetric")) cfg.gs_symmetric = to_bool(*v);
+    if (const JVal* v = get("tail")) {
+        static const std::map<std::string, int> kTail = {{"auto", -1}, {"off", 0}, {"on", 1}};
+        cfg.tail = lookup(kTail, to_string(*v), "tail");
+    }
+    if (const JVal* v = get("batch_rhs_per_thread")) {
+        cfg.batch_rpt = to_int(*v);
+        if (!(cfg.batch_rpt == 0 || cfg.batch_rpt == 1 || cfg.batch_rpt == 4 || cfg.batch_rpt == 8 ||
+              cfg.batch_rpt == 16))
+            throw Error("batch_rhs_per_thread must be 0 (autotuned), 1, 4, 8 or 16");
+    }
     if (const JVal* v = get("gpu_setup")) cfg.gpu_setup = to_bool(*v);
     if (const JVal* v = get("fuse")) {
         static const std::map<std::string, int> kFuse = {{"auto", -1}, {"off", 0}, {"on", 1}};

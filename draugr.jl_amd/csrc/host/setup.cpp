@@ -147,9 +147,12 @@ std::vector<HostHierarchy> amg_setup_host_many(const std::vector<CSR>& As, const
     const int nsys = (int)As.size();
     std::vector<HostHierarchy> out(nsys);
     if (nsys == 0) return out;
-    const int procs = std::max(1, omp_get_num_procs());
-    const int workers = std::min(nsys, std::max(1, std::min(8, procs / 2)));
-    const int per_worker = std::max(1, procs / workers);
+    // the thread budget is the calling thread's (OMP_NUM_THREADS or the caller's
+    // omp_set_num_threads), shared out: up to 8 systems at once, each worker
+    // with its share of the threads for its own parallel regions
+    const int budget = std::max(1, omp_get_max_threads());
+    const int workers = std::min(nsys, std::max(1, std::min(8, budget / 2)));
+    const int per_worker = std::max(1, budget / workers);
     std::atomic<int> next{0};
     std::vector<std::exception_ptr> errs(nsys);
     auto work = [&]() {

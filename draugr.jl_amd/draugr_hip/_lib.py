@@ -58,6 +58,9 @@ SIGNATURES = {
     "draugr_hip_batch_cycle_device": (_i32, [_i32, _vp, _vp, _i32, _vp]),
     "draugr_hip_batch_pcg_device": (_i32, [_i32, _vp, _vp, _i32, _f64, _i32, _vp, _vp, _vp]),
     "draugr_hip_batch_free": (_i32, [_i32]),
+    "draugr_hip_batch_tune_report": (_i32, [_i32, ctypes.c_char_p, _i32]),
+    "draugr_hip_batch_level_info": (_i32, [_i32, _i32, _i64p, _i32]),
+    "draugr_hip_batch_jacobi_pass": (_i32, [_i32, _i32, _i32, _vp, _vp, _vp, _vp]),
     "draugr_hip_ilu0_factor": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 

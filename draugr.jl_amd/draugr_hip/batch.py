@@ -59,6 +59,29 @@ class AMGBatch:
         m = check(_lib.lib().draugr_hip_batch_system_info(self.handle, s, out, 4), "draugr_hip_batch_system_info")
         return dict(zip(["n", "offset", "levels", "n_coarse"], [int(out[i]) for i in range(m)]))
 
+    def level_info(self, l: int) -> dict:
+        out = (ctypes.c_int64 * 5)()
+        m = check(_lib.lib().draugr_hip_batch_level_info(self.handle, l, out, 5), "draugr_hip_batch_level_info")
+        return dict(zip(["n_smooth", "n_vec", "nnz", "nnz_P", "n_Pt"], [int(out[i]) for i in range(m)]))
+
+    def tune_report(self) -> str:
+        L = _lib.lib()
+        n = check(L.draugr_hip_batch_tune_report(self.handle, None, 0), "draugr_hip_batch_tune_report")
+        buf = ctypes.create_string_buffer(n + 1)
+        L.draugr_hip_batch_tune_report(self.handle, buf, n + 1)
+        return buf.value.decode()
+
+    def jacobi_pass(self, l: int, prec: str, x, b, out):
+        """One K-wide Jacobi step on packed level l (test / measurement hook):
+        out = x + w d (b - A x) over the level's smoothing rows; tensors of
+        the level's vector rows * nrhs entries, float64 (prec "fp64") or
+        float32 ("fp32")."""
+        p = {"fp64": 0, "fp32": 1}[prec]
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        check(_lib.lib().draugr_hip_batch_jacobi_pass(self.handle, l, p, ptr(x), ptr(b), ptr(out), _torch_stream()),
+              "draugr_hip_batch_jacobi_pass")
+        return out
+
     # ── packed vectors ──────────────────────────────────────────────────────
     def pack(self, vectors: Sequence, device="cuda"):
         """List of (n_s, nrhs) arrays (or (n_s,) when nrhs = 1) -> packed tensor."""

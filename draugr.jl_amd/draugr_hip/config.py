@@ -177,6 +177,8 @@ class AMGConfig:
     gpu_setup: bool = True  # interpolation, coarse operators, triple maps and PMIS sweeps of the setup on the GPU
     gpu_setup_min_rows: int = 32768  # smaller levels interpolate on the host
     gs_symmetric: bool = False  # coloured GS post-sweeps in reverse colour order (a symmetric cycle, for PCG)
+    batch_rhs_per_thread: int = 0  # batched passes: right-hand sides per thread (0 autotuned, 1, 4, 8, 16)
+    tail: str = "auto"  # small levels and the coarse solve in one launch: "auto" (autotuned), "off", "on" (deepest)
 
     def __post_init__(self):
         if str(self.cycle_type).upper() not in ("V", "W"):
@@ -214,6 +216,8 @@ class AMGConfig:
             "fuse": self.fuse,
             "gpu_setup_min_rows": self.gpu_setup_min_rows,
             "gs_symmetric": self.gs_symmetric,
+            "batch_rhs_per_thread": self.batch_rhs_per_thread,
+            "tail": self.tail,
         }
         if hasattr(c, "interpolation"):
             d["interpolation"] = c.interpolation.to_json()

@@ -61,6 +61,9 @@ const char *draugr_amg_last_error(void);
  * splitting run on the GPU, identical to the host's, and the triple map stays
  * in HBM; levels of fewer than "gpu_setup_min_rows" rows (default 32768)
  * interpolate and sweep on the host),
+ * "gs_symmetric" (coloured GS post-smoothing sweeps the colours in reverse
+ * order: a symmetric cycle, which PCG requires of GS), "batch_rhs_per_thread"
+ * (0 = autotuned, 1, 4, 8 or 16: mapping of the batched passes),
  * "filtering", "filter_tol".  Returns a handle > 0, or -1.
  * Replaces c_api/draugr_amg.h:94. */
 int32_t draugr_amg_config_from_json(const char *json_str);
@@ -247,6 +250,22 @@ int32_t draugr_hip_batch_pcg_device(int32_t batch, double *x, const double *b, i
                                     int32_t maxiter, void *stream, int32_t *iters, double *relres);
 
 int32_t draugr_hip_batch_free(int32_t batch);
+
+/* The batch autotuner's choices (right-hand sides per thread of each level's
+ * four passes, with the times it measured) as text; returns the length.  The
+ * tuner runs on the first cycle of each precision. */
+int32_t draugr_hip_batch_tune_report(int32_t batch, char *buf, int32_t buflen);
+
+/* out[0..]: rows that smooth on packed level `level`, rows of its vectors,
+ * nnz(A), nnz(P), rows of P^T.  Returns the number of values written. */
+int32_t draugr_hip_batch_level_info(int32_t batch, int32_t level, int64_t *out, int32_t nout);
+
+/* One K-wide Jacobi / l1-Jacobi step on packed level `level` with the tuned
+ * mapping: out = x + w d (b - A x) for its smoothing rows (vectors rows x K,
+ * row-major; prec 0 FP64, 1 FP32 vectors and values).  Measurement and test
+ * hook. */
+int32_t draugr_hip_batch_jacobi_pass(int32_t batch, int32_t level, int32_t prec, const void *x, const void *b,
+                                     void *out, void *stream);
 
 /* ILU(0) of a CSR operator on the host (0-based, columns sorted), as the
  * ilu0 smoother factorizes each level (src/smoothers.jl:1016-1098): diag[n]

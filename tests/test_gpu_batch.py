@@ -275,3 +275,56 @@ def test_batch_setup_with_gpu_setup_hooks_concurrent(dh, torch_cuda):
             dh.amg_cycle_(xj, bj, h, dataclasses.replace(host_cfg, precision="fp64"), zero_guess=True)
             torch.cuda.synchronize()
             assert np.array_equal(got[s][:, j], xj.cpu().numpy()), (s, j)
+
+
+@pytest.mark.parametrize("rpt", [1, 4, 8, 16])
+def test_every_rhs_per_thread_mapping_is_bit_exact(dh, torch_cuda, rpt):
+    """Each mapping of the K-wide passes the batch autotuner chooses among
+    (1, 4, 8 or 16 right-hand sides per thread), forced on every pass, gives
+    the single-system cycles bit for bit at K = 16 (FP64 and FP32)."""
+    torch = torch_cuda
+    mats, cfg, batch, singles = _setup(dh, [9, 14, 11], 16, max_coarse_size=50, batch_rhs_per_thread=rpt)
+    for prec in ("fp64", "fp32"):
+        c = dataclasses.replace(cfg, precision=prec)
+        bs = _rhs(dh, batch, SEED + 660)
+        b = batch.pack(bs)
+        x = torch.zeros_like(b)
+        batch.cycle_(x, b, c)
+        torch.cuda.synchronize()
+        got = batch.unpack(x)
+        for s, h in enumerate(singles):
+            for j in range(16):
+                bj = torch.tensor(np.ascontiguousarray(bs[s][:, j]), device="cuda")
+                xj = torch.zeros_like(bj)
+                dh.amg_cycle_(xj, bj, h, c, zero_guess=True)
+                torch.cuda.synchronize()
+                assert np.array_equal(got[s][:, j], xj.cpu().numpy()), (rpt, prec, s, j)
+    assert ("%d, restrict %d" % (rpt, rpt)) in batch.tune_report()
+
+
+def test_batched_jacobi_pass_equals_single_level_ops(dh, torch_cuda):
+    """The measurement hook (one K-wide Jacobi step on packed level 0, tuned
+    mapping) equals the single-system Jacobi step of every system and
+    right-hand side bit for bit."""
+    torch = torch_cuda
+    mats, cfg, batch, singles = _setup(dh, C4_EDGES, 16, max_coarse_size=50)
+    info = batch.level_info(0)
+    assert info["n_smooth"] == info["n_vec"] == batch.rows
+    for prec, tdt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        xs = _rhs(dh, batch, SEED + 670)
+        bs = _rhs(dh, batch, SEED + 680)
+        x = batch.pack(xs).to(tdt)
+        b = batch.pack(bs).to(tdt)
+        out = torch.zeros_like(x)
+        batch.jacobi_pass(0, prec, x, b, out)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy().reshape(batch.rows, 16)
+        for s, h in enumerate(singles):
+            o = batch.offsets[s]
+            for j in (0, 7, 15):
+                xj = torch.tensor(np.ascontiguousarray(xs[s][:, j]), dtype=tdt, device="cuda")
+                bj = torch.tensor(np.ascontiguousarray(bs[s][:, j]), dtype=tdt, device="cuda")
+                oj = torch.zeros_like(xj)
+                h.level_op(0, 2, prec, xj, bj, oj)
+                torch.cuda.synchronize()
+                assert np.array_equal(got[o:o + batch.sizes[s], j], oj.cpu().numpy()), (prec, s, j)

@@ -148,3 +148,62 @@ def test_resetup_rejects_wrong_device_values(dh, torch_cuda):
     with pytest.raises(TypeError):
         dh.amg_resetup_(h, torch.stack([nz, nz], 1)[:, 0], cfg)  # strided view
     dh.amg_resetup_(h, nz, cfg)  # the valid form still works
+
+
+@pytest.mark.parametrize("name", ["hypre3d", "agg2d", "wide2d", "hub2d"])
+def test_tail_in_one_launch_is_bit_exact(dh, orc, torch_cuda, name):
+    """The last AMG level and the coarse solve in one single-workgroup launch
+    from LDS (config tail = "on") give the oracle's FP64 cycle bit for bit,
+    V-cycles with several step counts (the W-cycle keeps the per-level
+    launches), and the separate launches' FP32 / BF16 cycles bit for bit; PCG
+    matches the oracle's; a resetup refreshes the LDS image."""
+    import dataclasses
+    from test_gpu_parity import _case
+    torch = torch_cuda
+    A, cfg, sm = _case(dh, name)
+    on, off = dataclasses.replace(cfg, tail="on"), dataclasses.replace(cfg, tail="off")
+    h_on, h_off = dh.amg_setup(A, on), dh.amg_setup(A, off)
+    oh = oracle_of(h_on, orc, sm)
+    b = torch.tensor(dh.problems.splitmix_normal(SEED + 990, A.n), device="cuda")
+    x0 = torch.tensor(dh.problems.splitmix_normal(SEED + 991, A.n), device="cuda")
+    for steps in ((1, 1), (2, 2), (0, 2), (2, 1)):
+        for cyc in ("V", "W"):
+            c = dataclasses.replace(on, pre_smoothing_steps=steps[0], post_smoothing_steps=steps[1], cycle_type=cyc)
+            oh.set_options(cycle=cyc.lower(), pre=steps[0], post=steps[1])
+            x = x0.clone()
+            dh.amg_cycle_(x, b, h_on, c)
+            z = torch.zeros_like(b)
+            dh.amg_cycle_(z, b, h_on, c, zero_guess=True)
+            torch.cuda.synchronize()
+            print("tail", name, steps, cyc, flush=True)
+            same(x.cpu().numpy(), oh.cycle(x0.cpu().numpy(), b.cpu().numpy()))
+            same(z.cpu().numpy(), oh.cycle(np.zeros(A.n), b.cpu().numpy()))
+            for prec in ("fp32", "bf16"):
+                outs = []
+                for h, base in ((h_on, on), (h_off, off)):
+                    cp = dataclasses.replace(base, precision=prec, pre_smoothing_steps=steps[0],
+                                             post_smoothing_steps=steps[1], cycle_type=cyc)
+                    zz = torch.zeros_like(b)
+                    dh.amg_cycle_(zz, b, h, cp, zero_guess=True)
+                    outs.append(zz.cpu().numpy())
+                assert np.array_equal(outs[0], outs[1]), (prec, steps, cyc)
+    assert "tail: level" in h_on.tune_report() and "one launch from LDS" in h_on.tune_report()
+    oh.set_options(cycle="v", pre=1, post=1)
+    xd = torch.zeros_like(b)
+    _, it, rel = dh.pcg_solve_(xd, b, h_on, on, tol=1e-10, maxiter=300)
+    xo, ito, _ = oh.pcg(np.zeros(A.n), b.cpu().numpy(), 1e-10, 300)
+    assert abs(it - ito) <= 1 and rel_err(xd.cpu().numpy(), xo) < 1e-10
+    # new values on the same pattern: the image follows the resetup
+    s = np.exp(0.3 * np.random.default_rng(SEED + 992).standard_normal(A.n))
+    rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+    new = A.nzval * s[rows] * s[A.colval]
+    dh.amg_resetup_(h_on, dh.problems.CSR(A.rowptr, A.colval, new), on)
+    oh.resetup(new)
+    for prec in ("fp64", "fp32"):
+        z = torch.zeros_like(b)
+        dh.amg_cycle_(z, b, h_on, dataclasses.replace(on, precision=prec), zero_guess=True)
+        ref = oh.cycle(np.zeros(A.n), b.cpu().numpy())
+        if prec == "fp64":
+            same(z.cpu().numpy(), ref)
+        else:
+            assert rel_err(z.cpu().numpy(), ref) < 1e-5
