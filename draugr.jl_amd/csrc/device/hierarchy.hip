@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_stream_reduce(const BlkHdr* __restri
                                                           const uint16_t* __restrict__ c16, const TV* __restrict__ av,
                                                           XF xf, EPI epi, double* partial,
                                                           const uint16_t* __restrict__ slot) {
-    __shared__ TX prod[kStreamNnz];
+    __shared__ TX prod[PER * kBlock];
     __shared__ double sh[kBlock];
     double dot = 0.0;
     auto pref = [](int) { return NoPre{}; };
@@ -129,21 +129,24 @@ void launch_stream_reduce(const DevCSR& M, const TV* av, XF xf, EPI epi, FIN fin
             case 1: DRAUGR_SR(1, true, true, M.scol.p, sv, M.slot.p); break;
             case 2: DRAUGR_SR(2, true, true, M.scol.p, sv, M.slot.p); break;
             case 4: DRAUGR_SR(4, true, true, M.scol.p, sv, M.slot.p); break;
-            default: DRAUGR_SR(8, true, true, M.scol.p, sv, M.slot.p); break;
+            case 8: DRAUGR_SR(8, true, true, M.scol.p, sv, M.slot.p); break;
+            default: DRAUGR_SR(16, true, true, M.scol.p, sv, M.slot.p); break;
         }
     } else if (M.has16) {
         switch (M.per) {
             case 1: DRAUGR_SR(1, true, false, M.c16.p, av, nullptr); break;
             case 2: DRAUGR_SR(2, true, false, M.c16.p, av, nullptr); break;
             case 4: DRAUGR_SR(4, true, false, M.c16.p, av, nullptr); break;
-            default: DRAUGR_SR(8, true, false, M.c16.p, av, nullptr); break;
+            case 8: DRAUGR_SR(8, true, false, M.c16.p, av, nullptr); break;
+            default: DRAUGR_SR(16, true, false, M.c16.p, av, nullptr); break;
         }
     } else {
         switch (M.per) {
             case 1: DRAUGR_SR(1, false, false, M.c16.p, av, nullptr); break;
             case 2: DRAUGR_SR(2, false, false, M.c16.p, av, nullptr); break;
             case 4: DRAUGR_SR(4, false, false, M.c16.p, av, nullptr); break;
-            default: DRAUGR_SR(8, false, false, M.c16.p, av, nullptr); break;
+            case 8: DRAUGR_SR(8, false, false, M.c16.p, av, nullptr); break;
+            default: DRAUGR_SR(16, false, false, M.c16.p, av, nullptr); break;
         }
     }
 #undef DRAUGR_SR
@@ -635,7 +638,7 @@ void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
     d.rp.upload(h.rp, st);
     d.cv.upload(h.cv, st);
     d.v64.upload(h.nz, st);
-    const int budget = stream_budget(h.nnz());
+    const int budget = stream_budget(h.nnz(), h.nrow);
     const std::vector<int32_t> blk = build_row_blocks(h.rp, budget);
     std::vector<int32_t> bz(blk.size());
     for (size_t b = 0; b < blk.size(); ++b) bz[b] = h.rp[blk[b]];
@@ -1217,13 +1220,12 @@ bool DeviceHierarchy::tail_fits(TailLayout* lay) const {
 // copied on the device from the hierarchy's own arrays (again after every
 // resetup, since values, diagonal and factors change).
 template <typename TV, typename TX>
-void DeviceHierarchy::pack_tail() {
+void DeviceHierarchy::pack_tail(hipStream_t st) {
     typedef typename TransferT<TV>::type TP;
     const int pi = pidx<TV>();
     TailLayout y;
     if (!tail_fits<TV, TX>(&y)) return;
     const DevLevel& D = L_[nlevels() - 1];
-    hipStream_t st = stream_;
     tail_img_[pi].alloc(y.img_bytes);
     tail_lay_[pi] = y;
     uint8_t* im = tail_img_[pi].p;
@@ -1254,10 +1256,11 @@ void DeviceHierarchy::pack_tail() {
     copy_i32(y.perm, flag_.p + 1 + nc_, y.nc);
 }
 
-void DeviceHierarchy::refresh_tails(hipStream_t) {
-    if (tail_img_[0].p) pack_tail<double, double>();
-    if (tail_img_[1].p) pack_tail<float, float>();
-    if (tail_img_[2].p) pack_tail<bf16_t, float>();
+// after a resetup, on the stream that produced the new values
+void DeviceHierarchy::refresh_tails(hipStream_t st) {
+    if (tail_img_[0].p) pack_tail<double, double>(st);
+    if (tail_img_[1].p) pack_tail<float, float>(st);
+    if (tail_img_[2].p) pack_tail<bf16_t, float>(st);
 }
 
 template <typename TV, typename TX>
@@ -1278,7 +1281,7 @@ void DeviceHierarchy::tune_tail() {
     const int pi = pidx<TV>();
     tail_lt_[pi] = -1;
     if (forced_tail_ == 0 || !tail_fits<TV, TX>(nullptr)) return;
-    pack_tail<TV, TX>();
+    pack_tail<TV, TX>(stream_);  // the tuner has synchronized the device
     DRAUGR_HIP_CHECK(hipStreamSynchronize(stream_));
     if (forced_tail_ == 1) {
         tail_lt_[pi] = nlevels() - 1;

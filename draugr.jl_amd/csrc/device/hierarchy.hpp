@@ -220,7 +220,7 @@ private:
     DBuf<uint8_t> tail_img_[3];
     dev::TailLayout tail_lay_[3] = {};
     template <typename TV, typename TX> bool tail_fits(dev::TailLayout* lay) const;
-    template <typename TV, typename TX> void pack_tail();
+    template <typename TV, typename TX> void pack_tail(hipStream_t st);
     void refresh_tails(hipStream_t st);
     template <typename TV, typename TX> void tune_tail();
     template <typename TV, typename TX> void launch_tail(const CycleOpts& o, hipStream_t st);

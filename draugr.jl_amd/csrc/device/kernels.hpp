@@ -30,7 +30,7 @@ namespace draugr {
 namespace dev {
 
 constexpr int kBlock = 256;
-constexpr int kStreamNnz = 2048;    // products staged in LDS per block
+constexpr int kStreamNnz = 4096;    // products per row block at most (16 per thread)
 constexpr int kStreamRows = kBlock; // one epilogue thread per row
 
 typedef uint16_t bf16_t;
@@ -247,12 +247,13 @@ __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ h
     const int nz = z1 - z0;
     const int t = (int)threadIdx.x;
     const int row = r0 + t;
-    if (nz <= kStreamNnz) {
+    constexpr int S = PER * kBlock;  // the block's LDS stage
+    if (nz <= S) {
         const bool own = row < r1;
         const int rs = own ? rp[row] : 0, re = own ? rp[row + 1] : 0;  // issued before the barrier
         decltype(pref(0)) q{};
         if (own) q = pref(row);
-        if (nz <= PER * kBlock) {
+        {
             if (nz > 0) {
                 int cj[PER], sl[PER];
                 TX v[PER], g[PER];
@@ -279,9 +280,6 @@ __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ h
                 for (int u = 0; u < PER; ++u)
                     if (t + u * kBlock < nz) prod[sl[u]] = v[u] * g[u];
             }
-        } else {  // a single row longer than the budget (still summed in row order)
-#pragma unroll 4
-            for (int k = t; k < nz; k += kBlock) prod[k] = ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
         }
         __syncthreads();
         if (own) {
@@ -297,8 +295,8 @@ __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ h
             q = pref(r0);
             a = initf(q);
         }
-        for (int c0 = 0; c0 < nz; c0 += kStreamNnz) {
-            const int cn = min(kStreamNnz, nz - c0);
+        for (int c0 = 0; c0 < nz; c0 += S) {
+            const int cn = min(S, nz - c0);
             for (int k = t; k < cn; k += kBlock) prod[k] = ld<TX, TV>(av, z0 + c0 + k) * xf(ldi(cv, z0 + c0 + k));
             __syncthreads();
             if (t == 0)
@@ -322,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(const BlkHdr* __restrict_
                                                        const int* __restrict__ cv, const uint16_t* __restrict__ c16,
                                                        const TV* __restrict__ av, XF xf, EPI epi,
                                                        const uint16_t* __restrict__ slot) {
-    __shared__ TX prod[kStreamNnz];
+    __shared__ TX prod[PER * kBlock];
     const int b = XCD ? xcd_block() : (int)blockIdx.x;
     auto pref = [&](int row) { return epi.pre(row); };
     auto initf = [&](const typename EPI::Pre& q) { return epi.init(q); };
@@ -360,7 +358,8 @@ __global__ __launch_bounds__(kBlock) void k_csr_pipe(int nblk, const int* __rest
                                                      const int* __restrict__ bz, const int* __restrict__ rp,
                                                      const int* __restrict__ cv, const TV* __restrict__ av, XF xf,
                                                      EPI epi) {
-    __shared__ TX prod[kStreamNnz];
+    constexpr int S = PER * kBlock;  // the LDS stage
+    __shared__ TX prod[S];
     const int w = XCD ? xcd_block() : (int)blockIdx.x;
     const int per_wg = (nblk + (int)gridDim.x - 1) / (int)gridDim.x;
     const int b0 = w * per_wg, b1 = min(nblk, b0 + per_wg);
@@ -409,11 +408,6 @@ __global__ __launch_bounds__(kBlock) void k_csr_pipe(int nblk, const int* __rest
             }
             __syncthreads();
             if (own) epi_segment(epi, row, prod + (rs - z0), re - rs, q);
-        } else if (nz <= kStreamNnz) {  // one row longer than the budget: row order, no pipelining
-            for (int k = t; k < nz; k += kBlock) prod[k] = ld<TX, TV>(av, z0 + k) * xf(ldi(cv, z0 + k));
-            __syncthreads();
-            if (own) epi_segment(epi, row, prod + (rs - z0), re - rs, q);
-            if (nnz > 0 && nnz <= PER * kBlock) load(nz0, nnz);
         } else {  // a row longer than the LDS stage: chunk by chunk, row order, on thread 0
             typename EPI::Pre q0{};
             TX a = 0;
@@ -421,8 +415,8 @@ __global__ __launch_bounds__(kBlock) void k_csr_pipe(int nblk, const int* __rest
                 q0 = epi.pre(r0);
                 a = epi.init(q0);
             }
-            for (int c0 = 0; c0 < nz; c0 += kStreamNnz) {
-                const int cn = min(kStreamNnz, nz - c0);
+            for (int c0 = 0; c0 < nz; c0 += S) {
+                const int cn = min(S, nz - c0);
                 for (int k = t; k < cn; k += kBlock) prod[k] = ld<TX, TV>(av, z0 + c0 + k) * xf(ldi(cv, z0 + c0 + k));
                 __syncthreads();
                 if (t == 0)
@@ -455,7 +449,8 @@ __global__ __launch_bounds__(kBlock) void k_stream_pipe(int nblk, const BlkHdr* 
                                                         const int* __restrict__ rp, const int* __restrict__ cv,
                                                         const uint16_t* __restrict__ c16, const TV* __restrict__ av,
                                                         XF xf, EPI epi, const uint16_t* __restrict__ slot) {
-    __shared__ TX prod[kStreamNnz];
+    constexpr int S = PER * kBlock;  // the LDS stage
+    __shared__ TX prod[S];
     const int w = XCD ? xcd_block() : (int)blockIdx.x;
     const int per_wg = (nblk + (int)gridDim.x - 1) / (int)gridDim.x;
     const int b0 = w * per_wg, b1 = min(nblk, b0 + per_wg);
@@ -525,8 +520,8 @@ __global__ __launch_bounds__(kBlock) void k_stream_pipe(int nblk, const BlkHdr* 
                 q0 = epi.pre(hc.r0);
                 a = epi.init(q0);
             }
-            for (int c0 = 0; c0 < nz; c0 += kStreamNnz) {
-                const int cn = min(kStreamNnz, nz - c0);
+            for (int c0 = 0; c0 < nz; c0 += S) {
+                const int cn = min(S, nz - c0);
                 for (int k = t; k < cn; k += kBlock)
                     prod[k] = ld<TX, TV>(av, hc.z0 + c0 + k) * xf(ldi(cv, hc.z0 + c0 + k));
                 __syncthreads();
@@ -570,14 +565,16 @@ void launch_stream_pipe(int nblk, int per, const BlkHdr* hdr, const int* rp, con
             case 1: launch_stream_pipe_per<1, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
             case 2: launch_stream_pipe_per<2, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
             case 4: launch_stream_pipe_per<4, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-            default: launch_stream_pipe_per<8, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+            case 8: launch_stream_pipe_per<8, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+            default: launch_stream_pipe_per<16, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
         }
     } else {
         switch (per) {
             case 1: launch_stream_pipe_per<1, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
             case 2: launch_stream_pipe_per<2, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
             case 4: launch_stream_pipe_per<4, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-            default: launch_stream_pipe_per<8, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+            case 8: launch_stream_pipe_per<8, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+            default: launch_stream_pipe_per<16, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
         }
     }
 }
@@ -602,7 +599,8 @@ void launch_pipe(int nblk, int per, const int* blk, const int* bz, const int* rp
         case 1: launch_pipe_per<1, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
         case 2: launch_pipe_per<2, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
         case 4: launch_pipe_per<4, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
-        default: launch_pipe_per<8, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
+        case 8: launch_pipe_per<8, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
+        default: launch_pipe_per<16, TV, TX>(nblk, blk, bz, rp, cv, av, xf, epi, st); break;
     }
 }
 
@@ -629,11 +627,12 @@ void launch_stream_c(int nblk, int per, const BlkHdr* hdr, const int* rp, const 
         case 1: launch_stream_per<1, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
         case 2: launch_stream_per<2, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
         case 4: launch_stream_per<4, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-        default: launch_stream_per<8, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+        case 8: launch_stream_per<8, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+        default: launch_stream_per<16, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
     }
 }
 
-// per = the operator's block budget / kBlock (1, 2, 4 or 8); c16 non-null =
+// per = the operator's block budget / kBlock (1, 2, 4, 8 or 16); c16 non-null =
 // the 16-bit column form (bases in the block headers); slot non-null = its
 // column-sorted form (c16 and av sorted within each 16-bit block)
 template <typename TV, typename TX, class XF, class EPI>
@@ -655,10 +654,12 @@ void launch_rowseq(int nrows, const int* rp, const int* cv, const TV* av, XF xf,
 // Nonzeros per row block: kStreamNnz for large operators; halved (down to
 // 256) while the operator would give fewer than ~8 blocks per CU, so that
 // small coarse levels still spread over the chip and each block's serial
-// phases stay short.
-inline int stream_budget(int64_t nnz) {
+// phases stay short, and while a block of kStreamRows average rows would fill
+// less than half of it (its threads' clamped loads would mostly repeat).
+inline int stream_budget(int64_t nnz, int64_t nrow) {
     int budget = kStreamNnz;
-    while (budget > 256 && nnz / budget < 2048) budget >>= 1;
+    const double avg = nrow > 0 ? (double)nnz / (double)nrow : 0.0;
+    while (budget > 256 && (nnz / budget < 2048 || budget > 2.0 * avg * kStreamRows)) budget >>= 1;
     return budget;
 }
 
