@@ -522,6 +522,23 @@ int32_t draugr_hip_level_op(int32_t handle, int32_t level, int32_t op, int32_t p
     });
 }
 
+int32_t draugr_hip_level_op_repeat(int32_t handle, int32_t level, int32_t op, int32_t prec, const void* in1,
+                                   const void* in2, void* out, void* out2, int32_t reps, void* stream) {
+    return guarded([&]() -> int32_t {
+        auto H = get_hier(handle);
+        if (!H) throw Error("Invalid hierarchy handle: " + std::to_string(handle));
+        if (reps < 1) throw Error("ArgumentError: reps must be positive");
+        std::lock_guard<std::mutex> g(H->mu);
+        require_device(*H);
+        if (prec < 0 || prec > 2) throw Error("prec must be 0, 1 or 2");
+        hipStream_t st = stream_of(*H, stream);
+        H->dev->begin(st);
+        for (int32_t r = 0; r < reps; ++r) H->dev->level_op(level, op, prec, in1, in2, out, out2, st);
+        H->dev->end(st);
+        return 0;
+    });
+}
+
 int32_t draugr_hip_export_device_values(int32_t handle, int32_t level, int32_t which, double* dst) {
     return guarded([&]() -> int32_t {
         auto H = get_hier(handle);

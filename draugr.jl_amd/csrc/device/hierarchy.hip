@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_stream_reduce(const BlkHdr* __restri
                                                           const int* __restrict__ cv,
                                                           const uint16_t* __restrict__ c16, const TV* __restrict__ av,
                                                           XF xf, EPI epi, double* partial,
-                                                          const uint16_t* __restrict__ slot) {
+                                                          const uint32_t* __restrict__ cs) {
     __shared__ TX prod[PER * kBlock];
     __shared__ double sh[kBlock];
     double dot = 0.0;
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_stream_reduce(const BlkHdr* __restri
     auto accf = [](TX a, TX p) { return a + p; };
     auto finf = [&](int row, TX a, NoPre) { dot = epi(row, a); };
     stream_block<PER, C16, TV, TX, XF, decltype(pref), decltype(initf), decltype(accf), decltype(finf), SRT>(
-        xcd_block(), hdr, rp, cv, c16, av, xf, prod, pref, initf, accf, finf, slot);
+        xcd_block(), hdr, rp, cv, c16, av, xf, prod, pref, initf, accf, finf, cs);
     block_sum_store(dot, sh, partial, blockIdx.x);
 }
 
@@ -120,17 +120,17 @@ __global__ __launch_bounds__(kBlock) void k_stream_reduce(const BlkHdr* __restri
 template <typename TV, typename TX, class XF, class EPI, class FIN>
 void launch_stream_reduce(const DevCSR& M, const TV* av, XF xf, EPI epi, FIN fin, double* partial, hipStream_t st,
                           bool sorted = false) {
-#define DRAUGR_SR(PER, C, S, COLS, VALS, SLOT)                                                                   \
+#define DRAUGR_SR(PER, C, S, COLS, VALS, CS)                                                                   \
     hipLaunchKernelGGL((k_stream_reduce<PER, C, S, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.hdr.p,  \
-                       M.rp.p, M.cv.p, COLS, VALS, xf, epi, partial, SLOT)
+                       M.rp.p, M.cv.p, COLS, VALS, xf, epi, partial, CS)
     if (sorted && M.hasS) {
         const TV* sv = reinterpret_cast<const TV*>(M.sv64.p);
         switch (M.per) {
-            case 1: DRAUGR_SR(1, true, true, M.scol.p, sv, M.slot.p); break;
-            case 2: DRAUGR_SR(2, true, true, M.scol.p, sv, M.slot.p); break;
-            case 4: DRAUGR_SR(4, true, true, M.scol.p, sv, M.slot.p); break;
-            case 8: DRAUGR_SR(8, true, true, M.scol.p, sv, M.slot.p); break;
-            default: DRAUGR_SR(16, true, true, M.scol.p, sv, M.slot.p); break;
+            case 1: DRAUGR_SR(1, true, true, nullptr, sv, M.scs.p); break;
+            case 2: DRAUGR_SR(2, true, true, nullptr, sv, M.scs.p); break;
+            case 4: DRAUGR_SR(4, true, true, nullptr, sv, M.scs.p); break;
+            case 8: DRAUGR_SR(8, true, true, nullptr, sv, M.scs.p); break;
+            default: DRAUGR_SR(16, true, true, nullptr, sv, M.scs.p); break;
         }
     } else if (M.has16) {
         switch (M.per) {
@@ -560,12 +560,14 @@ void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
     else if (kind == kOpStream16 && M.has16)
         launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st, M.c16.p);
     else if (kind == kOpSorted && M.hasS && svals<TV>(M))
-        launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, svals<TV>(M), xf, epi, st, M.scol.p, M.slot.p);
+        launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, svals<TV>(M), xf, epi, st, nullptr, M.scs.p);
     else if (kind == kOpStream16Pipe && M.has16)
         launch_stream_pipe<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, M.c16.p, vals<TV>(M), xf, epi, st, nullptr);
     else if (kind == kOpSortedPipe && M.hasS && svals<TV>(M))
-        launch_stream_pipe<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, M.scol.p, svals<TV>(M), xf, epi, st,
-                                   M.slot.p);
+        launch_stream_pipe<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, nullptr, svals<TV>(M), xf, epi, st,
+                                   M.scs.p);
+    else if (kind == kOpSortedQuad && M.hasS && svals<TV>(M))
+        launch_stream_quad<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, M.scs.p, svals<TV>(M), xf, epi, st);
     else launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
 }
 
@@ -577,20 +579,21 @@ void op(const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
 }
 
 // Column-sorted form of the 16-bit blocks: a bitonic sort in LDS of the keys
-// (column offset << 12 | position in the block), unique per entry; blocks
-// without a base keep row order (slot = position).  The positions of a 16-bit
-// block are below its budget (<= kStreamNnz entries), so 12 bits hold them.
+// (column offset << 12 | position in the block), unique per entry, giving
+// cs = column offset | position << 16 and the values in that order; blocks
+// without a base keep row order (cs = position << 16, unread: those blocks
+// take their columns from the 32-bit array).  The positions of a 16-bit block
+// are below its budget (<= kStreamNnz entries), so 12 bits hold them.
 __global__ __launch_bounds__(kBlock) void k_sort_blocks(const BlkHdr* __restrict__ hdr, const uint16_t* __restrict__ c16,
-                                                        const double* __restrict__ v, uint16_t* __restrict__ scol,
-                                                        uint16_t* __restrict__ slot, double* __restrict__ sv) {
+                                                        const double* __restrict__ v, uint32_t* __restrict__ cs,
+                                                        double* __restrict__ sv) {
     __shared__ uint32_t key[kStreamNnz];
     const int b = blockIdx.x;
     const int z0 = hdr[b].z0, nz = hdr[b + 1].z0 - z0, cb = hdr[b].cb;
     const int t = (int)threadIdx.x;
     if (cb < 0 || nz > kStreamNnz) {
         for (int k = t; k < nz; k += kBlock) {
-            scol[z0 + k] = 0;
-            slot[z0 + k] = (uint16_t)k;
+            cs[z0 + k] = (uint32_t)(k & 0xFFFF) << 16;
             sv[z0 + k] = v[z0 + k];
         }
         return;
@@ -616,20 +619,28 @@ __global__ __launch_bounds__(kBlock) void k_sort_blocks(const BlkHdr* __restrict
     for (int k = t; k < nz; k += kBlock) {
         const uint32_t kk = key[k];
         const int pos = (int)(kk & 0xFFFu);
-        scol[z0 + k] = (uint16_t)(kk >> 12);
-        slot[z0 + k] = (uint16_t)pos;
+        cs[z0 + k] = (kk >> 12) | ((uint32_t)pos << 16);
         sv[z0 + k] = v[z0 + pos];
     }
 }
 
-// sv = the values in the sorted form's order (after new values in v)
+// sv = the values in the sorted form's order (after new values in v); blocks
+// without a base (row order, possibly one row longer than 65536 entries) copy
 __global__ __launch_bounds__(kBlock) void k_permute_blocks(const BlkHdr* __restrict__ hdr,
-                                                           const uint16_t* __restrict__ slot,
+                                                           const uint32_t* __restrict__ cs,
                                                            const double* __restrict__ v, double* __restrict__ sv) {
     const int b = blockIdx.x;
     const int z0 = hdr[b].z0, nz = hdr[b + 1].z0 - z0;
-    for (int k = (int)threadIdx.x; k < nz; k += kBlock) sv[z0 + k] = v[z0 + slot[z0 + k]];
+    if (hdr[b].cb < 0) {
+        for (int k = (int)threadIdx.x; k < nz; k += kBlock) sv[z0 + k] = v[z0 + k];
+        return;
+    }
+    for (int k = (int)threadIdx.x; k < nz; k += kBlock) sv[z0 + k] = v[z0 + (int)(cs[z0 + k] >> 16)];
 }
+
+// Entries allocated for an array of the column-sorted form: a multiple of 4,
+// so that the four-entry loads of the last block stay inside it.
+inline size_t sorted_alloc(int64_t nnz) { return (size_t)((nnz + 3) & ~(int64_t)3); }
 
 void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
     d.nrow = h.nrow;
@@ -681,14 +692,12 @@ void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
     d.hdr.upload(hdr, st);
     d.hasS = fits;
     if (fits) {
-        d.scol.alloc(d.nnz);
-        d.slot.alloc(d.nnz);
-        d.sv64.alloc(d.nnz);
-        hipLaunchKernelGGL(k_sort_blocks, dim3(d.nblk), dim3(kBlock), 0, st, d.hdr.p, d.c16.p, d.v64.p, d.scol.p,
-                           d.slot.p, d.sv64.p);
+        d.scs.alloc(sorted_alloc(d.nnz));
+        d.sv64.alloc(sorted_alloc(d.nnz));
+        hipLaunchKernelGGL(k_sort_blocks, dim3(d.nblk), dim3(kBlock), 0, st, d.hdr.p, d.c16.p, d.v64.p, d.scs.p,
+                           d.sv64.p);
     } else {
-        d.scol.release();
-        d.slot.release();
+        d.scs.release();
         d.sv64.release();
     }
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // blk, c16, hdr are pageable and local
@@ -842,7 +851,7 @@ int64_t DeviceHierarchy::bytes_resident() const {
     int64_t b = 0;
     auto csr = [&](const DevCSR& m) {
         b += m.rp.bytes() + m.cv.bytes() + m.v64.bytes() + m.v32.bytes() + m.v16.bytes() + m.blk.bytes();
-        b += m.c16.bytes() + m.hdr.bytes() + m.scol.bytes() + m.slot.bytes() + m.sv64.bytes() + m.sv32.bytes() +
+        b += m.c16.bytes() + m.hdr.bytes() + m.scs.bytes() + m.sv64.bytes() + m.sv32.bytes() +
              m.sv16.bytes();
     };
     for (auto& l : L_) {
@@ -974,7 +983,7 @@ static void refresh_mirrors(DevCSR& M, hipStream_t st) {
         launch_map(M.nnz, [=] __device__(int64_t i) { d[i] = float_to_bf16((float)s[i]); }, st);
     }
     if (M.hasS && M.nblk > 0) {
-        hipLaunchKernelGGL(k_permute_blocks, dim3(M.nblk), dim3(kBlock), 0, st, M.hdr.p, M.slot.p, M.v64.p, M.sv64.p);
+        hipLaunchKernelGGL(k_permute_blocks, dim3(M.nblk), dim3(kBlock), 0, st, M.hdr.p, M.scs.p, M.v64.p, M.sv64.p);
         const double* ss = M.sv64.p;
         if (M.sv32.p) {
             float* d = M.sv32.p;
@@ -1020,7 +1029,7 @@ void DeviceHierarchy::ensure_precision(Precision p) {
                 float* d = m->v32.p;
                 launch_map(m->nnz, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
                 if (m->hasS) {
-                    m->sv32.alloc(m->nnz);
+                    m->sv32.alloc(sorted_alloc(m->nnz));
                     const double* ss = m->sv64.p;
                     float* ds = m->sv32.p;
                     launch_map(m->nnz, [=] __device__(int64_t i) { ds[i] = (float)ss[i]; }, st);
@@ -1041,7 +1050,7 @@ void DeviceHierarchy::ensure_precision(Precision p) {
             bf16_t* d = D.A.v16.p;
             launch_map(D.A.nnz, [=] __device__(int64_t i) { d[i] = float_to_bf16((float)s[i]); }, st);
             if (D.A.hasS) {
-                D.A.sv16.alloc(D.A.nnz);
+                D.A.sv16.alloc(sorted_alloc(D.A.nnz));
                 const double* ss = D.A.sv64.p;
                 bf16_t* ds = D.A.sv16.p;
                 launch_map(D.A.nnz, [=] __device__(int64_t i) { ds[i] = float_to_bf16((float)ss[i]); }, st);
@@ -1382,7 +1391,7 @@ void DeviceHierarchy::tune() {
             const float skip = 1e30f;  // the 16-bit and sorted forms only where the operator has them
             auto absent = [&](const DevCSR& M) {
                 return ((k == kOpStream16 || k == kOpStream16Pipe) && !M.has16) ||
-                       ((k == kOpSorted || k == kOpSortedPipe) && !M.hasS);
+                       ((k == kOpSorted || k == kOpSortedPipe || k == kOpSortedQuad) && !M.hasS);
             };
             tA[k] = absent(D.A) ? skip : timeit([&] {
                 op_kind<TV, TX>(k, D.A, XVec<TX>{xin}, EpiJacobi<TX>{xin, bin, d, s1, w}, st);
@@ -1451,7 +1460,8 @@ std::string DeviceHierarchy::tune_report() const {
         if (!tuned_[p]) continue;
         for (int l = 0; l < nlevels(); ++l) {
             const DevLevel& D = L_[l];
-            const char* kn[kOpKinds] = {"stream", "rowseq", "pipe", "stream16", "sorted", "stream16pipe", "sortedpipe"};
+            const char* kn[kOpKinds] = {"stream",       "rowseq",     "pipe",      "stream16",
+                                        "sorted",       "stream16pipe", "sortedpipe", "sortedquad"};
             std::snprintf(buf, sizeof buf, "%s level %d: A=%s Pt=%s P=%s zero-guess=%s down=%s up=%s\n", names[p], l,
                           kn[D.A.kind[p]], kn[D.Pt.kind[p]], kn[D.P.kind[p]], D.fuse_zero[p] ? "fused" : "split",
                           D.fuse_down[p] ? "one-pass" : (D.down_state < 0 ? "separate(only)" : "separate"),
@@ -1775,7 +1785,7 @@ bool DeviceHierarchy::pcg_sorted(const DevCSR& A, hipStream_t st) {
     pcg_sorted_ = 0;
     if (!A.hasS) return false;
     if (forced_kind_ >= 0) {
-        pcg_sorted_ = forced_kind_ == kOpSorted || forced_kind_ == kOpSortedPipe;
+        pcg_sorted_ = forced_kind_ == kOpSorted || forced_kind_ == kOpSortedPipe || forced_kind_ == kOpSortedQuad;
         return pcg_sorted_ == 1;
     }
     const double* u = w_z_.p;
@@ -2097,7 +2107,8 @@ void DeviceHierarchy::resetup(const double* nzval, hipStream_t st) {
 static double index_bytes(const DevCSR& M, int pi) {
     if ((M.kind[pi] == kOpStream16 || M.kind[pi] == kOpStream16Pipe) && M.has16)
         return 2.0 * M.nnz16 + 4.0 * (M.nnz - M.nnz16) + 16.0 * M.nblk;  // + the block headers
-    if ((M.kind[pi] == kOpSorted || M.kind[pi] == kOpSortedPipe) && M.hasS)  // + the slot of each 16-bit entry
+    if ((M.kind[pi] == kOpSorted || M.kind[pi] == kOpSortedPipe || M.kind[pi] == kOpSortedQuad) &&
+        M.hasS)  // + the position of each 16-bit entry
         return 4.0 * M.nnz16 + 4.0 * (M.nnz - M.nnz16) + 16.0 * M.nblk;
     return 4.0 * M.nnz;
 }

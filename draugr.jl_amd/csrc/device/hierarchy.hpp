@@ -39,11 +39,13 @@ struct DevCSR {
     int64_t nnz16 = 0;        // nonzeros of the blocks read through 16-bit columns
     DBuf<uint16_t> c16;       // column - (the block's base) per nonzero
     DBuf<dev::BlkHdr> hdr;    // per block: first row, first nonzero, 16-bit column base (-1: 32-bit columns)
-    // Column-sorted form of the 16-bit blocks (kernel kind sorted): per entry
-    // the column offset and the value in column order, and the entry's slot in
-    // the block's row order; blocks without a base keep row order.
+    // Column-sorted form of the 16-bit blocks (kernel kinds sorted,
+    // sortedpipe, sortedquad): the values in column order and per entry one
+    // word, column offset | the entry's position in the block's row order << 16;
+    // blocks without a base keep row order.  Allocated to a multiple of 4
+    // entries (four-entry loads).
     bool hasS = false;
-    DBuf<uint16_t> scol, slot;
+    DBuf<uint32_t> scs;
     DBuf<double> sv64;
     DBuf<float> sv32;
     DBuf<dev::bf16_t> sv16;

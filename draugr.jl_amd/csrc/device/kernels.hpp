@@ -57,6 +57,7 @@ template <> __device__ __forceinline__ float ld<float, bf16_t>(const bf16_t* p, 
 }
 __device__ __forceinline__ int ldi(const int* p, int64_t k) { return __builtin_nontemporal_load(p + k); }
 __device__ __forceinline__ int ldc16(const uint16_t* p, int64_t k) { return (int)__builtin_nontemporal_load(p + k); }
+__device__ __forceinline__ uint32_t ldu(const uint32_t* p, int64_t k) { return __builtin_nontemporal_load(p + k); }
 
 // XCD-contiguous remap of the block index (bijective for any grid size).
 __device__ __forceinline__ int xcd_block() {
@@ -229,8 +230,9 @@ struct alignas(16) BlkHdr {
 // columns.
 //
 // SRT (with C16): the entries of each such block are stored sorted by column
-// (c16 and av in that order) with slot[k] = the entry's position in the block's
-// row order, where its product is staged.  A wave's gathers then touch a few
+// (av in that order) with one 32-bit word per entry, cs[k] = the 16-bit column
+// offset | the entry's position in the block's row order (where its product is
+// staged) << 16, so column and position arrive with one load.  A wave's gathers then touch a few
 // adjacent cache lines instead of the lines of every column cluster of its
 // rows; the staged products, and so the row sums, are the same.  Other blocks
 // keep row order (av in row order, columns from cv).
@@ -239,7 +241,7 @@ template <int PER, bool C16, typename TV, typename TX, class XF, class PREF, cla
 __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ hdr, const int* __restrict__ rp,
                                              const int* __restrict__ cv, const uint16_t* __restrict__ c16,
                                              const TV* __restrict__ av, XF xf, TX* prod, PREF pref, INITF initf,
-                                             ACCF accf, FINF finf, const uint16_t* __restrict__ slot = nullptr) {
+                                             ACCF accf, FINF finf, const uint32_t* __restrict__ cs = nullptr) {
     const BlkHdr h0 = hdr[b];
     const int r0 = h0.r0, r1 = h0.r1;
     const int z0 = h0.z0, z1 = hdr[b + 1].z0;
@@ -257,13 +259,22 @@ __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ h
             if (nz > 0) {
                 int cj[PER], sl[PER];
                 TX v[PER], g[PER];
-                if (C16 && cb >= 0) {  // block-uniform choice, outside the unrolled loads
+                if (SRT && cb >= 0) {  // block-uniform choices, outside the unrolled loads
+#pragma unroll
+                    for (int u = 0; u < PER; ++u) {
+                        const int k = min(t + u * kBlock, nz - 1);
+                        const uint32_t e = ldu(cs, z0 + k);
+                        cj[u] = cb + (int)(e & 0xFFFFu);
+                        v[u] = ld<TX, TV>(av, z0 + k);
+                        sl[u] = (int)(e >> 16);
+                    }
+                } else if (C16 && cb >= 0) {
 #pragma unroll
                     for (int u = 0; u < PER; ++u) {
                         const int k = min(t + u * kBlock, nz - 1);
                         cj[u] = cb + ldc16(c16, z0 + k);
                         v[u] = ld<TX, TV>(av, z0 + k);
-                        sl[u] = SRT ? ldc16(slot, z0 + k) : t + u * kBlock;
+                        sl[u] = t + u * kBlock;
                     }
                 } else {
 #pragma unroll
@@ -319,7 +330,7 @@ template <bool XCD, int PER, bool C16, typename TV, typename TX, class XF, class
 __global__ __launch_bounds__(kBlock) void k_csr_stream(const BlkHdr* __restrict__ hdr, const int* __restrict__ rp,
                                                        const int* __restrict__ cv, const uint16_t* __restrict__ c16,
                                                        const TV* __restrict__ av, XF xf, EPI epi,
-                                                       const uint16_t* __restrict__ slot) {
+                                                       const uint32_t* __restrict__ cs) {
     __shared__ TX prod[PER * kBlock];
     const int b = XCD ? xcd_block() : (int)blockIdx.x;
     auto pref = [&](int row) { return epi.pre(row); };
@@ -327,7 +338,117 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(const BlkHdr* __restrict_
     auto accf = [&](TX a, TX p) { return epi.acc(a, p); };
     auto finf = [&](int row, TX a, const typename EPI::Pre& q) { epi.fin(row, a, q); };
     stream_block<PER, C16, TV, TX, XF, decltype(pref), decltype(initf), decltype(accf), decltype(finf), SRT>(
-        b, hdr, rp, cv, c16, av, xf, prod, pref, initf, accf, finf, slot);
+        b, hdr, rp, cv, c16, av, xf, prod, pref, initf, accf, finf, cs);
+}
+
+// Four consecutive entries k0 .. k0 + 3 (k0 a multiple of 4) of a value array
+// of the column-sorted form, in one or two 16-byte loads (those arrays are
+// allocated to a multiple of 4 entries, so the load stays inside them).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+template <typename TX, typename TV> struct Quad;
+template <> struct Quad<double, double> {
+    static __device__ __forceinline__ void load(const double* p, int k0, double (&o)[4]) {
+        const f64x2 a = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p + k0));
+        const f64x2 b = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p + k0) + 1);
+        o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+    }
+};
+template <> struct Quad<float, float> {
+    static __device__ __forceinline__ void load(const float* p, int k0, float (&o)[4]) {
+        const f32x4 a = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p + k0));
+        o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    }
+};
+template <> struct Quad<float, bf16_t> {
+    static __device__ __forceinline__ void load(const bf16_t* p, int k0, float (&o)[4]) {
+        const u16x4 a = __builtin_nontemporal_load(reinterpret_cast<const u16x4*>(p + k0));
+        o[0] = bf16_to_float(a.x); o[1] = bf16_to_float(a.y); o[2] = bf16_to_float(a.z); o[3] = bf16_to_float(a.w);
+    }
+};
+
+// One row block of the column-sorted form with four entries per load: the
+// block's nonzero range [z0, z1) is covered by the aligned quads starting at
+// z0 & ~3; thread t loads quads t, t + kBlock, ... (one 16-byte load of four
+// packed column / position words, one or two of four values), gathers the
+// four x entries and stages the products of the entries inside [z0, z1) at
+// their positions; entries of the neighbouring blocks that share the first or
+// last quad are loaded and dropped (their gathers read the block's base
+// column instead).  Per product that is 1/4 column load, 1/4-1/2 value load
+// and one gather, against one of each in stream_block.  The staged products
+// and the row walk are stream_block's, so the values are the same bits.
+// Blocks without a 16-bit base, and the rare block whose quads exceed the
+// stage (nz within 3 of it and misaligned), take stream_block's path.
+template <int PER, typename TV, typename TX, class XF, class PREF, class INITF, class ACCF, class FINF>
+__device__ __forceinline__ void stream_block_quad(int b, const BlkHdr* __restrict__ hdr, const int* __restrict__ rp,
+                                                  const int* __restrict__ cv, const uint32_t* __restrict__ cs,
+                                                  const TV* __restrict__ av, XF xf, TX* prod, PREF pref, INITF initf,
+                                                  ACCF accf, FINF finf) {
+    static_assert(PER % 4 == 0, "four entries per load");
+    constexpr int G = PER / 4;
+    const BlkHdr h0 = hdr[b];
+    const int z0 = h0.z0, z1 = hdr[b + 1].z0, cb = h0.cb;
+    const int a0 = z0 & ~3;
+    const int nq = (z1 - a0 + 3) >> 2;
+    if (cb < 0 || nq > G * kBlock) {
+        stream_block<PER, true, TV, TX, XF, PREF, INITF, ACCF, FINF, true>(b, hdr, rp, cv, nullptr, av, xf, prod,
+                                                                          pref, initf, accf, finf, cs);
+        return;
+    }
+    // cb >= 0: 0 < z1 - z0 <= the budget (16-bit bases are given to such blocks only)
+    const int t = (int)threadIdx.x;
+    const int row = h0.r0 + t;
+    const bool own = row < h0.r1;
+    const int rs = own ? rp[row] : 0, re = own ? rp[row + 1] : 0;  // issued before the barrier
+    decltype(pref(0)) q{};
+    if (own) q = pref(row);
+    u32x4 c[G];
+    TX v[G][4], g[G][4];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int qi = min(t + u * kBlock, nq - 1);  // clamped: every load issued; the entries are masked below
+        c[u] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(cs + a0) + qi);
+        Quad<TX, TV>::load(av, a0 + 4 * qi, v[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int k = a0 + 4 * (t + u * kBlock) + e;  // unclamped: k >= z1 for a clamped quad
+            const bool in = k >= z0 && k < z1;
+            g[u][e] = xf(cb + (in ? (int)(c[u][e] & 0xFFFFu) : 0));
+        }
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int k = a0 + 4 * (t + u * kBlock) + e;
+            if (k >= z0 && k < z1) prod[c[u][e] >> 16] = v[u][e] * g[u][e];
+        }
+    __syncthreads();
+    if (own) {
+        const TX* seg = prod + (rs - z0);
+        auto a = initf(q);
+        for (int k = 0; k < re - rs; ++k) a = accf(a, seg[k]);
+        finf(row, a, q);
+    }
+}
+
+template <bool XCD, int PER, typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_csr_stream_quad(const BlkHdr* __restrict__ hdr, const int* __restrict__ rp,
+                                                            const int* __restrict__ cv,
+                                                            const uint32_t* __restrict__ cs,
+                                                            const TV* __restrict__ av, XF xf, EPI epi) {
+    __shared__ TX prod[PER * kBlock];
+    const int b = XCD ? xcd_block() : (int)blockIdx.x;
+    auto pref = [&](int row) { return epi.pre(row); };
+    auto initf = [&](const typename EPI::Pre& q) { return epi.init(q); };
+    auto accf = [&](TX a, TX p) { return epi.acc(a, p); };
+    auto finf = [&](int row, TX a, const typename EPI::Pre& q) { epi.fin(row, a, q); };
+    stream_block_quad<PER, TV, TX, XF, decltype(pref), decltype(initf), decltype(accf), decltype(finf)>(
+        b, hdr, rp, cv, cs, av, xf, prod, pref, initf, accf, finf);
 }
 
 // Thread-per-row variant without LDS: fewer dependent round trips, for the
@@ -437,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_pipe(int nblk, const int* __rest
 // Software-pipelined CSR-stream over the block-header table, for the 16-bit
 // column form (C16) and its column-sorted copy (SRT): workgroup w walks a
 // contiguous run of row blocks (the runs of one XCD are adjacent); block b+1's
-// column, value and slot loads are issued right after block b's gathers, so
+// column, value and position loads are issued right after block b's gathers, so
 // they are in flight while block b's gathers return, its products are staged
 // and its rows are summed.  Block headers are read two blocks ahead.  Same
 // products, same LDS slots and the same sequential row sums as k_csr_stream,
@@ -448,7 +569,7 @@ template <bool XCD, int PER, bool SRT, typename TV, typename TX, class XF, class
 __global__ __launch_bounds__(kBlock) void k_stream_pipe(int nblk, const BlkHdr* __restrict__ hdr,
                                                         const int* __restrict__ rp, const int* __restrict__ cv,
                                                         const uint16_t* __restrict__ c16, const TV* __restrict__ av,
-                                                        XF xf, EPI epi, const uint16_t* __restrict__ slot) {
+                                                        XF xf, EPI epi, const uint32_t* __restrict__ cs) {
     constexpr int S = PER * kBlock;  // the LDS stage
     __shared__ TX prod[S];
     const int w = XCD ? xcd_block() : (int)blockIdx.x;
@@ -461,13 +582,22 @@ __global__ __launch_bounds__(kBlock) void k_stream_pipe(int nblk, const BlkHdr* 
     // loads of a block of 0 < nz <= PER * kBlock nonzeros (indices past its end
     // clamped to its last entry; their products are discarded)
     auto load = [&](const BlkHdr& h, int nz) {
-        if (h.cb >= 0) {
+        if (SRT && h.cb >= 0) {
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int k = min(t + u * kBlock, nz - 1);
+                const uint32_t e = ldu(cs, h.z0 + k);
+                cj[u] = h.cb + (int)(e & 0xFFFFu);
+                v[u] = ld<TX, TV>(av, h.z0 + k);
+                sl[u] = (int)(e >> 16);
+            }
+        } else if (h.cb >= 0) {
 #pragma unroll
             for (int u = 0; u < PER; ++u) {
                 const int k = min(t + u * kBlock, nz - 1);
                 cj[u] = h.cb + ldc16(c16, h.z0 + k);
                 v[u] = ld<TX, TV>(av, h.z0 + k);
-                sl[u] = SRT ? ldc16(slot, h.z0 + k) : t + u * kBlock;
+                sl[u] = t + u * kBlock;
             }
         } else {
 #pragma unroll
@@ -544,37 +674,37 @@ constexpr int kPipeGrid = 2048;
 
 template <int PER, bool SRT, typename TV, typename TX, class XF, class EPI>
 void launch_stream_pipe_per(int nblk, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16,
-                            const TV* av, XF xf, EPI epi, hipStream_t st, const uint16_t* slot) {
+                            const TV* av, XF xf, EPI epi, hipStream_t st, const uint32_t* cs) {
     const int grid = std::min(nblk, kPipeGrid);
     if (grid >= 64)
         hipLaunchKernelGGL((k_stream_pipe<true, PER, SRT, TV, TX, XF, EPI>), dim3(grid), dim3(kBlock), 0, st, nblk,
-                           hdr, rp, cv, c16, av, xf, epi, slot);
+                           hdr, rp, cv, c16, av, xf, epi, cs);
     else
         hipLaunchKernelGGL((k_stream_pipe<false, PER, SRT, TV, TX, XF, EPI>), dim3(grid), dim3(kBlock), 0, st,
-                           nblk, hdr, rp, cv, c16, av, xf, epi, slot);
+                           nblk, hdr, rp, cv, c16, av, xf, epi, cs);
 }
 
-// per = the operator's block budget / kBlock; slot non-null = the
-// column-sorted form (c16 and av sorted within each 16-bit block)
+// per = the operator's block budget / kBlock; cs non-null = the
+// column-sorted form (cs and av sorted within each 16-bit block)
 template <typename TV, typename TX, class XF, class EPI>
 void launch_stream_pipe(int nblk, int per, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16,
-                        const TV* av, XF xf, EPI epi, hipStream_t st, const uint16_t* slot) {
+                        const TV* av, XF xf, EPI epi, hipStream_t st, const uint32_t* cs) {
     if (nblk <= 0) return;
-    if (slot) {
+    if (cs) {
         switch (per) {
-            case 1: launch_stream_pipe_per<1, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-            case 2: launch_stream_pipe_per<2, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-            case 4: launch_stream_pipe_per<4, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-            case 8: launch_stream_pipe_per<8, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-            default: launch_stream_pipe_per<16, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+            case 1: launch_stream_pipe_per<1, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
+            case 2: launch_stream_pipe_per<2, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
+            case 4: launch_stream_pipe_per<4, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
+            case 8: launch_stream_pipe_per<8, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
+            default: launch_stream_pipe_per<16, true, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
         }
     } else {
         switch (per) {
-            case 1: launch_stream_pipe_per<1, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-            case 2: launch_stream_pipe_per<2, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-            case 4: launch_stream_pipe_per<4, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-            case 8: launch_stream_pipe_per<8, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-            default: launch_stream_pipe_per<16, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+            case 1: launch_stream_pipe_per<1, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
+            case 2: launch_stream_pipe_per<2, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
+            case 4: launch_stream_pipe_per<4, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
+            case 8: launch_stream_pipe_per<8, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
+            default: launch_stream_pipe_per<16, false, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
         }
     }
 }
@@ -605,43 +735,69 @@ void launch_pipe(int nblk, int per, const int* blk, const int* bz, const int* rp
 }
 
 enum OpKernel : int {
-    kOpStream = 0, kOpRowSeq = 1, kOpPipe = 2, kOpStream16 = 3, kOpSorted = 4, kOpStream16Pipe = 5, kOpSortedPipe = 6
+    kOpStream = 0, kOpRowSeq = 1, kOpPipe = 2, kOpStream16 = 3, kOpSorted = 4, kOpStream16Pipe = 5, kOpSortedPipe = 6,
+    kOpSortedQuad = 7
 };
-constexpr int kOpKinds = 7;
+constexpr int kOpKinds = 8;
 
 template <int PER, bool C16, bool SRT, typename TV, typename TX, class XF, class EPI>
 void launch_stream_per(int nblk, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16, const TV* av,
-                       XF xf, EPI epi, hipStream_t st, const uint16_t* slot) {
+                       XF xf, EPI epi, hipStream_t st, const uint32_t* cs) {
     if (nblk >= 64)
         hipLaunchKernelGGL((k_csr_stream<true, PER, C16, TV, TX, XF, EPI, SRT>), dim3(nblk), dim3(kBlock), 0, st, hdr,
-                           rp, cv, c16, av, xf, epi, slot);
+                           rp, cv, c16, av, xf, epi, cs);
     else
         hipLaunchKernelGGL((k_csr_stream<false, PER, C16, TV, TX, XF, EPI, SRT>), dim3(nblk), dim3(kBlock), 0, st,
-                           hdr, rp, cv, c16, av, xf, epi, slot);
+                           hdr, rp, cv, c16, av, xf, epi, cs);
 }
 
 template <bool C16, bool SRT, typename TV, typename TX, class XF, class EPI>
 void launch_stream_c(int nblk, int per, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16,
-                     const TV* av, XF xf, EPI epi, hipStream_t st, const uint16_t* slot) {
+                     const TV* av, XF xf, EPI epi, hipStream_t st, const uint32_t* cs) {
     switch (per) {
-        case 1: launch_stream_per<1, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-        case 2: launch_stream_per<2, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-        case 4: launch_stream_per<4, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-        case 8: launch_stream_per<8, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
-        default: launch_stream_per<16, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, slot); break;
+        case 1: launch_stream_per<1, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
+        case 2: launch_stream_per<2, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
+        case 4: launch_stream_per<4, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
+        case 8: launch_stream_per<8, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
+        default: launch_stream_per<16, C16, SRT, TV, TX>(nblk, hdr, rp, cv, c16, av, xf, epi, st, cs); break;
     }
 }
 
 // per = the operator's block budget / kBlock (1, 2, 4, 8 or 16); c16 non-null =
-// the 16-bit column form (bases in the block headers); slot non-null = its
-// column-sorted form (c16 and av sorted within each 16-bit block)
+// the 16-bit column form (bases in the block headers); cs non-null = its
+// column-sorted form (cs and av sorted within each 16-bit block)
 template <typename TV, typename TX, class XF, class EPI>
 void launch_stream(int nblk, int per, const BlkHdr* hdr, const int* rp, const int* cv, const TV* av, XF xf, EPI epi,
-                   hipStream_t st, const uint16_t* c16 = nullptr, const uint16_t* slot = nullptr) {
+                   hipStream_t st, const uint16_t* c16 = nullptr, const uint32_t* cs = nullptr) {
     if (nblk <= 0) return;
-    if (c16 && slot) launch_stream_c<true, true, TV, TX>(nblk, per, hdr, rp, cv, c16, av, xf, epi, st, slot);
+    if (cs) launch_stream_c<true, true, TV, TX>(nblk, per, hdr, rp, cv, nullptr, av, xf, epi, st, cs);
     else if (c16) launch_stream_c<true, false, TV, TX>(nblk, per, hdr, rp, cv, c16, av, xf, epi, st, nullptr);
     else launch_stream_c<false, false, TV, TX>(nblk, per, hdr, rp, cv, nullptr, av, xf, epi, st, nullptr);
+}
+
+// The column-sorted form with four entries per load (budgets of 4, 8 or 16
+// per thread; smaller budgets take the entry-by-entry kernel).
+template <int PER, typename TV, typename TX, class XF, class EPI>
+void launch_stream_quad_per(int nblk, const BlkHdr* hdr, const int* rp, const int* cv, const uint32_t* cs,
+                            const TV* av, XF xf, EPI epi, hipStream_t st) {
+    if (nblk >= 64)
+        hipLaunchKernelGGL((k_csr_stream_quad<true, PER, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, hdr, rp,
+                           cv, cs, av, xf, epi);
+    else
+        hipLaunchKernelGGL((k_csr_stream_quad<false, PER, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, hdr,
+                           rp, cv, cs, av, xf, epi);
+}
+
+template <typename TV, typename TX, class XF, class EPI>
+void launch_stream_quad(int nblk, int per, const BlkHdr* hdr, const int* rp, const int* cv, const uint32_t* cs,
+                        const TV* av, XF xf, EPI epi, hipStream_t st) {
+    if (nblk <= 0) return;
+    switch (per) {
+        case 4: launch_stream_quad_per<4, TV, TX>(nblk, hdr, rp, cv, cs, av, xf, epi, st); break;
+        case 8: launch_stream_quad_per<8, TV, TX>(nblk, hdr, rp, cv, cs, av, xf, epi, st); break;
+        case 16: launch_stream_quad_per<16, TV, TX>(nblk, hdr, rp, cv, cs, av, xf, epi, st); break;
+        default: launch_stream_c<true, true, TV, TX>(nblk, per, hdr, rp, cv, nullptr, av, xf, epi, st, cs); break;
+    }
 }
 
 template <typename TV, typename TX, class XF, class EPI>

@@ -191,6 +191,12 @@ class AMGHierarchy:
         check(_lib.lib().draugr_hip_level_op(self.handle, l, op, PREC.get(prec, prec), p(in1), p(in2), p(out),
                                              p(out2), _torch_stream()), "draugr_hip_level_op")
 
+    def level_op_repeat(self, l, op, prec, reps, in1=None, in2=None, out=None, out2=None):
+        """level_op `reps` times back to back on torch's current stream (measurement hook)."""
+        p = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        check(_lib.lib().draugr_hip_level_op_repeat(self.handle, l, op, PREC.get(prec, prec), p(in1), p(in2), p(out),
+                                                    p(out2), reps, _torch_stream()), "draugr_hip_level_op_repeat")
+
     def device_values(self, l, which, count):
         a = np.empty(count)
         check(_lib.lib().draugr_hip_export_device_values(self.handle, l, which, a.ctypes.data_as(ctypes.c_void_p)),

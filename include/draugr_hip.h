@@ -51,7 +51,7 @@ const char *draugr_amg_last_error(void);
  * "seed" (setup RNG seed), "graphs" (true: the PCG / mixed-precision preconditioner cycle is
  * captured once into a hipGraph and replayed; false: launched eagerly),
  * "kernel" ("auto" | "stream" | "rowseq" | "pipe" | "stream16" | "sorted" | "stream16pipe" |
- * "sortedpipe"), "fuse" ("auto"
+ * "sortedpipe" | "sortedquad"), "fuse" ("auto"
  * | "off" | "on": a level's first smoothing step from zero fused with its
  * residual, that step, residual and restriction in one pass, and its
  * prolongation and first post-smoothing step in one pass; Jacobi-type
@@ -176,6 +176,12 @@ int32_t draugr_hip_resetup_device(int32_t handle, int64_t nnz, const double *nzv
  *        form used above 256 rows; same factors) */
 int32_t draugr_hip_level_op(int32_t handle, int32_t level, int32_t op, int32_t prec, const void *in1,
                             const void *in2, void *out, void *out2, void *stream);
+
+/* draugr_hip_level_op `reps` times back to back on one stream (no host round
+ * trip between the launches): the per-launch time of a kernel as a cycle sees
+ * it, for measurement. */
+int32_t draugr_hip_level_op_repeat(int32_t handle, int32_t level, int32_t op, int32_t prec, const void *in1,
+                                   const void *in2, void *out, void *out2, int32_t reps, void *stream);
 
 /* Device-side FP64 values to the host: which 0 A.nzval, 1 smoother inverse
  * diagonal, 2 coarse LU factors (col-major), 3 coarsest operator values,

@@ -90,7 +90,7 @@ def test_long_row_level_kernels(dh, orc, torch_cuda, prec):
     rng = np.random.default_rng(SEED + 975)
     x, b = rng.standard_normal(A.n), rng.standard_normal(A.n)
     rp, cv, nz = A.rowptr, A.colval, A.nzval
-    for kernel in ("stream", "stream16", "sorted", "pipe", "rowseq", "stream16pipe", "sortedpipe"):
+    for kernel in ("stream", "stream16", "sorted", "pipe", "rowseq", "stream16pipe", "sortedpipe", "sortedquad"):
         cfg = dataclasses.replace(base, kernel=kernel)
         h = dh.amg_setup(A, cfg)
         oh = oracle_of(h, orc, "jacobi")

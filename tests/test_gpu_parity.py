@@ -372,7 +372,8 @@ def test_preconditioner_adapter_with_scipy_cg(dh, torch_cuda):
     assert info2 == 0
 
 
-@pytest.mark.parametrize("kernel", ["stream", "rowseq", "pipe", "stream16", "sorted", "stream16pipe", "sortedpipe"])
+@pytest.mark.parametrize("kernel", ["stream", "rowseq", "pipe", "stream16", "sorted", "stream16pipe", "sortedpipe",
+                                    "sortedquad"])
 @pytest.mark.parametrize("name", ["hypre3d", "agg2d", "wide2d", "hub2d"])
 def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
     """Each of the five sparse-pass kernels the autotuner chooses among,
