@@ -561,6 +561,23 @@ int32_t draugr_hip_cycle_bytes(int32_t handle, int32_t config_handle, double* by
     });
 }
 
+int32_t draugr_hip_precondition_repeat(int32_t handle, int32_t config_handle, const double* b, int32_t reps,
+                                       void* stream) {
+    return guarded([&]() -> int32_t {
+        auto H = get_hier(handle);
+        Config cfg;
+        if (!H || !get_cfg(config_handle, cfg)) invalid(handle, config_handle);
+        if (reps < 1 || !b) throw Error("ArgumentError: reps must be positive and b non-null");
+        std::lock_guard<std::mutex> g(H->mu);
+        require_device(*H);
+        hipStream_t st = stream_of(*H, stream);
+        H->dev->begin(st);
+        H->dev->precondition_repeat(b, opts_of(cfg), reps, st);
+        H->dev->end(st);
+        return 0;
+    });
+}
+
 int32_t draugr_hip_op_bytes(int32_t handle, int32_t level, int32_t which, int32_t precision, int32_t as_stored,
                             double* bytes) {
     return guarded([&]() -> int32_t {

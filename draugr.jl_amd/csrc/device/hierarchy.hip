@@ -1729,6 +1729,26 @@ void DeviceHierarchy::cycle(double* x, const double* b, bool zero, const CycleOp
     else launch_map(n, [=] __device__(int64_t i) { x[i] += (double)z32[i]; }, st);
 }
 
+void DeviceHierarchy::precondition_repeat(const double* b, const CycleOpts& o, int reps, hipStream_t st) {
+    if (nlevels() == 0) throw Error("precondition_repeat: the hierarchy has no AMG levels");
+    const int n = n0_;
+    if (o.prec == Precision::FP64) {
+        double* r = w_r_.p;
+        double* z = w_z_.p;
+        launch_map(n, [=] __device__(int64_t i) { r[i] = b[i]; }, st);
+        for (int k = 0; k < reps; ++k) precondition<double, double>(z, r, o, st);
+        return;
+    }
+    ensure_precision(o.prec);
+    float* r32 = w_r32_.p;
+    float* z32 = w_z32_.p;
+    launch_map(n, [=] __device__(int64_t i) { r32[i] = (float)b[i]; }, st);
+    for (int k = 0; k < reps; ++k) {
+        if (o.prec == Precision::FP32) precondition<float, float>(z32, r32, o, st);
+        else precondition<bf16_t, float>(z32, r32, o, st);
+    }
+}
+
 double DeviceHierarchy::reduce_to_host(hipStream_t st, int slot) { return reduce_to_host(st, scal_.p, slot); }
 
 double DeviceHierarchy::reduce_to_host(hipStream_t st, const double* scal, int slot) {

@@ -174,6 +174,10 @@ public:
     void export_level_values(int lvl, int which, double* host_dst);  // 0 A vals, 1 dinv, 2 coarse LU, 3 coarsest vals
     int64_t bytes_resident() const;
     double cycle_bytes(const CycleOpts& o) const;  // algorithmic bytes of one V/W cycle (zero guess)
+    // The PCG's preconditioner application (zero-guess cycle, replayed from its
+    // graph when o.graphs) `reps` times back to back on the PCG's own residual
+    // and correction vectors, the residual set from b first (measurement hook)
+    void precondition_repeat(const double* b, const CycleOpts& o, int reps, hipStream_t st);
     // bytes of one pass over operator which (0 A, 1 P, 2 P^T) of level l at
     // precision p: values, column indices, row pointers (vectors excluded);
     // as_stored: indices in the form the tuned kernel reads (16-bit where it

@@ -45,6 +45,7 @@ SIGNATURES = {
     "draugr_hip_resetup_device": (_i32, [_i32, _i64, _vp, _vp]),
     "draugr_hip_level_op": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     "draugr_hip_level_op_repeat": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp]),
+    "draugr_hip_precondition_repeat": (_i32, [_i32, _i32, _vp, _i32, _vp]),
     "draugr_hip_export_device_values": (_i32, [_i32, _i32, _i32, _vp]),
     "draugr_hip_cycle_bytes": (_i32, [_i32, _i32, _f64p]),
     "draugr_hip_op_bytes": (_i32, [_i32, _i32, _i32, _i32, _i32, _f64p]),

@@ -197,6 +197,14 @@ class AMGHierarchy:
         check(_lib.lib().draugr_hip_level_op_repeat(self.handle, l, op, PREC.get(prec, prec), p(in1), p(in2), p(out),
                                                     p(out2), reps, _torch_stream()), "draugr_hip_level_op_repeat")
 
+    def precondition_repeat(self, b, config: AMGConfig, reps: int):
+        """The PCG's preconditioner application ``reps`` times back to back on
+        torch's current stream, the residual set from the float64 CUDA tensor b
+        (measurement hook)."""
+        c = _Config(config)
+        check(_lib.lib().draugr_hip_precondition_repeat(self.handle, c.handle, _dptr(b, self.n, "b"), reps,
+                                                        _torch_stream()), "draugr_hip_precondition_repeat")
+
     def device_values(self, l, which, count):
         a = np.empty(count)
         check(_lib.lib().draugr_hip_export_device_values(self.handle, l, which, a.ctypes.data_as(ctypes.c_void_p)),

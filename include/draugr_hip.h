@@ -183,6 +183,15 @@ int32_t draugr_hip_level_op(int32_t handle, int32_t level, int32_t op, int32_t p
 int32_t draugr_hip_level_op_repeat(int32_t handle, int32_t level, int32_t op, int32_t prec, const void *in1,
                                    const void *in2, void *out, void *out2, int32_t reps, void *stream);
 
+/* The PCG's preconditioner application (the zero-guess cycle in the config's
+ * precision, replayed from its hipGraph when "graphs" is set) `reps` times
+ * back to back on the hierarchy's own residual and correction vectors, the
+ * residual set from the device vector b first: the cycle time a PCG iteration
+ * sees, for measurement (draugr_hip_cycle_device adds the caller-vector
+ * copies and conversions around it). */
+int32_t draugr_hip_precondition_repeat(int32_t handle, int32_t config_handle, const double *b, int32_t reps,
+                                       void *stream);
+
 /* Device-side FP64 values to the host: which 0 A.nzval, 1 smoother inverse
  * diagonal, 2 coarse LU factors (col-major), 3 coarsest operator values,
  * 4 Chebyshev bounds (lambda_min, lambda_max) of the level, 5 the level's
