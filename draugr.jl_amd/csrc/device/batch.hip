@@ -168,6 +168,96 @@ __global__ __launch_bounds__(kBlock) void k_kpassR(int n, int ks, const int* __r
     for (int u = 0; u < LR; ++u) epi.fin(i, j0 + u, a[u]);
 }
 
+// LDS-staged forms of the one and four rhs-per-thread mappings: the
+// workgroup's rows are contiguous, so their nonzeros are one range; its
+// columns and values are first copied into LDS with coalesced loads, then
+// every thread walks its row there.  In the plain kernels each load of a
+// column or value touches one cache line per row of the wavefront (the rows
+// are a row length apart), as many line requests as the gathers make; here
+// the range's lines are each requested once.  Same products, same row-order
+// sums.  cap = the most nonzeros of any workgroup's rows (the stage's size).
+constexpr int kStageBytes = 32 * 1024;
+
+template <typename TX>
+__device__ __forceinline__ int* stage_cols(unsigned char* base, int cap) {
+    return reinterpret_cast<int*>(base + (((size_t)cap * sizeof(TX) + 15) & ~(size_t)15));
+}
+
+template <typename TV, typename TX>
+__device__ __forceinline__ void stage_range(int z0, int nz, const int* __restrict__ cv, const TV* __restrict__ av,
+                                            TX* sv, int* sc) {
+#pragma unroll 4
+    for (int k = (int)threadIdx.x; k < nz; k += kBlock) {
+        sv[k] = ld<TX, TV>(av, z0 + k);
+        sc[k] = ldi(cv, z0 + k);
+    }
+}
+
+template <typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_kpass_s(int n, int ks, const int* __restrict__ rp,
+                                                    const int* __restrict__ cv, const TV* __restrict__ av, XF xf,
+                                                    EPI epi, int cap) {
+    extern __shared__ __align__(16) unsigned char kst[];
+    TX* sv = reinterpret_cast<TX*>(kst);
+    int* sc = stage_cols<TX>(kst, cap);
+    const int rows = kBlock >> ks;
+    const int b = gridDim.x >= 64 ? xcd_block() : (int)blockIdx.x;
+    const int i0 = b * rows;
+    const int i = i0 + ((int)threadIdx.x >> ks);
+    const int j = (int)threadIdx.x & ((1 << ks) - 1);
+    const int z0 = rp[i0];
+    const bool own = i < n;
+    const int rs = own ? rp[i] - z0 : 0, re = own ? rp[i + 1] - z0 : 0;
+    stage_range<TV, TX>(z0, rp[min(i0 + rows, n)] - z0, cv, av, sv, sc);
+    __syncthreads();
+    if (!own) return;
+    TX a = epi.init(i, j);
+#pragma unroll 4
+    for (int k = rs; k < re; ++k) a = epi.acc(a, sv[k] * xf(sc[k], j));
+    epi.fin(i, j, a);
+}
+
+template <typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_kpass4_s(int n, int ks, const int* __restrict__ rp,
+                                                     const int* __restrict__ cv, const TV* __restrict__ av, XF xf,
+                                                     EPI epi, int cap) {
+    extern __shared__ __align__(16) unsigned char kst[];
+    TX* sv = reinterpret_cast<TX*>(kst);
+    int* sc = stage_cols<TX>(kst, cap);
+    const int q = ks - 2;
+    const int rows = kBlock >> q;
+    const int b = gridDim.x >= 64 ? xcd_block() : (int)blockIdx.x;
+    const int i0 = b * rows;
+    const int i = i0 + ((int)threadIdx.x >> q);
+    const int j0 = ((int)threadIdx.x & ((1 << q) - 1)) << 2;
+    const int z0 = rp[i0];
+    const bool own = i < n;
+    const int rs = own ? rp[i] - z0 : 0, re = own ? rp[i + 1] - z0 : 0;
+    stage_range<TV, TX>(z0, rp[min(i0 + rows, n)] - z0, cv, av, sv, sc);
+    __syncthreads();
+    if (!own) return;
+    TX a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = epi.init(i, j0 + u);
+#pragma unroll 4
+    for (int k = rs; k < re; ++k) {
+        const TX v = sv[k];
+        TX g[4];
+        xf.get4(sc[k], j0, g);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = epi.acc(a[u], v * g[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) epi.fin(i, j0 + u, a[u]);
+}
+
+// The most nonzeros of `rows` consecutive rows starting at a multiple of rows.
+inline int32_t stage_cap(const std::vector<int32_t>& rp, int32_t nrow, int rows) {
+    int32_t c = 0;
+    for (int32_t i0 = 0; i0 < nrow; i0 += rows) c = std::max(c, rp[std::min(i0 + rows, nrow)] - rp[i0]);
+    return c;
+}
+
 // Right-hand sides per thread of a K-wide pass: rpt in {1, 4, 8, 16}, at
 // most K (the per-level choice of the batch's autotuner; every choice gives
 // the same values).
@@ -179,9 +269,29 @@ inline int rpt_fit(int rpt, int K) {
     return 1;
 }
 
+// rpt < 0: the LDS-staged kernel of mapping -rpt (1 or 4) when the stage
+// (caps[0] / caps[1] nonzeros) fits kStageBytes, else the plain kernel.
 template <typename TV, typename TX, class XF, class EPI>
-void kpass(int n, int ks, const int* rp, const int* cv, const TV* av, XF xf, EPI epi, hipStream_t st, int rpt = 4) {
+void kpass(int n, int ks, const int* rp, const int* cv, const TV* av, XF xf, EPI epi, hipStream_t st, int rpt = 4,
+           const int32_t* caps = nullptr) {
     if (n <= 0) return;
+    if (rpt < 0) {
+        rpt = -rpt;
+        const int r = rpt_fit(rpt, 1 << ks);
+        const int cap = caps && (r == 1 || r == 4) ? caps[r == 4 ? 1 : 0] : -1;
+        const size_t lds = cap >= 0 ? ((size_t)cap * sizeof(TX) + 15) / 16 * 16 + (size_t)cap * 4 : 0;
+        if (cap >= 0 && lds <= (size_t)kStageBytes) {
+            const int rows = r == 4 ? kBlock >> (ks - 2) : kBlock >> ks;
+            const int nb = (n + rows - 1) / rows;
+            if (r == 4)
+                hipLaunchKernelGGL((k_kpass4_s<TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), lds, st, n, ks, rp, cv, av,
+                                   xf, epi, cap);
+            else
+                hipLaunchKernelGGL((k_kpass_s<TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), lds, st, n, ks, rp, cv, av,
+                                   xf, epi, cap);
+            return;
+        }
+    }
     const int r = rpt_fit(rpt, 1 << ks);
     const int lr = r == 16 ? 4 : (r == 8 ? 3 : (r == 4 ? 2 : 0));
     const int rows = kBlock >> (ks - lr);
@@ -308,14 +418,18 @@ __global__ __launch_bounds__(kBlock) void k_kpq(int ks, const int* __restrict__ 
 // K >= 4: the same pass with one thread per (row, four consecutive rhs) and
 // vector gathers.  Each (row, rhs) product sum is formed as in k_kpq; the
 // block's partial p.q per rhs is summed in another (fixed) order.
-template <typename TZ>
+// STAGE: the block's nonzero range is first copied into LDS (cap entries;
+// the staged pass kernels' reason), and the row walks read it there.
+template <typename TZ, bool STAGE>
 __global__ __launch_bounds__(kBlock) void k_kpq4(int ks, const int* __restrict__ br0, const int* __restrict__ br1,
                                                  const int* __restrict__ bsys, const int* __restrict__ rp,
                                                  const int* __restrict__ cv, const double* __restrict__ av,
                                                  const TZ* __restrict__ z, const double* __restrict__ pa,
                                                  double* __restrict__ pb, double* __restrict__ q,
-                                                 const double* __restrict__ beta, double* __restrict__ part) {
+                                                 const double* __restrict__ beta, double* __restrict__ part,
+                                                 int cap) {
     __shared__ double sh[kBlock * 4];
+    extern __shared__ __align__(16) unsigned char kst[];
     const int qs = ks - 2;                   // log2(threads per row)
     const int lanes = kBlock >> qs;          // rows in flight per block
     const int K = 1 << ks;
@@ -329,6 +443,18 @@ __global__ __launch_bounds__(kBlock) void k_kpq4(int ks, const int* __restrict__
         bt[u] = beta[kidx(s, j0 + u, ks)];
         dot[u] = 0.0;
     }
+    const double* sv = av;  // the values and columns the row walks read (LDS copies when STAGE)
+    const int* sc = cv;
+    int zs = 0;
+    if constexpr (STAGE) {
+        double* tv = reinterpret_cast<double*>(kst);
+        int* tc = stage_cols<double>(kst, cap);
+        zs = rp[br0[b]];
+        stage_range<double, double>(zs, rp[r1] - zs, cv, av, tv, tc);
+        __syncthreads();
+        sv = tv;
+        sc = tc;
+    }
     for (int i = br0[b] + lr; i < r1; i += lanes) {
         const size_t ii = kidx(i, j0, ks);
         TZ zi[4];
@@ -340,10 +466,10 @@ __global__ __launch_bounds__(kBlock) void k_kpq4(int ks, const int* __restrict__
             pn[u] = (double)zi[u] + bt[u] * pi4[u];
             a[u] = 0.0;
         }
-        const int e = rp[i + 1];
-        for (int k = rp[i]; k < e; ++k) {
-            const size_t cc = kidx(cv[k], j0, ks);
-            const double v = av[k];
+        const int e = rp[i + 1] - zs;
+        for (int k = rp[i] - zs; k < e; ++k) {
+            const size_t cc = kidx(sc[k], j0, ks);
+            const double v = sv[k];
             TZ zc[4];
             double pc[4];
             ld4(z + cc, zc);
@@ -596,6 +722,7 @@ BatchedHierarchy::BatchedHierarchy(const std::vector<const HostHierarchy*>& hs, 
     }
     hipStream_t st = nullptr;
     // packed operators
+    std::vector<int32_t> rp0;  // level 0's row offsets (the PCG pass's stage size)
     for (int l = 0; l < lmax_; ++l) {
         PLevel& D = L_[l];
         std::vector<int32_t> arp{0}, acv, prp{0}, pcv, trp{0}, tcv;
@@ -637,10 +764,13 @@ BatchedHierarchy::BatchedHierarchy(const std::vector<const HostHierarchy*>& hs, 
             M.rp.upload(rp, st);
             M.cv.upload(cv, st);
             M.v64.upload(nz, st);
+            M.cap[0] = stage_cap(rp, nrow, kBlock >> ks_);
+            M.cap[1] = ks_ >= 2 ? stage_cap(rp, nrow, kBlock >> (ks_ - 2)) : -1;
             DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
             upload_cast(M.v32, M.v64, M.nnz, st);
         };
         up(D.A, D.nA, D.nV, arp, acv, anz);
+        if (l == 0) rp0 = arp;
         up(D.P, D.nA, L_[l + 1].nV, prp, pcv, pnz);
         up(D.Pt, L_[l + 1].nV, D.nA, trp, tcv, tnz);
         D.d64.alloc(D.nA);
@@ -685,6 +815,10 @@ BatchedHierarchy::BatchedHierarchy(const std::vector<const HostHierarchy*>& hs, 
             sb.push_back((int32_t)r0.size());
         }
         nblk0_ = (int)r0.size();
+        // the most nonzeros of one reduction block's rows (-1: no staging)
+        pq_cap_ = rp0.empty() ? -1 : 0;
+        for (size_t k = 0; k < r0.size() && pq_cap_ >= 0; ++k)
+            pq_cap_ = r1[k] < (int32_t)rp0.size() ? std::max(pq_cap_, rp0[r1[k]] - rp0[r0[k]]) : -1;
         br0_.upload(r0, st);
         br1_.upload(r1, st);
         bs_.upload(bs, st);
@@ -774,9 +908,9 @@ void BatchedHierarchy::cycle_t(TX* f0, const TX* b0, hipStream_t st) {
         const TX* d = db(D);
         const int* rs = D.rpt[pi];
         kpass<TX, TX>(D.nA, ks, D.A.rp.p, D.A.cv.p, vals<TX>(D.A.v64, D.A.v32), KXZero<TX>{d, b, w, ks},
-                      KZeroJacobiResid<TX>{b, d, xb(D), rb(D), w, ks}, st, rs[0]);
+                      KZeroJacobiResid<TX>{b, d, xb(D), rb(D), w, ks}, st, rs[0], D.A.cap);
         kpass<TX, TX>(C.nV, ks, D.Pt.rp.p, D.Pt.cv.p, vals<TX>(D.Pt.v64, D.Pt.v32), KX<TX>{rb(D), ks},
-                      KStore<TX>{bb(C), ks}, st, rs[1]);
+                      KStore<TX>{bb(C), ks}, st, rs[1], D.Pt.cap);
         coarse(C, bb(C), fb(C));
     }
     for (int l = lmax_ - 1; l >= 0; --l) {
@@ -786,9 +920,9 @@ void BatchedHierarchy::cycle_t(TX* f0, const TX* b0, hipStream_t st) {
         TX* f = l == 0 ? f0 : fb(D);
         const int* rs = D.rpt[pi];
         kpass<TX, TX>(D.nA, ks, D.P.rp.p, D.P.cv.p, vals<TX>(D.P.v64, D.P.v32), KX<TX>{fb(C), ks},
-                      KAdd<TX>{xb(D), ks}, st, rs[2]);
+                      KAdd<TX>{xb(D), ks}, st, rs[2], D.P.cap);
         kpass<TX, TX>(D.nA, ks, D.A.rp.p, D.A.cv.p, vals<TX>(D.A.v64, D.A.v32), KX<TX>{xb(D), ks},
-                      KJacobi<TX>{xb(D), b, db(D), f, w, ks}, st, rs[3]);
+                      KJacobi<TX>{xb(D), b, db(D), f, w, ks}, st, rs[3], D.A.cap);
     }
 }
 
@@ -802,9 +936,10 @@ template <typename TX>
 void BatchedHierarchy::tune() {
     const int pi = sizeof(TX) == 8 ? 0 : 1;
     if (tuned_[pi]) return;
-    if (forced_rpt_ > 0) {
+    if (forced_rpt_ != 0) {  // negative: the LDS-staged kernel of that mapping
         for (auto& D : L_)
-            for (int k = 0; k < 4; ++k) D.rpt[pi][k] = rpt_fit(forced_rpt_, K_);
+            for (int k = 0; k < 4; ++k)
+                D.rpt[pi][k] = forced_rpt_ > 0 ? rpt_fit(forced_rpt_, K_) : -rpt_fit(-forced_rpt_, K_);
         tuned_[pi] = true;
         return;
     }
@@ -835,7 +970,7 @@ void BatchedHierarchy::tune() {
     auto bb = [](PLevel& D) -> TX* { if constexpr (sizeof(TX) == 8) return D.b64.p; else return D.b32.p; };
     auto fb = [](PLevel& D) -> TX* { if constexpr (sizeof(TX) == 8) return D.f64.p; else return D.f32.p; };
     auto db = [](PLevel& D) -> const TX* { if constexpr (sizeof(TX) == 8) return D.d64.p; else return D.d32.p; };
-    const int cands[4] = {1, 4, 8, 16};
+    const int cands[6] = {1, 4, 8, 16, -1, -4};  // negative: the LDS-staged kernel of that mapping
     for (int l = 0; l < lmax_; ++l) {
         PLevel& D = L_[l];
         PLevel& C = L_[l + 1];
@@ -851,23 +986,23 @@ void BatchedHierarchy::tune() {
         const TX* d = db(D);
         float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
         for (int c : cands) {
-            if (c > K_) continue;
+            if (std::abs(c) > K_) continue;
             const float t[4] = {
                 timeit([&] {
                     kpass<TX, TX>(D.nA, ks, D.A.rp.p, D.A.cv.p, vals<TX>(D.A.v64, D.A.v32), KXZero<TX>{d, r, w, ks},
-                                  KZeroJacobiResid<TX>{r, d, x, x, w, ks}, st, c);
+                                  KZeroJacobiResid<TX>{r, d, x, x, w, ks}, st, c, D.A.cap);
                 }),
                 timeit([&] {
                     kpass<TX, TX>(C.nV, ks, D.Pt.rp.p, D.Pt.cv.p, vals<TX>(D.Pt.v64, D.Pt.v32), KX<TX>{r, ks},
-                                  KStore<TX>{bc, ks}, st, c);
+                                  KStore<TX>{bc, ks}, st, c, D.Pt.cap);
                 }),
                 timeit([&] {
                     kpass<TX, TX>(D.nA, ks, D.P.rp.p, D.P.cv.p, vals<TX>(D.P.v64, D.P.v32), KX<TX>{fc, ks},
-                                  KAdd<TX>{x, ks}, st, c);
+                                  KAdd<TX>{x, ks}, st, c, D.P.cap);
                 }),
                 timeit([&] {
                     kpass<TX, TX>(D.nA, ks, D.A.rp.p, D.A.cv.p, vals<TX>(D.A.v64, D.A.v32), KX<TX>{r, ks},
-                                  KJacobi<TX>{r, r, d, x, w, ks}, st, c);
+                                  KJacobi<TX>{r, r, d, x, w, ks}, st, c, D.A.cap);
                 })};
             for (int k = 0; k < 4; ++k)
                 if (t[k] < best[k]) {
@@ -898,13 +1033,13 @@ void BatchedHierarchy::jacobi_pass(int l, Precision p, const void* x, const void
         kpass<double, double>(D.nA, ks, D.A.rp.p, D.A.cv.p, D.A.v64.p, KX<double>{(const double*)x, ks},
                               KJacobi<double>{(const double*)x, (const double*)b, D.d64.p, (double*)out,
                                               (double)omega_, ks},
-                              st, D.rpt[0][3]);
+                              st, D.rpt[0][3], D.A.cap);
     } else {
         ensure_vectors<float>();
         tune<float>();
         kpass<float, float>(D.nA, ks, D.A.rp.p, D.A.cv.p, D.A.v32.p, KX<float>{(const float*)x, ks},
                             KJacobi<float>{(const float*)x, (const float*)b, D.d32.p, (float*)out, (float)omega_, ks},
-                            st, D.rpt[1][3]);
+                            st, D.rpt[1][3], D.A.cap);
     }
 }
 
@@ -926,11 +1061,14 @@ std::string BatchedHierarchy::tune_report() const {
         if (!tuned_[p]) continue;
         for (int l = 0; l < lmax_; ++l) {
             const PLevel& D = L_[l];
+            auto m = [&](int k) {  // "4" or "4s" (LDS-staged)
+                return std::to_string(std::abs(D.rpt[p][k])) + (D.rpt[p][k] < 0 ? "s" : "");
+            };
             std::snprintf(buf, sizeof buf,
-                          "%s level %d (%d rows x %d rhs): rhs/thread zero-step %d, restrict %d, prolong %d, "
-                          "jacobi %d (%.1f / %.1f / %.1f / %.1f us)\n",
-                          pn[p], l, D.nA, K_, D.rpt[p][0], D.rpt[p][1], D.rpt[p][2], D.rpt[p][3], D.rpt_us[p][0],
-                          D.rpt_us[p][1], D.rpt_us[p][2], D.rpt_us[p][3]);
+                          "%s level %d (%d rows x %d rhs): rhs/thread zero-step %s, restrict %s, prolong %s, "
+                          "jacobi %s (%.1f / %.1f / %.1f / %.1f us)\n",
+                          pn[p], l, D.nA, K_, m(0).c_str(), m(1).c_str(), m(2).c_str(), m(3).c_str(),
+                          D.rpt_us[p][0], D.rpt_us[p][1], D.rpt_us[p][2], D.rpt_us[p][3]);
             out += buf;
         }
     }
@@ -1066,15 +1204,23 @@ int BatchedHierarchy::pcg(double* x, const double* b, double tol, int maxiter, c
         rz(F_RZ0, st);
         // the two graphs of an iteration (the first in two forms: p ping-pongs)
         auto step1 = [&](double* pold, double* pnew, hipStream_t s) {
-            if (mixed && ks >= 2)
-                hipLaunchKernelGGL(k_kpq4<float>, dim3(nb), dim3(kBlock), 0, s, ks, br0_.p, br1_.p, bs_.p,
-                                   D0.A.rp.p, D0.A.cv.p, D0.A.v64.p, z32, pold, pnew, q, st_beta_.p, part);
+            const size_t lds = pq_cap_ >= 0 ? ((size_t)pq_cap_ * 8 + 15) / 16 * 16 + (size_t)pq_cap_ * 4 : 0;
+            const bool stage = pq_cap_ >= 0 && lds <= (size_t)kStageBytes;
+            if (mixed && ks >= 2 && stage)
+                hipLaunchKernelGGL((k_kpq4<float, true>), dim3(nb), dim3(kBlock), lds, s, ks, br0_.p, br1_.p, bs_.p,
+                                   D0.A.rp.p, D0.A.cv.p, D0.A.v64.p, z32, pold, pnew, q, st_beta_.p, part, pq_cap_);
+            else if (mixed && ks >= 2)
+                hipLaunchKernelGGL((k_kpq4<float, false>), dim3(nb), dim3(kBlock), 0, s, ks, br0_.p, br1_.p, bs_.p,
+                                   D0.A.rp.p, D0.A.cv.p, D0.A.v64.p, z32, pold, pnew, q, st_beta_.p, part, 0);
             else if (mixed)
                 hipLaunchKernelGGL(k_kpq<float>, dim3(nb), dim3(kBlock), 0, s, ks, br0_.p, br1_.p, bs_.p, D0.A.rp.p,
                                    D0.A.cv.p, D0.A.v64.p, z32, pold, pnew, q, st_beta_.p, part);
+            else if (ks >= 2 && stage)
+                hipLaunchKernelGGL((k_kpq4<double, true>), dim3(nb), dim3(kBlock), lds, s, ks, br0_.p, br1_.p, bs_.p,
+                                   D0.A.rp.p, D0.A.cv.p, D0.A.v64.p, z, pold, pnew, q, st_beta_.p, part, pq_cap_);
             else if (ks >= 2)
-                hipLaunchKernelGGL(k_kpq4<double>, dim3(nb), dim3(kBlock), 0, s, ks, br0_.p, br1_.p, bs_.p,
-                                   D0.A.rp.p, D0.A.cv.p, D0.A.v64.p, z, pold, pnew, q, st_beta_.p, part);
+                hipLaunchKernelGGL((k_kpq4<double, false>), dim3(nb), dim3(kBlock), 0, s, ks, br0_.p, br1_.p, bs_.p,
+                                   D0.A.rp.p, D0.A.cv.p, D0.A.v64.p, z, pold, pnew, q, st_beta_.p, part, 0);
             else
                 hipLaunchKernelGGL(k_kpq<double>, dim3(nb), dim3(kBlock), 0, s, ks, br0_.p, br1_.p, bs_.p,
                                    D0.A.rp.p, D0.A.cv.p, D0.A.v64.p, z, pold, pnew, q, st_beta_.p, part);

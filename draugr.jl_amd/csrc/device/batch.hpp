@@ -69,6 +69,10 @@ private:
         DBuf<int32_t> rp, cv;
         DBuf<double> v64;
         DBuf<float> v32;
+        // the most nonzeros of one workgroup's rows under the one (cap[0]) and
+        // four (cap[1]; -1 for fewer than 4 rhs) rhs-per-thread mappings: the
+        // LDS stage of the staged pass kernels
+        int32_t cap[2] = {-1, -1};
     };
     struct PLevel {
         int32_t nA = 0;   // rows of the systems that smooth on this level (a prefix)
@@ -82,7 +86,8 @@ private:
         DBuf<float> x32, r32, b32, f32;
         // right-hand sides per thread of the zero-guess step, restriction,
         // prolongation and Jacobi passes, per precision (the autotuner's
-        // choice) and the time it measured for each
+        // choice; negative: the LDS-staged kernel of that mapping) and the
+        // time it measured for each
         int rpt[2][4] = {{4, 4, 4, 4}, {4, 4, 4, 4}};
         float rpt_us[2][4] = {};
     };
@@ -107,6 +112,7 @@ private:
     bool have_vec_[2] = {false, false};
     // level-0 blocks for the PCG reductions: never straddle two systems
     int nblk0_ = 0;
+    int32_t pq_cap_ = -1;  // the most nonzeros of one reduction block's rows of level 0 (-1: not staged)
     DBuf<int32_t> br0_, br1_, bs_, sysb_;
     // PCG state (per system and rhs) and workspaces
     DBuf<double> st_bb_, st_rr_, st_rz_, st_pq_, st_alpha_, st_beta_, st_tol_, part_;

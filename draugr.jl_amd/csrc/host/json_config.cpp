@@ -273,8 +273,8 @@ Config config_from_json(const char* json) {
     if (const JVal* v = get("batch_rhs_per_thread")) {
         cfg.batch_rpt = to_int(*v);
         if (!(cfg.batch_rpt == 0 || cfg.batch_rpt == 1 || cfg.batch_rpt == 4 || cfg.batch_rpt == 8 ||
-              cfg.batch_rpt == 16))
-            throw Error("batch_rhs_per_thread must be 0 (autotuned), 1, 4, 8 or 16");
+              cfg.batch_rpt == 16 || cfg.batch_rpt == -1 || cfg.batch_rpt == -4))
+            throw Error("batch_rhs_per_thread must be 0 (autotuned), 1, 4, 8, 16, -1 or -4 (LDS-staged 1 or 4)");
     }
     if (const JVal* v = get("gpu_setup")) cfg.gpu_setup = to_bool(*v);
     if (const JVal* v = get("fuse")) {

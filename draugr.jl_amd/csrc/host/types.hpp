@@ -85,7 +85,8 @@ struct Config {
     bool gpu_setup = true;                  // interpolation, coarse operators and triple maps built on the GPU
     int gpu_setup_min_rows = 32768;         // levels below this size interpolate on the host (faster there)
     bool gs_symmetric = false;              // coloured GS post-smoothing sweeps the colours in reverse order
-    int batch_rpt = 0;                      // batched passes: right-hand sides per thread (0: autotuned; 1, 4, 8, 16)
+    int batch_rpt = 0;                      // batched passes: right-hand sides per thread (0: autotuned; 1, 4, 8, 16;
+                                            // -1, -4: the LDS-staged kernels of 1 and 4)
     int tail = -1;                          // small levels + coarse solve in one launch: -1 autotuned, 0 never, 1 deepest
 };
 

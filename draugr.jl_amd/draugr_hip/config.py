@@ -177,7 +177,7 @@ class AMGConfig:
     gpu_setup: bool = True  # interpolation, coarse operators, triple maps and PMIS sweeps of the setup on the GPU
     gpu_setup_min_rows: int = 32768  # smaller levels interpolate on the host
     gs_symmetric: bool = False  # coloured GS post-sweeps in reverse colour order (a symmetric cycle, for PCG)
-    batch_rhs_per_thread: int = 0  # batched passes: right-hand sides per thread (0 autotuned, 1, 4, 8, 16)
+    batch_rhs_per_thread: int = 0  # batched passes: rhs per thread (0 autotuned, 1, 4, 8, 16; -1, -4 LDS-staged 1, 4)
     tail: str = "auto"  # small levels and the coarse solve in one launch: "auto" (autotuned), "off", "on" (deepest)
 
     def __post_init__(self):

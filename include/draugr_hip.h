@@ -63,7 +63,8 @@ const char *draugr_amg_last_error(void);
  * interpolate and sweep on the host),
  * "gs_symmetric" (coloured GS post-smoothing sweeps the colours in reverse
  * order: a symmetric cycle, which PCG requires of GS), "batch_rhs_per_thread"
- * (0 = autotuned, 1, 4, 8 or 16: mapping of the batched passes),
+ * (0 = autotuned, 1, 4, 8 or 16: mapping of the batched passes; -1 or -4: the
+ * LDS-staged kernels of mappings 1 and 4),
  * "filtering", "filter_tol".  Returns a handle > 0, or -1.
  * Replaces c_api/draugr_amg.h:94. */
 int32_t draugr_amg_config_from_json(const char *json_str);

@@ -277,11 +277,12 @@ def test_batch_setup_with_gpu_setup_hooks_concurrent(dh, torch_cuda):
             assert np.array_equal(got[s][:, j], xj.cpu().numpy()), (s, j)
 
 
-@pytest.mark.parametrize("rpt", [1, 4, 8, 16])
+@pytest.mark.parametrize("rpt", [1, 4, 8, 16, -1, -4])
 def test_every_rhs_per_thread_mapping_is_bit_exact(dh, torch_cuda, rpt):
     """Each mapping of the K-wide passes the batch autotuner chooses among
-    (1, 4, 8 or 16 right-hand sides per thread), forced on every pass, gives
-    the single-system cycles bit for bit at K = 16 (FP64 and FP32)."""
+    (1, 4, 8 or 16 right-hand sides per thread; -1 and -4 the LDS-staged
+    kernels of 1 and 4), forced on every pass, gives the single-system cycles
+    bit for bit at K = 16 (FP64 and FP32)."""
     torch = torch_cuda
     mats, cfg, batch, singles = _setup(dh, [9, 14, 11], 16, max_coarse_size=50, batch_rhs_per_thread=rpt)
     for prec in ("fp64", "fp32"):
@@ -299,7 +300,8 @@ def test_every_rhs_per_thread_mapping_is_bit_exact(dh, torch_cuda, rpt):
                 dh.amg_cycle_(xj, bj, h, c, zero_guess=True)
                 torch.cuda.synchronize()
                 assert np.array_equal(got[s][:, j], xj.cpu().numpy()), (rpt, prec, s, j)
-    assert ("%d, restrict %d" % (rpt, rpt)) in batch.tune_report()
+    tag = "%d%s" % (abs(rpt), "s" if rpt < 0 else "")
+    assert ("%s, restrict %s" % (tag, tag)) in batch.tune_report()
 
 
 def test_batched_jacobi_pass_equals_single_level_ops(dh, torch_cuda):
