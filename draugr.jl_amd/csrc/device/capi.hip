@@ -437,7 +437,8 @@ int32_t draugr_hip_level_info(int32_t handle, int32_t level, int64_t* out, int32
         const HostLevel& L = H->host.levels[level];
         const int64_t v[] = {L.A.nrow, L.A.nnz(), L.P.ncol, L.P.nnz(), L.R.ntriples(), L.coloring.num_colors,
                              L.has_rmap ? 1 : 0, H->dev ? H->dev->level_blocks(level) : 0,
-                             H->dev && H->dev->level_has16(level) ? 1 : 0};
+                             H->dev && H->dev->level_has16(level) ? 1 : 0,
+                             H->dev ? H->dev->level_nnz_sorted(level) : 0};
         const int32_t m = std::min<int32_t>(nout, (int32_t)(sizeof v / sizeof v[0]));
         for (int32_t i = 0; i < m; ++i) out[i] = v[i];
         return m;

@@ -121,8 +121,8 @@ template <typename TV, typename TX, class XF, class EPI, class FIN>
 void launch_stream_reduce(const DevCSR& M, const TV* av, XF xf, EPI epi, FIN fin, double* partial, hipStream_t st,
                           bool sorted = false) {
 #define DRAUGR_SR(PER, C, S, COLS, VALS, CS)                                                                   \
-    hipLaunchKernelGGL((k_stream_reduce<PER, C, S, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.hdr.p,  \
-                       M.rp.p, M.cv.p, COLS, VALS, xf, epi, partial, CS)
+    hipLaunchKernelGGL((k_stream_reduce<PER, C, S, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st,         \
+                       S ? M.shdr.p : M.hdr.p, M.rp.p, M.cv.p, COLS, VALS, xf, epi, partial, CS)
     if (sorted && M.hasS) {
         const TV* sv = reinterpret_cast<const TV*>(M.sv64.p);
         switch (M.per) {
@@ -560,14 +560,14 @@ void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
     else if (kind == kOpStream16 && M.has16)
         launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st, M.c16.p);
     else if (kind == kOpSorted && M.hasS && svals<TV>(M))
-        launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, svals<TV>(M), xf, epi, st, nullptr, M.scs.p);
+        launch_stream<TV, TX>(M.nblk, M.per, M.shdr.p, M.rp.p, M.cv.p, svals<TV>(M), xf, epi, st, nullptr, M.scs.p);
     else if (kind == kOpStream16Pipe && M.has16)
         launch_stream_pipe<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, M.c16.p, vals<TV>(M), xf, epi, st, nullptr);
     else if (kind == kOpSortedPipe && M.hasS && svals<TV>(M))
-        launch_stream_pipe<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, nullptr, svals<TV>(M), xf, epi, st,
+        launch_stream_pipe<TV, TX>(M.nblk, M.per, M.shdr.p, M.rp.p, M.cv.p, nullptr, svals<TV>(M), xf, epi, st,
                                    M.scs.p);
     else if (kind == kOpSortedQuad && M.hasS && svals<TV>(M))
-        launch_stream_quad<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, M.scs.p, svals<TV>(M), xf, epi, st);
+        launch_stream_quad<TV, TX>(M.nblk, M.per, M.shdr.p, M.rp.p, M.cv.p, M.scs.p, svals<TV>(M), xf, epi, st);
     else launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
 }
 
@@ -578,29 +578,32 @@ void op(const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
     op_kind<TV, TX>(M.kind[pidx<TV>()], M, xf, epi, st);
 }
 
-// Column-sorted form of the 16-bit blocks: a bitonic sort in LDS of the keys
-// (column offset << 12 | position in the block), unique per entry, giving
-// cs = column offset | position << 16 and the values in that order; blocks
-// without a base keep row order (cs = position << 16, unread: those blocks
-// take their columns from the 32-bit array).  The positions of a 16-bit block
-// are below its budget (<= kStreamNnz entries), so 12 bits hold them.
-__global__ __launch_bounds__(kBlock) void k_sort_blocks(const BlkHdr* __restrict__ hdr, const uint16_t* __restrict__ c16,
+// Column-sorted form: a bitonic sort in LDS of each block's keys (column
+// offset << 12 | position in the block), unique per entry, giving cs = column
+// offset | position << kSortColBits and the values in that order; blocks
+// without a sorted base keep row order (cs = position << kSortColBits, unread:
+// those blocks take their columns from the 32-bit array).  A sorted block
+// holds at most its budget (<= kStreamNnz = 2^12 entries) and spans fewer than
+// 2^kSortColBits columns, so a key fits 32 bits.
+static_assert(kSortColBits + 12 == 32, "column offset and position share one word");
+__global__ __launch_bounds__(kBlock) void k_sort_blocks(const BlkHdr* __restrict__ shdr, const int* __restrict__ cv,
                                                         const double* __restrict__ v, uint32_t* __restrict__ cs,
                                                         double* __restrict__ sv) {
     __shared__ uint32_t key[kStreamNnz];
     const int b = blockIdx.x;
-    const int z0 = hdr[b].z0, nz = hdr[b + 1].z0 - z0, cb = hdr[b].cb;
+    const int z0 = shdr[b].z0, nz = shdr[b + 1].z0 - z0, cb = shdr[b].cb;
     const int t = (int)threadIdx.x;
     if (cb < 0 || nz > kStreamNnz) {
         for (int k = t; k < nz; k += kBlock) {
-            cs[z0 + k] = (uint32_t)(k & 0xFFFF) << 16;
+            cs[z0 + k] = (uint32_t)(k & 0xFFF) << kSortColBits;
             sv[z0 + k] = v[z0 + k];
         }
         return;
     }
     int m = 1;
     while (m < nz) m <<= 1;
-    for (int k = t; k < m; k += kBlock) key[k] = k < nz ? ((uint32_t)c16[z0 + k] << 12) | (uint32_t)k : 0xFFFFFFFFu;
+    for (int k = t; k < m; k += kBlock)
+        key[k] = k < nz ? ((uint32_t)(cv[z0 + k] - cb) << 12) | (uint32_t)k : 0xFFFFFFFFu;
     __syncthreads();
     for (int size = 2; size <= m; size <<= 1)
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -619,23 +622,23 @@ __global__ __launch_bounds__(kBlock) void k_sort_blocks(const BlkHdr* __restrict
     for (int k = t; k < nz; k += kBlock) {
         const uint32_t kk = key[k];
         const int pos = (int)(kk & 0xFFFu);
-        cs[z0 + k] = (kk >> 12) | ((uint32_t)pos << 16);
+        cs[z0 + k] = (kk >> 12) | ((uint32_t)pos << kSortColBits);
         sv[z0 + k] = v[z0 + pos];
     }
 }
 
 // sv = the values in the sorted form's order (after new values in v); blocks
 // without a base (row order, possibly one row longer than 65536 entries) copy
-__global__ __launch_bounds__(kBlock) void k_permute_blocks(const BlkHdr* __restrict__ hdr,
+__global__ __launch_bounds__(kBlock) void k_permute_blocks(const BlkHdr* __restrict__ shdr,
                                                            const uint32_t* __restrict__ cs,
                                                            const double* __restrict__ v, double* __restrict__ sv) {
     const int b = blockIdx.x;
-    const int z0 = hdr[b].z0, nz = hdr[b + 1].z0 - z0;
-    if (hdr[b].cb < 0) {
+    const int z0 = shdr[b].z0, nz = shdr[b + 1].z0 - z0;
+    if (shdr[b].cb < 0) {
         for (int k = (int)threadIdx.x; k < nz; k += kBlock) sv[z0 + k] = v[z0 + k];
         return;
     }
-    for (int k = (int)threadIdx.x; k < nz; k += kBlock) sv[z0 + k] = v[z0 + (int)(cs[z0 + k] >> 16)];
+    for (int k = (int)threadIdx.x; k < nz; k += kBlock) sv[z0 + k] = v[z0 + (int)(cs[z0 + k] >> kSortColBits)];
 }
 
 // Entries allocated for an array of the column-sorted form: a multiple of 4,
@@ -690,13 +693,33 @@ void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
     for (int b = 0; b <= d.nblk; ++b)
         hdr[b] = BlkHdr{blk[b], bz[b], fits ? cbase[b] : -1, blk[std::min(b + 1, d.nblk)]};
     d.hdr.upload(hdr, st);
-    d.hasS = fits;
-    if (fits) {
+    // the column-sorted form: every branch-free block spanning fewer than
+    // 2^kSortColBits columns, with its own headers
+    std::vector<BlkHdr> shdr(d.nblk + 1);
+    d.nnzS = 0;
+    for (int b = 0; b <= d.nblk; ++b) {
+        int32_t base = -1;
+        if (b < d.nblk) {
+            const int z0 = bz[b], z1 = bz[b + 1];
+            if (z1 > z0 && z1 - z0 <= budget) {
+                const auto mm = std::minmax_element(h.cv.begin() + z0, h.cv.begin() + z1);
+                if ((int64_t)*mm.second - *mm.first < ((int64_t)1 << kSortColBits)) {
+                    base = *mm.first;
+                    d.nnzS += z1 - z0;
+                }
+            }
+        }
+        shdr[b] = BlkHdr{blk[b], bz[b], base, blk[std::min(b + 1, d.nblk)]};
+    }
+    d.hasS = d.nnzS > 0;
+    if (d.hasS) {
+        d.shdr.upload(shdr, st);
         d.scs.alloc(sorted_alloc(d.nnz));
         d.sv64.alloc(sorted_alloc(d.nnz));
-        hipLaunchKernelGGL(k_sort_blocks, dim3(d.nblk), dim3(kBlock), 0, st, d.hdr.p, d.c16.p, d.v64.p, d.scs.p,
+        hipLaunchKernelGGL(k_sort_blocks, dim3(d.nblk), dim3(kBlock), 0, st, d.shdr.p, d.cv.p, d.v64.p, d.scs.p,
                            d.sv64.p);
     } else {
+        d.shdr.release();
         d.scs.release();
         d.sv64.release();
     }
@@ -851,7 +874,7 @@ int64_t DeviceHierarchy::bytes_resident() const {
     int64_t b = 0;
     auto csr = [&](const DevCSR& m) {
         b += m.rp.bytes() + m.cv.bytes() + m.v64.bytes() + m.v32.bytes() + m.v16.bytes() + m.blk.bytes();
-        b += m.c16.bytes() + m.hdr.bytes() + m.scs.bytes() + m.sv64.bytes() + m.sv32.bytes() +
+        b += m.c16.bytes() + m.hdr.bytes() + m.shdr.bytes() + m.scs.bytes() + m.sv64.bytes() + m.sv32.bytes() +
              m.sv16.bytes();
     };
     for (auto& l : L_) {
@@ -983,7 +1006,7 @@ static void refresh_mirrors(DevCSR& M, hipStream_t st) {
         launch_map(M.nnz, [=] __device__(int64_t i) { d[i] = float_to_bf16((float)s[i]); }, st);
     }
     if (M.hasS && M.nblk > 0) {
-        hipLaunchKernelGGL(k_permute_blocks, dim3(M.nblk), dim3(kBlock), 0, st, M.hdr.p, M.scs.p, M.v64.p, M.sv64.p);
+        hipLaunchKernelGGL(k_permute_blocks, dim3(M.nblk), dim3(kBlock), 0, st, M.shdr.p, M.scs.p, M.v64.p, M.sv64.p);
         const double* ss = M.sv64.p;
         if (M.sv32.p) {
             float* d = M.sv32.p;
@@ -2128,8 +2151,8 @@ static double index_bytes(const DevCSR& M, int pi) {
     if ((M.kind[pi] == kOpStream16 || M.kind[pi] == kOpStream16Pipe) && M.has16)
         return 2.0 * M.nnz16 + 4.0 * (M.nnz - M.nnz16) + 16.0 * M.nblk;  // + the block headers
     if ((M.kind[pi] == kOpSorted || M.kind[pi] == kOpSortedPipe || M.kind[pi] == kOpSortedQuad) &&
-        M.hasS)  // + the position of each 16-bit entry
-        return 4.0 * M.nnz16 + 4.0 * (M.nnz - M.nnz16) + 16.0 * M.nblk;
+        M.hasS)  // one column / position word per sorted entry, 32-bit columns elsewhere
+        return 4.0 * M.nnzS + 4.0 * (M.nnz - M.nnzS) + 16.0 * M.nblk;
     return 4.0 * M.nnz;
 }
 

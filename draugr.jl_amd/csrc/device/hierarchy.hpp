@@ -39,12 +39,16 @@ struct DevCSR {
     int64_t nnz16 = 0;        // nonzeros of the blocks read through 16-bit columns
     DBuf<uint16_t> c16;       // column - (the block's base) per nonzero
     DBuf<dev::BlkHdr> hdr;    // per block: first row, first nonzero, 16-bit column base (-1: 32-bit columns)
-    // Column-sorted form of the 16-bit blocks (kernel kinds sorted,
-    // sortedpipe, sortedquad): the values in column order and per entry one
-    // word, column offset | the entry's position in the block's row order << 16;
-    // blocks without a base keep row order.  Allocated to a multiple of 4
-    // entries (four-entry loads).
+    // Column-sorted form (kernel kinds sorted, sortedpipe, sortedquad) of the
+    // branch-free blocks spanning fewer than 2^20 columns: its own block
+    // headers (base = the block's first column, -1 for the other blocks, which
+    // keep row order and read the 32-bit columns), the values in column order
+    // and per entry one word, column offset | the entry's position in the
+    // block's row order << 20.  Allocated to a multiple of 4 entries
+    // (four-entry loads).
     bool hasS = false;
+    int64_t nnzS = 0;          // nonzeros of the blocks that have the sorted form
+    DBuf<dev::BlkHdr> shdr;
     DBuf<uint32_t> scs;
     DBuf<double> sv64;
     DBuf<float> sv32;
@@ -141,6 +145,7 @@ public:
     int nlevels() const { return (int)L_.size(); }
     int level_blocks(int l) const { return l < nlevels() ? L_[l].A.nblk : Ac_.nblk; }
     bool level_has16(int l) const { return l < nlevels() ? L_[l].A.has16 : Ac_.has16; }
+    int64_t level_nnz_sorted(int l) const { return l < nlevels() ? L_[l].A.nnzS : Ac_.nnzS; }
     hipStream_t stream() const { return stream_; }
 
     // One V/W cycle on device vectors x (in/out) and b; zero_guess asserts x == 0

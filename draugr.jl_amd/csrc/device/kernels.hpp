@@ -32,6 +32,11 @@ namespace dev {
 constexpr int kBlock = 256;
 constexpr int kStreamNnz = 4096;    // products per row block at most (16 per thread)
 constexpr int kStreamRows = kBlock; // one epilogue thread per row
+// Column-sorted form: one 32-bit word per entry, the column offset from the
+// block's base in the low kSortColBits bits and the entry's position in the
+// block's row order (< kStreamNnz = 2^12) above them.
+constexpr int kSortColBits = 20;
+constexpr uint32_t kSortColMask = (1u << kSortColBits) - 1u;
 
 typedef uint16_t bf16_t;
 
@@ -229,13 +234,15 @@ struct alignas(16) BlkHdr {
 // fewer than 65536 columns (cb >= 0); wider and longer blocks read the 32-bit
 // columns.
 //
-// SRT (with C16): the entries of each such block are stored sorted by column
-// (av in that order) with one 32-bit word per entry, cs[k] = the 16-bit column
-// offset | the entry's position in the block's row order (where its product is
-// staged) << 16, so column and position arrive with one load.  A wave's gathers then touch a few
-// adjacent cache lines instead of the lines of every column cluster of its
-// rows; the staged products, and so the row sums, are the same.  Other blocks
-// keep row order (av in row order, columns from cv).
+// SRT (launched with the column-sorted form's block headers, whose base cb is
+// set for the branch-free blocks spanning fewer than 2^kSortColBits columns):
+// the entries of each such block are stored sorted by column (av in that
+// order) with one 32-bit word per entry, cs[k] = the column offset | the
+// entry's position in the block's row order (where its product is staged)
+// << kSortColBits, so column and position arrive with one load.  A wave's
+// gathers then touch a few adjacent cache lines instead of the lines of every
+// column cluster of its rows; the staged products, and so the row sums, are
+// the same.  Other blocks keep row order (av in row order, columns from cv).
 template <int PER, bool C16, typename TV, typename TX, class XF, class PREF, class INITF, class ACCF, class FINF,
           bool SRT = false>
 __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ hdr, const int* __restrict__ rp,
@@ -264,9 +271,9 @@ __device__ __forceinline__ void stream_block(int b, const BlkHdr* __restrict__ h
                     for (int u = 0; u < PER; ++u) {
                         const int k = min(t + u * kBlock, nz - 1);
                         const uint32_t e = ldu(cs, z0 + k);
-                        cj[u] = cb + (int)(e & 0xFFFFu);
+                        cj[u] = cb + (int)(e & kSortColMask);
                         v[u] = ld<TX, TV>(av, z0 + k);
-                        sl[u] = (int)(e >> 16);
+                        sl[u] = (int)(e >> kSortColBits);
                     }
                 } else if (C16 && cb >= 0) {
 #pragma unroll
@@ -418,14 +425,14 @@ __device__ __forceinline__ void stream_block_quad(int b, const BlkHdr* __restric
         for (int e = 0; e < 4; ++e) {
             const int k = a0 + 4 * (t + u * kBlock) + e;  // unclamped: k >= z1 for a clamped quad
             const bool in = k >= z0 && k < z1;
-            g[u][e] = xf(cb + (in ? (int)(c[u][e] & 0xFFFFu) : 0));
+            g[u][e] = xf(cb + (in ? (int)(c[u][e] & kSortColMask) : 0));
         }
 #pragma unroll
     for (int u = 0; u < G; ++u)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int k = a0 + 4 * (t + u * kBlock) + e;
-            if (k >= z0 && k < z1) prod[c[u][e] >> 16] = v[u][e] * g[u][e];
+            if (k >= z0 && k < z1) prod[c[u][e] >> kSortColBits] = v[u][e] * g[u][e];
         }
     __syncthreads();
     if (own) {
@@ -587,9 +594,9 @@ __global__ __launch_bounds__(kBlock) void k_stream_pipe(int nblk, const BlkHdr* 
             for (int u = 0; u < PER; ++u) {
                 const int k = min(t + u * kBlock, nz - 1);
                 const uint32_t e = ldu(cs, h.z0 + k);
-                cj[u] = h.cb + (int)(e & 0xFFFFu);
+                cj[u] = h.cb + (int)(e & kSortColMask);
                 v[u] = ld<TX, TV>(av, h.z0 + k);
-                sl[u] = (int)(e >> 16);
+                sl[u] = (int)(e >> kSortColBits);
             }
         } else if (h.cb >= 0) {
 #pragma unroll

@@ -106,9 +106,9 @@ class AMGHierarchy:
         return self.info()["nlevels"]
 
     def level_info(self, l: int) -> dict:
-        out = (ctypes.c_int64 * 9)()
-        m = check(_lib.lib().draugr_hip_level_info(self.handle, l, out, 9), "draugr_hip_level_info")
-        keys = ["n", "nnz", "n_coarse", "nnz_P", "ntriples", "num_colors", "has_rmap", "nblk", "has16"]
+        out = (ctypes.c_int64 * 10)()
+        m = check(_lib.lib().draugr_hip_level_info(self.handle, l, out, 10), "draugr_hip_level_info")
+        keys = ["n", "nnz", "n_coarse", "nnz_P", "ntriples", "num_colors", "has_rmap", "nblk", "has16", "nnz_sorted"]
         return dict(zip(keys, [int(out[i]) for i in range(m)]))
 
     def _export(self, l, which, count, dtype):

@@ -116,7 +116,8 @@ int32_t draugr_hip_hierarchy_info(int32_t handle, int64_t *out, int32_t nout);
 
 /* out[0..]: n, nnz(A), n_coarse, nnz(P), ntriples, num_colors, has_rmap,
  * row blocks of A on the device (0 before upload), A stored with 16-bit
- * block-relative columns on the device (0/1). */
+ * block-relative columns on the device (0/1), nonzeros of A in the
+ * column-sorted form on the device. */
 int32_t draugr_hip_level_info(int32_t handle, int32_t level, int64_t *out, int32_t nout);
 
 /* Copy a host array of level `level` (0-based) into dst, 0-based indices:

@@ -8,6 +8,9 @@
   through a whole V-cycle, the oracle's cycle;
 * rows longer than a row block's LDS stage (hub rows), summed chunk by chunk
   in row order: level kernels bit for bit against the oracle;
+* row blocks spanning more than 65,536 columns (no 16-bit form) but fewer
+  than 2^20: the column-sorted kernels read them through 20-bit offsets;
+  level kernels bit for bit against the oracle;
 * the stationary solve's non-convergence semantics (src/cycle.jl:135-159):
   a non-finite residual never passes the test, the iteration count returned
   is maxiter and no error is raised.
@@ -207,3 +210,50 @@ def test_tail_in_one_launch_is_bit_exact(dh, orc, torch_cuda, name):
             same(z.cpu().numpy(), ref)
         else:
             assert rel_err(z.cpu().numpy(), ref) < 1e-5
+
+
+def _wide_band(dh, n, w):
+    """SPD, diagonally dominant: 4 on the diagonal, -1 at i +- 1 and i +- w.
+    With w = 70,001 every row block spans about 2 w columns."""
+    import scipy.sparse as sp
+    off = [np.full(n - 1, -1.0), np.full(n - 1, -1.0), np.full(n - w, -1.0), np.full(n - w, -1.0)]
+    B = sp.diags([np.full(n, 4.0)] + off, [0, 1, -1, w, -w], shape=(n, n), format="csr")
+    B.sort_indices()
+    return dh.as_csr(B)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_wide_blocks_column_sorted_kernels(dh, orc, torch_cuda, prec):
+    """A 500,000-row band matrix whose row blocks span ~140,000 columns: no
+    block has the 16-bit form, every one the column-sorted form (20-bit
+    offsets).  SpMV, residual, Jacobi and the fused zero-guess step under the
+    three column-sorted kernel kinds (budget 1024: the four-entry kernel runs)
+    are the oracle's bit for bit in FP64 (1e-5 in FP32)."""
+    import dataclasses
+    torch = torch_cuda
+    A = _wide_band(dh, 500_000, 70_001)
+    base = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType())
+    tdt = torch.float64 if prec == "fp64" else torch.float32
+    rng = np.random.default_rng(SEED + 985)
+    x, b = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    rp, cv, nz = A.rowptr, A.colval, A.nzval
+    for kernel in ("sorted", "sortedquad", "sortedpipe"):
+        cfg = dataclasses.replace(base, kernel=kernel)
+        h = dh.amg_setup(A, cfg)
+        li = h.level_info(0)
+        assert li["has16"] == 0 and li["nnz_sorted"] == A.nnz, li
+        oh = oracle_of(h, orc, "jacobi")
+        d = oh.dinv(0)
+        dev = lambda a: torch.tensor(a, dtype=tdt, device="cuda")  # noqa: E731
+        out, out2 = torch.zeros(A.n, dtype=tdt, device="cuda"), torch.zeros(A.n, dtype=tdt, device="cuda")
+        print("wide blocks", kernel, prec, flush=True)
+        h.level_op(0, 0, prec, dev(x), None, out)
+        same(out.cpu().numpy(), orc.spmv(rp, cv, nz, x), prec)
+        h.level_op(0, 1, prec, dev(x), dev(b), out)
+        same(out.cpu().numpy(), orc.residual(rp, cv, nz, x, b), prec)
+        h.level_op(0, 2, prec, dev(x), dev(b), out)
+        same(out.cpu().numpy(), orc.jacobi_step(rp, cv, nz, x, b, d, cfg.jacobi_omega), prec)
+        h.level_op(0, 3, prec, None, dev(b), out, out2)
+        x1 = orc.jacobi_step(rp, cv, nz, np.zeros(A.n), b, d, cfg.jacobi_omega)
+        same(out.cpu().numpy(), x1, prec)
+        same(out2.cpu().numpy(), orc.residual(rp, cv, nz, x1, b), prec)
