@@ -15,7 +15,8 @@ median of full solves) ride along in the line.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--precision fp32|bf16]
        python bench.py --kernel-only   (setup, a 256 MiB calibration copy and the
                                         dominant-kernel launches only: the run the
-                                        rocprofv3 --pmc passes profile)
+                                        rocprofv3 --pmc passes profile; with
+                                        --workload batched48 the batched pass)
 """
 import argparse
 import json
@@ -282,6 +283,18 @@ def workload_batched48(args, torch, dh):
     info = batch.info()
     log("batch: %d systems, %d rows, %d rhs, deepest %d levels, setup %.1fs, resident %.1f GB"
         % (info["nsys"], info["rows"], K, info["max_levels"], t_setup, info["resident_bytes"] / 1e9))
+    if args.kernel_only:  # the calibration copy and the dominant pass only (for counter runs)
+        li = batch.level_info(0)
+        xa = torch.randn(li["n_vec"] * K, dtype=torch.float32, device="cuda")
+        ba = torch.randn(li["n_vec"] * K, dtype=torch.float32, device="cuda")
+        oa = torch.empty_like(xa)
+        batch.jacobi_pass(0, "fp32", xa, ba, oa)  # tunes the FP32 mappings
+        torch.cuda.synchronize()
+        calibration_copy(torch)
+        t_k = event_time(torch, lambda: batch.jacobi_pass(0, "fp32", xa, ba, oa), 10)
+        print(json.dumps({"kernel": "batched48_level0_jacobi_fp32", "avg_launch_us": t_k * 1e6,
+                          "mapping": batch.tune_report().splitlines()[0] if batch.tune_report() else ""}))
+        return
     b = torch.tensor(P.splitmix_normal(SEED + 14, info["rows"] * K), device="cuda")
     out = {"workload": "batched48", "systems": 48, "rhs": K, "rows": info["rows"],
            "edges": edges, "setup_s": t_setup}
@@ -318,7 +331,9 @@ def workload_batched48(args, torch, dh):
     t_k = min(event_time(torch, lambda: batch.jacobi_pass(0, "fp32", xa, ba, oa), 20) for _ in range(3))
     kbytes = nnz0 * (4 + 4) + (n0 + 1) * 4 + n0 * 4 + 3 * K * n0 * 4  # A, d, then x, b, out (K wide)
     out["roofline"] = {"bound": "hbm", "achieved": kbytes / t_k / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": kbytes / t_k / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                       "frac": kbytes / t_k / 1e9 / HBM_PEAK_GBS,
+                       "traffic": measured_traffic(os.path.join(ROOT, "profiles", "batched48_kernel_stats.md"),
+                                                   "batched48_level0_jacobi_fp32"),
                        "kernel": "K-wide l1-Jacobi step on packed level 0 (%d rows x %d rhs, %d nnz), FP32"
                                  % (n0, K, nnz0),
                        "bytes_per_launch": kbytes, "avg_launch_us": t_k * 1e6}
