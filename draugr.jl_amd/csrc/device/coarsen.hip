@@ -87,6 +87,12 @@ void pmis_sweeps_device(std::vector<int8_t>& cf, int32_t n, const int64_t* off, 
     DBuf<uint8_t> d_s;
     DBuf<double> d_meas;
     DBuf<int8_t> d_cf, d_nxt;
+    int* pinned = nullptr;  // the sweep's flag, read by the host each pass
+    DRAUGR_HIP_CHECK(hipHostMalloc((void**)&pinned, sizeof(int), hipHostMallocDefault));
+    struct PinnedOwner {
+        int* p;
+        ~PinnedOwner() { (void)hipHostFree(p); }
+    } pinned_owner{pinned};
     struct SyncOnExit {  // runs before the buffers above are freed, also when a step throws
         hipStream_t s;
         ~SyncOnExit() { (void)hipStreamSynchronize(s); }
@@ -99,12 +105,6 @@ void pmis_sweeps_device(std::vector<int8_t>& cf, int32_t n, const int64_t* off, 
     d_nxt.upload(cf, st);
     d_any.alloc(1);
     const uint8_t* sp = s ? d_s.p : nullptr;
-    int* pinned = nullptr;
-    DRAUGR_HIP_CHECK(hipHostMalloc((void**)&pinned, sizeof(int), hipHostMallocDefault));
-    struct PinnedOwner {
-        int* p;
-        ~PinnedOwner() { (void)hipHostFree(p); }
-    } pinned_owner{pinned};
     for (int64_t iter = 0; iter <= (int64_t)n; ++iter) {
         hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, st, d_any.p);
         hipLaunchKernelGGL(k_pmis_select, dim3(grid_of(n)), dim3(kCB), 0, st, n, d_off.p, d_adj.p, sp, d_meas.p,
