@@ -568,6 +568,10 @@ void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
                                    M.scs.p);
     else if (kind == kOpSortedQuad && M.hasS && svals<TV>(M))
         launch_stream_quad<TV, TX>(M.nblk, M.per, M.shdr.p, M.rp.p, M.cv.p, M.scs.p, svals<TV>(M), xf, epi, st);
+    else if (kind == kOpVector && sizeof(TX) == 4) {
+        if constexpr (sizeof(TX) == 4)  // FP32 / BF16 only: not the reference's sequential sum order
+            launch_vector<TV, TX>(M.nrow, M.nnz, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
+    }
     else launch_stream<TV, TX>(M.nblk, M.per, M.hdr.p, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
 }
 
@@ -1414,7 +1418,8 @@ void DeviceHierarchy::tune() {
             const float skip = 1e30f;  // the 16-bit and sorted forms only where the operator has them
             auto absent = [&](const DevCSR& M) {
                 return ((k == kOpStream16 || k == kOpStream16Pipe) && !M.has16) ||
-                       ((k == kOpSorted || k == kOpSortedPipe || k == kOpSortedQuad) && !M.hasS);
+                       ((k == kOpSorted || k == kOpSortedPipe || k == kOpSortedQuad) && !M.hasS) ||
+                       (k == kOpVector && sizeof(TX) == 8);  // FP64 keeps the sequential row sums
             };
             tA[k] = absent(D.A) ? skip : timeit([&] {
                 op_kind<TV, TX>(k, D.A, XVec<TX>{xin}, EpiJacobi<TX>{xin, bin, d, s1, w}, st);
@@ -1483,8 +1488,8 @@ std::string DeviceHierarchy::tune_report() const {
         if (!tuned_[p]) continue;
         for (int l = 0; l < nlevels(); ++l) {
             const DevLevel& D = L_[l];
-            const char* kn[kOpKinds] = {"stream",       "rowseq",     "pipe",      "stream16",
-                                        "sorted",       "stream16pipe", "sortedpipe", "sortedquad"};
+            const char* kn[kOpKinds] = {"stream",       "rowseq",     "pipe",       "stream16",  "sorted",
+                                        "stream16pipe", "sortedpipe", "sortedquad", "vector"};
             std::snprintf(buf, sizeof buf, "%s level %d: A=%s Pt=%s P=%s zero-guess=%s down=%s up=%s\n", names[p], l,
                           kn[D.A.kind[p]], kn[D.Pt.kind[p]], kn[D.P.kind[p]], D.fuse_zero[p] ? "fused" : "split",
                           D.fuse_down[p] ? "one-pass" : (D.down_state < 0 ? "separate(only)" : "separate"),

@@ -458,6 +458,66 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream_quad(const BlkHdr* __rest
         b, hdr, rp, cv, cs, av, xf, prod, pref, initf, accf, finf);
 }
 
+// Vector CSR (FP32 / BF16 cycles only): LPR consecutive lanes per row, lane l
+// summing the row's products l, l + LPR, ... with the epilogue's own
+// accumulation from zero, the LPR partial sums combined by xor shuffles, then
+// a = init(q) + sum and fin -- LPR chosen by the operator's average row
+// length (about four products per lane; 64 = a wavefront per row).  No LDS
+// stage and no barrier: three dependent round trips (row bounds, columns and
+// values, gathers).  The sum order differs from the reference's sequential
+// one, so the FP64 cycle never uses this kernel (its kind falls back to the
+// stream kernel there) and FP32 / BF16 results under it are within rounding,
+// not bit-equal, to the other kinds'.  Every epilogue accumulates a +- p, so
+// splitting the sum over lanes is valid.
+template <int LPR, typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_csr_vector(int nrows, const int* __restrict__ rp,
+                                                       const int* __restrict__ cv, const TV* __restrict__ av, XF xf,
+                                                       EPI epi) {
+    const int t = (int)threadIdx.x;
+    const int lane = t & (LPR - 1);
+    const int b = gridDim.x >= 64 ? xcd_block() : (int)blockIdx.x;
+    const int row = b * (kBlock / LPR) + t / LPR;
+    const bool own = row < nrows;
+    typename EPI::Pre q{};
+    if (own && lane == 0) q = epi.pre(row);
+    TX sum = 0;
+    if (own) {
+        const int re = rp[row + 1];
+#pragma unroll 4
+        for (int k = rp[row] + lane; k < re; k += LPR) sum = epi.acc(sum, ld<TX, TV>(av, k) * xf(ldi(cv, k)));
+    }
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, LPR);  // every lane takes part
+    if (own && lane == 0) {
+        auto a = epi.init(q);
+        a = a + sum;
+        epi.fin(row, a, q);
+    }
+}
+
+// Lanes per row of the vector CSR kernel: about four products per lane.
+inline int vector_lanes(int64_t nnz, int64_t nrow) {
+    const double avg = nrow > 0 ? (double)nnz / (double)nrow : 0.0;
+    int l = 4;
+    while (l < 64 && 4.0 * l < avg) l <<= 1;
+    return l;
+}
+
+template <typename TV, typename TX, class XF, class EPI>
+void launch_vector(int nrows, int64_t nnz, const int* rp, const int* cv, const TV* av, XF xf, EPI epi,
+                   hipStream_t st) {
+    if (nrows <= 0) return;
+    const int lpr = vector_lanes(nnz, nrows);
+    const int nb = (int)(((int64_t)nrows * lpr + kBlock - 1) / kBlock);
+    switch (lpr) {
+        case 4: hipLaunchKernelGGL((k_csr_vector<4, TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, nrows, rp, cv, av, xf, epi); break;
+        case 8: hipLaunchKernelGGL((k_csr_vector<8, TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, nrows, rp, cv, av, xf, epi); break;
+        case 16: hipLaunchKernelGGL((k_csr_vector<16, TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, nrows, rp, cv, av, xf, epi); break;
+        case 32: hipLaunchKernelGGL((k_csr_vector<32, TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, nrows, rp, cv, av, xf, epi); break;
+        default: hipLaunchKernelGGL((k_csr_vector<64, TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, nrows, rp, cv, av, xf, epi); break;
+    }
+}
+
 // Thread-per-row variant without LDS: fewer dependent round trips, for the
 // small coarse levels where a kernel's latency, not bandwidth, is the cost.
 // Same per-row operation order as the stream kernel.
@@ -743,9 +803,9 @@ void launch_pipe(int nblk, int per, const int* blk, const int* bz, const int* rp
 
 enum OpKernel : int {
     kOpStream = 0, kOpRowSeq = 1, kOpPipe = 2, kOpStream16 = 3, kOpSorted = 4, kOpStream16Pipe = 5, kOpSortedPipe = 6,
-    kOpSortedQuad = 7
+    kOpSortedQuad = 7, kOpVector = 8
 };
-constexpr int kOpKinds = 8;
+constexpr int kOpKinds = 9;
 
 template <int PER, bool C16, bool SRT, typename TV, typename TX, class XF, class EPI>
 void launch_stream_per(int nblk, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16, const TV* av,

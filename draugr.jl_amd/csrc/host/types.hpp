@@ -81,7 +81,8 @@ struct Config {
     int fuse = -1;                          // zero-guess step + residual, one-pass down / up sweeps: -1 autotuned, 0 never, 1 always
     int kernel = -1;                        // sparse-pass kernel: -1 autotuned, 0 stream, 1 row-sequential, 2 pipelined,
                                             // 3 stream with 16-bit columns, 4 column-sorted, 5 pipelined 16-bit,
-                                            // 6 pipelined column-sorted, 7 column-sorted with four-entry loads
+                                            // 6 pipelined column-sorted, 7 column-sorted with four-entry loads,
+                                            // 8 vector CSR (FP32 / BF16 only)
     bool gpu_setup = true;                  // interpolation, coarse operators and triple maps built on the GPU
     int gpu_setup_min_rows = 32768;         // levels below this size interpolate on the host (faster there)
     bool gs_symmetric = false;              // coloured GS post-smoothing sweeps the colours in reverse order

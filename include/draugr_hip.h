@@ -51,7 +51,8 @@ const char *draugr_amg_last_error(void);
  * "seed" (setup RNG seed), "graphs" (true: the PCG / mixed-precision preconditioner cycle is
  * captured once into a hipGraph and replayed; false: launched eagerly),
  * "kernel" ("auto" | "stream" | "rowseq" | "pipe" | "stream16" | "sorted" | "stream16pipe" |
- * "sortedpipe" | "sortedquad"), "fuse" ("auto"
+ * "sortedpipe" | "sortedquad" | "vector" (lanes per row by row length; FP32 / BF16
+ * cycles only, FP64 passes take "stream")), "fuse" ("auto"
  * | "off" | "on": a level's first smoothing step from zero fused with its
  * residual, that step, residual and restriction in one pass, and its
  * prolongation and first post-smoothing step in one pass; Jacobi-type

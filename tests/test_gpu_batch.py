@@ -46,7 +46,10 @@ def _setup(dh, edges, K, **kw):
         mats = _systems(dh, edges)
         cfg = _cfg(dh, **kw)
         batch = dh.amg_batch_setup(mats, cfg, nrhs=K)
-        singles = [dh.amg_setup(A, cfg) for A in mats]
+        # the single-system references sum every row in row order (the stream
+        # kernel): the batched passes do, while the single-system autotuner may
+        # pick the vector CSR kernel for FP32 passes, which sums in another order
+        singles = [dh.amg_setup(A, dataclasses.replace(cfg, kernel="stream")) for A in mats]
         _cache[key] = (mats, cfg, batch, singles)
     return _cache[key]
 

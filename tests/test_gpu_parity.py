@@ -426,6 +426,46 @@ def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
     assert out[0][0] == out[1][0] and out[0][1] == out[1][1] and np.array_equal(out[0][2], out[1][2])
 
 
+@pytest.mark.parametrize("name", ["hypre3d", "agg2d", "hub2d"])
+def test_vector_kernel(dh, orc, torch_cuda, name):
+    """The vector CSR kernel (lanes per row chosen by row length, partial sums
+    combined by shuffles; FP32 / BF16 only): forced on every level, the FP64
+    cycle keeps the stream kernel and is the oracle's bit for bit; the level
+    kernels in FP32 are within 1e-5 of the oracle's; the FP32 and BF16 cycles
+    are within 1e-5 / 1e-2 of the FP64 cycle; their PCG reaches 1e-8 with at
+    most one iteration more than the FP64 cycle's."""
+    import dataclasses
+    torch = torch_cuda
+    A, cfg, sm = _case(dh, name)
+    c = dataclasses.replace(cfg, kernel="vector")
+    h = dh.amg_setup(A, c)
+    oh = oracle_of(h, orc, sm)
+    b = dh.problems.splitmix_normal(SEED + 990, A.n)
+    x0 = dh.problems.splitmix_normal(SEED + 991, A.n)
+    x = torch.tensor(x0, device="cuda")
+    dh.amg_cycle_(x, torch.tensor(b, device="cuda"), h, c)
+    ref = oh.cycle(x0, b)
+    assert np.array_equal(x.cpu().numpy(), ref)
+    rp, cv, nz = A.rowptr, A.colval, A.nzval
+    d = oh.dinv(0)
+    f32 = lambda a: torch.tensor(a, dtype=torch.float32, device="cuda")  # noqa: E731
+    out = torch.zeros(A.n, dtype=torch.float32, device="cuda")
+    h.level_op(0, 1, "fp32", f32(x0), f32(b), out)
+    same(out.cpu().numpy(), orc.residual(rp, cv, nz, x0, b), "fp32")
+    h.level_op(0, 2, "fp32", f32(x0), f32(b), out)
+    same(out.cpu().numpy(), orc.jacobi_step(rp, cv, nz, x0, b, d, cfg.jacobi_omega), "fp32")
+    z64 = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+    _, it64, _ = dh.pcg_solve_(z64, torch.tensor(b, device="cuda"), h, c, tol=1e-8, maxiter=400)
+    for prec, tol in (("fp32", 1e-5), ("bf16", 1e-2)):
+        cp = dataclasses.replace(c, precision=prec)
+        x = torch.tensor(x0, device="cuda")
+        dh.amg_cycle_(x, torch.tensor(b, device="cuda"), h, cp)
+        assert rel_err(x.cpu().numpy(), ref) < tol, prec
+        z = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+        _, it, rel = dh.pcg_solve_(z, torch.tensor(b, device="cuda"), h, cp, tol=1e-8, maxiter=400)
+        assert rel < 1e-8 and it <= it64 + 1 + (prec == "bf16") * 6, (prec, it, it64)
+
+
 @pytest.mark.parametrize("name", ["hypre3d", "agg2d"])
 @pytest.mark.parametrize("steps", [(1, 1), (2, 3), (1, 2)])
 def test_one_pass_sweeps_are_bit_exact(dh, orc, torch_cuda, name, steps):
@@ -437,7 +477,10 @@ def test_one_pass_sweeps_are_bit_exact(dh, orc, torch_cuda, name, steps):
     import dataclasses
     torch = torch_cuda
     A, cfg, sm = _case(dh, name)
-    cfg = dataclasses.replace(cfg, pre_smoothing_steps=steps[0], post_smoothing_steps=steps[1])
+    # one row-order kernel on both hierarchies, so that they differ only in the
+    # fusions (each autotuner could pick the vector CSR kernel, whose FP32 sums
+    # are in another order, on different levels)
+    cfg = dataclasses.replace(cfg, pre_smoothing_steps=steps[0], post_smoothing_steps=steps[1], kernel="sorted")
     b = torch.tensor(dh.problems.splitmix_normal(SEED + 960, A.n), device="cuda")
     x0 = torch.tensor(dh.problems.splitmix_normal(SEED + 961, A.n), device="cuda")
 
