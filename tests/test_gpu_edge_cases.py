@@ -8,6 +8,9 @@
   through a whole V-cycle, the oracle's cycle;
 * rows longer than a row block's LDS stage (hub rows), summed chunk by chunk
   in row order: level kernels bit for bit against the oracle;
+* row blocks whose four-entry quads exceed the LDS stage (a block within
+  three nonzeros of the budget starting off a multiple of four): the
+  four-entry kernel's entry-by-entry fallback, bit for bit against the oracle;
 * row blocks spanning more than 65,536 columns (no 16-bit form) but fewer
   than 2^20: the column-sorted kernels read them through 20-bit offsets;
   level kernels bit for bit against the oracle;
@@ -252,6 +255,72 @@ def test_wide_blocks_column_sorted_kernels(dh, orc, torch_cuda, prec):
         dev = lambda a: torch.tensor(a, dtype=tdt, device="cuda")  # noqa: E731
         out, out2 = torch.zeros(A.n, dtype=tdt, device="cuda"), torch.zeros(A.n, dtype=tdt, device="cuda")
         print("wide blocks", kernel, prec, flush=True)
+        h.level_op(0, 0, prec, dev(x), None, out)
+        same(out.cpu().numpy(), orc.spmv(rp, cv, nz, x), prec)
+        h.level_op(0, 1, prec, dev(x), dev(b), out)
+        same(out.cpu().numpy(), orc.residual(rp, cv, nz, x, b), prec)
+        h.level_op(0, 2, prec, dev(x), dev(b), out)
+        same(out.cpu().numpy(), orc.jacobi_step(rp, cv, nz, x, b, d, cfg.jacobi_omega), prec)
+        h.level_op(0, 3, prec, None, dev(b), out, out2)
+        x1 = orc.jacobi_step(rp, cv, nz, np.zeros(A.n), b, d, cfg.jacobi_omega)
+        same(out.cpu().numpy(), x1, prec)
+        same(out2.cpu().numpy(), orc.residual(rp, cv, nz, x1, b), prec)
+
+
+def _misaligned_blocks(dh, n, w):
+    """4 on the diagonal, -1 at i - 1, i + 1 and i + w: row 0 has three
+    entries and most others four, so with the 1024-nonzero budget nearly
+    every row block holds exactly 1024 entries starting 3 past a multiple of
+    four, and its quads (257) exceed the stage (256 x 4)."""
+    import scipy.sparse as sp
+    idx = np.arange(n)
+    rows, cols = [], []
+    for off in (0, -1, 1, w):
+        j = idx + off
+        m = (j >= 0) & (j < n)
+        rows.append(idx[m])
+        cols.append(j[m])
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    B = sp.csr_matrix((np.where(r == c, 4.0, -1.0), (r, c)), shape=(n, n))
+    B.sort_indices()
+    return dh.as_csr(B)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_quad_kernel_fallback_blocks(dh, orc, torch_cuda, prec):
+    """Nearly every row block of this matrix takes the four-entry kernel's
+    entry-by-entry path (checked on the host here): SpMV, residual, Jacobi
+    and the fused zero-guess step with the sortedquad and sorted kinds are
+    the oracle's bit for bit in FP64 (1e-5 in FP32)."""
+    import dataclasses
+    torch = torch_cuda
+    A = _misaligned_blocks(dh, 600_000, 1000)
+    rp, cv, nz = A.rowptr, A.colval, A.nzval
+    # the host's view of the blocks: budget 1024, blocks of <= 256 rows
+    lens = np.diff(rp)
+    starts, acc, rows = [0], 0, 0
+    for i, L in enumerate(lens):
+        if rows > 0 and (acc + L > 1024 or rows == 256):
+            starts.append(i)
+            acc = rows = 0
+        acc += L
+        rows += 1
+    z = rp[np.array(starts + [A.n])]
+    nq = (z[1:] - (z[:-1] & ~3) + 3) >> 2
+    assert np.mean(nq > 256) > 0.99
+    base = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType())
+    tdt = torch.float64 if prec == "fp64" else torch.float32
+    rng = np.random.default_rng(SEED + 995)
+    x, b = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    for kernel in ("sortedquad", "sorted"):
+        cfg = dataclasses.replace(base, kernel=kernel)
+        h = dh.amg_setup(A, cfg)
+        assert h.level_info(0)["nblk"] == len(starts)
+        oh = oracle_of(h, orc, "jacobi")
+        d = oh.dinv(0)
+        dev = lambda a: torch.tensor(a, dtype=tdt, device="cuda")  # noqa: E731
+        out, out2 = torch.zeros(A.n, dtype=tdt, device="cuda"), torch.zeros(A.n, dtype=tdt, device="cuda")
+        print("quad fallback", kernel, prec, flush=True)
         h.level_op(0, 0, prec, dev(x), None, out)
         same(out.cpu().numpy(), orc.spmv(rp, cv, nz, x), prec)
         h.level_op(0, 1, prec, dev(x), dev(b), out)
