@@ -207,15 +207,61 @@ __global__ __launch_bounds__(kBlock) void k_ilu_bwd(int cnt, const int* __restri
 // Galerkin numeric product into the fixed coarse pattern:
 // Ac[k] = sum_t P[pi_t] * A[ai_t] * P[pj_t] over the triples of nonzero k in
 // generation order (src/galerkin.jl:194-222).  One thread per coarse nonzero.
+// A workgroup takes kBlock consecutive coarse nonzeros, whose triples are one
+// contiguous range of the map.  The range is processed kGalChunk triples at a
+// time: every thread loads kGalPer triples of the chunk (coalesced, indices
+// past its end clamped to its last entry and their products dropped), issues
+// their three gathers together and stages P[pi] * A[ai] * P[pj] in LDS; then
+// each thread adds the products of its own coarse nonzero that fall in the
+// chunk, in triple order, to its running sum.  Same products, same order as
+// one thread walking its triples, so the values are the same bits; a thread
+// walking its triples alone would touch one cache line per coarse nonzero of
+// its wavefront on every step of the map loads.
+constexpr int kGalPer = 8;
+constexpr int kGalChunk = kGalPer * kBlock;
 __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nnzc, const int64_t* __restrict__ off,
                                                      const int* __restrict__ pi, const int* __restrict__ ai,
                                                      const int* __restrict__ pj, const double* __restrict__ P,
                                                      const double* __restrict__ A, double* __restrict__ Ac) {
-    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (k >= nnzc) return;
+    __shared__ double prod[kGalChunk];
+    const int b = gridDim.x >= 64 ? xcd_block() : (int)blockIdx.x;
+    const int t = (int)threadIdx.x;
+    const int64_t k0 = (int64_t)b * kBlock;
+    const int64_t k = k0 + t;
+    const int64_t kend = k0 + kBlock < nnzc ? k0 + kBlock : nnzc;
+    const int64_t T0 = off[k0], T1 = off[kend];
+    const bool own = k < nnzc;
+    const int64_t ts = own ? off[k] : 0, te = own ? off[k + 1] : 0;
     double acc = 0.0;
-    for (int64_t q = off[k]; q < off[k + 1]; ++q) acc += P[pi[q]] * A[ai[q]] * P[pj[q]];
-    Ac[k] = acc;
+    for (int64_t c0 = T0; c0 < T1; c0 += kGalChunk) {
+        const int cn = (int)(T1 - c0 < kGalChunk ? T1 - c0 : kGalChunk);
+        int ip[kGalPer], ia[kGalPer], jp[kGalPer];
+#pragma unroll
+        for (int u = 0; u < kGalPer; ++u) {
+            const int64_t q = c0 + min(t + u * kBlock, cn - 1);
+            ip[u] = pi[q];
+            ia[u] = ai[q];
+            jp[u] = pj[q];
+        }
+        double vp[kGalPer], va[kGalPer], vq[kGalPer];
+#pragma unroll
+        for (int u = 0; u < kGalPer; ++u) {
+            vp[u] = P[ip[u]];
+            va[u] = A[ia[u]];
+            vq[u] = P[jp[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kGalPer; ++u)
+            if (t + u * kBlock < cn) prod[t + u * kBlock] = vp[u] * va[u] * vq[u];
+        __syncthreads();
+        if (own) {
+            const int64_t a = ts > c0 ? ts : c0;
+            const int64_t e = te < c0 + cn ? te : c0 + cn;
+            for (int64_t q = a; q < e; ++q) acc += prod[q - c0];
+        }
+        __syncthreads();
+    }
+    if (own) Ac[k] = acc;
 }
 
 // Dense LU with partial pivoting of the coarsest operator, one workgroup
