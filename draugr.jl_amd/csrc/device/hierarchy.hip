@@ -177,6 +177,39 @@ __global__ __launch_bounds__(kBlock) void k_invdiag(int n, const int* __restrict
     }
 }
 
+// The l1 inverse diagonal over the operator's row blocks: a block's |a_k|
+// staged in LDS with coalesced loads, then each row's sum on its own thread in
+// row order (the same sums as k_invdiag<1>); a row longer than the stage
+// (alone in its block) is summed by one thread from memory.
+__global__ __launch_bounds__(kBlock) void k_invdiag_l1_blocks(const BlkHdr* __restrict__ hdr,
+                                                              const int* __restrict__ rp,
+                                                              const double* __restrict__ av, double* __restrict__ d) {
+    __shared__ double term[kStreamNnz];
+    const int b = gridDim.x >= 64 ? xcd_block() : (int)blockIdx.x;
+    const BlkHdr h0 = hdr[b];
+    const int r0 = h0.r0, r1 = h0.r1, z0 = h0.z0, nz = hdr[b + 1].z0 - z0;
+    const int t = (int)threadIdx.x;
+    const double eps = 2.220446049250313e-16;
+    if (nz > kStreamNnz) {
+        if (t == 0) {
+            double nrm = 0.0;
+            for (int k = 0; k < nz; ++k) nrm += fabs(av[z0 + k]);
+            d[r0] = nrm > eps ? 1.0 / nrm : 0.0;
+        }
+        return;
+    }
+    const int row = r0 + t;
+    const bool own = row < r1;
+    const int rs = own ? rp[row] - z0 : 0, re = own ? rp[row + 1] - z0 : 0;
+    for (int k = t; k < nz; k += kBlock) term[k] = fabs(av[z0 + k]);
+    __syncthreads();
+    if (own) {
+        double nrm = 0.0;
+        for (int k = rs; k < re; ++k) nrm += term[k];
+        d[row] = nrm > eps ? 1.0 / nrm : 0.0;
+    }
+}
+
 // ILU(0) triangular solves by level schedule (src/smoothers.jl:1383-1411):
 // rows of one level are independent; each row subtracts in column order.
 template <typename TF, typename TX>
@@ -677,18 +710,31 @@ __global__ __launch_bounds__(kBlock) void k_sort_blocks(const BlkHdr* __restrict
     }
 }
 
-// sv = the values in the sorted form's order (after new values in v); blocks
-// without a base (row order, possibly one row longer than 65536 entries) copy
+// sv = the values in the sorted form's order (after new values in v), and
+// its FP32 / BF16 copies when they exist (null otherwise).  A sorted block
+// (<= kStreamNnz entries) first copies its values into LDS with coalesced
+// loads, then writes each entry from its position there; blocks without a
+// sorted base (row order, possibly one row longer than 65536 entries) copy.
 __global__ __launch_bounds__(kBlock) void k_permute_blocks(const BlkHdr* __restrict__ shdr,
                                                            const uint32_t* __restrict__ cs,
-                                                           const double* __restrict__ v, double* __restrict__ sv) {
+                                                           const double* __restrict__ v, double* __restrict__ sv,
+                                                           float* __restrict__ sv32, bf16_t* __restrict__ sv16) {
+    __shared__ double sh[kStreamNnz];
     const int b = blockIdx.x;
     const int z0 = shdr[b].z0, nz = shdr[b + 1].z0 - z0;
+    const int t = (int)threadIdx.x;
+    auto put = [&](int k, double x) {
+        sv[z0 + k] = x;
+        if (sv32) sv32[z0 + k] = (float)x;
+        if (sv16) sv16[z0 + k] = float_to_bf16((float)x);
+    };
     if (shdr[b].cb < 0) {
-        for (int k = (int)threadIdx.x; k < nz; k += kBlock) sv[z0 + k] = v[z0 + k];
+        for (int k = t; k < nz; k += kBlock) put(k, v[z0 + k]);
         return;
     }
-    for (int k = (int)threadIdx.x; k < nz; k += kBlock) sv[z0 + k] = v[z0 + (int)(cs[z0 + k] >> kSortColBits)];
+    for (int k = t; k < nz; k += kBlock) sh[k] = v[z0 + k];
+    __syncthreads();
+    for (int k = t; k < nz; k += kBlock) put(k, sh[cs[z0 + k] >> kSortColBits]);
 }
 
 // Entries allocated for an array of the column-sorted form: a multiple of 4,
@@ -946,8 +992,12 @@ void DeviceHierarchy::compute_dinv(int l, hipStream_t st) {
     DevLevel& D = L_[l];
     const int n = D.A.nrow;
     if (n == 0) return;
-    invdiag_device(n, D.A.rp.p, D.A.cv.p, D.A.v64.p, D.d64.p, is_l1(smoother_) ? 1 : (smoother_ == Smoother::SPAI0 ? 2 : 0),
-                   st);
+    if (is_l1(smoother_) && D.A.nblk > 0)
+        hipLaunchKernelGGL(k_invdiag_l1_blocks, dim3(D.A.nblk), dim3(kBlock), 0, st, D.A.hdr.p, D.A.rp.p, D.A.v64.p,
+                           D.d64.p);
+    else
+        invdiag_device(n, D.A.rp.p, D.A.cv.p, D.A.v64.p, D.d64.p,
+                       is_l1(smoother_) ? 1 : (smoother_ == Smoother::SPAI0 ? 2 : 0), st);
     if (D.d32.p) {
         const double* s = D.d64.p;
         float* d = D.d32.p;
@@ -1045,28 +1095,23 @@ void DeviceHierarchy::coarse_factor(hipStream_t st, bool stepwise) {
 }
 
 // the reduced-precision and column-sorted copies of a matrix's values
+// The reduced-precision and column-sorted copies of M's values after new
+// FP64 values: one pass over the values for the row-order copies, one for the
+// sorted form and its copies.
 static void refresh_mirrors(DevCSR& M, hipStream_t st) {
     const double* s = M.v64.p;
-    if (M.v32.p) {
-        float* d = M.v32.p;
-        launch_map(M.nnz, [=] __device__(int64_t i) { d[i] = (float)s[i]; }, st);
+    if (M.v32.p || M.v16.p) {
+        float* d32 = M.v32.p;
+        bf16_t* d16 = M.v16.p;
+        launch_map(M.nnz, [=] __device__(int64_t i) {
+            const double x = s[i];
+            if (d32) d32[i] = (float)x;
+            if (d16) d16[i] = float_to_bf16((float)x);
+        }, st);
     }
-    if (M.v16.p) {
-        bf16_t* d = M.v16.p;
-        launch_map(M.nnz, [=] __device__(int64_t i) { d[i] = float_to_bf16((float)s[i]); }, st);
-    }
-    if (M.hasS && M.nblk > 0) {
-        hipLaunchKernelGGL(k_permute_blocks, dim3(M.nblk), dim3(kBlock), 0, st, M.shdr.p, M.scs.p, M.v64.p, M.sv64.p);
-        const double* ss = M.sv64.p;
-        if (M.sv32.p) {
-            float* d = M.sv32.p;
-            launch_map(M.nnz, [=] __device__(int64_t i) { d[i] = (float)ss[i]; }, st);
-        }
-        if (M.sv16.p) {
-            bf16_t* d = M.sv16.p;
-            launch_map(M.nnz, [=] __device__(int64_t i) { d[i] = float_to_bf16((float)ss[i]); }, st);
-        }
-    }
+    if (M.hasS && M.nblk > 0)
+        hipLaunchKernelGGL(k_permute_blocks, dim3(M.nblk), dim3(kBlock), 0, st, M.shdr.p, M.scs.p, M.v64.p, M.sv64.p,
+                           M.sv32.p, M.sv16.p);
 }
 
 void DeviceHierarchy::refresh_level_mirrors(int l, hipStream_t st) { refresh_mirrors(L_[l].A, st); }
