@@ -331,3 +331,34 @@ def test_quad_kernel_fallback_blocks(dh, orc, torch_cuda, prec):
         x1 = orc.jacobi_step(rp, cv, nz, np.zeros(A.n), b, d, cfg.jacobi_omega)
         same(out.cpu().numpy(), x1, prec)
         same(out2.cpu().numpy(), orc.residual(rp, cv, nz, x1, b), prec)
+
+
+def test_l1_inverse_diagonal_long_row(dh, orc, torch_cuda):
+    """The l1 inverse diagonal over row blocks (|a| staged in LDS, row-order
+    sums) with a 5,000-entry hub row, longer than the stage and summed by one
+    thread from memory: every level's values are the oracle's bit for bit,
+    also after a resetup with new values."""
+    torch = torch_cuda
+    B = hub_matrix(dh.problems, 100, 5000)
+    A = dh.as_csr(B)
+    assert np.diff(A.rowptr).max() == 5000
+    cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.L1JacobiSmootherType())
+    h = dh.amg_setup(A, cfg)
+    oh = oracle_of(h, orc, "l1_jacobi")
+    for stage in ("setup", "resetup"):
+        if stage == "resetup":
+            s = np.exp(0.3 * np.random.default_rng(SEED + 996).standard_normal(A.n))
+            rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+            new = A.nzval * s[rows] * s[A.colval]
+            dh.amg_resetup_(h, dh.problems.CSR(A.rowptr, A.colval, new), cfg)
+            oh.resetup(new)
+        for l in range(h.nlevels):
+            n = h.level_info(l)["n"]
+            dd = torch.zeros(n, dtype=torch.float64, device="cuda")
+            h.level_op(l, 9, 0, None, None, dd)
+            torch.cuda.synchronize()
+            same(dd.cpu().numpy(), oh.dinv(l))
+    # the oracle's l1 diagonal itself, on the fine level, from its definition
+    d0 = oh.dinv(0)
+    ref0 = 1.0 / np.add.reduceat(np.abs(new), A.rowptr[:-1])
+    assert np.allclose(d0, ref0, rtol=1e-14, atol=0)
