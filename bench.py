@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the calibration copy and the dominant-kernel launches (for counter runs)")
     ap.add_argument("--kernel", default="auto", choices=["auto", "stream", "rowseq", "pipe", "stream16", "sorted", "stream16pipe",
-                                                  "sortedpipe", "sortedquad", "vector"],
+                                                  "sortedpipe", "sortedquad", "vector", "pattern"],
                     help="sparse-pass kernel (config key kernel); a counter run forces the kernel the "
                          "autotuner picks without a profiler, whose timings it would perturb")
     ap.add_argument("--fuse", default="auto", choices=["auto", "off", "on"],
@@ -108,7 +108,8 @@ def dominant_kernel(torch, h, prec, reps):
 
 KERNEL_OF_KIND = {"stream": "k_csr_stream", "stream16": "k_csr_stream", "sorted": "k_csr_stream",
                   "rowseq": "k_csr_rowseq", "pipe": "k_csr_pipe", "stream16pipe": "k_stream_pipe",
-                  "sortedpipe": "k_stream_pipe", "sortedquad": "k_csr_stream_quad", "vector": "k_csr_vector"}
+                  "sortedpipe": "k_stream_pipe", "sortedquad": "k_csr_stream_quad", "vector": "k_csr_vector",
+                  "pattern": "k_csr_pattern"}
 
 
 def h_blocks(h, l):

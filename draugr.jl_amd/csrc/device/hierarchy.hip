@@ -13,6 +13,7 @@
 #include <mutex>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 
 namespace draugr {
 
@@ -116,14 +117,45 @@ __global__ __launch_bounds__(kBlock) void k_stream_reduce(const BlkHdr* __restri
     block_sum_store(dot, sh, partial, blockIdx.x);
 }
 
-// sorted: the column-sorted form (values from M.sv64; TV = double)
+// The same pass over the pattern form: the same row blocks, rows on the
+// same threads and partial per block, so p.q is the same bits.
+template <int KW, typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_pattern_reduce(const BlkHdr* __restrict__ hdr, int npat, int w,
+                                                           const uint16_t* __restrict__ pid,
+                                                           const int* __restrict__ tab, const TV* __restrict__ tv,
+                                                           XF xf, EPI epi, double* partial) {
+    __shared__ int stab[kPatMax * (kPatSlots + 1)];
+    __shared__ double sh[kBlock];
+    double dot = 0.0;
+    auto pref = [](int) { return NoPre{}; };
+    auto initf = [](NoPre) { return (TX)0; };
+    auto accf = [](TX a, TX p) { return a + p; };
+    auto finf = [&](int row, TX a, NoPre) { dot = epi(row, a); };
+    pattern_block<KW, TV, TX, XF, decltype(pref), decltype(initf), decltype(accf), decltype(finf)>(
+        xcd_block(), hdr, npat, w, pid, tab, tv, xf, stab, pref, initf, accf, finf);
+    block_sum_store(dot, sh, partial, blockIdx.x);
+}
+
+// Forms of the PCG's q = A p pass (TV = double): the row-order stream kernel
+// (16-bit columns where the operator has them), the column-sorted form
+// (values from M.sv64), the pattern form (values from M.tv64).
+enum PcgForm : int { kPcgStream = 0, kPcgSorted = 1, kPcgPattern = 2 };
+
 template <typename TV, typename TX, class XF, class EPI, class FIN>
 void launch_stream_reduce(const DevCSR& M, const TV* av, XF xf, EPI epi, FIN fin, double* partial, hipStream_t st,
-                          bool sorted = false) {
+                          int form = kPcgStream) {
 #define DRAUGR_SR(PER, C, S, COLS, VALS, CS)                                                                   \
     hipLaunchKernelGGL((k_stream_reduce<PER, C, S, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st,         \
                        S ? M.shdr.p : M.hdr.p, M.rp.p, M.cv.p, COLS, VALS, xf, epi, partial, CS)
-    if (sorted && M.hasS) {
+    if (form == kPcgPattern && M.hasT) {
+        const TV* tv = reinterpret_cast<const TV*>(M.tv64.p);
+        if (M.tw <= 8)
+            hipLaunchKernelGGL((k_pattern_reduce<8, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.hdr.p,
+                               M.npat, M.tw, M.tpid.p, M.ttab.p, tv, xf, epi, partial);
+        else
+            hipLaunchKernelGGL((k_pattern_reduce<16, TV, TX, XF, EPI>), dim3(M.nblk), dim3(kBlock), 0, st, M.hdr.p,
+                               M.npat, M.tw, M.tpid.p, M.ttab.p, tv, xf, epi, partial);
+    } else if (form == kPcgSorted && M.hasS) {
         const TV* sv = reinterpret_cast<const TV*>(M.sv64.p);
         switch (M.per) {
             case 1: DRAUGR_SR(1, true, true, nullptr, sv, M.scs.p); break;
@@ -523,6 +555,11 @@ template <typename TV> const TV* svals(const DevCSR& m);
 template <> const double* svals<double>(const DevCSR& m) { return m.sv64.p; }
 template <> const float* svals<float>(const DevCSR& m) { return m.sv32.p; }
 template <> const bf16_t* svals<bf16_t>(const DevCSR& m) { return m.sv16.p; }
+// the values of the pattern form
+template <typename TV> const TV* tvals(const DevCSR& m);
+template <> const double* tvals<double>(const DevCSR& m) { return m.tv64.p; }
+template <> const float* tvals<float>(const DevCSR& m) { return m.tv32.p; }
+template <> const bf16_t* tvals<bf16_t>(const DevCSR& m) { return m.tv16.p; }
 
 // Transfer operators use FP32 values when the cycle stores A in BF16.
 template <typename TV> using TransferT = TransferOf<TV>;
@@ -647,6 +684,8 @@ void op_kind(int kind, const DevCSR& M, XF xf, EPI epi, hipStream_t st) {
                                    M.scs.p);
     else if (kind == kOpSortedQuad && M.hasS && svals<TV>(M))
         launch_stream_quad<TV, TX>(M.nblk, M.per, M.shdr.p, M.rp.p, M.cv.p, M.scs.p, svals<TV>(M), xf, epi, st);
+    else if (kind == kOpPattern && M.hasT && tvals<TV>(M))
+        launch_pattern<TV, TX>(M.nblk, M.hdr.p, M.npat, M.tw, M.tpid.p, M.ttab.p, tvals<TV>(M), xf, epi, st);
     else if (kind == kOpVector && sizeof(TX) == 4) {
         if constexpr (sizeof(TX) == 4)  // FP32 / BF16 only: not the reference's sequential sum order
             launch_vector<TV, TX>(M.nrow, M.nnz, M.rp.p, M.cv.p, vals<TV>(M), xf, epi, st);
@@ -741,6 +780,104 @@ __global__ __launch_bounds__(kBlock) void k_permute_blocks(const BlkHdr* __restr
 // so that the four-entry loads of the last block stay inside it.
 inline size_t sorted_alloc(int64_t nnz) { return (size_t)((nnz + 3) & ~(int64_t)3); }
 
+// The pattern form's values from the FP64 values v (row-order CSR): slot
+// (row, j) holds entry rp[row] + j, zero past the row's end, with the FP32
+// copy in the same pass; then the BF16 copy (paired layout) when it exists.
+void fill_pattern(const DevCSR& d, hipStream_t st) {
+    const int n = d.nrow, w = d.tw;
+    const int* rp = d.rp.p;
+    const double* v = d.v64.p;
+    double* t64 = d.tv64.p;
+    float* t32 = d.tv32.p;
+    bf16_t* t16 = d.tv16.p;
+    auto value = [=] __device__(int row, int j) -> double {
+        if (row >= n) return 0.0;
+        const int k = rp[row] + j;
+        return k < rp[row + 1] ? v[k] : 0.0;
+    };
+    launch_map(pattern_slots(n, w), [=] __device__(int64_t s) {
+        const int lane = (int)(s & (kPatSlice - 1));
+        const int64_t q = s / kPatSlice;
+        const double x = value((int)(q / w) * kPatSlice + lane, (int)(q % w));
+        t64[s] = x;
+        if (t32) t32[s] = (float)x;
+    }, st);
+    if (t16) {
+        const int wh = (w + 1) >> 1;
+        launch_map(pattern_slots2(n, w), [=] __device__(int64_t e) {
+            const int64_t pair = e >> 1;
+            const int lane = (int)(pair & (kPatSlice - 1));
+            const int64_t q = pair / kPatSlice;
+            t16[e] = float_to_bf16((float)value((int)(q / wh) * kPatSlice + lane, 2 * (int)(q % wh) + (int)(e & 1)));
+        }, st);
+    }
+}
+
+// The pattern form of h, when its rows use at most kPatMax distinct column
+// patterns relative to the row (col - row, in CSR order), none longer than
+// kPatSlots, and the 64-row slices pad the values by at most 1/8.  Rows are
+// matched against the previous row's pattern first (a stencil's interior
+// rows repeat it), then by a hash of the offsets.
+void build_pattern(DevCSR& d, const CSR& h, hipStream_t st) {
+    d.hasT = false;
+    d.tw = d.npat = 0;
+    const int n = h.nrow;
+    int w = 0;
+    for (int i = 0; i < n; ++i) w = std::max(w, h.rp[i + 1] - h.rp[i]);
+    const bool eligible = n > 0 && w > 0 && w <= kPatSlots && pattern_slots(n, w) * 8 <= (int64_t)h.nnz() * 9;
+    std::vector<int32_t> tab;
+    std::vector<uint16_t> pid(eligible ? n : 0);
+    auto same = [&](int p, int i) {
+        const int32_t* e = tab.data() + (size_t)p * (w + 1);
+        const int z0 = h.rp[i], len = h.rp[i + 1] - z0;
+        if (e[0] != len) return false;
+        for (int j = 0; j < len; ++j)
+            if (e[1 + j] != h.cv[z0 + j] - i) return false;
+        return true;
+    };
+    std::unordered_map<uint64_t, std::vector<int>> seen;
+    bool ok = eligible;
+    int npat = 0, prev = -1;
+    for (int i = 0; ok && i < n; ++i) {
+        if (prev >= 0 && same(prev, i)) {
+            pid[i] = (uint16_t)prev;
+            continue;
+        }
+        const int z0 = h.rp[i], len = h.rp[i + 1] - z0;
+        uint64_t key = 1469598103934665603ull ^ (uint64_t)len;
+        for (int j = 0; j < len; ++j) key = (key ^ (uint32_t)(h.cv[z0 + j] - i)) * 1099511628211ull;
+        std::vector<int>& cand = seen[key];
+        int found = -1;
+        for (int p : cand)
+            if (same(p, i)) { found = p; break; }
+        if (found < 0) {
+            if (npat == kPatMax) { ok = false; break; }
+            found = npat++;
+            tab.push_back(len);
+            for (int j = 0; j < w; ++j) tab.push_back(j < len ? h.cv[z0 + j] - i : 0);
+            cand.push_back(found);
+        }
+        pid[i] = (uint16_t)found;
+        prev = found;
+    }
+    if (!ok) {
+        d.tpid.release();
+        d.ttab.release();
+        d.tv64.release();
+        d.tv32.release();
+        d.tv16.release();
+        return;
+    }
+    d.hasT = true;
+    d.tw = w;
+    d.npat = npat;
+    d.tpid.upload(pid, st);
+    d.ttab.upload(tab, st);
+    d.tv64.alloc(pattern_slots(n, w));
+    fill_pattern(d, st);
+    DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // pid, tab are pageable and local
+}
+
 void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
     d.nrow = h.nrow;
     d.ncol = h.ncol;
@@ -820,6 +957,7 @@ void upload_csr(DevCSR& d, const CSR& h, hipStream_t st) {
         d.sv64.release();
     }
     DRAUGR_HIP_CHECK(hipStreamSynchronize(st));  // blk, c16, hdr are pageable and local
+    build_pattern(d, h, st);
 }
 
 bool is_l1(Smoother s) { return s == Smoother::L1Jacobi || s == Smoother::L1ColoredGS; }
@@ -1112,6 +1250,7 @@ static void refresh_mirrors(DevCSR& M, hipStream_t st) {
     if (M.hasS && M.nblk > 0)
         hipLaunchKernelGGL(k_permute_blocks, dim3(M.nblk), dim3(kBlock), 0, st, M.shdr.p, M.scs.p, M.v64.p, M.sv64.p,
                            M.sv32.p, M.sv16.p);
+    if (M.hasT) fill_pattern(M, st);
 }
 
 void DeviceHierarchy::refresh_level_mirrors(int l, hipStream_t st) { refresh_mirrors(L_[l].A, st); }
@@ -1152,6 +1291,10 @@ void DeviceHierarchy::ensure_precision(Precision p) {
                     float* ds = m->sv32.p;
                     launch_map(m->nnz, [=] __device__(int64_t i) { ds[i] = (float)ss[i]; }, st);
                 }
+                if (m->hasT) {
+                    m->tv32.alloc(pattern_slots(m->nrow, m->tw));
+                    fill_pattern(*m, st);
+                }
             }
             const double* s = D.d64.p;
             float* d = D.d32.p;
@@ -1172,6 +1315,10 @@ void DeviceHierarchy::ensure_precision(Precision p) {
                 const double* ss = D.A.sv64.p;
                 bf16_t* ds = D.A.sv16.p;
                 launch_map(D.A.nnz, [=] __device__(int64_t i) { ds[i] = float_to_bf16((float)ss[i]); }, st);
+            }
+            if (D.A.hasT) {
+                D.A.tv16.alloc(pattern_slots2(D.A.nrow, D.A.tw));
+                fill_pattern(D.A, st);  // also rewrites the FP64 / FP32 copies with the same values
             }
         }
         have16_ = true;
@@ -1510,6 +1657,7 @@ void DeviceHierarchy::tune() {
             auto absent = [&](const DevCSR& M) {
                 return ((k == kOpStream16 || k == kOpStream16Pipe) && !M.has16) ||
                        ((k == kOpSorted || k == kOpSortedPipe || k == kOpSortedQuad) && !M.hasS) ||
+                       (k == kOpPattern && !M.hasT) ||
                        (k == kOpVector && sizeof(TX) == 8);  // FP64 keeps the sequential row sums
             };
             tA[k] = absent(D.A) ? skip : timeit([&] {
@@ -1580,12 +1728,18 @@ std::string DeviceHierarchy::tune_report() const {
         for (int l = 0; l < nlevels(); ++l) {
             const DevLevel& D = L_[l];
             const char* kn[kOpKinds] = {"stream",       "rowseq",     "pipe",       "stream16",  "sorted",
-                                        "stream16pipe", "sortedpipe", "sortedquad", "vector"};
+                                        "stream16pipe", "sortedpipe", "sortedquad", "vector",
+                                        "pattern"};
             std::snprintf(buf, sizeof buf, "%s level %d: A=%s Pt=%s P=%s zero-guess=%s down=%s up=%s\n", names[p], l,
                           kn[D.A.kind[p]], kn[D.Pt.kind[p]], kn[D.P.kind[p]], D.fuse_zero[p] ? "fused" : "split",
                           D.fuse_down[p] ? "one-pass" : (D.down_state < 0 ? "separate(only)" : "separate"),
                           D.fuse_up[p] ? "one-pass" : "separate");
             out += buf;
+            if (p == 0 && D.A.hasT) {
+                std::snprintf(buf, sizeof buf, "level %d: A has the pattern form (%d row patterns, width %d)\n", l,
+                              D.A.npat, D.A.tw);
+                out += buf;
+            }
             if (D.A.nnz >= (1 << 20) && forced_kind_ < 0) {  // the big levels: every kind's time for the Jacobi pass
                 std::string tl = std::string(names[p]) + " level " + std::to_string(l) + " Jacobi pass us:";
                 for (int k = 0; k < kOpKinds; ++k) {
@@ -1611,9 +1765,12 @@ std::string DeviceHierarchy::tune_report() const {
                           "choice is fixed)\n", names[p]);
         out += buf;
     }
-    if (pcg_sorted_ >= 0) {  // the form of the PCG's q = A p pass, once a PCG has run
+    if (pcg_form_ >= 0) {  // the form of the PCG's q = A p pass, once a PCG has run
         const DevCSR& A0 = nlevels() > 0 ? L_[0].A : Ac_;
-        out += pcg_sorted_ ? "pcg q = A p: sorted\n" : (A0.has16 ? "pcg q = A p: stream16\n" : "pcg q = A p: stream\n");
+        out += pcg_form_ == kPcgSorted    ? "pcg q = A p: sorted\n"
+               : pcg_form_ == kPcgPattern ? "pcg q = A p: pattern\n"
+               : A0.has16                 ? "pcg q = A p: stream16\n"
+                                          : "pcg q = A p: stream\n";
     }
     return out;
 }
@@ -1915,17 +2072,20 @@ int DeviceHierarchy::solve(double* x, const double* b, double tol, int maxiter, 
     return maxiter;
 }
 
-// Whether the PCG's q = A p pass reads the column-sorted form: timed once
-// against the 16-bit form (4 back-to-back launches, best of 3, on scratch
-// vectors with the pass's two gathers per nonzero); a forced kernel kind
-// decides instead.  Both forms give the same values.
-bool DeviceHierarchy::pcg_sorted(const DevCSR& A, hipStream_t st) {
-    if (pcg_sorted_ >= 0) return pcg_sorted_ == 1;
-    pcg_sorted_ = 0;
-    if (!A.hasS) return false;
+// The form of the PCG's q = A p pass: timed once among the forms the
+// operator has (4 back-to-back launches, best of 3, on scratch vectors with
+// the pass's two gathers per nonzero); a forced kernel kind decides instead.
+// Every form gives the same values.
+int DeviceHierarchy::pcg_form(const DevCSR& A, hipStream_t st) {
+    if (pcg_form_ >= 0) return pcg_form_;
+    pcg_form_ = kPcgStream;
+    if (!A.hasS && !A.hasT) return pcg_form_;
     if (forced_kind_ >= 0) {
-        pcg_sorted_ = forced_kind_ == kOpSorted || forced_kind_ == kOpSortedPipe || forced_kind_ == kOpSortedQuad;
-        return pcg_sorted_ == 1;
+        if (A.hasS && (forced_kind_ == kOpSorted || forced_kind_ == kOpSortedPipe || forced_kind_ == kOpSortedQuad))
+            pcg_form_ = kPcgSorted;
+        else if (A.hasT && forced_kind_ == kOpPattern)
+            pcg_form_ = kPcgPattern;
+        return pcg_form_;
     }
     const double* u = w_z_.p;
     const double* v = w_p0_.p;
@@ -1939,9 +2099,11 @@ bool DeviceHierarchy::pcg_sorted(const DevCSR& A, hipStream_t st) {
     hipEvent_t e0, e1;
     DRAUGR_HIP_CHECK(hipEventCreate(&e0));
     DRAUGR_HIP_CHECK(hipEventCreate(&e1));
-    float best[2] = {1e30f, 1e30f};
+    float best[3] = {1e30f, 1e30f, 1e30f};
+    const bool have[3] = {true, A.hasS, A.hasT};
     for (int rep = 0; rep < 3; ++rep)
-        for (int s = 0; s < 2; ++s) {
+        for (int s = 0; s < 3; ++s) {
+            if (!have[s]) continue;
             DRAUGR_HIP_CHECK(hipEventRecord(e0, st));
             for (int k = 0; k < 4; ++k) launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin, partial_.p, st, s);
             DRAUGR_HIP_CHECK(hipEventRecord(e1, st));
@@ -1952,8 +2114,8 @@ bool DeviceHierarchy::pcg_sorted(const DevCSR& A, hipStream_t st) {
         }
     DRAUGR_HIP_CHECK(hipEventDestroy(e0));
     DRAUGR_HIP_CHECK(hipEventDestroy(e1));
-    pcg_sorted_ = best[1] < best[0];
-    return pcg_sorted_ == 1;
+    pcg_form_ = (int)(std::min_element(best, best + 3) - best);
+    return pcg_form_;
 }
 
 // FP64 preconditioned CG (Hestenes-Stiefel) with z = V(0; r).  Stopping test
@@ -2000,7 +2162,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
     if (ws) {
         const int pi = mixed ? (o.prec == Precision::FP32 ? 1 : 2) : 0;
         const bool mirrors = !mixed || (have32_ && (o.prec != Precision::BF16 || have16_));
-        if (!mirrors || (nlevels() > 0 && !tuned_[pi]) || pcg_sorted_ < 0)
+        if (!mirrors || (nlevels() > 0 && !tuned_[pi]) || pcg_form_ < 0)
             throw Error("allocation-free PCG: the hierarchy is not prepared for this precision (call the "
                         "allocating PCG once with the same config first)");
         const int64_t need = std::max<int64_t>(kRedBlocks, A.nblk);
@@ -2021,7 +2183,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
         DRAUGR_HIP_CHECK(hipStreamSynchronize(st));
         return 0;
     }
-    const bool srt = pcg_sorted(A, st);
+    const int form = pcg_form(A, st);
     op<double, double>(A, XVec<double>{x}, EpiResid<double>{b, r}, st);  // r = b - A x
     launch_reduce(n, [=] __device__(int64_t i) { return r[i] * r[i]; },
                   [=] __device__(double s) { scal[S_RR] = s; }, part, st);
@@ -2078,7 +2240,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                     q[i] = s;
                     return pn * s;
                 };
-                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, part, st, srt);
+                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, part, st, form);
             } else {
                 auto xf = [=] __device__(int j) -> double { return z[j] + *betap * pa[j]; };
                 auto epi = [=] __device__(int i, double s) -> double {
@@ -2087,7 +2249,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                     q[i] = s;
                     return pn * s;
                 };
-                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, part, st, srt);
+                launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, part, st, form);
             }
             // x += alpha p; r -= alpha q; r.r  (and the FP32 copy of r).  The
             // finalize also writes r.r and the iteration number straight into

@@ -53,6 +53,18 @@ struct DevCSR {
     DBuf<double> sv64;
     DBuf<float> sv32;
     DBuf<dev::bf16_t> sv16;
+    // Pattern form (kernel kind pattern), present when the rows use at most
+    // dev::kPatMax column patterns relative to the row, none longer than
+    // dev::kPatSlots entries: per row its pattern index, per pattern
+    // [length, col - row ...] in tw + 1 ints, the values in 64-row slices
+    // (dev::pattern_slot), refreshed with every new set of values.
+    bool hasT = false;
+    int tw = 0, npat = 0;
+    DBuf<uint16_t> tpid;
+    DBuf<int32_t> ttab;
+    DBuf<double> tv64;
+    DBuf<float> tv32;
+    DBuf<dev::bf16_t> tv16;
 };
 
 struct DevLevel {
@@ -261,8 +273,8 @@ private:
     uint64_t seed_ = 0;
     int forced_kind_ = -1;  // config "kernel": every sparse pass uses this kernel (no tuning)
     int forced_fuse_ = -1;  // config "fuse": -1 tuned, 0 never, 1 always (Jacobi-type smoothers)
-    int pcg_sorted_ = -1;   // PCG's q = A p pass on the column-sorted form: -1 not timed yet, 0 no, 1 yes
-    bool pcg_sorted(const DevCSR& A, hipStream_t st);
+    int pcg_form_ = -1;     // form of the PCG's q = A p pass: -1 not timed yet, else stream / sorted / pattern
+    int pcg_form(const DevCSR& A, hipStream_t st);
     bool have32_ = false, have16_ = false;
     bool tuned_[3] = {false, false, false};
     // solver workspaces at the finest size

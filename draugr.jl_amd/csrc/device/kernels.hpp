@@ -518,6 +518,127 @@ void launch_vector(int nrows, int64_t nnz, const int* rp, const int* cv, const T
     }
 }
 
+// Pattern form, for operators whose rows repeat a few column patterns
+// relative to the row index, as a stencil on a structured grid does (the
+// 7-point operator of a 3-D grid has 27: the interior, 6 faces, 12 edges and
+// 8 corners).  Per row a 16-bit pattern index; per pattern its length and its
+// column offsets col - row in CSR order (npat rows of W + 1 ints, W = the
+// longest row; at most kPatMax patterns of at most kPatSlots entries), staged
+// in LDS by every workgroup.  The values sit in slices of 64 rows with entry
+// j of the slice's rows contiguous (slot ((row >> 6) * W + j) * 64 +
+// (row & 63), zero where a row is shorter), so a wavefront's value loads are
+// coalesced and its gathers x[row + off] fall on one or two lines per
+// offset (the BF16 copy pairs two entries per 32-bit word, pattern_slot2).
+// No column index is read: 2 bytes per row instead of 4 per entry.
+// Workgroup b takes the CSR-stream row block b, row r0 + t on thread t (the
+// rows and threads of stream_block), and each row sums its products in CSR
+// order, each rounded on its own: the same bits as every other kind.
+constexpr int kPatMax = 64;
+constexpr int kPatSlots = 16;
+constexpr int kPatSlice = 64;
+
+__host__ __device__ inline int64_t pattern_slot(int row, int j, int w) {
+    return ((int64_t)(row >> 6) * w + j) * kPatSlice + (row & (kPatSlice - 1));
+}
+
+// Slots of the pattern form's value arrays for n rows of width w.
+inline int64_t pattern_slots(int n, int w) { return (int64_t)((n + kPatSlice - 1) / kPatSlice) * kPatSlice * w; }
+
+// The BF16 copy holds entries j and j + 1 (j even) of a row in one 32-bit
+// word, low half first, the words in the same slice layout over ceil(w / 2)
+// pairs: a wavefront loads two entries of its 64 rows per dword load (2-byte
+// loads per lane run at a fraction of the rate).
+__host__ __device__ inline int64_t pattern_slot2(int row, int j, int w) {
+    return 2 * pattern_slot(row, j >> 1, (w + 1) >> 1) + (j & 1);
+}
+inline int64_t pattern_slots2(int n, int w) { return 2 * pattern_slots(n, (w + 1) >> 1); }
+
+// The values of a row's slots j < w (KW >= w; the rest zero)
+template <int KW, typename TX, typename TV> struct PatternLoad {
+    static __device__ __forceinline__ void load(const TV* tv, int row, int w, TX (&v)[KW]) {
+        const int64_t s0 = pattern_slot(row, 0, w);
+#pragma unroll
+        for (int j = 0; j < KW; ++j) v[j] = j < w ? ld<TX, TV>(tv, s0 + (int64_t)j * kPatSlice) : (TX)0;
+    }
+};
+template <int KW> struct PatternLoad<KW, float, bf16_t> {
+    static __device__ __forceinline__ void load(const bf16_t* tv, int row, int w, float (&v)[KW]) {
+        const int wh = (w + 1) >> 1;
+        const uint32_t* p = reinterpret_cast<const uint32_t*>(tv) + pattern_slot(row, 0, wh);
+#pragma unroll
+        for (int jj = 0; jj < KW / 2; ++jj) {
+            const uint32_t u = jj < wh ? __builtin_nontemporal_load(p + (int64_t)jj * kPatSlice) : 0u;
+            v[2 * jj] = bf16_to_float((bf16_t)(u & 0xFFFFu));
+            v[2 * jj + 1] = bf16_to_float((bf16_t)(u >> 16));
+        }
+    }
+};
+
+template <int KW, typename TV, typename TX, class XF, class PREF, class INITF, class ACCF, class FINF>
+__device__ __forceinline__ void pattern_block(int b, const BlkHdr* __restrict__ hdr, int npat, int w,
+                                              const uint16_t* __restrict__ pid, const int* __restrict__ tab,
+                                              const TV* __restrict__ tv, XF xf, int* stab, PREF pref, INITF initf,
+                                              ACCF accf, FINF finf) {
+    const int t = (int)threadIdx.x;
+    const BlkHdr h0 = hdr[b];
+    const int row = h0.r0 + t;
+    const bool own = row < h0.r1;
+    for (int k = t; k < npat * (w + 1); k += kBlock) stab[k] = tab[k];
+    int p = 0;
+    TX v[KW];
+    decltype(pref(0)) q{};
+    if (own) {  // issued before the barrier: the pattern index, every value slot, the epilogue's loads
+        p = (int)pid[row];
+        PatternLoad<KW, TX, TV>::load(tv, row, w, v);
+        q = pref(row);
+    }
+    __syncthreads();
+    if (own) {
+        const int* pt = stab + p * (w + 1);
+        const int len = pt[0];
+        TX g[KW];
+#pragma unroll
+        for (int j = 0; j < KW; ++j) g[j] = j < len ? xf(row + pt[1 + j]) : (TX)0;
+        auto a = initf(q);
+#pragma unroll
+        for (int j = 0; j < KW; ++j)
+            if (j < len) a = accf(a, v[j] * g[j]);
+        finf(row, a, q);
+    }
+}
+
+template <bool XCD, int KW, typename TV, typename TX, class XF, class EPI>
+__global__ __launch_bounds__(kBlock) void k_csr_pattern(const BlkHdr* __restrict__ hdr, int npat, int w,
+                                                        const uint16_t* __restrict__ pid, const int* __restrict__ tab,
+                                                        const TV* __restrict__ tv, XF xf, EPI epi) {
+    __shared__ int stab[kPatMax * (kPatSlots + 1)];
+    const int b = XCD ? xcd_block() : (int)blockIdx.x;
+    auto pref = [&](int row) { return epi.pre(row); };
+    auto initf = [&](const typename EPI::Pre& q) { return epi.init(q); };
+    auto accf = [&](TX a, TX p) { return epi.acc(a, p); };
+    auto finf = [&](int row, TX a, const typename EPI::Pre& q) { epi.fin(row, a, q); };
+    pattern_block<KW, TV, TX, XF, decltype(pref), decltype(initf), decltype(accf), decltype(finf)>(
+        b, hdr, npat, w, pid, tab, tv, xf, stab, pref, initf, accf, finf);
+}
+
+// w = the form's width (<= kPatSlots): 8 or 16 unrolled slots
+template <typename TV, typename TX, class XF, class EPI>
+void launch_pattern(int nblk, const BlkHdr* hdr, int npat, int w, const uint16_t* pid, const int* tab, const TV* tv,
+                    XF xf, EPI epi, hipStream_t st) {
+    if (nblk <= 0) return;
+#define DRAUGR_PAT(X, KW)                                                                                    \
+    hipLaunchKernelGGL((k_csr_pattern<X, KW, TV, TX, XF, EPI>), dim3(nblk), dim3(kBlock), 0, st, hdr, npat, w, \
+                       pid, tab, tv, xf, epi)
+    if (w <= 8) {
+        if (nblk >= 64) DRAUGR_PAT(true, 8);
+        else DRAUGR_PAT(false, 8);
+    } else {
+        if (nblk >= 64) DRAUGR_PAT(true, 16);
+        else DRAUGR_PAT(false, 16);
+    }
+#undef DRAUGR_PAT
+}
+
 // Thread-per-row variant without LDS: fewer dependent round trips, for the
 // small coarse levels where a kernel's latency, not bandwidth, is the cost.
 // Same per-row operation order as the stream kernel.
@@ -803,9 +924,9 @@ void launch_pipe(int nblk, int per, const int* blk, const int* bz, const int* rp
 
 enum OpKernel : int {
     kOpStream = 0, kOpRowSeq = 1, kOpPipe = 2, kOpStream16 = 3, kOpSorted = 4, kOpStream16Pipe = 5, kOpSortedPipe = 6,
-    kOpSortedQuad = 7, kOpVector = 8
+    kOpSortedQuad = 7, kOpVector = 8, kOpPattern = 9
 };
-constexpr int kOpKinds = 9;
+constexpr int kOpKinds = 10;
 
 template <int PER, bool C16, bool SRT, typename TV, typename TX, class XF, class EPI>
 void launch_stream_per(int nblk, const BlkHdr* hdr, const int* rp, const int* cv, const uint16_t* c16, const TV* av,

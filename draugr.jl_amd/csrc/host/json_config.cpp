@@ -286,7 +286,8 @@ Config config_from_json(const char* json) {
         static const std::map<std::string, int> kKernel = {{"auto", -1}, {"stream", 0}, {"rowseq", 1}, {"pipe", 2},
                                                                  {"stream16", 3}, {"sorted", 4},
                                                                  {"stream16pipe", 5}, {"sortedpipe", 6},
-                                                                 {"sortedquad", 7}, {"vector", 8}};
+                                                                 {"sortedquad", 7}, {"vector", 8},
+                                                                 {"pattern", 9}};
         cfg.kernel = lookup(kKernel, to_string(*v), "kernel");
     }
     if (const JVal* v = get("filtering")) {

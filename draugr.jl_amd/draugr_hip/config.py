@@ -171,7 +171,9 @@ class AMGConfig:
     seed: int = 0x5EED
     graphs: bool = True
     # sparse-pass kernel: "auto" (autotuned), "stream", "rowseq", "pipe", "stream16", "sorted", "stream16pipe",
-    # "sortedpipe", "sortedquad", "vector" (FP32 / BF16 cycles; FP64 passes keep the stream kernel)
+    # "sortedpipe", "sortedquad", "vector" (FP32 / BF16 cycles; FP64 passes keep the stream kernel), "pattern"
+    # (operators whose rows repeat a few column patterns relative to the row, e.g. a grid stencil; others take
+    # "stream")
     kernel: str = "auto"
     fuse: str = "auto"  # zero-guess step + residual, one-pass down / up sweeps: "auto" (autotuned), "off", "on"
     gpu_setup: bool = True  # interpolation, coarse operators, triple maps and PMIS sweeps of the setup on the GPU

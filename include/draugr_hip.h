@@ -52,7 +52,9 @@ const char *draugr_amg_last_error(void);
  * captured once into a hipGraph and replayed; false: launched eagerly),
  * "kernel" ("auto" | "stream" | "rowseq" | "pipe" | "stream16" | "sorted" | "stream16pipe" |
  * "sortedpipe" | "sortedquad" | "vector" (lanes per row by row length; FP32 / BF16
- * cycles only, FP64 passes take "stream")), "fuse" ("auto"
+ * cycles only, FP64 passes take "stream") | "pattern" (operators whose rows repeat at most
+ * 64 column patterns relative to the row, as a grid stencil does: a 16-bit pattern index per
+ * row instead of a column per entry; other operators take "stream")), "fuse" ("auto"
  * | "off" | "on": a level's first smoothing step from zero fused with its
  * residual, that step, residual and restriction in one pass, and its
  * prolongation and first post-smoothing step in one pass; Jacobi-type

@@ -373,7 +373,7 @@ def test_preconditioner_adapter_with_scipy_cg(dh, torch_cuda):
 
 
 @pytest.mark.parametrize("kernel", ["stream", "rowseq", "pipe", "stream16", "sorted", "stream16pipe", "sortedpipe",
-                                    "sortedquad"])
+                                    "sortedquad", "pattern"])
 @pytest.mark.parametrize("name", ["hypre3d", "agg2d", "wide2d", "hub2d"])
 def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
     """Each of the five sparse-pass kernels the autotuner chooses among,
@@ -395,6 +395,8 @@ def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
     x = torch.tensor(x0, device="cuda")
     dh.amg_cycle_(x, torch.tensor(b, device="cuda"), h, c)
     assert np.array_equal(x.cpu().numpy(), oh.cycle(x0, b))
+    if kernel == "pattern":  # the grid operators have the form on their finest level; the hub row is too long
+        assert ("level 0: A has the pattern form" in h.tune_report()) == (name != "hub2d")
     z = torch.zeros(A.n, dtype=torch.float64, device="cuda")
     c32 = dataclasses.replace(c, precision="fp32")
     _, it, rel = dh.pcg_solve_(z, torch.tensor(b, device="cuda"), h, c32, tol=1e-8)
@@ -424,6 +426,72 @@ def test_every_kernel_choice_is_bit_exact(dh, orc, torch_cuda, kernel, name):
         _, it2, rel2 = dh.pcg_solve_(zz, torch.tensor(b, device="cuda"), hh, cc, tol=1e-8, maxiter=400)
         out.append((it2, rel2, zz.cpu().numpy()))
     assert out[0][0] == out[1][0] and out[0][1] == out[1][1] and np.array_equal(out[0][2], out[1][2])
+
+
+def _row_patterns(A):
+    """Distinct (length, col - row ...) tuples of a CSR matrix, counted
+    independently of the library (numpy over the rows)."""
+    rp, cv = np.asarray(A.rowptr), np.asarray(A.colval)
+    return {tuple((cv[rp[i]:rp[i + 1]] - i).tolist()) for i in range(len(rp) - 1)}
+
+
+@pytest.mark.parametrize("grid", ["7pt24", "5pt64"])
+def test_pattern_form(dh, orc, torch_cuda, grid):
+    """The pattern form (a 16-bit pattern index per row, values in 64-row
+    slices, no column indices): present on the finest operator of a 3-D
+    7-point grid (27 row patterns of width 7) and of a 2-D 5-point grid (9 of
+    width 5), the counts matching an independent count of the rows'
+    col - row tuples.  Forced on every level, the FP64 cycle is the oracle's
+    bit for bit and the FP32 and BF16 cycles are the stream kernel's bit for
+    bit, before and after a resetup with new values; the FP64 PCG, whose
+    q = A p pass then reads the pattern form, is the stream form's bit for bit."""
+    import dataclasses
+    torch = torch_cuda
+    P = dh.problems
+    if grid == "7pt24":
+        A = P.heterogeneous3d(24, P.log_permeability(24, 1.5, SEED + 950))
+        cfg = dh.hypre_default_config(smoother=dh.L1JacobiSmootherType())
+        sm, want = "l1_jacobi", (27, 7)
+    else:
+        A = P.poisson2d(64)
+        cfg = dh.AMGConfig(coarsening=dh.AggregationCoarsening(0.25), smoother=dh.JacobiSmootherType())
+        sm, want = "jacobi", (9, 5)
+    pats = _row_patterns(A)
+    assert (len(pats), max(len(t) for t in pats)) == want
+    cp = dataclasses.replace(cfg, kernel="pattern")
+    cs = dataclasses.replace(cfg, kernel="stream")
+    hp, hs = dh.amg_setup(A, cp), dh.amg_setup(A, cs)
+    oh = oracle_of(hp, orc, sm)
+    b = torch.tensor(P.splitmix_normal(SEED + 951, A.n), device="cuda")
+    x0 = P.splitmix_normal(SEED + 952, A.n)
+    for stage in ("setup", "resetup"):
+        if stage == "resetup":
+            s = np.exp(0.3 * np.random.default_rng(SEED + 953).standard_normal(A.n))
+            rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+            new = A.nzval * s[rows] * s[A.colval]
+            for hh, cc in ((hp, cp), (hs, cs)):
+                dh.amg_resetup_(hh, P.CSR(A.rowptr, A.colval, new), cc)
+            oh.resetup(new)
+        x = torch.tensor(x0, device="cuda")
+        dh.amg_cycle_(x, b, hp, cp)
+        assert np.array_equal(x.cpu().numpy(), oh.cycle(x0, b.cpu().numpy()))
+        if stage == "setup":
+            assert "level 0: A has the pattern form (%d row patterns, width %d)" % want in hp.tune_report()
+        for prec in ("fp32", "bf16"):
+            y = []
+            for hh, cc in ((hp, cp), (hs, cs)):
+                yy = torch.tensor(x0, device="cuda")
+                dh.amg_cycle_(yy, b, hh, dataclasses.replace(cc, precision=prec))
+                y.append(yy.cpu().numpy())
+            assert np.array_equal(y[0], y[1]), (stage, prec)
+        out = []
+        for hh, cc in ((hp, cp), (hs, cs)):
+            zz = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+            _, it, rel = dh.pcg_solve_(zz, b, hh, cc, tol=1e-8, maxiter=400)
+            out.append((it, rel, zz.cpu().numpy()))
+        assert out[0][1] < 1e-8 and out[0][0] == out[1][0] and out[0][1] == out[1][1]
+        assert np.array_equal(out[0][2], out[1][2])
+    assert "pcg q = A p: pattern" in hp.tune_report()
 
 
 @pytest.mark.parametrize("name", ["hypre3d", "agg2d", "hub2d"])
