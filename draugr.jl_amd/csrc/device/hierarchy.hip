@@ -2120,10 +2120,11 @@ int DeviceHierarchy::pcg_form(const DevCSR& A, hipStream_t st) {
 
 // FP64 preconditioned CG (Hestenes-Stiefel) with z = V(0; r).  Stopping test
 // on the recurrence residual ||r||/||b|| < tol.  Kernel fusion: p = z + beta p
-// is formed inside the q = A p pass (ping-pong p), the pass also reduces p.q
-// and sets alpha on the device; the x/r update reduces r.r.  The cycle for the
-// next z is enqueued before the host reads r.r, so the GPU never idles on the
-// per-iteration convergence check (the last cycle is speculative).
+// is formed inside the q = A p pass (ping-pong p), the pass also reduces p.q,
+// sets alpha on the device and applies the previous iteration's x update; the
+// r update reduces r.r.  The cycle for the next z is enqueued before the host
+// reads r.r, so the GPU never idles on the per-iteration convergence check
+// (the last cycle is speculative).
 //
 // ws (allocation-free entry point): the device scalars and the reduction
 // partials live in caller-provided memory instead of the hierarchy's, and the
@@ -2229,13 +2230,20 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
     int result = maxiter;
     try {
         for (int it = 1; it <= maxiter; ++it) {
-            // p_new = z + beta p_old; q = A p_new; alpha = rz / (p_new . q)
+            // p_new = z + beta p_old; q = A p_new; alpha = rz / (p_new . q).
+            // The pass also applies the previous iteration's x += alpha p_old
+            // (it reads p_old anyway; alpha still holds that iteration's value
+            // until this pass's finalize): the same update, one pass later.
             const double* betap = scal + S_BETA;
+            const double* alphap = scal + S_ALPHA;
+            const bool xupd = it > 1;
             auto fin_pq = [=] __device__(double pq) { scal[S_PQ] = pq; scal[S_ALPHA] = scal[S_RZ] / pq; };
             if (mixed) {
                 auto xf = [=] __device__(int j) -> double { return (double)z32[j] + *betap * pa[j]; };
                 auto epi = [=] __device__(int i, double s) -> double {
-                    const double pn = (double)z32[i] + *betap * pa[i];
+                    const double po = pa[i];
+                    const double pn = (double)z32[i] + *betap * po;
+                    if (xupd) x[i] += *alphap * po;
                     pb[i] = pn;
                     q[i] = s;
                     return pn * s;
@@ -2244,17 +2252,18 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
             } else {
                 auto xf = [=] __device__(int j) -> double { return z[j] + *betap * pa[j]; };
                 auto epi = [=] __device__(int i, double s) -> double {
-                    const double pn = z[i] + *betap * pa[i];
+                    const double po = pa[i];
+                    const double pn = z[i] + *betap * po;
+                    if (xupd) x[i] += *alphap * po;
                     pb[i] = pn;
                     q[i] = s;
                     return pn * s;
                 };
                 launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, part, st, form);
             }
-            // x += alpha p; r -= alpha q; r.r  (and the FP32 copy of r).  The
-            // finalize also writes r.r and the iteration number straight into
-            // mapped pinned host memory, so no copy is enqueued for the check.
-            const double* alphap = scal + S_ALPHA;
+            // r -= alpha q; r.r  (and the FP32 copy of r).  The finalize also
+            // writes r.r and the iteration number straight into mapped pinned
+            // host memory, so no copy is enqueued for the check.
             double* hp = pinned_dev_;
             const double itd = (double)it;
             auto fin_rr = [=] __device__(double s) {
@@ -2266,7 +2275,6 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                 launch_reduce(n,
                               [=] __device__(int64_t i) {
                                   const double a = *alphap;
-                                  x[i] += a * pb[i];
                                   const double ri = r[i] - a * q[i];
                                   r[i] = ri;
                                   const float rf = (float)ri;
@@ -2279,7 +2287,6 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                 launch_reduce(n,
                               [=] __device__(int64_t i) {
                                   const double a = *alphap;
-                                  x[i] += a * pb[i];
                                   const double ri = r[i] - a * q[i];
                                   r[i] = ri;
                                   if (x1_0) t64[i] = (w64 * d64[i]) * ri;
@@ -2307,6 +2314,11 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
             if (verbose >= 2) std::printf("  PCG iter %4d: relative residual = %.6e\n", it, rel);
             if (rel < tol) { result = it; break; }
             if (!std::isfinite(rel)) throw Error("PCG diverged (non-finite residual)");
+        }
+        if (maxiter >= 1) {  // the last iteration's x += alpha p (p in pa after the swap)
+            const double* alphap = scal + S_ALPHA;
+            const double* pl = pa;
+            launch_map(n, [=] __device__(int64_t i) { x[i] += *alphap * pl[i]; }, st);
         }
     } catch (...) {
         (void)hipStreamSynchronize(st);
