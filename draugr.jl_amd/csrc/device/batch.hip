@@ -306,39 +306,16 @@ void kpass(int n, int ks, const int* rp, const int* cv, const TV* av, XF xf, EPI
         hipLaunchKernelGGL((k_kpass<TV, TX, XF, EPI>), dim3(nb), dim3(kBlock), 0, st, n, ks, rp, cv, av, xf, epi);
 }
 
-__device__ __forceinline__ double bcast(double v, int lane) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-
 // Coarsest solves, one wavefront per (system, rhs), n <= 64: LAPACK getrs
-// order (forward substitution in ascending, back substitution in descending
-// column order), as the single-system wave kernel.
+// order, the single-system wave kernel's solve (wave_lu_solve).
 template <typename TX>
 __global__ __launch_bounds__(64) void k_kcoarse_wave(const CoarseDesc* __restrict__ cd, int ks,
                                                      const TX* __restrict__ b, TX* __restrict__ x) {
     const CoarseDesc c = cd[blockIdx.x];
     const int j = blockIdx.y, i = threadIdx.x, n = c.n;
     double row[64];
-#pragma unroll
-    for (int q = 0; q < 64; ++q) row[q] = (q < n && i < n) ? c.lu[(size_t)q * n + i] : 0.0;
-    double xi = i < n ? (double)b[kidx(c.row0 + c.perm[i], j, ks)] : 0.0;
-#pragma unroll
-    for (int q = 0; q < 64; ++q) {
-        if (q < n) {
-            const double xq = bcast(xi, q);
-            if (i > q && i < n) xi -= row[q] * xq;
-        }
-    }
-#pragma unroll
-    for (int q = 63; q >= 0; --q) {
-        if (q < n) {
-            if (i == q) xi /= row[q];
-            const double xq = bcast(xi, q);
-            if (i < q) xi -= row[q] * xq;
-        }
-    }
+    dev::wave_lu_row(row, n, i, [&](int q) { return c.lu[(size_t)q * n + i]; });
+    const double xi = dev::wave_lu_solve(i < n ? (double)b[kidx(c.row0 + c.perm[i], j, ks)] : 0.0, row, n, i);
     if (i < n) x[kidx(c.row0 + i, j, ks)] = (TX)xi;
 }
 

@@ -510,39 +510,17 @@ __global__ __launch_bounds__(kBlock) void k_coarse_lu_solve(int n, const double*
 }
 
 // The same solve for n_c <= 64 in one wavefront without barriers: lane i holds
-// row i of the factors in registers and x_i; x_j is broadcast from lane j with
-// readlane at step j.  Per-element operation order is that of
-// k_coarse_lu_solve; the row interchanges are applied as one permutation.
-__device__ __forceinline__ double bcast_lane(double v, int lane) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-
+// row i of the factors in registers and x_i (wave_lu_solve); per-element
+// operation order is that of k_coarse_lu_solve; the row interchanges are
+// applied as one permutation.
 template <typename TX>
 __global__ __launch_bounds__(64) void k_coarse_lu_solve_wave(int n, const double* __restrict__ lu,
                                                              const int* __restrict__ perm,
                                                              const TX* __restrict__ b, TX* __restrict__ x) {
     const int i = threadIdx.x;
     double row[64];
-#pragma unroll
-    for (int j = 0; j < 64; ++j) row[j] = (j < n && i < n) ? lu[(size_t)j * n + i] : 0.0;
-    double xi = i < n ? (double)b[perm[i]] : 0.0;
-#pragma unroll
-    for (int j = 0; j < 64; ++j) {
-        if (j < n) {
-            const double xj = bcast_lane(xi, j);
-            if (i > j && i < n) xi -= row[j] * xj;
-        }
-    }
-#pragma unroll
-    for (int j = 63; j >= 0; --j) {
-        if (j < n) {
-            if (i == j) xi /= row[j];
-            const double xj = bcast_lane(xi, j);
-            if (i < j) xi -= row[j] * xj;
-        }
-    }
+    wave_lu_row(row, n, i, [&](int j) { return lu[(size_t)j * n + i]; });
+    const double xi = wave_lu_solve(i < n ? (double)b[perm[i]] : 0.0, row, n, i);
     if (i < n) x[i] = (TX)xi;
 }
 

@@ -31,6 +31,55 @@ namespace dev {
 constexpr int kTailThreads = 512;
 constexpr int kTailLds = 156 * 1024;
 
+__device__ __forceinline__ double bcast_lane(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Forward and back substitution in one wavefront for n <= 64 unknowns, lane i
+// holding row i of the LU factors and x_i: the operations of the reference's
+// getrs loops -- x_i -= L_ij x_j for i > j with j ascending, then x_j /= U_jj
+// and x_i -= U_ij x_j for i < j with j descending -- x_j broadcast from lane
+// j with readlane.  Every lane forms a step's candidate and keeps it by a
+// select (no divergent branch); the pivot division is formed on the broadcast
+// operands x_j and U_jj, the same IEEE quotient in every lane.  Lanes i >= n
+// hold the identity row and x_i = 0 (wave_lu_row), so that the steps j >= n
+// inside the last group of 8 leave every value unchanged (x - 0 * 0); one
+// uniform branch per group of 8 steps, not per step, skips the rest.
+template <class AT>
+__device__ __forceinline__ void wave_lu_row(double (&row)[64], int n, int i, AT at) {  // at(j) = LU(i, j)
+#pragma unroll
+    for (int j = 0; j < 64; ++j) row[j] = (j < n && i < n) ? at(j) : (i == j ? 1.0 : 0.0);
+}
+
+__device__ __forceinline__ double wave_lu_solve(double xi, const double (&row)[64], int n, int i) {
+    const bool in = i < n;
+#pragma unroll
+    for (int j0 = 0; j0 < 64; j0 += 8) {
+        if (j0 < n) {
+#pragma unroll
+            for (int j = j0; j < j0 + 8; ++j) {
+                const double xj = bcast_lane(xi, j);
+                const double t = xi - row[j] * xj;
+                xi = (in && i > j) ? t : xi;
+            }
+        }
+    }
+#pragma unroll
+    for (int j0 = 56; j0 >= 0; j0 -= 8) {
+        if (j0 < n) {
+#pragma unroll
+            for (int j = j0 + 7; j >= j0; --j) {
+                const double xd = bcast_lane(xi, j) / bcast_lane(row[j], j);
+                const double t = xi - row[j] * xd;
+                xi = i == j ? xd : (i < j ? t : xi);
+            }
+        }
+    }
+    return xi;
+}
+
 // Byte offsets of the image's segments (16-byte aligned) and of the LDS
 // vectors that follow the image.
 struct TailLayout {
@@ -137,14 +186,24 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_lds(TailRun<TX> R) {
         else return (TX)av[k];
     };
     // one sparse pass: every product of the operator formed in parallel (each
-    // rounded on its own), then every row summed on its own thread in row
-    // order: a = init(i); a = acc(a, p_k); fin(i, a)
+    // rounded on its own; eight per thread formed before any is stored, so
+    // their LDS reads overlap), then every row summed on its own thread in
+    // row order: a = init(i); a = acc(a, p_k); fin(i, a)
     auto pass = [&](const int* rp, int nrow, int nnz, auto prodf, auto initf, auto accf, auto finf) {
-        for (int k = t; k < nnz; k += kTailThreads) prod[k] = prodf(k);
+        constexpr int kU = 8;
+        for (int k0 = t; k0 < nnz; k0 += kU * kTailThreads) {
+            TX v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) v[u] = prodf(min(k0 + u * kTailThreads, nnz - 1));
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (k0 + u * kTailThreads < nnz) prod[k0 + u * kTailThreads] = v[u];
+        }
         __syncthreads();
         for (int i = t; i < nrow; i += kTailThreads) {
             auto a = initf(i);
             const int e = rp[i + 1];
+#pragma unroll 8
             for (int k = rp[i]; k < e; ++k) a = accf(a, prod[k]);
             finf(i, a);
         }
@@ -178,30 +237,9 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_lds(TailRun<TX> R) {
     // coarsest solve by wave 0: the operations of k_coarse_lu_solve_wave
     if (t < 64) {
         const int i = t;
-        auto bcast = [](double v, int lane) {
-            const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-            const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-            return __hiloint2double(hi, lo);
-        };
         double row[64];
-#pragma unroll
-        for (int j = 0; j < 64; ++j) row[j] = (j < nc && i < nc) ? lu[j * nc + i] : 0.0;
-        double xi = i < nc ? (double)bc[perm[i]] : 0.0;
-#pragma unroll
-        for (int j = 0; j < 64; ++j) {
-            if (j < nc) {
-                const double xj = bcast(xi, j);
-                if (i > j && i < nc) xi -= row[j] * xj;
-            }
-        }
-#pragma unroll
-        for (int j = 63; j >= 0; --j) {
-            if (j < nc) {
-                if (i == j) xi /= row[j];
-                const double xj = bcast(xi, j);
-                if (i < j) xi -= row[j] * xj;
-            }
-        }
+        wave_lu_row(row, nc, i, [&](int j) { return lu[j * nc + i]; });
+        const double xi = wave_lu_solve(i < nc ? (double)bc[perm[i]] : 0.0, row, nc, i);
         if (i < nc) xc[i] = (TX)xi;
     }
     __syncthreads();
