@@ -89,12 +89,24 @@ __global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, F f, double* parti
     block_sum_store(acc, sh, partial, blockIdx.x);
 }
 
+// Grid of the vector reductions over n elements (at most kRedBlocks).
+inline int red_blocks(int64_t n) {
+    const int64_t nb = (n + kBlock - 1) / kBlock;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(nb, kRedBlocks));
+}
+
 template <class F, class FIN>
 void launch_reduce(int64_t n, F f, FIN fin, double* partial, hipStream_t st) {
-    int64_t nb = (n + kBlock - 1) / kBlock;
-    nb = std::max<int64_t>(1, std::min<int64_t>(nb, kRedBlocks));
+    const int nb = red_blocks(n);
     hipLaunchKernelGGL((k_reduce<F>), dim3((unsigned)nb), dim3(kBlock), 0, st, n, f, partial);
-    hipLaunchKernelGGL((k_finalize<FIN>), dim3(1), dim3(kBlock), 0, st, partial, (int)nb, fin);
+    hipLaunchKernelGGL((k_finalize<FIN>), dim3(1), dim3(kBlock), 0, st, partial, nb, fin);
+}
+
+// The vector pass and its per-block partials only (red_blocks(n) of them),
+// for a sum the host finishes.
+template <class F>
+void launch_reduce_partials(int64_t n, F f, double* partial, hipStream_t st) {
+    hipLaunchKernelGGL((k_reduce<F>), dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, st, n, f, partial);
 }
 
 // CSR-stream pass whose per-row epilogue also returns a contribution to a
@@ -1042,9 +1054,10 @@ void DeviceHierarchy::build(const HostHierarchy& h) {
     scal_.alloc(S_COUNT_);
     partial_.alloc(max_blocks);
     DRAUGR_HIP_CHECK(hipMemsetAsync(scal_.p, 0, S_COUNT_ * sizeof(double), st));
-    DRAUGR_HIP_CHECK(hipHostMalloc((void**)&pinned_, 16 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+    DRAUGR_HIP_CHECK(hipHostMalloc((void**)&pinned_, (kPinPart + kRedBlocks) * sizeof(double),
+                                   hipHostMallocMapped | hipHostMallocCoherent));
     DRAUGR_HIP_CHECK(hipHostGetDevicePointer((void**)&pinned_dev_, pinned_, 0));
-    for (int i = 0; i < 16; ++i) pinned_[i] = 0.0;
+    for (int i = 0; i < kPinPart + kRedBlocks; ++i) pinned_[i] = 0.0;
     for (int l = 0; l < nlevels(); ++l) compute_dinv(l, st);
     if (smoother_ == Smoother::Chebyshev) chebyshev_bounds(st);
     coarse_factor(st);
@@ -1082,12 +1095,29 @@ void DeviceHierarchy::release_handles() {
     last_valid_ = false;
 }
 
+// The sum k_finalize forms from nblocks partials (pinned_ + kPinPart), in its
+// order: lane t adds partials t, t + 256, ... one at a time, then the lanes'
+// sums are added in the same tree, so the value is the same bits.
+double DeviceHierarchy::host_finalize(int nblocks) const {
+    const volatile double* p = pinned_ + kPinPart;
+    double acc[kBlock];
+    for (int t = 0; t < kBlock; ++t) {
+        double a = 0.0;
+        for (int i = t; i < nblocks; i += kBlock) a += p[i];
+        acc[t] = a;
+    }
+    for (int s = kBlock / 2; s > 0; s >>= 1)
+        for (int t = 0; t < s; ++t) acc[t] += acc[t + s];
+    return acc[0];
+}
+
 int64_t DeviceHierarchy::bytes_resident() const {
     int64_t b = 0;
     auto csr = [&](const DevCSR& m) {
         b += m.rp.bytes() + m.cv.bytes() + m.v64.bytes() + m.v32.bytes() + m.v16.bytes() + m.blk.bytes();
         b += m.c16.bytes() + m.hdr.bytes() + m.shdr.bytes() + m.scs.bytes() + m.sv64.bytes() + m.sv32.bytes() +
              m.sv16.bytes();
+        b += m.tpid.bytes() + m.ttab.bytes() + m.tv64.bytes() + m.tv32.bytes() + m.tv16.bytes();
     };
     for (auto& l : L_) {
         csr(l.A); csr(l.P); csr(l.Pt);
@@ -2239,18 +2269,14 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                 };
                 launch_stream_reduce<double, double>(A, A.v64.p, xf, epi, fin_pq, part, st, form);
             }
-            // r -= alpha q; r.r  (and the FP32 copy of r).  The finalize also
-            // writes r.r and the iteration number straight into mapped pinned
-            // host memory, so no copy is enqueued for the check.
-            double* hp = pinned_dev_;
-            const double itd = (double)it;
-            auto fin_rr = [=] __device__(double s) {
-                scal[S_RR] = s;
-                hp[kPinRR] = s;
-                hp[kPinIter] = itd;
-            };
+            // r -= alpha q; r.r  (and the FP32 copy of r).  The per-block
+            // partials of r.r go straight into mapped pinned host memory and
+            // the host sums them (host_finalize): no finalize launch, and no
+            // copy is enqueued for the convergence check.
+            double* hp = pinned_dev_ + kPinPart;
+            const int nbr = red_blocks(n);
             if (mixed)
-                launch_reduce(n,
+                launch_reduce_partials(n,
                               [=] __device__(int64_t i) {
                                   const double a = *alphap;
                                   const double ri = r[i] - a * q[i];
@@ -2260,9 +2286,9 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                                   if (x1_0) t32[i] = (w32 * d32[i]) * rf;
                                   return ri * ri;
                               },
-                              fin_rr, part, st);
+                              hp, st);
             else
-                launch_reduce(n,
+                launch_reduce_partials(n,
                               [=] __device__(int64_t i) {
                                   const double a = *alphap;
                                   const double ri = r[i] - a * q[i];
@@ -2270,7 +2296,7 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
                                   if (x1_0) t64[i] = (w64 * d64[i]) * ri;
                                   return ri * ri;
                               },
-                              fin_rr, part, st);
+                              hp, st);
             DRAUGR_HIP_CHECK(hipEventRecord(ev, st));
             // speculative: next preconditioned residual and beta
             apply_prec(oit);
@@ -2283,10 +2309,8 @@ int DeviceHierarchy::pcg(double* x, const double* b, double tol, int maxiter, co
             else
                 launch_reduce(n, [=] __device__(int64_t i) { return r[i] * z[i]; }, fin_rz, part, st);
             std::swap(pa, pb);
-            DRAUGR_HIP_CHECK(hipEventSynchronize(ev));
-            double rr = pinned_[kPinRR];
-            if (pinned_[kPinIter] != (double)it)  // not visible yet: read the device copy
-                DRAUGR_HIP_CHECK(hipMemcpy(&rr, scal + S_RR, sizeof(double), hipMemcpyDeviceToHost));
+            DRAUGR_HIP_CHECK(hipEventSynchronize(ev));  // the r update has completed: its partials are visible
+            const double rr = host_finalize(nbr);
             rel = std::sqrt(rr / bb);
             if (relres) *relres = rel;
             if (verbose >= 2) std::printf("  PCG iter %4d: relative residual = %.6e\n", it, rel);

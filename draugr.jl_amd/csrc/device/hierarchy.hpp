@@ -281,9 +281,13 @@ private:
     DBuf<double> w_r_, w_p0_, w_p1_, w_q_, w_z_, w_x_;
     DBuf<float> w_r32_, w_z32_;
     DBuf<double> scal_, partial_;
-    double* pinned_ = nullptr;      // mapped, coherent host memory
+    // mapped, coherent host memory: [0, 16) scalars read back by the host,
+    // [kPinPart, kPinPart + kRedBlocks) the per-block partials of the PCG's
+    // r.r, which the host sums itself
+    double* pinned_ = nullptr;
     double* pinned_dev_ = nullptr;  // its device address
-    static constexpr int kPinRR = 8, kPinIter = 9;
+    static constexpr int kPinPart = 16;
+    double host_finalize(int nblocks) const;
     hipStream_t stream_ = nullptr;
     hipEvent_t last_ev_ = nullptr;
     hipEvent_t pcg_ev_ = nullptr;  // marks each PCG iteration's r.r (the host waits on it)

@@ -148,8 +148,10 @@ int32_t draugr_hip_pcg(int32_t handle, int32_t n, double *x, const double *b, in
  * reference; for callers that cannot allocate device memory during a solve
  * (tight serving loops, code that must not call hipMalloc).  Same arguments,
  * kernels and results (bit for bit) as draugr_hip_pcg_device.  The PCG's
- * device scalars and reduction partials live in a static device array of the
- * library (513 KiB per GPU, kept until the process ends), so the finest
+ * device scalars and the partials of its device-side sums live in a static
+ * device array of the library (513 KiB per GPU, kept until the process ends;
+ * the partials of r.r, which the host sums, go to the hierarchy's own pinned
+ * host buffer, made with the hierarchy), so the finest
  * operator may have at most 65536 row blocks (about 16 M rows); larger
  * hierarchies get -1 with a message.  The hierarchy must already be prepared
  * for the config's precision by one call of draugr_hip_pcg_device (which
