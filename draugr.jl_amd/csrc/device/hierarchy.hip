@@ -1545,13 +1545,29 @@ void DeviceHierarchy::refresh_tails(hipStream_t st) {
     if (tail_img_[2].p) pack_tail<bf16_t, float>(st);
 }
 
+// rin non-null: the parent's restriction of rin is formed in the launch (else
+// the tail reads the parent's bc); xpar non-null: the parent's prolongation
+// into xpar is applied in the launch (else the parent applies it).
 template <typename TV, typename TX>
-void DeviceHierarchy::launch_tail(const CycleOpts& o, hipStream_t st) {
+void DeviceHierarchy::launch_tail(const CycleOpts& o, hipStream_t st, const TX* rin, TX* xpar) {
     typedef typename TransferT<TV>::type TP;
     const int pi = pidx<TV>();
     DevLevel& parent = L_[nlevels() - 2];
     TailRun<TX> run{tail_img_[pi].p, bc_of<TX>(parent), xc_of<TX>(parent), (TX)weight(), o.pre, o.post,
                     tail_lay_[pi]};
+    if (rin) {
+        run.trp = parent.Pt.rp.p;
+        run.tcv = parent.Pt.cv.p;
+        run.tvv = vals<TP>(parent.Pt);
+        run.rin = rin;
+    }
+    if (xpar) {
+        run.prp = parent.P.rp.p;
+        run.pcv = parent.P.cv.p;
+        run.pvv = vals<TP>(parent.P);
+        run.xpar = xpar;
+        run.npar = parent.A.nrow;
+    }
     hipLaunchKernelGGL((k_tail_lds<TV, TP, TX>), dim3(1), dim3(kTailThreads), 0, st, run);
 }
 
@@ -1890,7 +1906,14 @@ void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const Cy
     const bool child_x1 = zero_step_is_map<TV>(l + 1, o.pre);
     TX* x1c = child_x1 ? t_of<TX>(L_[l + 1]) : nullptr;
     const TX* dc = child_x1 ? d_of<TX>(L_[l + 1]) : nullptr;
-    if (zero && jac && o.pre == 1 && D.fuse_down[pi]) {
+    // the next level is the LDS-resident tail: this level's restriction and
+    // prolongation ride in its launch unless a one-pass sweep does them
+    const bool tail_next = l + 1 < nlevels() && l + 1 == tail_lt_[pi] && !o.w_cycle && jac;
+    const bool down_fused = zero && jac && o.pre == 1 && D.fuse_down[pi];
+    const bool up_fused = jac && o.post >= 1 && D.fuse_up[pi];
+    const bool fold_in = tail_next && !down_fused;
+    const bool fold_out = tail_next && !up_fused;
+    if (down_fused) {
         // first step from zero, residual and restriction in one launch
         launch_down<TV, TX>(l, b, xalt, bc, dc, x1c, st);
         std::swap(xcur, xalt);
@@ -1898,13 +1921,17 @@ void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const Cy
         bool resid_done = false;
         smooth<TV, TX>(l, xcur, xalt, b, o.pre, zero, true, resid_done, st, x1_ready && zero);
         if (!resid_done) op<TV, TX>(D.A, XVec<TX>{xcur}, EpiResid<TX>{b, r}, st);
-        if (child_x1)
+        if (fold_in) {
+            // the restriction is formed by the tail's launch
+        } else if (child_x1) {
             op<TP, TX>(D.Pt, XVec<TX>{r}, EpiStoreJac0<TX>{bc, dc, x1c, (TX)weight()}, st);
-        else
+        } else {
             op<TP, TX>(D.Pt, XVec<TX>{r}, EpiStore<TX>{bc}, st);
+        }
     }
-    if (l + 1 < nlevels() && l + 1 == tail_lt_[pi] && !o.w_cycle && jac) {
-        launch_tail<TV, TX>(o, st);  // the last level and the coarse solve in one launch
+    if (tail_next) {
+        // the last level and the coarse solve in one launch
+        launch_tail<TV, TX>(o, st, fold_in ? r : nullptr, fold_out ? xcur : nullptr);
     } else if (l + 1 < nlevels()) {
         const int reps = o.w_cycle ? 2 : 1;
         for (int rep = 0; rep < reps; ++rep) cycle_level<TV, TX>(l + 1, xc, bc, rep == 0, o, st, rep == 0 && child_x1);
@@ -1919,7 +1946,7 @@ void DeviceHierarchy::cycle_level(int l, TX* x, const TX* b, bool zero, const Cy
         std::swap(xcur, xalt);
         smooth<TV, TX>(l, xcur, xalt, b, o.post - 1, z, false, dummy, st, false, o.gs_sym);
     } else {
-        op<TP, TX>(D.P, XVec<TX>{xc}, EpiAdd<TX>{xcur}, st);
+        if (!fold_out) op<TP, TX>(D.P, XVec<TX>{xc}, EpiAdd<TX>{xcur}, st);
         smooth<TV, TX>(l, xcur, xalt, b, o.post, z, false, dummy, st, false, o.gs_sym);
     }
     if (xcur != x) {

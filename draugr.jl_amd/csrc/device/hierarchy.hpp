@@ -246,7 +246,8 @@ private:
     template <typename TV, typename TX> void pack_tail(hipStream_t st);
     void refresh_tails(hipStream_t st);
     template <typename TV, typename TX> void tune_tail();
-    template <typename TV, typename TX> void launch_tail(const CycleOpts& o, hipStream_t st);
+    template <typename TV, typename TX>
+    void launch_tail(const CycleOpts& o, hipStream_t st, const TX* rin = nullptr, TX* xpar = nullptr);
     void compute_dinv(int l, hipStream_t st);
     double estimate_rho(int l, hipStream_t st);  // power iteration on D^-1 A (FP64)
     void chebyshev_bounds(hipStream_t st);

@@ -11,7 +11,9 @@
 // prolongation, post-smoothing -- and writes the level's iterate back.  Every
 // row is summed on its own thread in row order with the products rounded one
 // by one and the stream kernels' epilogue arithmetic, so the values are the
-// same bits as the separate launches'.
+// same bits as the separate launches'.  The parent level's restriction (into
+// this level's right-hand side) and prolongation (of this level's iterate)
+// can be folded into the same launch, read from global memory (TailRun).
 //
 // TV: values of A (double, float or bf16); TP: values of P and P^T; TX:
 // vectors.  Eligible (checked on the host): a Jacobi-type smoother, a
@@ -133,6 +135,21 @@ template <typename TX> struct TailRun {
     TX w;                // smoother weight
     int pre, post;
     TailLayout lay;
+    // The parent level's restriction and prolongation folded into the launch
+    // (values of P^T and P in the transfer type TP of the kernel):
+    // trp set: b = P^T rin formed here (rows of P^T in global memory, each
+    // summed in row order as the restriction pass sums it) instead of read;
+    // prp set: xpar += P x at the end (each of the parent's npar rows in row
+    // order, as the prolongation pass adds it).
+    const int* trp = nullptr;
+    const int* tcv = nullptr;
+    const void* tvv = nullptr;
+    const TX* rin = nullptr;
+    const int* prp = nullptr;
+    const int* pcv = nullptr;
+    const void* pvv = nullptr;
+    TX* xpar = nullptr;
+    int npar = 0;
 };
 
 template <typename TV, typename TP, typename TX>
@@ -177,7 +194,17 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_lds(TailRun<TX> R) {
     TX* r = reinterpret_cast<TX*>(sm + Y.v_r);
     TX* bc = reinterpret_cast<TX*>(sm + Y.v_bc);
     TX* xc = reinterpret_cast<TX*>(sm + Y.v_xc);
-    for (int i = t; i < n; i += kTailThreads) b[i] = R.b[i];
+    if (R.trp) {  // the parent's restriction: b_I = sum_k P^T_Ik rin_k, products rounded one by one, row order
+        const TP* ptv = static_cast<const TP*>(R.tvv);
+        for (int I = t; I < n; I += kTailThreads) {
+            TX a = (TX)0;
+            const int e = R.trp[I + 1];
+            for (int k = R.trp[I]; k < e; ++k) a = a + ld<TX, TP>(ptv, k) * R.rin[R.tcv[k]];
+            b[I] = a;
+        }
+    } else {
+        for (int i = t; i < n; i += kTailThreads) b[i] = R.b[i];
+    }
     TX* prod = reinterpret_cast<TX*>(sm + Y.v_prod);
     __syncthreads();
     const TX w = R.w;
@@ -251,6 +278,15 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_lds(TailRun<TX> R) {
         TX* q = x; x = alt; alt = q;
     }
     for (int i = t; i < n; i += kTailThreads) R.x[i] = x[i];
+    if (R.prp) {  // the parent's prolongation: xpar_i += sum_k P_ik x_k, one product at a time
+        const TP* ppv = static_cast<const TP*>(R.pvv);
+        for (int i = t; i < R.npar; i += kTailThreads) {
+            TX a = R.xpar[i];
+            const int e = R.prp[i + 1];
+            for (int k = R.prp[i]; k < e; ++k) a = a + ld<TX, TP>(ppv, k) * x[R.pcv[k]];
+            R.xpar[i] = a;
+        }
+    }
 }
 
 }  // namespace dev

@@ -157,13 +157,17 @@ def test_resetup_rejects_wrong_device_values(dh, torch_cuda):
     dh.amg_resetup_(h, nz, cfg)  # the valid form still works
 
 
+@pytest.mark.parametrize("fuse", ["auto", "off", "on"])
 @pytest.mark.parametrize("name", ["hypre3d", "agg2d", "wide2d", "hub2d"])
-def test_tail_in_one_launch_is_bit_exact(dh, orc, torch_cuda, name):
+def test_tail_in_one_launch_is_bit_exact(dh, orc, torch_cuda, name, fuse):
     """The last AMG level and the coarse solve in one single-workgroup launch
     from LDS (config tail = "on") give the oracle's FP64 cycle bit for bit,
     V-cycles with several step counts (the W-cycle keeps the per-level
     launches), and the separate launches' FP32 / BF16 cycles bit for bit; PCG
-    matches the oracle's; a resetup refreshes the LDS image."""
+    matches the oracle's; a resetup refreshes the LDS image.  With the
+    one-pass sweeps off the parent level's restriction and prolongation ride
+    in the tail's launch; with them on (where they apply) the parent's own
+    one-pass launches do them."""
     import dataclasses
     from test_gpu_parity import _case
     torch = torch_cuda
@@ -171,7 +175,7 @@ def test_tail_in_one_launch_is_bit_exact(dh, orc, torch_cuda, name):
     # one row-order kernel on both hierarchies, so that the two differ only in
     # the tail (each autotuner could pick the vector CSR kernel, whose FP32
     # sums are in another order, on different levels)
-    cfg = dataclasses.replace(cfg, kernel="sorted")
+    cfg = dataclasses.replace(cfg, kernel="sorted", fuse=fuse)
     on, off = dataclasses.replace(cfg, tail="on"), dataclasses.replace(cfg, tail="off")
     h_on, h_off = dh.amg_setup(A, on), dh.amg_setup(A, off)
     oh = oracle_of(h_on, orc, sm)
