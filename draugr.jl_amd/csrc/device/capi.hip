@@ -111,13 +111,17 @@ CSR csr_from_raw(int32_t n, int32_t nnz, const int32_t* rp, const int32_t* cv, c
     return A;
 }
 
-CycleOpts opts_of(const Config& c) {
+// graphs "auto": replayed graphs for the batched PCG (its iterations are
+// many launches over all systems), eager launches for one system (the
+// calling thread keeps ahead of the GPU, and a switch between graph and
+// eager submissions on a stream costs more than the launches it saves)
+CycleOpts opts_of(const Config& c, bool batched = false) {
     CycleOpts o;
     o.w_cycle = c.w_cycle;
     o.pre = c.pre_smoothing_steps;
     o.post = c.post_smoothing_steps;
     o.prec = c.precision;
-    o.graphs = c.graphs;
+    o.graphs = c.graphs < 0 ? batched : c.graphs != 0;
     o.gs_sym = c.gs_symmetric;
     return o;
 }
@@ -723,7 +727,7 @@ int32_t draugr_hip_batch_cycle_device(int32_t batch, double* x, const double* b,
         std::lock_guard<std::mutex> g(B->mu);
         hipStream_t st = (hipStream_t)stream;
         batch_begin(*B, st);
-        B->dev->cycle(x, b, opts_of(cfg), st);
+        B->dev->cycle(x, b, opts_of(cfg, true), st);
         batch_end(*B, st);
         return 0;
     });
@@ -740,7 +744,7 @@ int32_t draugr_hip_batch_pcg_device(int32_t batch, double* x, const double* b, i
         std::lock_guard<std::mutex> g(B->mu);
         hipStream_t st = (hipStream_t)stream;
         batch_begin(*B, st);
-        const int it = B->dev->pcg(x, b, tol, maxiter, opts_of(cfg), st, iters, relres);
+        const int it = B->dev->pcg(x, b, tol, maxiter, opts_of(cfg, true), st, iters, relres);
         batch_end(*B, st);
         return it;
     });

@@ -264,7 +264,14 @@ Config config_from_json(const char* json) {
     // backend extensions
     if (const JVal* v = get("precision")) cfg.precision = lookup(kPrecision, to_string(*v), "precision");
     if (const JVal* v = get("seed")) cfg.seed = (uint64_t)std::llround(to_float(*v));
-    if (const JVal* v = get("graphs")) cfg.graphs = to_bool(*v);
+    if (const JVal* v = get("graphs")) {
+        if (v->kind == JVal::Str) {
+            static const std::map<std::string, int> kGraphs = {{"auto", -1}};
+            cfg.graphs = lookup(kGraphs, to_string(*v), "graphs");
+        } else {
+            cfg.graphs = to_bool(*v) ? 1 : 0;
+        }
+    }
     if (const JVal* v = get("gs_symmetric")) cfg.gs_symmetric = to_bool(*v);
     if (const JVal* v = get("tail")) {
         static const std::map<std::string, int> kTail = {{"auto", -1}, {"off", 0}, {"on", 1}};

@@ -77,7 +77,9 @@ struct Config {
     // ── extensions of this backend (no counterpart in the reference config) ──
     Precision precision = Precision::FP64;  // arithmetic of the V/W cycle
     uint64_t seed = 0x5EEDull;              // seed of the setup RNG (PMIS/HMIS measures)
-    bool graphs = true;                     // replay the preconditioner cycle from a captured hipGraph
+    int graphs = -1;                        // -1 auto (batched PCG: replayed graphs; one system: eager launches),
+                                            // 1: replay the preconditioner cycle / PCG iteration from a captured
+                                            // hipGraph, 0: launch eagerly
     int fuse = -1;                          // zero-guess step + residual, one-pass down / up sweeps: -1 autotuned, 0 never, 1 always
     int kernel = -1;                        // sparse-pass kernel: -1 autotuned, 0 stream, 1 row-sequential, 2 pipelined,
                                             // 3 stream with 16-bit columns, 4 column-sorted, 5 pipelined 16-bit,

@@ -169,7 +169,9 @@ class AMGConfig:
     # backend extensions
     precision: str = "fp64"
     seed: int = 0x5EED
-    graphs: bool = True
+    # True: replay the preconditioner cycle (batched: each PCG iteration) from a captured hipGraph; False:
+    # eager launches; "auto": graphs for batches, eager launches for one system
+    graphs: object = "auto"
     # sparse-pass kernel: "auto" (autotuned), "stream", "rowseq", "pipe", "stream16", "sorted", "stream16pipe",
     # "sortedpipe", "sortedquad", "vector" (FP32 / BF16 cycles; FP64 passes keep the stream kernel), "pattern"
     # (operators whose rows repeat a few column patterns relative to the row, e.g. a grid stencil; others take

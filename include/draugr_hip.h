@@ -48,8 +48,9 @@ const char *draugr_amg_last_error(void);
 /* Config from JSON; all keys optional, unknown keys ignored, defaults of
  * AMGConfig() (src/types.jl:514-536).  Keys of c_api/draugr_amg.h:58-89 plus
  * "precision" ("fp64" | "fp32" | "bf16": arithmetic of the V/W cycle),
- * "seed" (setup RNG seed), "graphs" (true: the PCG / mixed-precision preconditioner cycle is
- * captured once into a hipGraph and replayed; false: launched eagerly),
+ * "seed" (setup RNG seed), "graphs" (true: the PCG / mixed-precision preconditioner cycle, and
+ * each batched PCG iteration, is captured once into a hipGraph and replayed; false: launched
+ * eagerly; "auto", the default: graphs for batches, eager launches for one system),
  * "kernel" ("auto" | "stream" | "rowseq" | "pipe" | "stream16" | "sorted" | "stream16pipe" |
  * "sortedpipe" | "sortedquad" | "vector" (lanes per row by row length; FP32 / BF16
  * cycles only, FP64 passes take "stream") | "pattern" (operators whose rows repeat at most
