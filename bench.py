@@ -433,8 +433,9 @@ def main():
     # work can see the GPU still clocking up)
     cyc = {}
     for name, c in (("fp64", cfg64), (args.precision, cfg32), (other, cfg_o)):
-        # the cycle as the PCG applies it: graph replays on its own vectors,
-        # 20 back to back from one call
+        # the cycle as the PCG applies it (launched as the config says: eager
+        # by default, graph replays with graphs=true) on its own vectors, 20
+        # back to back from one call
         h.precondition_repeat(b, c, 2)
         t_c = min(event_time(torch, lambda: h.precondition_repeat(b, c, 20), 1) / 20 for _ in range(3))
         # the same cycle through the C-ABI call on caller vectors (adds the

@@ -192,7 +192,8 @@ int32_t draugr_hip_level_op_repeat(int32_t handle, int32_t level, int32_t op, in
                                    const void *in2, void *out, void *out2, int32_t reps, void *stream);
 
 /* The PCG's preconditioner application (the zero-guess cycle in the config's
- * precision, replayed from its hipGraph when "graphs" is set) `reps` times
+ * precision, replayed from its hipGraph when "graphs" is true, else launched
+ * eagerly) `reps` times
  * back to back on the hierarchy's own residual and correction vectors, the
  * residual set from the device vector b first: the cycle time a PCG iteration
  * sees, for measurement (draugr_hip_cycle_device adds the caller-vector
