@@ -266,7 +266,8 @@ Config config_from_json(const char* json) {
     if (const JVal* v = get("seed")) cfg.seed = (uint64_t)std::llround(to_float(*v));
     if (const JVal* v = get("graphs")) {
         if (v->kind == JVal::Str) {
-            static const std::map<std::string, int> kGraphs = {{"auto", -1}};
+            static const std::map<std::string, int> kGraphs = {{"auto", -1}, {"true", 1}, {"false", 0},
+                                                                {"1", 1},     {"0", 0}};
             cfg.graphs = lookup(kGraphs, to_string(*v), "graphs");
         } else {
             cfg.graphs = to_bool(*v) ? 1 : 0;

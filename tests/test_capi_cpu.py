@@ -41,7 +41,13 @@ def test_config_json_defaults_and_errors(dh):
     assert L.draugr_amg_config_free(h) == 0
     assert L.draugr_amg_config_free(h) == -1
     assert "Invalid config handle" in dh.last_error()
-    for bad, msg in [(b'{"coarsening": "foo"}', "Unknown coarsening"),
+    # graphs: a boolean, a string-encoded boolean or "auto"
+    for g in (b'true', b'false', b'"true"', b'"FALSE"', b'"auto"', b'"1"'):
+        h = L.draugr_amg_config_from_json(b'{"graphs": ' + g + b'}')
+        assert h > 0, (g, dh.last_error())
+        assert L.draugr_amg_config_free(h) == 0
+    for bad, msg in [(b'{"graphs": "sometimes"}', "Unknown graphs"),
+                     (b'{"coarsening": "foo"}', "Unknown coarsening"),
                      (b'{"smoother": "bar"}', "Unknown smoother"),
                      (b'{"interpolation": "baz"}', "Unknown interpolation"),
                      (b'{"cycle": "x"}', "Unknown cycle"),
